@@ -1,0 +1,142 @@
+// Device-side building blocks of the ring-Ising statevector engine (gfx950).
+//
+// Amplitudes are complex128 stored as double2 {re, im}; wire w is bit w of the
+// basis index.  All arithmetic is IEEE FP64.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace rie {
+
+// cos/sin of the half angles of one wire's RY(theta) and RZ(phi)
+// (reference: cpp/ising_cuda_rotation_kernels.cu:661-675 computes the same
+// quantities on the host; here they live in a device table so that captured
+// graphs can re-read parameters).
+struct WireCoef {
+    double c;   // cos(theta / 2)
+    double s;   // sin(theta / 2)
+    double ch;  // cos(phi / 2)
+    double sh;  // sin(phi / 2)
+};
+
+constexpr int kWave = 64;
+
+__device__ __forceinline__ uint64_t low_mask(int n) {
+    return n >= 64 ? ~uint64_t{0} : ((uint64_t{1} << n) - 1);
+}
+
+// Image of index i under the ring-CNOT layer P = CNOT(n-1,0)...CNOT(0,1):
+// bits k >= 1 become the XOR of bits 0..k, bit 0 the parity of bits 1..n-1.
+// (reference: cpp/ising_cuda_kernels.cu:178-205, forward branch)
+__device__ __forceinline__ uint64_t ring_fwd(uint64_t i, int n) {
+    uint64_t p = i;
+    p ^= p << 1;
+    p ^= p << 2;
+    p ^= p << 4;
+    p ^= p << 8;
+    p ^= p << 16;
+    p ^= p << 32;
+    p &= low_mask(n);
+    return (p & ~uint64_t{1}) | (uint64_t)(__popcll(i >> 1) & 1);
+}
+
+// P^-1(i) = (i ^ (i << 1)) & mask, with bits 0 and 1 flipped when the top bit
+// of i is set (reference: cpp/ising_cuda_kernels.cu:178-205, inverse branch).
+__device__ __forceinline__ uint64_t ring_inv(uint64_t i, int n) {
+    uint64_t t = (i ^ (i << 1)) & low_mask(n);
+    if ((i >> (n - 1)) & 1U) {
+        t ^= 3U;
+    }
+    return t;
+}
+
+// 32-bit versions for tile-local offsets (n <= 31).
+__device__ __forceinline__ uint32_t ring_fwd32(uint32_t i, int n) {
+    uint32_t p = i;
+    p ^= p << 1;
+    p ^= p << 2;
+    p ^= p << 4;
+    p ^= p << 8;
+    p ^= p << 16;
+    p &= (1u << n) - 1u;
+    return (p & ~1u) | (uint32_t)(__popc(i >> 1) & 1);
+}
+
+__device__ __forceinline__ uint32_t ring_inv32(uint32_t i, int n) {
+    uint32_t t = (i ^ (i << 1)) & ((1u << n) - 1u);
+    if ((i >> (n - 1)) & 1U) {
+        t ^= 3U;
+    }
+    return t;
+}
+
+// Diagonal of -sum_w Z_w Z_{w+1 mod n}: 2 * (#domain walls) - n
+// (reference: cpp/ising_cuda_kernels.cu:313-330).
+__device__ __forceinline__ double ham_diag(uint64_t i, int n) {
+    const uint64_t m = low_mask(n);
+    const uint64_t rot = ((i << 1) & m) | (i >> (n - 1));
+    const int walls = (int)__popcll((i ^ rot) & m);  // __popcll is unsigned
+    return (double)(2 * walls - n);
+}
+
+// (a0, a1) <- RZ(phi) RY(theta) (a0, a1) on one wire pair.
+__device__ __forceinline__ void fwd_pair(double2 &a0, double2 &a1, const WireCoef &k) {
+    const double b0x = k.c * a0.x - k.s * a1.x;
+    const double b0y = k.c * a0.y - k.s * a1.y;
+    const double b1x = k.s * a0.x + k.c * a1.x;
+    const double b1y = k.s * a0.y + k.c * a1.y;
+    a0.x = k.ch * b0x + k.sh * b0y;
+    a0.y = k.ch * b0y - k.sh * b0x;
+    a1.x = k.ch * b1x - k.sh * b1y;
+    a1.y = k.ch * b1y + k.sh * b1x;
+}
+
+// (a0, a1) <- (RZ(phi) RY(theta))^dagger (a0, a1).
+__device__ __forceinline__ void inv_pair(double2 &a0, double2 &a1, const WireCoef &k) {
+    const double p0x = k.ch * a0.x - k.sh * a0.y;
+    const double p0y = k.ch * a0.y + k.sh * a0.x;
+    const double p1x = k.ch * a1.x + k.sh * a1.y;
+    const double p1y = k.ch * a1.y - k.sh * a1.x;
+    a0.x = k.c * p0x + k.s * p1x;
+    a0.y = k.c * p0y + k.s * p1y;
+    a1.x = k.c * p1x - k.s * p0x;
+    a1.y = k.c * p1y - k.s * p0y;
+}
+
+// One adjoint step of U = RZ(phi) RY(theta) on a pair.  On entry (c, l) are
+// the state and the adjoint state just after U; on exit just before it.
+//   dE/dphi   += Re<l_after | -iZ | c_after>   = Im(l0* c0) - Im(l1* c1)
+//   dE/dtheta += Re<l_before| -iY | c_before>  = Re(l1* c0) - Re(l0* c1)
+// (same quantities as cpp/ising_cuda_statevector_grad_kernels.cu:223-319,
+//  which forms them from the half-rotated lambda.)
+__device__ __forceinline__ void bwd_pair(double2 &c0, double2 &c1, double2 &l0, double2 &l1,
+                                         const WireCoef &k, double &dt, double &dp) {
+    dp += (l0.x * c0.y - l0.y * c0.x) - (l1.x * c1.y - l1.y * c1.x);
+    inv_pair(c0, c1, k);
+    inv_pair(l0, l1, k);
+    dt += (l1.x * c0.x + l1.y * c0.y) - (l0.x * c1.x + l0.y * c1.y);
+}
+
+// Components of U|0> for one wire: u0 = c e^{-i phi/2}, u1 = s e^{+i phi/2}
+// (reference: cpp/ising_cuda_rotation_kernels.cu:686-703).
+__device__ __forceinline__ double2 u0_of(const WireCoef &k) {
+    return make_double2(k.c * k.ch, -k.c * k.sh);
+}
+__device__ __forceinline__ double2 u1_of(const WireCoef &k) {
+    return make_double2(k.s * k.ch, k.s * k.sh);
+}
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+        v += __shfl_xor(v, off, kWave);
+    }
+    return v;
+}
+
+}  // namespace rie

@@ -1,0 +1,918 @@
+// Host orchestration and C ABI of the ring-Ising statevector engine.
+//
+// One engine owns the device buffers of one (n, L, h, strategy) problem
+// (reference: RingIsingCudaBackend::Impl, cpp/ising_cuda_backend.cu:333-401)
+// and issues the HIP kernels of rie_kernels.hip on one stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "rie.h"
+#include "rie_engine_internal.hpp"
+#include "rie_kernels.hpp"
+
+namespace rie {
+
+thread_local std::string g_last_error;
+
+Error::Error(rie_status st, const std::string &msg) : std::runtime_error(msg), status(st) {}
+
+void fail(rie_status st, const std::string &msg) { throw Error(st, msg); }
+
+void check(hipError_t e, const char *ctx) {
+    if (e == hipSuccess) {
+        return;
+    }
+    const rie_status st =
+        (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) ? RIE_OUT_OF_MEMORY
+                                                                     : RIE_DEVICE_ERROR;
+    fail(st, std::string(ctx) + ": " + hipGetErrorString(e));
+}
+
+void check_launch(const char *ctx) { check(hipGetLastError(), ctx); }
+
+void *device_alloc(size_t bytes, const char *what) {
+    void *p = nullptr;
+    if (bytes == 0) {
+        return nullptr;
+    }
+    check(hipMalloc(&p, bytes), what);
+    return p;
+}
+
+void device_free(void *p) {
+    if (p != nullptr) {
+        (void)hipFree(p);
+    }
+}
+
+int parse_strategy(const std::string &name) {
+    // reference: cpp/ising_cuda_backend.cu:249-264
+    if (name == "inverse_walk_cuQuantum" || name == "inverse_walk_cuquantum") {
+        return RIE_STRATEGY_INVERSE_WALK_GATE;
+    }
+    if (name == "structured_adjoint") {
+        return RIE_STRATEGY_STRUCTURED_ADJOINT;
+    }
+    if (name == "dense_scan") {
+        return RIE_STRATEGY_DENSE_SCAN;
+    }
+    fail(RIE_INVALID_ARGUMENT,
+         "Unknown gradient strategy. Expected one of: inverse_walk_cuQuantum, "
+         "structured_adjoint, dense_scan.");
+}
+
+// ---------------------------------------------------------------------------
+// Plan: the tile passes of one rotation layer.
+// ---------------------------------------------------------------------------
+
+static int pass_grid(uint64_t tiles, int max_grid) {
+    return (int)std::min<uint64_t>(tiles, (uint64_t)max_grid);
+}
+
+void Engine::build_plan() {
+    passes.clear();
+    if (n <= kSmallMaxLdsQubits) {
+        small = true;
+        return;
+    }
+    small = false;
+    R = 4;
+    T = 3 * R;
+    const uint64_t tiles = uint64_t{1} << (n - T);
+    const uint64_t full = (n >= 64) ? ~uint64_t{0} : ((uint64_t{1} << n) - 1);
+    const uint64_t tile_mask = (uint64_t{1} << T) - 1;
+    PassGeom low{};
+    low.n = n;
+    low.a = T;
+    low.h0 = T;
+    low.rot_lo = 0;
+    low.rot_hi = T;
+    low.num_tiles = tiles;
+    low.hi_mask = full & ~tile_mask;
+    passes.push_back({low, pass_grid(tiles, max_grid), 0, T});
+    // High wires [T, n) in groups of at most T - a (a = 3 contiguous low bits).
+    const int a = 3;
+    const int width = T - a;
+    const int rest = n - T;
+    const int groups = (rest + width - 1) / width;
+    int g0 = T;
+    for (int gi = 0; gi < groups; ++gi) {
+        const int bg = rest / groups + (gi < rest % groups ? 1 : 0);
+        PassGeom hg{};
+        hg.n = n;
+        hg.a = a;
+        hg.h0 = std::min(g0, n - width);
+        hg.rot_lo = a + (g0 - hg.h0);
+        hg.rot_hi = hg.rot_lo + bg;
+        hg.num_tiles = tiles;
+        hg.hi_mask = 0;
+        passes.push_back({hg, pass_grid(tiles, max_grid), g0, g0 + bg});
+        g0 += bg;
+    }
+}
+
+uint64_t Engine::wires_before_pass(size_t p) const {
+    uint64_t mask = 0;
+    for (size_t i = 0; i < p; ++i) {
+        for (int w = passes[i].first_wire; w < passes[i].end_wire; ++w) {
+            mask |= uint64_t{1} << w;
+        }
+    }
+    return mask;
+}
+
+Engine::Engine(size_t num_qubits, size_t num_layers, double field, const std::string &strat,
+               size_t chunk_width, bool double_buffer_)
+    : n((int)num_qubits), L((int)num_layers), h(field), strategy(parse_strategy(strat)),
+      chunk_width(chunk_width), double_buffer(double_buffer_) {
+    // reference: validate_and_get_state_size (cpp/ising_cuda_backend.cu:229-241),
+    // Impl ctor width checks (:388-397), dense_scan limit (dense_modes.inc:353-355).
+    if (num_qubits < 2) {
+        fail(RIE_INVALID_ARGUMENT, "num_qubits must be at least 2.");
+    }
+    if (num_qubits > 34) {
+        fail(RIE_INVALID_ARGUMENT, "num_qubits must be at most 34 on one device.");
+    }
+    if (num_layers < 1) {
+        fail(RIE_INVALID_ARGUMENT, "layers must be at least 1.");
+    }
+    if (strategy == RIE_STRATEGY_STRUCTURED_ADJOINT) {
+        if (chunk_width == 0) {
+            fail(RIE_INVALID_ARGUMENT, "structured_rotation_chunk_width must be at least 1.");
+        }
+        if (chunk_width > 8) {
+            fail(RIE_INVALID_ARGUMENT,
+                 "structured_rotation_chunk_width exceeds supported maximum.");
+        }
+    }
+    if (strategy == RIE_STRATEGY_DENSE_SCAN && num_qubits > 8) {
+        fail(RIE_INVALID_ARGUMENT, "dense_scan requires num_qubits <= 8.");
+    }
+    N = uint64_t{1} << n;
+    num_params = (size_t)2 * n * L;
+    check(hipGetDevice(&device), "hipGetDevice");
+    hipDeviceProp_t prop;
+    check(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
+    num_cus = prop.multiProcessorCount;
+    max_grid = std::max(64, num_cus * 8);
+    check(hipStreamCreateWithFlags(&own_stream, hipStreamNonBlocking), "hipStreamCreate");
+    check(hipEventCreate(&ev_a), "hipEventCreate");
+    check(hipEventCreate(&ev_b), "hipEventCreate");
+    build_plan();
+
+    d_params = (double *)device_alloc(sizeof(double) * num_params, "alloc params");
+    d_grad = (double *)device_alloc(sizeof(double) * num_params, "alloc gradient");
+    d_energy = (double *)device_alloc(sizeof(double) * 4, "alloc energy");
+    d_coef = (WireCoef *)device_alloc(sizeof(WireCoef) * (size_t)n * L, "alloc coefficients");
+
+    if (strategy == RIE_STRATEGY_STRUCTURED_ADJOINT && small) {
+        // the whole circuit lives in one workgroup's LDS
+    } else if (strategy == RIE_STRATEGY_STRUCTURED_ADJOINT) {
+        for (int i = 0; i < 4; ++i) {
+            st[i] = (double2 *)device_alloc(sizeof(double2) * N, "alloc statevector");
+        }
+        size_t part = 0;
+        for (const auto &p : passes) {
+            part = std::max<size_t>(part, (size_t)p.grid * 2 * T);
+        }
+        size_t eparts = 0;
+        for (const auto &p : passes) {
+            eparts += (size_t)p.grid;
+        }
+        partial_count = std::max(part, eparts);
+        d_partials = (double *)device_alloc(sizeof(double) * partial_count, "alloc partials");
+        d_epartials = (double *)device_alloc(sizeof(double) * eparts, "alloc energy partials");
+    } else {
+        // gate-level and dense strategies: psi, lambda
+        for (int i = 0; i < 2; ++i) {
+            st[i] = (double2 *)device_alloc(sizeof(double2) * N, "alloc statevector");
+        }
+        partial_count = (size_t)elementwise_grid(N);
+        d_partials = (double *)device_alloc(sizeof(double) * partial_count, "alloc partials");
+        d_epartials = (double *)device_alloc(sizeof(double) * partial_count, "alloc energy partials");
+    }
+    if (strategy == RIE_STRATEGY_DENSE_SCAN) {
+        dense_init();
+    }
+}
+
+Engine::~Engine() {
+    if (own_stream != nullptr) {
+        (void)hipStreamSynchronize(own_stream);
+    }
+    for (auto &c : ckpts) {
+        device_free(c);
+    }
+    for (int i = 0; i < 4; ++i) {
+        device_free(st[i]);
+    }
+    device_free(d_params);
+    device_free(d_grad);
+    device_free(d_energy);
+    device_free(d_coef);
+    device_free(d_partials);
+    device_free(d_epartials);
+    device_free(d_drift);
+    dense_free();
+    if (ev_a != nullptr) {
+        (void)hipEventDestroy(ev_a);
+    }
+    if (ev_b != nullptr) {
+        (void)hipEventDestroy(ev_b);
+    }
+    if (own_stream != nullptr) {
+        (void)hipStreamDestroy(own_stream);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Stage timing (reference stage names: structured_adjoint_modes.inc:317-370)
+// ---------------------------------------------------------------------------
+
+void Engine::stage_begin(hipStream_t s) {
+    if (profiling) {
+        check(hipEventRecord(ev_a, s), "hipEventRecord");
+    }
+}
+
+void Engine::stage_end(hipStream_t s, const char *name) {
+    if (!profiling) {
+        return;
+    }
+    check(hipEventRecord(ev_b, s), "hipEventRecord");
+    check(hipEventSynchronize(ev_b), "hipEventSynchronize");
+    float ms = 0.0f;
+    check(hipEventElapsedTime(&ms, ev_a, ev_b), "hipEventElapsedTime");
+    timings.emplace_back(name, (double)ms);
+}
+
+// ---------------------------------------------------------------------------
+// Structured adjoint: tile-pass path (n > 12)
+// ---------------------------------------------------------------------------
+
+void Engine::structured_forward(hipStream_t s, bool save_ckpts) {
+    // Forward working buffers are st[0] and st[1]; boundary states that are
+    // checkpointed are written straight into their checkpoint buffer by the
+    // low (scatter) pass and read from there by the next layer's first pass.
+    launch_init_product_cnot(st[0], d_coef, n, s);
+    check_launch("init_product_cnot");
+    const double2 *cur = st[0];
+    int cur_buf = 0;  // -1 while cur is a checkpoint buffer
+    for (int l = 1; l < L; ++l) {
+        const WireCoef *lc = d_coef + (size_t)l * n;
+        for (size_t p = 1; p < passes.size(); ++p) {
+            double2 *out = (cur_buf >= 0) ? st[cur_buf] : st[0];
+            launch_fwd_pass(R, false, cur, out, lc, passes[p].g, passes[p].grid, s);
+            check_launch("fwd_pass");
+            if (cur_buf < 0) {
+                cur_buf = 0;
+            }
+            cur = out;
+        }
+        const int boundary = l + 1;  // state after layer l
+        const bool ckpt_due = save_ckpts && ckpt_every > 0 && boundary % (int)ckpt_every == 0 &&
+                              boundary < L;
+        double2 *out;
+        int out_buf;
+        if (ckpt_due) {
+            out = ckpts[(size_t)boundary / ckpt_every - 1];
+            out_buf = -1;
+        } else {
+            out_buf = (cur_buf == 0) ? 1 : 0;
+            out = st[out_buf];
+        }
+        launch_fwd_pass(R, true, cur, out, lc, passes[0].g, passes[0].grid, s);
+        check_launch("fwd_pass scatter");
+        cur = out;
+        cur_buf = out_buf;
+    }
+    final_cur = cur_buf;  // the last boundary (L) is never a checkpoint
+}
+
+const double2 *Engine::final_state() const { return st[final_cur]; }
+
+int Engine::free_buffer(std::initializer_list<int> used) const {
+    for (int i = 0; i < 4; ++i) {
+        bool taken = false;
+        for (int u : used) {
+            taken = taken || (u == i);
+        }
+        if (!taken) {
+            return i;
+        }
+    }
+    fail(RIE_DEVICE_ERROR, "internal: no free state buffer");
+}
+
+void Engine::structured_hamiltonian(hipStream_t s, bool write_lam) {
+    const double2 *psi = final_state();
+    lam_idx = write_lam ? free_buffer({final_cur}) : -1;
+    double2 *lam = write_lam ? st[lam_idx] : nullptr;
+    size_t off = 0;
+    for (size_t p = 0; p < passes.size(); ++p) {
+        launch_ham_pass(R, p == 0, write_lam && p > 0, write_lam, psi, lam, lam, passes[p].g, h,
+                        d_epartials + off, passes[p].grid, s);
+        check_launch("ham_pass");
+        off += (size_t)passes[p].grid;
+    }
+    energy_partials = off;
+}
+
+void Engine::structured_backward(hipStream_t s, const double *scale, bool full_walk) {
+    // Backward sweep: layer L-1 .. 0; in each layer the low pass (with the
+    // inverse ring CNOT on load) comes first, then the high passes in place.
+    int cur = final_cur;          // may be -1 (state in a checkpoint)
+    const double2 *cur_in = final_state();
+    int lam = lam_idx;
+    for (int l = L - 1; l >= 0; --l) {
+        const WireCoef *lc = d_coef + (size_t)l * n;
+        const bool analytic = (l == 0) && !full_walk;
+        const bool last_layer = (l == 0);
+        const int ncur = analytic ? -1 : free_buffer({cur, lam});
+        const int nlam = free_buffer({cur, lam, ncur});
+        const bool only_low = passes.size() == 1;
+        const bool write_lam0 = !(last_layer && only_low && !full_walk);
+        launch_bwd_pass(R, true, analytic, !analytic, write_lam0, cur_in, st[lam],
+                        analytic ? nullptr : st[ncur], st[nlam], lc, passes[0].g, 0, d_partials,
+                        passes[0].grid, s);
+        check_launch("bwd_pass gather");
+        launch_reduce_grad(d_partials, passes[0].grid, T, passes[0].g, l, d_grad, scale, s);
+        check_launch("reduce_grad");
+        cur = ncur;
+        lam = nlam;
+        for (size_t p = 1; p < passes.size(); ++p) {
+            const bool final_pass = last_layer && (p + 1 == passes.size()) && !full_walk;
+            const uint64_t processed = analytic ? wires_before_pass(p) : 0;
+            launch_bwd_pass(R, false, analytic, !analytic, !final_pass,
+                            analytic ? nullptr : st[cur], st[lam], analytic ? nullptr : st[cur],
+                            st[lam], lc, passes[p].g, processed, d_partials, passes[p].grid, s);
+            check_launch("bwd_pass");
+            launch_reduce_grad(d_partials, passes[p].grid, T, passes[p].g, l, d_grad, scale, s);
+            check_launch("reduce_grad");
+        }
+        if (l > 0) {
+            // Restore the stored boundary state (after layer l-1) if there is one.
+            if (!analytic && ckpt_every > 0 && (l % (int)ckpt_every) == 0 &&
+                (size_t)(l / ckpt_every) <= ckpts.size()) {
+                cur_in = ckpts[(size_t)l / ckpt_every - 1];
+                cur = -2;  // not one of the working buffers
+            } else {
+                cur_in = st[cur];
+            }
+        }
+    }
+    walked_cur = full_walk ? cur : -1;
+}
+
+// ---------------------------------------------------------------------------
+// Structured adjoint: whole-circuit path (n <= 12)
+// ---------------------------------------------------------------------------
+
+void Engine::small_run(hipStream_t s, bool want_grad, const double *scale, double *energy_out,
+                       double *grad_out, double *drift_out) {
+    SmallJob job{};
+    job.n = n;
+    job.layers = L;
+    job.field = h;
+    job.coef = d_coef;
+    job.energy = energy_out;
+    job.grad = want_grad ? grad_out : nullptr;
+    job.drift = drift_out;
+    job.gstate = nullptr;
+    job.scale = scale;
+    launch_small(nullptr, job, 1, n, true, s);
+    check_launch("small_circuit");
+}
+
+// ---------------------------------------------------------------------------
+// Gate-level adjoint (inverse_walk_cuQuantum)
+// ---------------------------------------------------------------------------
+
+void Engine::gate_forward(hipStream_t s) {
+    launch_zero_state(st[0], N, s);
+    check_launch("zero_state");
+    for (int l = 0; l < L; ++l) {
+        for (int w = 0; w < n; ++w) {
+            const WireCoef *k = d_coef + (size_t)l * n + w;
+            launch_gate_ry(st[0], n, w, k, false, s);
+            launch_gate_rz(st[0], n, w, k, false, s);
+        }
+        for (int w = 0; w < n; ++w) {
+            launch_gate_cnot(st[0], n, w, (w + 1) % n, s);
+        }
+    }
+    check_launch("gate forward");
+}
+
+void Engine::gate_hamiltonian(hipStream_t s) {
+    const int grid = elementwise_grid(N);
+    launch_ham_gather(st[0], st[1], n, h, d_epartials, grid, s);
+    check_launch("ham_gather");
+    energy_partials = (size_t)grid;
+}
+
+void Engine::gate_backward(hipStream_t s, const double *scale) {
+    const int grid = elementwise_grid(N / 2);
+    for (int l = L - 1; l >= 0; --l) {
+        for (int w = n - 1; w >= 0; --w) {
+            launch_gate_cnot(st[0], n, w, (w + 1) % n, s);
+            launch_gate_cnot(st[1], n, w, (w + 1) % n, s);
+        }
+        for (int w = n - 1; w >= 0; --w) {
+            const size_t base = ((size_t)l * n + w) * 2;
+            const WireCoef *k = d_coef + (size_t)l * n + w;
+            launch_gate_rz_step(st[0], st[1], n, w, k, d_partials, grid, s);
+            launch_reduce_to(d_partials, grid, d_grad + base + 1, scale, s);
+            launch_gate_ry_step(st[0], st[1], n, w, k, d_partials, grid, s);
+            launch_reduce_to(d_partials, grid, d_grad + base, scale, s);
+        }
+    }
+    check_launch("gate backward");
+}
+
+// ---------------------------------------------------------------------------
+// Device-resident drivers
+// ---------------------------------------------------------------------------
+
+void Engine::forward_device(const double *params, double *energy, hipStream_t s, bool keep_lam) {
+    launch_coef(params, d_coef, n * L, s);
+    check_launch("coef");
+    if (strategy == RIE_STRATEGY_DENSE_SCAN) {
+        dense_run(s, false, nullptr, energy, nullptr);
+        return;
+    }
+    if (strategy == RIE_STRATEGY_INVERSE_WALK_GATE) {
+        gate_forward(s);
+        gate_hamiltonian(s);
+        launch_reduce_sum(d_epartials, (int)energy_partials, energy, s);
+        check_launch("energy reduce");
+        resident = true;
+        return;
+    }
+    if (small) {
+        small_run(s, false, nullptr, energy, nullptr, nullptr);
+        resident = false;
+        return;
+    }
+    structured_forward(s, ckpt_every > 0);
+    structured_hamiltonian(s, keep_lam);
+    launch_reduce_sum(d_epartials, (int)energy_partials, energy, s);
+    check_launch("energy reduce");
+    resident = keep_lam;
+}
+
+void Engine::backward_device(const double *params, const double *scale, double *grad,
+                             hipStream_t s) {
+    if (strategy == RIE_STRATEGY_DENSE_SCAN) {
+        launch_coef(params, d_coef, n * L, s);
+        dense_run(s, true, scale, d_energy + 1, grad);
+        return;
+    }
+    if (strategy == RIE_STRATEGY_INVERSE_WALK_GATE) {
+        if (!resident) {
+            forward_device(params, d_energy + 1, s, true);
+        }
+        gate_backward(s, scale);
+        copy_grad_if_needed(grad, s);
+        resident = false;
+        return;
+    }
+    if (small) {
+        launch_coef(params, d_coef, n * L, s);
+        small_run(s, true, scale, d_energy + 1, grad, nullptr);
+        return;
+    }
+    if (!resident) {
+        forward_device(params, d_energy + 1, s, true);
+    }
+    structured_backward(s, scale, false);
+    copy_grad_if_needed(grad, s);
+    resident = false;
+}
+
+void Engine::copy_grad_if_needed(double *grad, hipStream_t s) {
+    if (grad != d_grad) {
+        check(hipMemcpyAsync(grad, d_grad, sizeof(double) * num_params, hipMemcpyDeviceToDevice, s),
+              "copy gradient");
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host-pointer driver with stage profiling
+// ---------------------------------------------------------------------------
+
+void Engine::energy_and_grad_host(const double *params, size_t count, bool want_grad,
+                                  bool profile, double *energy_out, double *grad_out) {
+    if (count != num_params) {
+        fail(RIE_INVALID_ARGUMENT, "num_params does not match the circuit.");
+    }
+    if (params == nullptr || energy_out == nullptr || (want_grad && grad_out == nullptr)) {
+        fail(RIE_INVALID_ARGUMENT, "null pointer argument.");
+    }
+    hipStream_t s = own_stream;
+    timings.clear();
+    profiling = profile;
+    auto t0 = std::chrono::steady_clock::now();
+    check(hipMemcpyAsync(d_params, params, sizeof(double) * num_params, hipMemcpyHostToDevice, s),
+          "upload params");
+    if (profile) {
+        check(hipStreamSynchronize(s), "sync");
+        const double host_ms =
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
+                .count();
+        timings.emplace_back("build_ops_host_ms", host_ms);
+    }
+    if (strategy == RIE_STRATEGY_DENSE_SCAN) {
+        stage_begin(s);
+        launch_coef(d_params, d_coef, n * L, s);
+        dense_run(s, want_grad, nullptr, d_energy, want_grad ? d_grad : nullptr);
+        stage_end(s, "dense_scan_ms");
+    } else if (strategy == RIE_STRATEGY_STRUCTURED_ADJOINT && small) {
+        stage_begin(s);
+        launch_coef(d_params, d_coef, n * L, s);
+        stage_end(s, "zero_state_ms");
+        stage_begin(s);
+        small_run(s, want_grad, nullptr, d_energy, want_grad ? d_grad : nullptr, nullptr);
+        stage_end(s, want_grad ? "fused_circuit_ms" : "forward_apply_ms");
+    } else {
+        const bool gate = strategy == RIE_STRATEGY_INVERSE_WALK_GATE;
+        stage_begin(s);
+        launch_coef(d_params, d_coef, n * L, s);
+        check_launch("coef");
+        stage_end(s, "zero_state_ms");
+        stage_begin(s);
+        if (gate) {
+            gate_forward(s);
+        } else {
+            structured_forward(s, want_grad && ckpt_every > 0);
+        }
+        stage_end(s, "forward_apply_ms");
+        stage_begin(s);
+        if (gate) {
+            gate_hamiltonian(s);
+        } else {
+            structured_hamiltonian(s, want_grad);
+        }
+        stage_end(s, "hamiltonian_ms");
+        stage_begin(s);
+        launch_reduce_sum(d_epartials, (int)energy_partials, d_energy, s);
+        check_launch("energy reduce");
+        stage_end(s, "energy_dot_ms");
+        if (want_grad) {
+            stage_begin(s);
+            stage_end(s, "gradient_zero_ms");
+            stage_begin(s);
+            if (gate) {
+                gate_backward(s, nullptr);
+            } else {
+                structured_backward(s, nullptr, false);
+            }
+            stage_end(s, "backward_sweep_ms");
+        }
+    }
+    stage_begin(s);
+    check(hipMemcpyAsync(energy_out, d_energy, sizeof(double), hipMemcpyDeviceToHost, s),
+          "copy energy");
+    if (want_grad) {
+        check(hipMemcpyAsync(grad_out, d_grad, sizeof(double) * num_params, hipMemcpyDeviceToHost,
+                             s),
+              "copy gradient");
+    }
+    check(hipStreamSynchronize(s), "synchronize");
+    stage_end(s, "gradient_copy_ms");
+    profiling = false;
+    resident = false;
+}
+
+double Engine::inverse_walk_drift(const double *params, size_t count) {
+    if (count != num_params) {
+        fail(RIE_INVALID_ARGUMENT, "num_params does not match the circuit.");
+    }
+    if (strategy != RIE_STRATEGY_STRUCTURED_ADJOINT) {
+        fail(RIE_INVALID_ARGUMENT, "inverse-walk drift requires structured_adjoint.");
+    }
+    hipStream_t s = own_stream;
+    check(hipMemcpyAsync(d_params, params, sizeof(double) * num_params, hipMemcpyHostToDevice, s),
+          "upload params");
+    launch_coef(d_params, d_coef, n * L, s);
+    if (d_drift == nullptr) {
+        d_drift = (double *)device_alloc(sizeof(double) * 8200, "alloc drift");
+    }
+    double drift2 = 0.0;
+    if (small) {
+        small_run(s, true, nullptr, d_energy, d_grad, d_drift);
+        check(hipMemcpyAsync(&drift2, d_drift, sizeof(double), hipMemcpyDeviceToHost, s),
+              "copy drift");
+    } else {
+        structured_forward(s, ckpt_every > 0);
+        structured_hamiltonian(s, true);
+        structured_backward(s, nullptr, true);
+        const int grid = elementwise_grid(N);
+        launch_drift(st[walked_cur], N, d_drift, grid, s);
+        launch_reduce_sum(d_drift, grid, d_drift + 8192, s);
+        check_launch("drift");
+        check(hipMemcpyAsync(&drift2, d_drift + 8192, sizeof(double), hipMemcpyDeviceToHost, s),
+              "copy drift");
+    }
+    check(hipStreamSynchronize(s), "synchronize");
+    return std::sqrt(drift2);
+}
+
+void Engine::set_checkpoint_interval(size_t every) {
+    for (auto &c : ckpts) {
+        device_free(c);
+    }
+    ckpts.clear();
+    ckpt_every = every;
+    if (every == 0 || small || strategy != RIE_STRATEGY_STRUCTURED_ADJOINT) {
+        return;
+    }
+    const size_t count = (size_t)(L - 1) / every;
+    for (size_t i = 0; i < count; ++i) {
+        ckpts.push_back((double2 *)device_alloc(sizeof(double2) * N, "alloc checkpoint"));
+    }
+}
+
+void Engine::benchmark_forward_graph(const double *params, size_t count, size_t repeats,
+                                     double *normal_ms, double *graph_ms) {
+    // reference: benchmark_structured_forward_graph (cpp/ising_cuda_backend.cu:444-541)
+    if (count != num_params) {
+        fail(RIE_INVALID_ARGUMENT, "num_params does not match the circuit.");
+    }
+    if (strategy != RIE_STRATEGY_STRUCTURED_ADJOINT) {
+        fail(RIE_INVALID_ARGUMENT, "CUDA graph forward benchmark requires structured_adjoint.");
+    }
+    if (repeats == 0) {
+        fail(RIE_INVALID_ARGUMENT, "repeats must be at least 1.");
+    }
+    hipStream_t s = own_stream;
+    check(hipMemcpyAsync(d_params, params, sizeof(double) * num_params, hipMemcpyHostToDevice, s),
+          "upload params");
+    auto forward = [&]() {
+        launch_coef(d_params, d_coef, n * L, s);
+        if (small) {
+            small_run_captured(s);
+        } else {
+            structured_forward(s, false);
+        }
+    };
+    forward();
+    check(hipStreamSynchronize(s), "warmup sync");
+    auto time_it = [&](auto &&body) {
+        check(hipEventRecord(ev_a, s), "event");
+        for (size_t r = 0; r < repeats; ++r) {
+            body();
+        }
+        check(hipEventRecord(ev_b, s), "event");
+        check(hipEventSynchronize(ev_b), "event sync");
+        float ms = 0.0f;
+        check(hipEventElapsedTime(&ms, ev_a, ev_b), "elapsed");
+        return (double)ms / (double)repeats;
+    };
+    *normal_ms = time_it(forward);
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    check(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed), "begin capture");
+    forward();
+    check(hipStreamEndCapture(s, &graph), "end capture");
+    check(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "instantiate");
+    check(hipGraphLaunch(exec, s), "graph launch");
+    check(hipStreamSynchronize(s), "graph warmup");
+    *graph_ms = time_it([&]() { check(hipGraphLaunch(exec, s), "graph launch"); });
+    (void)hipGraphExecDestroy(exec);
+    (void)hipGraphDestroy(graph);
+}
+
+void Engine::small_run_captured(hipStream_t s) {
+    small_run(s, false, nullptr, d_energy + 2, nullptr, nullptr);
+}
+
+void Engine::copy_state(double *host_out, size_t count) {
+    if (count != 2 * N) {
+        fail(RIE_INVALID_ARGUMENT, "state buffer must hold 2 * 2^n doubles.");
+    }
+    if (strategy != RIE_STRATEGY_STRUCTURED_ADJOINT || small) {
+        fail(RIE_UNSUPPORTED, "copy_state requires structured_adjoint with num_qubits > 12.");
+    }
+    check(hipMemcpy(host_out, final_state(), sizeof(double2) * N, hipMemcpyDeviceToHost),
+          "copy state");
+}
+
+}  // namespace rie
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+
+using rie::Engine;
+
+namespace {
+
+template <typename F>
+rie_status guard(F &&f) {
+    try {
+        f();
+        return RIE_OK;
+    } catch (const rie::Error &e) {
+        rie::g_last_error = e.what();
+        return e.status;
+    } catch (const std::bad_alloc &) {
+        rie::g_last_error = "host allocation failed";
+        return RIE_OUT_OF_MEMORY;
+    } catch (const std::exception &e) {
+        rie::g_last_error = e.what();
+        return RIE_DEVICE_ERROR;
+    }
+}
+
+Engine *as_engine(rie_engine *e) {
+    if (e == nullptr) {
+        rie::fail(RIE_INVALID_ARGUMENT, "null engine");
+    }
+    return reinterpret_cast<Engine *>(e);
+}
+
+hipStream_t stream_of(Engine *e, void *stream) {
+    return stream != nullptr ? static_cast<hipStream_t>(stream) : e->own_stream;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *rie_last_error(void) { return rie::g_last_error.c_str(); }
+
+int rie_abi_version(void) { return RIE_ABI_VERSION; }
+
+rie_status rie_parse_strategy(const char *name, int *strategy_out) {
+    return guard([&]() {
+        if (name == nullptr || strategy_out == nullptr) {
+            rie::fail(RIE_INVALID_ARGUMENT, "null pointer argument.");
+        }
+        *strategy_out = rie::parse_strategy(name);
+    });
+}
+
+int rie_device_count(void) {
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return count;
+}
+
+rie_status rie_engine_create(size_t num_qubits, size_t num_layers, double field,
+                             const char *gradient_strategy, size_t chunk_width, int double_buffer,
+                             rie_engine **engine_out) {
+    return guard([&]() {
+        if (gradient_strategy == nullptr || engine_out == nullptr) {
+            rie::fail(RIE_INVALID_ARGUMENT, "null pointer argument.");
+        }
+        *engine_out = nullptr;
+        auto engine = std::make_unique<Engine>(num_qubits, num_layers, field, gradient_strategy,
+                                               chunk_width, double_buffer != 0);
+        *engine_out = reinterpret_cast<rie_engine *>(engine.release());
+    });
+}
+
+void rie_engine_destroy(rie_engine *engine) { delete reinterpret_cast<Engine *>(engine); }
+
+rie_status rie_engine_energy_and_grad(rie_engine *engine, const double *params, size_t num_params,
+                                      int compute_gradient, int profile, double *energy_out,
+                                      double *gradient_out) {
+    return guard([&]() {
+        as_engine(engine)->energy_and_grad_host(params, num_params, compute_gradient != 0,
+                                                profile != 0, energy_out, gradient_out);
+    });
+}
+
+size_t rie_engine_timing_count(const rie_engine *engine) {
+    if (engine == nullptr) {
+        return 0;
+    }
+    return reinterpret_cast<const Engine *>(engine)->timings.size();
+}
+
+rie_status rie_engine_timing(const rie_engine *engine, size_t index, const char **name_out,
+                             double *ms_out) {
+    return guard([&]() {
+        if (engine == nullptr || name_out == nullptr || ms_out == nullptr) {
+            rie::fail(RIE_INVALID_ARGUMENT, "null pointer argument.");
+        }
+        const auto *e = reinterpret_cast<const Engine *>(engine);
+        if (index >= e->timings.size()) {
+            rie::fail(RIE_INVALID_ARGUMENT, "timing index out of range.");
+        }
+        *name_out = e->timings[index].first.c_str();
+        *ms_out = e->timings[index].second;
+    });
+}
+
+rie_status rie_engine_benchmark_forward_graph(rie_engine *engine, const double *params,
+                                              size_t num_params, size_t repeats,
+                                              double *normal_forward_ms, double *graph_forward_ms) {
+    return guard([&]() {
+        if (params == nullptr || normal_forward_ms == nullptr || graph_forward_ms == nullptr) {
+            rie::fail(RIE_INVALID_ARGUMENT, "null pointer argument.");
+        }
+        as_engine(engine)->benchmark_forward_graph(params, num_params, repeats, normal_forward_ms,
+                                                   graph_forward_ms);
+    });
+}
+
+rie_status rie_energy_and_grad(size_t num_qubits, size_t num_layers, double field,
+                               const char *gradient_strategy, const double *params,
+                               size_t num_params, int compute_gradient, int profile,
+                               size_t chunk_width, double *energy_out, double *gradient_out) {
+    return guard([&]() {
+        if (gradient_strategy == nullptr) {
+            rie::fail(RIE_INVALID_ARGUMENT, "null pointer argument.");
+        }
+        Engine engine(num_qubits, num_layers, field, gradient_strategy, chunk_width, false);
+        engine.energy_and_grad_host(params, num_params, compute_gradient != 0, profile != 0,
+                                    energy_out, gradient_out);
+    });
+}
+
+size_t rie_engine_num_qubits(const rie_engine *engine) {
+    return engine == nullptr ? 0 : (size_t)reinterpret_cast<const Engine *>(engine)->n;
+}
+
+size_t rie_engine_num_layers(const rie_engine *engine) {
+    return engine == nullptr ? 0 : (size_t)reinterpret_cast<const Engine *>(engine)->L;
+}
+
+int rie_engine_strategy(const rie_engine *engine) {
+    return engine == nullptr ? -1 : reinterpret_cast<const Engine *>(engine)->strategy;
+}
+
+rie_status rie_engine_forward_device(rie_engine *engine, const double *d_params, double *d_energy,
+                                     void *stream) {
+    return guard([&]() {
+        Engine *e = as_engine(engine);
+        if (d_params == nullptr || d_energy == nullptr) {
+            rie::fail(RIE_INVALID_ARGUMENT, "null pointer argument.");
+        }
+        e->forward_device(d_params, d_energy, stream_of(e, stream), true);
+    });
+}
+
+rie_status rie_engine_backward_device(rie_engine *engine, const double *d_params,
+                                      const double *d_scale, double *d_gradient, void *stream) {
+    return guard([&]() {
+        Engine *e = as_engine(engine);
+        if (d_params == nullptr || d_gradient == nullptr) {
+            rie::fail(RIE_INVALID_ARGUMENT, "null pointer argument.");
+        }
+        e->backward_device(d_params, d_scale, d_gradient, stream_of(e, stream));
+    });
+}
+
+rie_status rie_engine_energy_and_grad_device(rie_engine *engine, const double *d_params,
+                                             double *d_energy, double *d_gradient, void *stream) {
+    return guard([&]() {
+        Engine *e = as_engine(engine);
+        if (d_params == nullptr || d_energy == nullptr || d_gradient == nullptr) {
+            rie::fail(RIE_INVALID_ARGUMENT, "null pointer argument.");
+        }
+        hipStream_t s = stream_of(e, stream);
+        e->forward_device(d_params, d_energy, s, true);
+        e->backward_device(d_params, nullptr, d_gradient, s);
+    });
+}
+
+rie_status rie_engine_set_checkpoint_interval(rie_engine *engine, size_t every_k_layers) {
+    return guard([&]() { as_engine(engine)->set_checkpoint_interval(every_k_layers); });
+}
+
+rie_status rie_engine_inverse_walk_drift(rie_engine *engine, const double *params,
+                                         size_t num_params, double *drift_out) {
+    return guard([&]() {
+        if (params == nullptr || drift_out == nullptr) {
+            rie::fail(RIE_INVALID_ARGUMENT, "null pointer argument.");
+        }
+        *drift_out = as_engine(engine)->inverse_walk_drift(params, num_params);
+    });
+}
+
+rie_status rie_engine_copy_state(rie_engine *engine, double *host_out, size_t num_doubles) {
+    return guard([&]() {
+        if (host_out == nullptr) {
+            rie::fail(RIE_INVALID_ARGUMENT, "null pointer argument.");
+        }
+        as_engine(engine)->copy_state(host_out, num_doubles);
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,128 @@
+// Engine state shared by the host-side translation units of librie.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <initializer_list>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "rie.h"
+#include "rie_kernels.hpp"
+
+namespace rie {
+
+struct Error : std::runtime_error {
+    Error(rie_status st, const std::string &msg);
+    rie_status status;
+};
+
+[[noreturn]] void fail(rie_status st, const std::string &msg);
+void check(hipError_t e, const char *ctx);
+void check_launch(const char *ctx);
+void *device_alloc(size_t bytes, const char *what);
+void device_free(void *p);
+int parse_strategy(const std::string &name);
+
+extern thread_local std::string g_last_error;
+
+struct PassPlan {
+    PassGeom g;
+    int grid;
+    int first_wire;  // wires [first_wire, end_wire) are processed by the pass
+    int end_wire;
+};
+
+struct DenseState;  // dense_scan buffers (rie_dense.hip)
+
+struct Engine {
+    Engine(size_t num_qubits, size_t num_layers, double field, const std::string &strategy,
+           size_t chunk_width, bool double_buffer);
+    ~Engine();
+    Engine(const Engine &) = delete;
+    Engine &operator=(const Engine &) = delete;
+
+    // problem
+    int n;
+    int L;
+    double h;
+    int strategy;
+    size_t chunk_width;
+    bool double_buffer;
+    uint64_t N = 0;
+    size_t num_params = 0;
+
+    // device / plan
+    int device = 0;
+    int num_cus = 0;
+    int max_grid = 0;
+    hipStream_t own_stream = nullptr;
+    bool small = false;
+    int R = 4;
+    int T = 12;
+    std::vector<PassPlan> passes;  // [0]: low pass (a = T); [1..]: high passes
+
+    // device buffers
+    double2 *st[4] = {nullptr, nullptr, nullptr, nullptr};
+    double *d_params = nullptr;
+    double *d_grad = nullptr;
+    double *d_energy = nullptr;
+    WireCoef *d_coef = nullptr;
+    double *d_partials = nullptr;
+    double *d_epartials = nullptr;
+    double *d_drift = nullptr;
+    size_t partial_count = 0;
+    size_t energy_partials = 0;
+    std::vector<double2 *> ckpts;
+    size_t ckpt_every = 0;
+    DenseState *dense = nullptr;
+
+    // resident-state bookkeeping
+    int final_cur = 0;
+    int lam_idx = -1;
+    int walked_cur = -1;
+    bool resident = false;
+
+    // profiling
+    bool profiling = false;
+    hipEvent_t ev_a = nullptr;
+    hipEvent_t ev_b = nullptr;
+    std::vector<std::pair<std::string, double>> timings;
+
+    void build_plan();
+    uint64_t wires_before_pass(size_t p) const;
+    const double2 *final_state() const;
+    int free_buffer(std::initializer_list<int> used) const;
+
+    void stage_begin(hipStream_t s);
+    void stage_end(hipStream_t s, const char *name);
+
+    void structured_forward(hipStream_t s, bool save_ckpts);
+    void structured_hamiltonian(hipStream_t s, bool write_lam);
+    void structured_backward(hipStream_t s, const double *scale, bool full_walk);
+    void small_run(hipStream_t s, bool want_grad, const double *scale, double *energy_out,
+                   double *grad_out, double *drift_out);
+    void small_run_captured(hipStream_t s);
+    void gate_forward(hipStream_t s);
+    void gate_hamiltonian(hipStream_t s);
+    void gate_backward(hipStream_t s, const double *scale);
+    void dense_init();
+    void dense_free();
+    void dense_run(hipStream_t s, bool want_grad, const double *scale, double *energy_out,
+                   double *grad_out);
+
+    void forward_device(const double *params, double *energy, hipStream_t s, bool keep_lam);
+    void backward_device(const double *params, const double *scale, double *grad, hipStream_t s);
+    void copy_grad_if_needed(double *grad, hipStream_t s);
+    void energy_and_grad_host(const double *params, size_t count, bool want_grad, bool profile,
+                              double *energy_out, double *grad_out);
+    double inverse_walk_drift(const double *params, size_t count);
+    void set_checkpoint_interval(size_t every);
+    void benchmark_forward_graph(const double *params, size_t count, size_t repeats,
+                                 double *normal_ms, double *graph_ms);
+    void copy_state(double *host_out, size_t count);
+};
+
+}  // namespace rie

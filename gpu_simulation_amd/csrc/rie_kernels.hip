@@ -1,0 +1,1177 @@
+// HIP kernels of the ring-Ising statevector engine for gfx950 (MI355X).
+//
+// Design (see the header of each kernel):
+//  * Large states stream through HBM in tile passes.  A workgroup loads a
+//    tile of 2^T amplitudes (T = 3R) into registers, 2^R per thread, applies
+//    the rotations of the R tile bits held in registers, and exchanges the
+//    tile through LDS twice so every tile bit is register-resident once.
+//    Global loads/stores are 16-byte-per-lane and contiguous across lanes.
+//  * The ring-CNOT layer is a GF(2)-linear index permutation.  The image of a
+//    low tile (index bits [0, T)) is two contiguous runs, so the permutation
+//    rides on the last LDS exchange of the forward low pass (scatter) and on
+//    the first exchange of the backward low pass (gather).
+//  * Small circuits (n <= 12) run entirely in one workgroup's LDS.
+//  * Reductions (energy, gradients) are per-workgroup partials followed by a
+//    fixed-order reduction kernel: results are bitwise reproducible.
+#include "rie_kernels.hpp"
+
+#include <algorithm>
+
+namespace rie {
+
+// ---------------------------------------------------------------------------
+// Tile helpers
+// ---------------------------------------------------------------------------
+
+template <int R>
+struct Tile {
+    static constexpr int T = 3 * R;
+    static constexpr uint32_t SIZE = 1u << T;
+    static constexpr int THREADS = 1 << (2 * R);
+    static constexpr int REGS = 1 << R;
+    static constexpr int WAVES = THREADS / kWave;
+    static constexpr uint32_t SLOTS = SIZE + (SIZE >> R);
+};
+
+// Register/thread mappings of a tile offset o in [0, 2^T):
+//   M = 0 (A): registers hold offset bits [2R, 3R), threads bits [0, 2R)
+//   M = 1 (B): registers hold bits [0, R),  threads bits [R, 3R)
+//   M = 2 (C): registers hold bits [R, 2R), threads bits [0, R) and [2R, 3R)
+template <int R, int M>
+__device__ __forceinline__ uint32_t omap(uint32_t t, int k) {
+    if constexpr (M == 0) {
+        return t | ((uint32_t)k << (2 * R));
+    } else if constexpr (M == 1) {
+        return (uint32_t)k | (t << R);
+    } else {
+        return (t & ((1u << R) - 1u)) | ((uint32_t)k << R) | ((t >> R) << (2 * R));
+    }
+}
+
+template <int R, int M>
+constexpr int phase_first_bit() {
+    return M == 0 ? 2 * R : (M == 1 ? 0 : R);
+}
+
+// LDS slot of offset o: one pad amplitude every 2^R keeps the strided
+// mapping B conflict-free.
+template <int R>
+__device__ __forceinline__ uint32_t lds_slot(uint32_t o) {
+    return o + (o >> R);
+}
+
+__device__ __forceinline__ uint64_t tile_base(uint64_t g, const PassGeom &G, int T) {
+    const int gap = G.h0 - G.a;
+    const uint64_t lo = g & ((uint64_t{1} << gap) - 1);
+    const uint64_t hi = g >> gap;
+    return (lo << G.a) | (hi << (G.h0 + T - G.a));
+}
+
+__device__ __forceinline__ uint64_t tile_index(uint64_t base, uint32_t o, const PassGeom &G) {
+    const uint64_t lo = o & ((1u << G.a) - 1u);
+    const uint64_t hi = (uint64_t)(o >> G.a) << G.h0;
+    return base | lo | hi;
+}
+
+__device__ __forceinline__ int tile_wire(int j, const PassGeom &G) {
+    return j < G.a ? j : G.h0 + (j - G.a);
+}
+
+// Per-thread global index offsets of mappings A and C and the uniform index
+// step of their register index k (k covers offset bits [2R, 3R) in A and
+// [R, 2R) in C, which lie entirely in the low or the high part of the tile).
+struct TileAddr {
+    uint64_t offA;
+    uint64_t offC;
+    uint64_t strideA;
+    uint64_t strideC;
+};
+
+template <int R>
+__device__ __forceinline__ TileAddr tile_addr(uint32_t t, const PassGeom &G) {
+    TileAddr ta;
+    ta.offA = tile_index(0, omap<R, 0>(t, 0), G);
+    ta.offC = tile_index(0, omap<R, 2>(t, 0), G);
+    if (G.a >= 3 * R) {
+        ta.strideA = uint64_t{1} << (2 * R);
+        ta.strideC = uint64_t{1} << R;
+    } else {
+        ta.strideA = uint64_t{1} << (G.h0 + 2 * R - G.a);
+        ta.strideC = uint64_t{1} << (G.h0 + R - G.a);
+    }
+    return ta;
+}
+
+template <int R, int FROM, int TO>
+__device__ __forceinline__ void exchange(double2 (&v)[Tile<R>::REGS], double2 *lds, uint32_t t) {
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < Tile<R>::REGS; ++k) {
+        lds[lds_slot<R>(omap<R, FROM>(t, k))] = v[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < Tile<R>::REGS; ++k) {
+        v[k] = lds[lds_slot<R>(omap<R, TO>(t, k))];
+    }
+}
+
+template <int R, int M>
+__device__ __forceinline__ void phase_fwd(double2 (&v)[Tile<R>::REGS],
+                                          const WireCoef *__restrict__ lc, const PassGeom &G) {
+    constexpr int P0 = phase_first_bit<R, M>();
+#pragma unroll
+    for (int rb = 0; rb < R; ++rb) {
+        const int j = P0 + rb;
+        if (j >= G.rot_lo && j < G.rot_hi) {
+            const WireCoef k = lc[tile_wire(j, G)];
+#pragma unroll
+            for (int q = 0; q < Tile<R>::REGS; ++q) {
+                if (!(q & (1 << rb))) {
+                    fwd_pair(v[q], v[q | (1 << rb)], k);
+                }
+            }
+        }
+    }
+}
+
+template <int R, int M>
+__device__ __forceinline__ void phase_bwd(double2 (&c)[Tile<R>::REGS], double2 (&l)[Tile<R>::REGS],
+                                          const WireCoef *__restrict__ lc, const PassGeom &G,
+                                          double (&dt)[3 * R], double (&dp)[3 * R]) {
+    constexpr int P0 = phase_first_bit<R, M>();
+#pragma unroll
+    for (int rb = R - 1; rb >= 0; --rb) {
+        const int j = P0 + rb;
+        if (j >= G.rot_lo && j < G.rot_hi) {
+            const WireCoef k = lc[tile_wire(j, G)];
+#pragma unroll
+            for (int q = 0; q < Tile<R>::REGS; ++q) {
+                if (!(q & (1 << rb))) {
+                    bwd_pair(c[q], c[q | (1 << rb)], l[q], l[q | (1 << rb)], k, dt[j], dp[j]);
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Parameters -> coefficient table
+// ---------------------------------------------------------------------------
+
+__global__ void coef_kernel(const double *__restrict__ params, WireCoef *__restrict__ coef,
+                            int count) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) {
+        return;
+    }
+    WireCoef k;
+    sincos(0.5 * params[2 * i], &k.s, &k.c);
+    sincos(0.5 * params[2 * i + 1], &k.sh, &k.ch);
+    coef[i] = k;
+}
+
+void launch_coef(const double *d_params, WireCoef *d_coef, int count, hipStream_t s) {
+    const int threads = 128;
+    coef_kernel<<<(count + threads - 1) / threads, threads, 0, s>>>(d_params, d_coef, count);
+}
+
+// ---------------------------------------------------------------------------
+// Layer 0 + ring CNOT as one write pass: out[j] = prod_w u_w[bit_w(P^-1 j)].
+// The product over up to 40 wires is split into 5 byte-wide LDS tables.
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void init_product_cnot_kernel(double2 *__restrict__ out,
+                                                                const WireCoef *__restrict__ lc0,
+                                                                int n, uint64_t size) {
+    __shared__ double2 table[5][256];
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int b = 0; b < 5; ++b) {
+        double2 v = make_double2(1.0, 0.0);
+        for (int bit = 0; bit < 8; ++bit) {
+            const int w = 8 * b + bit;
+            if (w < n) {
+                const WireCoef k = lc0[w];
+                v = cmul(v, ((t >> bit) & 1) ? u1_of(k) : u0_of(k));
+            }
+        }
+        table[b][t] = v;
+    }
+    __syncthreads();
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + t; j < size; j += stride) {
+        const uint64_t x = ring_inv(j, n);
+        double2 v = cmul(table[0][x & 255], table[1][(x >> 8) & 255]);
+        v = cmul(v, table[2][(x >> 16) & 255]);
+        v = cmul(v, table[3][(x >> 24) & 255]);
+        v = cmul(v, table[4][(x >> 32) & 255]);
+        out[j] = v;
+    }
+}
+
+int elementwise_grid(uint64_t work_items) {
+    const uint64_t blocks = (work_items + 255) / 256;
+    return (int)std::min<uint64_t>(std::max<uint64_t>(blocks, 1), 8192);
+}
+
+void launch_init_product_cnot(double2 *out, const WireCoef *coef_layer0, int n, hipStream_t s) {
+    const uint64_t size = uint64_t{1} << n;
+    init_product_cnot_kernel<<<elementwise_grid(size), 256, 0, s>>>(out, coef_layer0, n, size);
+}
+
+// ---------------------------------------------------------------------------
+// Forward tile pass.
+// ---------------------------------------------------------------------------
+
+template <int R, bool kScatter>
+__global__ __launch_bounds__(Tile<R>::THREADS) void fwd_pass_kernel(
+    const double2 *in, double2 *out, const WireCoef *__restrict__ lc, PassGeom G) {
+    using TL = Tile<R>;
+    __shared__ double2 lds[TL::SLOTS];
+    const uint32_t t = threadIdx.x;
+    const TileAddr ta = tile_addr<R>(t, G);
+    for (uint64_t g = blockIdx.x; g < G.num_tiles; g += gridDim.x) {
+        const uint64_t base = tile_base(g, G, TL::T);
+        double2 v[TL::REGS];
+        const double2 *src = in + base + ta.offA;
+#pragma unroll
+        for (int k = 0; k < TL::REGS; ++k) {
+            v[k] = src[k * ta.strideA];
+        }
+        phase_fwd<R, 0>(v, lc, G);
+        exchange<R, 0, 1>(v, lds, t);
+        phase_fwd<R, 1>(v, lc, G);
+        exchange<R, 1, 2>(v, lds, t);
+        phase_fwd<R, 2>(v, lc, G);
+        if constexpr (!kScatter) {
+            double2 *dst = out + base + ta.offC;
+#pragma unroll
+            for (int k = 0; k < TL::REGS; ++k) {
+                dst[k * ta.strideC] = v[k];
+            }
+        } else {
+            // Low tile (a == T): P(base ^ o) = P(base) ^ P_T(o) ^ (parity(o) ? hi : 0),
+            // and bit T-1 of P_T(o) is parity(o).
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < TL::REGS; ++k) {
+                lds[lds_slot<R>(ring_fwd32(omap<R, 2>(t, k), TL::T))] = v[k];
+            }
+            __syncthreads();
+            // q = t | k << 2R is contiguous in the image run; P(base) has zero
+            // bits [1, T) and its bit 0 only swaps neighbours inside the run.
+            const uint64_t pbase = ring_fwd(base, G.n);
+            const uint64_t lo_flip = pbase & 1u;
+            const uint64_t run = (pbase & ~uint64_t{TL::SIZE - 1}) + (t ^ lo_flip);
+            double2 *dst0 = out + run;
+            double2 *dst1 = out + (run ^ G.hi_mask);
+#pragma unroll
+            for (int k = 0; k < TL::REGS; ++k) {
+                const double2 val = lds[lds_slot<R>(omap<R, 0>(t, k))];
+                if (k < TL::REGS / 2) {
+                    dst0[(uint64_t)k << (2 * R)] = val;
+                } else {
+                    dst1[(uint64_t)k << (2 * R)] = val;
+                }
+            }
+        }
+    }
+}
+
+void launch_fwd_pass(int R, bool scatter, const double2 *in, double2 *out,
+                     const WireCoef *layer_coef, const PassGeom &g, int grid, hipStream_t s) {
+    if (R == 4) {
+        if (scatter) {
+            fwd_pass_kernel<4, true><<<grid, Tile<4>::THREADS, 0, s>>>(in, out, layer_coef, g);
+        } else {
+            fwd_pass_kernel<4, false><<<grid, Tile<4>::THREADS, 0, s>>>(in, out, layer_coef, g);
+        }
+    } else {
+        if (scatter) {
+            fwd_pass_kernel<3, true><<<grid, Tile<3>::THREADS, 0, s>>>(in, out, layer_coef, g);
+        } else {
+            fwd_pass_kernel<3, false><<<grid, Tile<3>::THREADS, 0, s>>>(in, out, layer_coef, g);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint tile pass.
+// ---------------------------------------------------------------------------
+
+// Factor of wire w in the layer-0 state at the start of a pass: U_w|0> if w
+// is still rotated, |0> if an earlier pass already un-rotated it.
+__device__ __forceinline__ double2 layer0_factor(const WireCoef *__restrict__ lc0,
+                                                 uint64_t processed, int w, int bit) {
+    if ((processed >> w) & 1U) {
+        return make_double2(bit ? 0.0 : 1.0, 0.0);
+    }
+    const WireCoef k = lc0[w];
+    return bit ? u1_of(k) : u0_of(k);
+}
+
+template <int R, int M>
+__device__ __forceinline__ double2 analytic_thread_factor(const WireCoef *__restrict__ lc0,
+                                                          const PassGeom &G, uint64_t processed,
+                                                          uint32_t t) {
+    // Product over the tile bits taken from the thread index in mapping M.
+    double2 v = make_double2(1.0, 0.0);
+    const uint32_t o_t = omap<R, M>(t, 0);
+    constexpr int P0 = phase_first_bit<R, M>();
+    for (int j = 0; j < 3 * R; ++j) {
+        if (j >= P0 && j < P0 + R) {
+            continue;  // register bit
+        }
+        v = cmul(v, layer0_factor(lc0, processed, tile_wire(j, G), (o_t >> j) & 1));
+    }
+    return v;
+}
+
+template <int R, int M>
+__device__ __forceinline__ double2 analytic_reg_factor(const WireCoef *__restrict__ lc0,
+                                                       const PassGeom &G, uint64_t processed,
+                                                       int k) {
+    double2 v = make_double2(1.0, 0.0);
+    constexpr int P0 = phase_first_bit<R, M>();
+    for (int rb = 0; rb < R; ++rb) {
+        v = cmul(v, layer0_factor(lc0, processed, tile_wire(P0 + rb, G), (k >> rb) & 1));
+    }
+    return v;
+}
+
+__device__ __forceinline__ double2 analytic_outer_factor(const WireCoef *__restrict__ lc0,
+                                                         const PassGeom &G, uint64_t processed,
+                                                         uint64_t base, int T) {
+    double2 v = make_double2(1.0, 0.0);
+    const int hi_end = G.h0 + T - G.a;
+    for (int w = 0; w < G.n; ++w) {
+        const bool in_tile = (w < G.a) || (w >= G.h0 && w < hi_end);
+        if (!in_tile) {
+            v = cmul(v, layer0_factor(lc0, processed, w, (int)((base >> w) & 1U)));
+        }
+    }
+    return v;
+}
+
+template <int R>
+__device__ __forceinline__ void block_reduce_grad(double (&dt)[3 * R], double (&dp)[3 * R],
+                                                  double *__restrict__ partials, const PassGeom &G) {
+    using TL = Tile<R>;
+    constexpr int T = TL::T;
+    __shared__ double red[TL::WAVES][2 * T];
+    const int t = threadIdx.x;
+    const int lane = t & (kWave - 1);
+    const int wave = t / kWave;
+#pragma unroll
+    for (int j = 0; j < T; ++j) {
+        const double a = wave_sum(dt[j]);
+        const double b = wave_sum(dp[j]);
+        if (lane == 0) {
+            red[wave][2 * j] = a;
+            red[wave][2 * j + 1] = b;
+        }
+    }
+    __syncthreads();
+    if (t < 2 * T) {
+        double acc = 0.0;
+#pragma unroll
+        for (int w = 0; w < TL::WAVES; ++w) {
+            acc += red[w][t];
+        }
+        partials[(uint64_t)blockIdx.x * (2 * T) + t] = acc;
+    }
+    (void)G;
+}
+
+template <int R, bool kGather, bool kAnalytic, bool kWriteCur, bool kWriteLam>
+__global__ __launch_bounds__(Tile<R>::THREADS) void bwd_pass_kernel(
+    const double2 *cur_in, const double2 *lam_in, double2 *cur_out, double2 *lam_out,
+    const WireCoef *__restrict__ lc, PassGeom G, uint64_t processed,
+    double *__restrict__ partials) {
+    using TL = Tile<R>;
+    constexpr int T = TL::T;
+    __shared__ double2 lds[TL::SLOTS];
+    __shared__ double2 reg_factor[kAnalytic ? TL::REGS : 1];
+    const uint32_t t = threadIdx.x;
+    double dt[T];
+    double dp[T];
+#pragma unroll
+    for (int j = 0; j < T; ++j) {
+        dt[j] = 0.0;
+        dp[j] = 0.0;
+    }
+    // The first register mapping of the pass: B after a gather, A otherwise.
+    constexpr int M0 = kGather ? 1 : 0;
+    double2 thread_factor = make_double2(1.0, 0.0);
+    if constexpr (kAnalytic) {
+        thread_factor = analytic_thread_factor<R, M0>(lc, G, processed, t);
+        if (t < (uint32_t)TL::REGS) {
+            reg_factor[t] = analytic_reg_factor<R, M0>(lc, G, processed, (int)t);
+        }
+        __syncthreads();
+    }
+
+    const TileAddr ta = tile_addr<R>(t, G);
+    for (uint64_t g = blockIdx.x; g < G.num_tiles; g += gridDim.x) {
+        const uint64_t base = tile_base(g, G, T);
+        double2 c[TL::REGS];
+        double2 l[TL::REGS];
+        if constexpr (kGather) {
+            // Low tile in the pre-CNOT frame: phi[base ^ o] = psi[P(base) ^ P_T(o) ^ ...].
+            // Load contiguous q = P_T(o) runs, place at o = P_T^-1(q) in LDS.
+            const uint64_t pbase = ring_fwd(base, G.n);
+            const uint64_t lo_flip = pbase & 1u;
+            const uint64_t run = (pbase & ~uint64_t{TL::SIZE - 1}) + (t ^ lo_flip);
+            const uint64_t run1 = run ^ G.hi_mask;
+#pragma unroll
+            for (int k = 0; k < TL::REGS; ++k) {
+                const uint64_t src = (k < TL::REGS / 2 ? run : run1) + ((uint64_t)k << (2 * R));
+                l[k] = lam_in[src];
+                if constexpr (!kAnalytic) {
+                    c[k] = cur_in[src];
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < TL::REGS; ++k) {
+                lds[lds_slot<R>(ring_inv32(omap<R, 0>(t, k), T))] = l[k];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < TL::REGS; ++k) {
+                l[k] = lds[lds_slot<R>(omap<R, 1>(t, k))];
+            }
+            if constexpr (!kAnalytic) {
+                __syncthreads();
+#pragma unroll
+                for (int k = 0; k < TL::REGS; ++k) {
+                    lds[lds_slot<R>(ring_inv32(omap<R, 0>(t, k), T))] = c[k];
+                }
+                __syncthreads();
+#pragma unroll
+                for (int k = 0; k < TL::REGS; ++k) {
+                    c[k] = lds[lds_slot<R>(omap<R, 1>(t, k))];
+                }
+            }
+        } else {
+            const uint64_t row = base + ta.offA;
+#pragma unroll
+            for (int k = 0; k < TL::REGS; ++k) {
+                const uint64_t idx = row + k * ta.strideA;
+                l[k] = lam_in[idx];
+                if constexpr (!kAnalytic) {
+                    c[k] = cur_in[idx];
+                }
+            }
+        }
+        if constexpr (kAnalytic) {
+            const double2 f = cmul(analytic_outer_factor(lc, G, processed, base, T), thread_factor);
+#pragma unroll
+            for (int k = 0; k < TL::REGS; ++k) {
+                c[k] = cmul(f, reg_factor[k]);
+            }
+        }
+
+        if constexpr (kGather) {
+            phase_bwd<R, 1>(c, l, lc, G, dt, dp);
+            exchange<R, 1, 2>(c, lds, t);
+            exchange<R, 1, 2>(l, lds, t);
+            phase_bwd<R, 2>(c, l, lc, G, dt, dp);
+            exchange<R, 2, 0>(c, lds, t);
+            exchange<R, 2, 0>(l, lds, t);
+            phase_bwd<R, 0>(c, l, lc, G, dt, dp);
+            const uint64_t row = base + ta.offA;
+#pragma unroll
+            for (int k = 0; k < TL::REGS; ++k) {
+                const uint64_t idx = row + k * ta.strideA;
+                if constexpr (kWriteCur) {
+                    cur_out[idx] = c[k];
+                }
+                if constexpr (kWriteLam) {
+                    lam_out[idx] = l[k];
+                }
+            }
+        } else {
+            phase_bwd<R, 0>(c, l, lc, G, dt, dp);
+            exchange<R, 0, 1>(c, lds, t);
+            exchange<R, 0, 1>(l, lds, t);
+            phase_bwd<R, 1>(c, l, lc, G, dt, dp);
+            exchange<R, 1, 2>(c, lds, t);
+            exchange<R, 1, 2>(l, lds, t);
+            phase_bwd<R, 2>(c, l, lc, G, dt, dp);
+            const uint64_t row = base + ta.offC;
+#pragma unroll
+            for (int k = 0; k < TL::REGS; ++k) {
+                const uint64_t idx = row + k * ta.strideC;
+                if constexpr (kWriteCur) {
+                    cur_out[idx] = c[k];
+                }
+                if constexpr (kWriteLam) {
+                    lam_out[idx] = l[k];
+                }
+            }
+        }
+    }
+    block_reduce_grad<R>(dt, dp, partials, G);
+}
+
+template <int R>
+static void launch_bwd_pass_r(bool gather, bool analytic, bool write_cur, bool write_lam,
+                              const double2 *cur_in, const double2 *lam_in, double2 *cur_out,
+                              double2 *lam_out, const WireCoef *lc, const PassGeom &g,
+                              uint64_t processed, double *partials, int grid, hipStream_t s) {
+    constexpr int TH = Tile<R>::THREADS;
+#define RIE_BWD(GA, AN, WC, WL)                                                           \
+    if (gather == GA && analytic == AN && write_cur == WC && write_lam == WL) {           \
+        bwd_pass_kernel<R, GA, AN, WC, WL><<<grid, TH, 0, s>>>(cur_in, lam_in, cur_out,   \
+                                                              lam_out, lc, g, processed,  \
+                                                              partials);                  \
+        return;                                                                           \
+    }
+    RIE_BWD(true, false, true, true)
+    RIE_BWD(false, false, true, true)
+    RIE_BWD(true, true, false, true)
+    RIE_BWD(false, true, false, true)
+    RIE_BWD(true, true, false, false)
+    RIE_BWD(false, true, false, false)
+    RIE_BWD(true, false, false, true)
+    RIE_BWD(false, false, false, true)
+    RIE_BWD(true, false, false, false)
+    RIE_BWD(false, false, false, false)
+#undef RIE_BWD
+}
+
+void launch_bwd_pass(int R, bool gather, bool analytic, bool write_cur, bool write_lam,
+                     const double2 *cur_in, const double2 *lam_in, double2 *cur_out,
+                     double2 *lam_out, const WireCoef *layer_coef, const PassGeom &g,
+                     uint64_t processed_mask, double *partials, int grid, hipStream_t s) {
+    if (R == 4) {
+        launch_bwd_pass_r<4>(gather, analytic, write_cur, write_lam, cur_in, lam_in, cur_out,
+                             lam_out, layer_coef, g, processed_mask, partials, grid, s);
+    } else {
+        launch_bwd_pass_r<3>(gather, analytic, write_cur, write_lam, cur_in, lam_in, cur_out,
+                             lam_out, layer_coef, g, processed_mask, partials, grid, s);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Hamiltonian tile pass.
+// ---------------------------------------------------------------------------
+
+template <int R, bool kDiag, bool kReadLam, bool kWriteLam>
+__global__ __launch_bounds__(Tile<R>::THREADS) void ham_pass_kernel(
+    const double2 *__restrict__ psi, const double2 *lam_in, double2 *lam_out, PassGeom G,
+    double field, double *__restrict__ e_partials) {
+    using TL = Tile<R>;
+    constexpr int T = TL::T;
+    __shared__ double2 lds[TL::SLOTS];
+    __shared__ double red[TL::WAVES];
+    const uint32_t t = threadIdx.x;
+    const TileAddr ta = tile_addr<R>(t, G);
+    double e = 0.0;
+    for (uint64_t g = blockIdx.x; g < G.num_tiles; g += gridDim.x) {
+        const uint64_t base = tile_base(g, G, T);
+        const uint64_t row = base + ta.offA;
+        double2 v[TL::REGS];
+#pragma unroll
+        for (int k = 0; k < TL::REGS; ++k) {
+            v[k] = psi[row + k * ta.strideA];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < TL::REGS; ++k) {
+            lds[lds_slot<R>(omap<R, 0>(t, k))] = v[k];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < TL::REGS; ++k) {
+            const uint32_t o = omap<R, 0>(t, k);
+            const uint64_t idx = row + k * ta.strideA;
+            double2 flips = make_double2(0.0, 0.0);
+            for (int j = G.rot_lo; j < G.rot_hi; ++j) {
+                const double2 p = lds[lds_slot<R>(o ^ (1u << j))];
+                flips.x += p.x;
+                flips.y += p.y;
+            }
+            double2 contrib = make_double2(-field * flips.x, -field * flips.y);
+            if constexpr (kDiag) {
+                const double d = ham_diag(idx, G.n);
+                contrib.x += d * v[k].x;
+                contrib.y += d * v[k].y;
+            }
+            e += v[k].x * contrib.x + v[k].y * contrib.y;
+            if constexpr (kWriteLam) {
+                double2 out = contrib;
+                if constexpr (kReadLam) {
+                    const double2 prev = lam_in[idx];
+                    out.x += prev.x;
+                    out.y += prev.y;
+                }
+                lam_out[idx] = out;
+            }
+        }
+    }
+    const double ws = wave_sum(e);
+    if ((t & (kWave - 1)) == 0) {
+        red[t / kWave] = ws;
+    }
+    __syncthreads();
+    if (t == 0) {
+        double acc = 0.0;
+        for (int w = 0; w < TL::WAVES; ++w) {
+            acc += red[w];
+        }
+        e_partials[blockIdx.x] = acc;
+    }
+}
+
+template <int R>
+static void launch_ham_pass_r(bool diag, bool read_lam, bool write_lam, const double2 *psi,
+                              const double2 *lam_in, double2 *lam_out, const PassGeom &g,
+                              double field, double *e_partials, int grid, hipStream_t s) {
+    constexpr int TH = Tile<R>::THREADS;
+#define RIE_HAM(DG, RL, WL)                                                                  \
+    if (diag == DG && read_lam == RL && write_lam == WL) {                                   \
+        ham_pass_kernel<R, DG, RL, WL><<<grid, TH, 0, s>>>(psi, lam_in, lam_out, g, field,   \
+                                                           e_partials);                      \
+        return;                                                                              \
+    }
+    RIE_HAM(true, false, true)
+    RIE_HAM(false, true, true)
+    RIE_HAM(true, false, false)
+    RIE_HAM(false, false, false)
+#undef RIE_HAM
+}
+
+void launch_ham_pass(int R, bool diag, bool read_lam, bool write_lam, const double2 *psi,
+                     const double2 *lam_in, double2 *lam_out, const PassGeom &g, double field,
+                     double *e_partials, int grid, hipStream_t s) {
+    if (R == 4) {
+        launch_ham_pass_r<4>(diag, read_lam, write_lam, psi, lam_in, lam_out, g, field,
+                             e_partials, grid, s);
+    } else {
+        launch_ham_pass_r<3>(diag, read_lam, write_lam, psi, lam_in, lam_out, g, field,
+                             e_partials, grid, s);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Fixed-order reductions.
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(64) void reduce_grad_kernel(const double *__restrict__ partials,
+                                                         int num_blocks, int T, PassGeom G,
+                                                         int layer, double *__restrict__ grad,
+                                                         const double *__restrict__ scale) {
+    const int col = 2 * G.rot_lo + blockIdx.x;  // 2 * tile bit + component
+    const int j = col >> 1;
+    const int comp = col & 1;
+    double acc = 0.0;
+    for (int b = threadIdx.x; b < num_blocks; b += 64) {
+        acc += partials[(uint64_t)b * (2 * T) + col];
+    }
+    acc = wave_sum(acc);
+    if (threadIdx.x == 0) {
+        const int wire = tile_wire(j, G);
+        const double f = scale != nullptr ? *scale : 1.0;
+        grad[((uint64_t)layer * G.n + wire) * 2 + comp] = acc * f;
+    }
+}
+
+void launch_reduce_grad(const double *partials, int num_blocks, int T, const PassGeom &g,
+                        int layer, double *grad, const double *scale, hipStream_t s) {
+    const int cols = 2 * (g.rot_hi - g.rot_lo);
+    reduce_grad_kernel<<<cols, 64, 0, s>>>(partials, num_blocks, T, g, layer, grad, scale);
+}
+
+__global__ __launch_bounds__(256) void reduce_sum_kernel(const double *__restrict__ values,
+                                                         int count, double *__restrict__ out,
+                                                         const double *__restrict__ scale) {
+    __shared__ double red[4];
+    const int t = threadIdx.x;
+    double acc = 0.0;
+    for (int i = t; i < count; i += 256) {
+        acc += values[i];
+    }
+    acc = wave_sum(acc);
+    if ((t & 63) == 0) {
+        red[t >> 6] = acc;
+    }
+    __syncthreads();
+    if (t == 0) {
+        const double f = scale != nullptr ? *scale : 1.0;
+        *out = (red[0] + red[1] + red[2] + red[3]) * f;
+    }
+}
+
+void launch_reduce_sum(const double *values, int count, double *out, hipStream_t s) {
+    reduce_sum_kernel<<<1, 256, 0, s>>>(values, count, out, nullptr);
+}
+
+void launch_reduce_to(const double *partials, int count, double *out, const double *scale,
+                      hipStream_t s) {
+    reduce_sum_kernel<<<1, 256, 0, s>>>(partials, count, out, scale);
+}
+
+__global__ __launch_bounds__(256) void drift_kernel(const double2 *__restrict__ state,
+                                                    uint64_t size, double *__restrict__ partials) {
+    __shared__ double red[4];
+    const int t = threadIdx.x;
+    double acc = 0.0;
+    const uint64_t stride = (uint64_t)gridDim.x * 256;
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + t; i < size; i += stride) {
+        const double2 v = state[i];
+        const double re = v.x - (i == 0 ? 1.0 : 0.0);
+        acc += re * re + v.y * v.y;
+    }
+    acc = wave_sum(acc);
+    if ((t & 63) == 0) {
+        red[t >> 6] = acc;
+    }
+    __syncthreads();
+    if (t == 0) {
+        partials[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+    }
+}
+
+void launch_drift(const double2 *state, uint64_t size, double *partials, int grid, hipStream_t s) {
+    drift_kernel<<<grid, 256, 0, s>>>(state, size, partials);
+}
+
+// ---------------------------------------------------------------------------
+// Whole-circuit kernel for small n: one workgroup per circuit, the state and
+// the adjoint state in LDS (n <= 12) or in a global workspace.
+// ---------------------------------------------------------------------------
+
+template <bool kLds>
+__global__ __launch_bounds__(kSmallThreads) void small_circuit_kernel(
+    const SmallJob *__restrict__ jobs, SmallJob single, int num_jobs) {
+    extern __shared__ double2 smem[];
+    __shared__ double red[kSmallThreads / kWave][2 * kSmallMaxQubits + 2];
+    if ((int)blockIdx.x >= num_jobs) {
+        return;
+    }
+    const SmallJob job = (jobs != nullptr) ? jobs[blockIdx.x] : single;
+    const int n = job.n;
+    const int L = job.layers;
+    const uint32_t N = 1u << n;
+    const int t = threadIdx.x;
+    const int lane = t & (kWave - 1);
+    const int wave = t / kWave;
+    double2 *cur = kLds ? smem : job.gstate;
+    double2 *lam = kLds ? smem + N : job.gstate + N;
+    double2 *tmp = kLds ? nullptr : job.gstate + 2 * N;  // permutation target (global variant)
+    const WireCoef *__restrict__ coef = job.coef;
+    constexpr int MAXR = (1 << kSmallMaxLdsQubits) / kSmallThreads;  // 16 amplitudes/thread
+
+    // Layer 0 as a product state, then the ring CNOT: cur[j] = prod u_w[bit_w(P^-1 j)].
+    for (uint32_t j = t; j < N; j += kSmallThreads) {
+        const uint32_t x = ring_inv32(j, n);
+        double2 v = make_double2(1.0, 0.0);
+        for (int w = 0; w < n; ++w) {
+            const WireCoef k = coef[w];
+            v = cmul(v, ((x >> w) & 1) ? u1_of(k) : u0_of(k));
+        }
+        cur[j] = v;
+    }
+    __syncthreads();
+
+    const uint32_t pairs = N >> 1;
+    for (int l = 1; l < L; ++l) {
+        for (int w = 0; w < n; ++w) {
+            const WireCoef k = coef[l * n + w];
+            for (uint32_t p = t; p < pairs; p += kSmallThreads) {
+                const uint32_t i0 = ((p >> w) << (w + 1)) | (p & ((1u << w) - 1u));
+                const uint32_t i1 = i0 | (1u << w);
+                double2 a0 = cur[i0];
+                double2 a1 = cur[i1];
+                fwd_pair(a0, a1, k);
+                cur[i0] = a0;
+                cur[i1] = a1;
+            }
+            __syncthreads();
+        }
+        if constexpr (kLds) {
+            double2 r[MAXR];
+#pragma unroll
+            for (int q = 0; q < MAXR; ++q) {
+                const uint32_t i = t + q * kSmallThreads;
+                if (i < N) {
+                    r[q] = cur[i];
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < MAXR; ++q) {
+                const uint32_t i = t + q * kSmallThreads;
+                if (i < N) {
+                    cur[ring_fwd32(i, n)] = r[q];
+                }
+            }
+        } else {
+            for (uint32_t i = t; i < N; i += kSmallThreads) {
+                tmp[ring_fwd32(i, n)] = cur[i];
+            }
+            double2 *sw = cur;
+            cur = tmp;
+            tmp = sw;
+        }
+        __syncthreads();
+    }
+
+    // lam = H psi and the energy.
+    double e = 0.0;
+    for (uint32_t i = t; i < N; i += kSmallThreads) {
+        const double2 v = cur[i];
+        const double d = ham_diag(i, n);
+        double fx = 0.0;
+        double fy = 0.0;
+        for (int w = 0; w < n; ++w) {
+            const double2 p = cur[i ^ (1u << w)];
+            fx += p.x;
+            fy += p.y;
+        }
+        const double2 hv = make_double2(d * v.x - job.field * fx, d * v.y - job.field * fy);
+        lam[i] = hv;
+        e += v.x * hv.x + v.y * hv.y;
+    }
+    e = wave_sum(e);
+    if (lane == 0) {
+        red[wave][0] = e;
+    }
+    __syncthreads();
+    if (t == 0) {
+        double acc = 0.0;
+        for (int w = 0; w < kSmallThreads / kWave; ++w) {
+            acc += red[w][0];
+        }
+        *job.energy = acc;
+    }
+    __syncthreads();
+    if (job.grad == nullptr) {
+        return;
+    }
+    const double gscale = job.scale != nullptr ? *job.scale : 1.0;
+
+    for (int l = L - 1; l >= 0; --l) {
+        // Inverse ring CNOT on both states: new[i] = old[P(i)].
+        if constexpr (kLds) {
+            double2 rc[MAXR];
+            double2 rl[MAXR];
+#pragma unroll
+            for (int q = 0; q < MAXR; ++q) {
+                const uint32_t i = t + q * kSmallThreads;
+                if (i < N) {
+                    const uint32_t src = ring_fwd32(i, n);
+                    rc[q] = cur[src];
+                    rl[q] = lam[src];
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < MAXR; ++q) {
+                const uint32_t i = t + q * kSmallThreads;
+                if (i < N) {
+                    cur[i] = rc[q];
+                    lam[i] = rl[q];
+                }
+            }
+        } else {
+            for (uint32_t i = t; i < N; i += kSmallThreads) {
+                tmp[i] = cur[ring_fwd32(i, n)];
+            }
+            __syncthreads();
+            double2 *sw = cur;
+            cur = tmp;
+            tmp = sw;
+            for (uint32_t i = t; i < N; i += kSmallThreads) {
+                tmp[i] = lam[ring_fwd32(i, n)];
+            }
+            sw = lam;
+            lam = tmp;
+            tmp = sw;
+        }
+        __syncthreads();
+        for (int w = n - 1; w >= 0; --w) {
+            const WireCoef k = coef[l * n + w];
+            double dt = 0.0;
+            double dp = 0.0;
+            for (uint32_t p = t; p < pairs; p += kSmallThreads) {
+                const uint32_t i0 = ((p >> w) << (w + 1)) | (p & ((1u << w) - 1u));
+                const uint32_t i1 = i0 | (1u << w);
+                double2 c0 = cur[i0];
+                double2 c1 = cur[i1];
+                double2 l0 = lam[i0];
+                double2 l1 = lam[i1];
+                bwd_pair(c0, c1, l0, l1, k, dt, dp);
+                cur[i0] = c0;
+                cur[i1] = c1;
+                lam[i0] = l0;
+                lam[i1] = l1;
+            }
+            dt = wave_sum(dt);
+            dp = wave_sum(dp);
+            if (lane == 0) {
+                red[wave][2 * w] = dt;
+                red[wave][2 * w + 1] = dp;
+            }
+            __syncthreads();
+        }
+        if (t < 2 * n) {
+            double acc = 0.0;
+            for (int w = 0; w < kSmallThreads / kWave; ++w) {
+                acc += red[w][t];
+            }
+            job.grad[(uint64_t)l * n * 2 + t] = acc * gscale;
+        }
+        __syncthreads();
+    }
+
+    if (job.drift != nullptr) {
+        double acc = 0.0;
+        for (uint32_t i = t; i < N; i += kSmallThreads) {
+            const double2 v = cur[i];
+            const double re = v.x - (i == 0 ? 1.0 : 0.0);
+            acc += re * re + v.y * v.y;
+        }
+        acc = wave_sum(acc);
+        if (lane == 0) {
+            red[wave][0] = acc;
+        }
+        __syncthreads();
+        if (t == 0) {
+            double s = 0.0;
+            for (int w = 0; w < kSmallThreads / kWave; ++w) {
+                s += red[w][0];
+            }
+            *job.drift = s;
+        }
+    }
+}
+
+void launch_small(const SmallJob *d_jobs, const SmallJob &single, int num_jobs, int max_n,
+                  bool lds, hipStream_t s) {
+    if (lds) {
+        const size_t bytes = sizeof(double2) * 2 * (size_t{1} << max_n);
+        small_circuit_kernel<true><<<num_jobs, kSmallThreads, bytes, s>>>(d_jobs, single,
+                                                                          num_jobs);
+    } else {
+        small_circuit_kernel<false><<<num_jobs, kSmallThreads, 0, s>>>(d_jobs, single, num_jobs);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Gate-level kernels (PennyLane's gate order: RY, RZ per wire, then n CNOTs).
+// ---------------------------------------------------------------------------
+
+__global__ void zero_state_kernel(double2 *__restrict__ s, uint64_t size) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < size; i += stride) {
+        s[i] = make_double2(i == 0 ? 1.0 : 0.0, 0.0);
+    }
+}
+
+void launch_zero_state(double2 *state, uint64_t size, hipStream_t s) {
+    zero_state_kernel<<<elementwise_grid(size), 256, 0, s>>>(state, size);
+}
+
+__device__ __forceinline__ uint64_t pair_low(uint64_t p, int w) {
+    return ((p >> w) << (w + 1)) | (p & ((uint64_t{1} << w) - 1));
+}
+
+__global__ void gate_ry_kernel(double2 *__restrict__ st, uint64_t pairs, int w,
+                               const WireCoef *__restrict__ kp, int adjoint) {
+    const double c = kp->c;
+    const double sn = adjoint ? -kp->s : kp->s;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < pairs; p += stride) {
+        const uint64_t i0 = pair_low(p, w);
+        const uint64_t i1 = i0 | (uint64_t{1} << w);
+        const double2 a0 = st[i0];
+        const double2 a1 = st[i1];
+        st[i0] = make_double2(c * a0.x - sn * a1.x, c * a0.y - sn * a1.y);
+        st[i1] = make_double2(sn * a0.x + c * a1.x, sn * a0.y + c * a1.y);
+    }
+}
+
+__global__ void gate_rz_kernel(double2 *__restrict__ st, uint64_t pairs, int w,
+                               const WireCoef *__restrict__ kp, int adjoint) {
+    const double ch = kp->ch;
+    const double sh = adjoint ? -kp->sh : kp->sh;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < pairs; p += stride) {
+        const uint64_t i0 = pair_low(p, w);
+        const uint64_t i1 = i0 | (uint64_t{1} << w);
+        const double2 a0 = st[i0];
+        const double2 a1 = st[i1];
+        // bit 0: e^{-i phi/2}; bit 1: e^{+i phi/2}
+        st[i0] = make_double2(ch * a0.x + sh * a0.y, ch * a0.y - sh * a0.x);
+        st[i1] = make_double2(ch * a1.x - sh * a1.y, ch * a1.y + sh * a1.x);
+    }
+}
+
+__global__ void gate_cnot_kernel(double2 *__restrict__ st, uint64_t pairs, int control,
+                                 int target) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < pairs; p += stride) {
+        const uint64_t i0 = pair_low(p, target);
+        if ((i0 >> control) & 1U) {
+            const uint64_t i1 = i0 | (uint64_t{1} << target);
+            const double2 a0 = st[i0];
+            st[i0] = st[i1];
+            st[i1] = a0;
+        }
+    }
+}
+
+void launch_gate_ry(double2 *state, int n, int wire, const WireCoef *k, bool adjoint,
+                    hipStream_t s) {
+    const uint64_t pairs = uint64_t{1} << (n - 1);
+    gate_ry_kernel<<<elementwise_grid(pairs), 256, 0, s>>>(state, pairs, wire, k, adjoint ? 1 : 0);
+}
+
+void launch_gate_rz(double2 *state, int n, int wire, const WireCoef *k, bool adjoint,
+                    hipStream_t s) {
+    const uint64_t pairs = uint64_t{1} << (n - 1);
+    gate_rz_kernel<<<elementwise_grid(pairs), 256, 0, s>>>(state, pairs, wire, k, adjoint ? 1 : 0);
+}
+
+void launch_gate_cnot(double2 *state, int n, int control, int target, hipStream_t s) {
+    const uint64_t pairs = uint64_t{1} << (n - 1);
+    gate_cnot_kernel<<<elementwise_grid(pairs), 256, 0, s>>>(state, pairs, control, target);
+}
+
+__global__ void ring_permute_kernel(const double2 *__restrict__ in, double2 *__restrict__ out,
+                                    int n, uint64_t size, int inverse) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < size; i += stride) {
+        // gather form: (P x)[i] = x[P^-1 i], (P^-1 x)[i] = x[P i]
+        const uint64_t src = inverse ? ring_fwd(i, n) : ring_inv(i, n);
+        out[i] = in[src];
+    }
+}
+
+void launch_ring_permute(const double2 *in, double2 *out, int n, bool inverse, hipStream_t s) {
+    const uint64_t size = uint64_t{1} << n;
+    ring_permute_kernel<<<elementwise_grid(size), 256, 0, s>>>(in, out, n, size, inverse ? 1 : 0);
+}
+
+__device__ __forceinline__ void block_partial(double v, double *__restrict__ partials) {
+    __shared__ double red[4];
+    v = wave_sum(v);
+    const int t = threadIdx.x;
+    if ((t & 63) == 0) {
+        red[t >> 6] = v;
+    }
+    __syncthreads();
+    if (t == 0) {
+        partials[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+    }
+}
+
+// psi, lam <- RY^dagger (psi, lam); partial of 2 Re<lam_after | dRY psi_before>
+// (PennyLane adjoint step; dRY = [[-s/2, -c/2], [c/2, -s/2]]).
+__global__ __launch_bounds__(256) void gate_ry_step_kernel(double2 *__restrict__ psi,
+                                                           double2 *__restrict__ lam,
+                                                           uint64_t pairs, int w,
+                                                           const WireCoef *__restrict__ kp,
+                                                           double *__restrict__ partials) {
+    const double c = kp->c;
+    const double s = kp->s;
+    double acc = 0.0;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < pairs; p += stride) {
+        const uint64_t i0 = pair_low(p, w);
+        const uint64_t i1 = i0 | (uint64_t{1} << w);
+        const double2 a0 = psi[i0];
+        const double2 a1 = psi[i1];
+        const double2 b0 = make_double2(c * a0.x + s * a1.x, c * a0.y + s * a1.y);
+        const double2 b1 = make_double2(c * a1.x - s * a0.x, c * a1.y - s * a0.y);
+        psi[i0] = b0;
+        psi[i1] = b1;
+        const double2 d0 = make_double2(-0.5 * (s * b0.x + c * b1.x), -0.5 * (s * b0.y + c * b1.y));
+        const double2 d1 = make_double2(0.5 * (c * b0.x - s * b1.x), 0.5 * (c * b0.y - s * b1.y));
+        const double2 l0 = lam[i0];
+        const double2 l1 = lam[i1];
+        acc += l0.x * d0.x + l0.y * d0.y + l1.x * d1.x + l1.y * d1.y;
+        lam[i0] = make_double2(c * l0.x + s * l1.x, c * l0.y + s * l1.y);
+        lam[i1] = make_double2(c * l1.x - s * l0.x, c * l1.y - s * l0.y);
+    }
+    block_partial(2.0 * acc, partials);
+}
+
+// dRZ = diag(-i/2 e^{-i phi/2}, +i/2 e^{+i phi/2}).
+__global__ __launch_bounds__(256) void gate_rz_step_kernel(double2 *__restrict__ psi,
+                                                           double2 *__restrict__ lam,
+                                                           uint64_t pairs, int w,
+                                                           const WireCoef *__restrict__ kp,
+                                                           double *__restrict__ partials) {
+    const double ch = kp->ch;
+    const double sh = kp->sh;
+    double acc = 0.0;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < pairs; p += stride) {
+        const uint64_t i0 = pair_low(p, w);
+        const uint64_t i1 = i0 | (uint64_t{1} << w);
+        // before = after * e^{+-i phi/2} (inverse phase)
+        const double2 a0 = psi[i0];
+        const double2 a1 = psi[i1];
+        const double2 b0 = make_double2(ch * a0.x - sh * a0.y, ch * a0.y + sh * a0.x);
+        const double2 b1 = make_double2(ch * a1.x + sh * a1.y, ch * a1.y - sh * a1.x);
+        psi[i0] = b0;
+        psi[i1] = b1;
+        // dRZ psi_before: bit0 -> (-i/2) e^{-i phi/2} b0 = (-i/2) a0 ; bit1 -> (+i/2) a1
+        const double2 d0 = make_double2(0.5 * a0.y, -0.5 * a0.x);
+        const double2 d1 = make_double2(-0.5 * a1.y, 0.5 * a1.x);
+        const double2 l0 = lam[i0];
+        const double2 l1 = lam[i1];
+        acc += l0.x * d0.x + l0.y * d0.y + l1.x * d1.x + l1.y * d1.y;
+        lam[i0] = make_double2(ch * l0.x - sh * l0.y, ch * l0.y + sh * l0.x);
+        lam[i1] = make_double2(ch * l1.x + sh * l1.y, ch * l1.y - sh * l1.x);
+    }
+    block_partial(2.0 * acc, partials);
+}
+
+void launch_gate_ry_step(double2 *psi, double2 *lam, int n, int wire, const WireCoef *k,
+                         double *partials, int grid, hipStream_t s) {
+    const uint64_t pairs = uint64_t{1} << (n - 1);
+    gate_ry_step_kernel<<<grid, 256, 0, s>>>(psi, lam, pairs, wire, k, partials);
+}
+
+void launch_gate_rz_step(double2 *psi, double2 *lam, int n, int wire, const WireCoef *k,
+                         double *partials, int grid, hipStream_t s) {
+    const uint64_t pairs = uint64_t{1} << (n - 1);
+    gate_rz_step_kernel<<<grid, 256, 0, s>>>(psi, lam, pairs, wire, k, partials);
+}
+
+__global__ __launch_bounds__(256) void ham_gather_kernel(const double2 *__restrict__ psi,
+                                                         double2 *__restrict__ lam, int n,
+                                                         uint64_t size, double field,
+                                                         double *__restrict__ partials) {
+    double acc = 0.0;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < size; i += stride) {
+        const double2 v = psi[i];
+        const double d = ham_diag(i, n);
+        double fx = 0.0;
+        double fy = 0.0;
+        for (int w = 0; w < n; ++w) {
+            const double2 p = psi[i ^ (uint64_t{1} << w)];
+            fx += p.x;
+            fy += p.y;
+        }
+        const double2 hv = make_double2(d * v.x - field * fx, d * v.y - field * fy);
+        lam[i] = hv;
+        acc += v.x * hv.x + v.y * hv.y;
+    }
+    block_partial(acc, partials);
+}
+
+void launch_ham_gather(const double2 *psi, double2 *lam, int n, double field, double *e_partials,
+                       int grid, hipStream_t s) {
+    const uint64_t size = uint64_t{1} << n;
+    ham_gather_kernel<<<grid, 256, 0, s>>>(psi, lam, n, size, field, e_partials);
+}
+
+}  // namespace rie

@@ -1,0 +1,109 @@
+// Host-visible launch interface of the engine's HIP kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "rie_device.hpp"
+
+namespace rie {
+
+// Geometry of one tile pass.  A tile holds 2^T amplitudes (T = 3R): its
+// offset bits [0, a) are index bits [0, a) (contiguous in memory) and its
+// offset bits [a, T) are index bits [h0, h0 + T - a).  The remaining index
+// bits enumerate the tiles.  Only tile bits in [rot_lo, rot_hi) are
+// processed; the others ride along for coalescing.  Either a == T (the low
+// tile) or a <= R (a high tile), so that the register index k of the A and C
+// mappings moves the global index by one uniform stride.
+struct PassGeom {
+    int n;
+    int a;
+    int h0;
+    int rot_lo;
+    int rot_hi;
+    uint64_t num_tiles;
+    uint64_t hi_mask;  // index bits [T, n): the ring-CNOT image of a low tile
+};
+
+// One independent circuit evaluated by a single workgroup (small n).
+struct SmallJob {
+    int n;
+    int layers;
+    double field;
+    const WireCoef *coef;  // layers * n entries
+    double *energy;        // 1 value
+    double *grad;          // 2 * n * layers values, or nullptr for energy only
+    double *drift;         // || walked state - |0> ||^2, or nullptr
+    double2 *gstate;       // 3 * 2^n amplitudes of global workspace (non-LDS variant)
+    const double *scale;   // optional chain-rule factor for grad (nullptr = 1)
+};
+
+constexpr int kSmallThreads = 256;
+constexpr int kSmallMaxLdsQubits = 12;
+constexpr int kSmallMaxQubits = 16;  // global-workspace variant
+
+// Coefficient table from device parameters: coef[i] from params[2i], params[2i+1].
+void launch_coef(const double *d_params, WireCoef *d_coef, int count, hipStream_t s);
+
+// Whole-circuit kernel, one workgroup per job (n <= 12 in LDS, else global).
+// With d_jobs == nullptr the single job passed by value is run.
+void launch_small(const SmallJob *d_jobs, const SmallJob &single, int num_jobs, int max_n,
+                  bool lds, hipStream_t s);
+
+// Layer 0 as a product state followed by the ring-CNOT permutation.
+void launch_init_product_cnot(double2 *out, const WireCoef *coef_layer0, int n, hipStream_t s);
+
+// Forward tile pass (rotations on the pass's tile wires, in place) or, with
+// scatter, the low-tile pass followed by the ring-CNOT permutation into out.
+void launch_fwd_pass(int R, bool scatter, const double2 *in, double2 *out,
+                     const WireCoef *layer_coef, const PassGeom &g, int grid, hipStream_t s);
+
+// Adjoint tile pass: un-rotates the pass's wires on (cur, lam) and writes
+// per-workgroup gradient partials [grid][2T].  gather = apply the inverse
+// ring-CNOT on load (low pass, out of place).  analytic = layer 0, whose
+// state is the product state and need not be read (processed_mask lists the
+// layer-0 wires already un-rotated by earlier passes).
+void launch_bwd_pass(int R, bool gather, bool analytic, bool write_cur, bool write_lam,
+                     const double2 *cur_in, const double2 *lam_in, double2 *cur_out,
+                     double2 *lam_out, const WireCoef *layer_coef, const PassGeom &g,
+                     uint64_t processed_mask, double *partials, int grid, hipStream_t s);
+
+// Hamiltonian tile pass: adds the X terms of the pass's wires (and the ZZ
+// diagonal if diag) of H psi, optionally into lam, and writes per-workgroup
+// energy partials sum_i Re(conj(psi_i) * contribution_i).
+void launch_ham_pass(int R, bool diag, bool read_lam, bool write_lam, const double2 *psi,
+                     const double2 *lam_in, double2 *lam_out, const PassGeom &g, double field,
+                     double *e_partials, int grid, hipStream_t s);
+
+// grad[(layer*n + wire)*2 + comp] = (sum over blocks) * scale for the pass's wires.
+void launch_reduce_grad(const double *partials, int num_blocks, int T, const PassGeom &g,
+                        int layer, double *grad, const double *scale, hipStream_t s);
+
+// *out = sum of count values (fixed order).
+void launch_reduce_sum(const double *values, int count, double *out, hipStream_t s);
+
+// Squared distance of a state from |0...0>.
+void launch_drift(const double2 *state, uint64_t size, double *partials, int grid, hipStream_t s);
+
+// Gate-level kernels (PennyLane gate order).
+void launch_zero_state(double2 *state, uint64_t size, hipStream_t s);
+void launch_gate_ry(double2 *state, int n, int wire, const WireCoef *k, bool adjoint, hipStream_t s);
+void launch_gate_rz(double2 *state, int n, int wire, const WireCoef *k, bool adjoint, hipStream_t s);
+void launch_gate_cnot(double2 *state, int n, int control, int target, hipStream_t s);
+void launch_ring_permute(const double2 *in, double2 *out, int n, bool inverse, hipStream_t s);
+// Adjoint step of RY / RZ: psi, lam <- G^dagger (psi, lam) and per-block partials
+// of 2 Re<lam_after | dG psi_before>.
+void launch_gate_ry_step(double2 *psi, double2 *lam, int n, int wire, const WireCoef *k,
+                         double *partials, int grid, hipStream_t s);
+void launch_gate_rz_step(double2 *psi, double2 *lam, int n, int wire, const WireCoef *k,
+                         double *partials, int grid, hipStream_t s);
+// lam = H psi by gathering all n flips per amplitude, with energy partials.
+void launch_ham_gather(const double2 *psi, double2 *lam, int n, double field, double *e_partials,
+                       int grid, hipStream_t s);
+// out[idx] = sum of partials[0..count) * scale (one value).
+void launch_reduce_to(const double *partials, int count, double *out, const double *scale,
+                      hipStream_t s);
+
+int elementwise_grid(uint64_t work_items);
+
+}  // namespace rie

@@ -1,0 +1,57 @@
+"""Locating and loading the built native code.
+
+``librie.so`` (HIP kernels + C ABI) and the pybind11 module ``_cuda_backend``
+are built in-tree by ``gpu_simulation_amd/build.py``.  There is no CPU
+fallback: if they are missing or the GPU is absent, the calls fail loudly.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import importlib
+
+from . import build as _build
+
+LIB_PATH = _build.LIB_PATH
+EXT_PATH = _build.EXT_PATH
+
+# Exported C ABI symbols (include/rie.h).
+C_ABI_SYMBOLS = (
+    "rie_last_error",
+    "rie_abi_version",
+    "rie_parse_strategy",
+    "rie_device_count",
+    "rie_engine_create",
+    "rie_engine_destroy",
+    "rie_engine_energy_and_grad",
+    "rie_engine_timing_count",
+    "rie_engine_timing",
+    "rie_engine_benchmark_forward_graph",
+    "rie_energy_and_grad",
+    "rie_engine_num_qubits",
+    "rie_engine_num_layers",
+    "rie_engine_strategy",
+    "rie_engine_forward_device",
+    "rie_engine_backward_device",
+    "rie_engine_energy_and_grad_device",
+    "rie_engine_set_checkpoint_interval",
+    "rie_engine_inverse_walk_drift",
+    "rie_engine_copy_state",
+)
+
+
+def ensure_built() -> None:
+    """Build once if nothing is built yet (never silently substitutes code)."""
+    if not _build.is_built():
+        _build.build()
+
+
+def load_library() -> ctypes.CDLL:
+    ensure_built()
+    return ctypes.CDLL(str(LIB_PATH))
+
+
+def load_extension():
+    """Import the pybind11 module ``_cuda_backend`` built from this tree."""
+    ensure_built()
+    return importlib.import_module("gpu_simulation_amd.standalone_backend._cuda_backend")

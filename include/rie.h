@@ -1,0 +1,164 @@
+/*
+ * rie.h - C ABI of the MI355X ring-Ising statevector engine (librie.so).
+ *
+ * The engine computes E(theta) = <psi|H|psi> and dE/dtheta for the circuit
+ * "RY(theta) RZ(phi) on every wire, then ring CNOTs i -> i+1 mod n" with
+ * H = -sum_i Z_i Z_{i+1 mod n} - h sum_i X_i, in complex128 on one AMD
+ * Instinct MI355X (gfx950).  Parameters are a flat float64 array of 2*n*L
+ * values indexed (l*n + w)*2 + {0: theta_RY, 1: phi_RZ}
+ * (reference: cpp/ising_cuda_structured_adjoint_modes.inc:73-77).
+ *
+ * Every entry point returns an rie_status; on failure rie_last_error()
+ * returns a thread-local message.  RIE_INVALID_ARGUMENT corresponds to the
+ * reference's std::invalid_argument (surfaced in Python as ValueError) and
+ * RIE_DEVICE_ERROR to its std::runtime_error (RuntimeError)
+ * (reference: cpp/ising_cuda_kernels.cu:18-44, cpp/ising_cuda_bindings.cpp).
+ *
+ * No torch or HIP types appear in these signatures: streams are passed as
+ * void* (a hipStream_t, NULL for the engine's own stream) and device buffers
+ * as plain pointers.
+ */
+#ifndef RIE_H
+#define RIE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RIE_ABI_VERSION 1
+
+typedef enum rie_status {
+    RIE_OK = 0,
+    RIE_INVALID_ARGUMENT = 1,
+    RIE_DEVICE_ERROR = 2,
+    RIE_OUT_OF_MEMORY = 3,
+    RIE_UNSUPPORTED = 4
+} rie_status;
+
+/* Gradient strategies; names as accepted by the reference's parser
+ * (cpp/ising_cuda_backend.cu:249-264, ring_ising/config.py:8-16). */
+typedef enum rie_strategy {
+    RIE_STRATEGY_INVERSE_WALK_GATE = 0, /* "inverse_walk_cuQuantum" / "inverse_walk_cuquantum" */
+    RIE_STRATEGY_STRUCTURED_ADJOINT = 1, /* "structured_adjoint" */
+    RIE_STRATEGY_DENSE_SCAN = 2          /* "dense_scan" (num_qubits <= 8) */
+} rie_strategy;
+
+typedef struct rie_engine rie_engine;
+
+/* Thread-local message describing the last failure on this thread. */
+const char *rie_last_error(void);
+
+/* ABI version of the loaded library (RIE_ABI_VERSION it was built with). */
+int rie_abi_version(void);
+
+/* Parse a strategy name; replaces parse_gradient_strategy
+ * (cpp/ising_cuda_backend.cu:249-264).  Unknown names -> RIE_INVALID_ARGUMENT. */
+rie_status rie_parse_strategy(const char *name, int *strategy_out);
+
+/* Number of visible HIP devices (0 when none); never fails. */
+int rie_device_count(void);
+
+/* Create an engine and allocate its device state buffers on the current
+ * device.  Replaces RingIsingCudaBackend::RingIsingCudaBackend
+ * (cpp/ising_cuda_backend.hpp:29-31, cpp/ising_cuda_backend.cu:374-419):
+ * num_qubits in [2, 34], num_layers >= 1, structured_rotation_chunk_width in
+ * [1, 8] for structured_adjoint, dense_scan requires num_qubits <= 8.
+ * double_buffer is accepted for interface compatibility. */
+rie_status rie_engine_create(size_t num_qubits, size_t num_layers, double field,
+                             const char *gradient_strategy,
+                             size_t structured_rotation_chunk_width,
+                             int double_buffer, rie_engine **engine_out);
+
+/* Free an engine and its device buffers; NULL is ignored.  Replaces
+ * RingIsingCudaBackend::~RingIsingCudaBackend (cpp/ising_cuda_backend.hpp:32). */
+void rie_engine_destroy(rie_engine *engine);
+
+/* Energy and, if compute_gradient, the 2*n*L gradient (host pointers,
+ * synchronous).  gradient_out may be NULL when compute_gradient == 0.
+ * profile != 0 records stage timings readable with rie_engine_timing*.
+ * Replaces RingIsingCudaBackend::energy_and_grad
+ * (cpp/ising_cuda_backend.hpp:39-41, cpp/ising_cuda_backend.cu:429-442). */
+rie_status rie_engine_energy_and_grad(rie_engine *engine, const double *params,
+                                      size_t num_params, int compute_gradient,
+                                      int profile, double *energy_out,
+                                      double *gradient_out);
+
+/* Stage timings of the last profiled call, in the reference's stage names
+ * (cpp/ising_cuda_structured_adjoint_modes.inc:317-370). */
+size_t rie_engine_timing_count(const rie_engine *engine);
+rie_status rie_engine_timing(const rie_engine *engine, size_t index,
+                             const char **name_out, double *ms_out);
+
+/* Time the structured forward pass issued normally vs. replayed from a HIP
+ * graph.  Replaces RingIsingCudaBackend::benchmark_structured_forward_graph
+ * (cpp/ising_cuda_backend.hpp:42-45, cpp/ising_cuda_backend.cu:444-541). */
+rie_status rie_engine_benchmark_forward_graph(rie_engine *engine,
+                                              const double *params,
+                                              size_t num_params, size_t repeats,
+                                              double *normal_forward_ms,
+                                              double *graph_forward_ms);
+
+/* One-shot helper constructing a temporary engine.  Replaces the free
+ * function standalone_backend::energy_and_grad
+ * (cpp/ising_cuda_backend.hpp:49-56, cpp/ising_cuda_backend.cu:543-554). */
+rie_status rie_energy_and_grad(size_t num_qubits, size_t num_layers, double field,
+                               const char *gradient_strategy, const double *params,
+                               size_t num_params, int compute_gradient, int profile,
+                               size_t structured_rotation_chunk_width,
+                               double *energy_out, double *gradient_out);
+
+/* Engine properties. */
+size_t rie_engine_num_qubits(const rie_engine *engine);
+size_t rie_engine_num_layers(const rie_engine *engine);
+int rie_engine_strategy(const rie_engine *engine);
+
+/* ---- Device-resident entry points (no host synchronisation) ------------
+ * These have no counterpart in the reference; they serve callers that keep
+ * parameters on the GPU (the torch.autograd wrapper, captured optimizer
+ * steps).  d_params, d_energy and d_gradient are device pointers; stream is
+ * a hipStream_t or NULL for the engine's own stream.  The calls are
+ * asynchronous with respect to the host. */
+
+/* Energy only; leaves the final state and lambda = H psi resident so that a
+ * following rie_engine_backward_device can run the adjoint sweep without
+ * recomputing the forward pass. */
+rie_status rie_engine_forward_device(rie_engine *engine, const double *d_params,
+                                     double *d_energy, void *stream);
+
+/* Adjoint sweep on the state left by the last rie_engine_forward_device with
+ * the same d_params; d_gradient receives 2*n*L values, scaled by *d_scale
+ * when d_scale is non-NULL (the chain-rule factor of an autograd backward). */
+rie_status rie_engine_backward_device(rie_engine *engine, const double *d_params,
+                                      const double *d_scale, double *d_gradient,
+                                      void *stream);
+
+/* Forward + adjoint in one call. */
+rie_status rie_engine_energy_and_grad_device(rie_engine *engine,
+                                             const double *d_params,
+                                             double *d_energy, double *d_gradient,
+                                             void *stream);
+
+/* Store the statevector at every k-th layer boundary during the forward pass
+ * and restore it there during the backward sweep instead of continuing the
+ * inverse walk (k = 0 disables).  Costs one extra state buffer per stored
+ * boundary. */
+rie_status rie_engine_set_checkpoint_interval(rie_engine *engine, size_t every_k_layers);
+
+/* Run energy_and_grad with a full inverse walk through layer 0 and report
+ * || psi_walked - |0...0> ||_2, the accumulated drift of recomputing states
+ * by inverse gates (host pointers, synchronous). */
+rie_status rie_engine_inverse_walk_drift(rie_engine *engine, const double *params,
+                                         size_t num_params, double *drift_out);
+
+/* Copy the final statevector of the last forward pass to host memory
+ * (2^n complex128 values, interleaved re/im). */
+rie_status rie_engine_copy_state(rie_engine *engine, double *host_out, size_t num_doubles);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RIE_H */

@@ -1,0 +1,139 @@
+"""GPU parity: every strategy of the HIP engine against the FP64 CPU oracle.
+
+The bar is |dE|, |dgrad| <= 1e-10 absolute in complex128 (the reference's own
+tests use 1e-9 / 1e-8, tests/test_backends_parity.py:27-31).  Cases cover the
+reference's test shapes (n = 2, 4, 6 at L = 2; n = 8 and 10 cross-strategy;
+dense at 6q x 32L and 8q x 8L) plus every tile-pass geometry of the engine
+(n = 13: one narrow high pass; 21: one full high pass; 22: two high passes).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10
+
+
+def _params(seed: int, n: int, layers: int, scale: float | None = None) -> np.ndarray:
+    rng = np.random.default_rng(seed)
+    if scale is None:
+        return rng.uniform(0.0, 2.0 * np.pi, size=(layers, n, 2))
+    return scale * rng.standard_normal((layers, n, 2))
+
+
+def _run(backend_module, n, layers, field, strategy, params, width=8, compute_gradient=True):
+    eng = backend_module.RingIsingCudaBackend(n, layers, field, strategy, width, False)
+    out = eng.energy_and_grad(np.ascontiguousarray(params.reshape(-1)), compute_gradient)
+    return out["energy"], np.asarray(out["gradient"])
+
+
+def _check(oracle, backend_module, n, layers, field, strategy, seed, width=8, scale=None):
+    params = _params(seed, n, layers, scale)
+    print(f"case strategy={strategy} n={n} L={layers} h={field} seed={seed}", flush=True)
+    e_ref, g_ref = oracle.energy_and_grad(n, layers, field, params)
+    e, g = _run(backend_module, n, layers, field, strategy, params, width)
+    assert abs(e - e_ref) <= TOL, (e, e_ref)
+    assert g.shape == g_ref.shape
+    assert np.max(np.abs(g - g_ref)) <= TOL, np.max(np.abs(g - g_ref))
+    e_only, g_only = _run(backend_module, n, layers, field, strategy, params, width, False)
+    assert abs(e_only - e_ref) <= TOL
+    assert g_only.size == 0
+
+
+@pytest.mark.parametrize("strategy", ["structured_adjoint", "inverse_walk_cuQuantum", "dense_scan"])
+@pytest.mark.parametrize("n,field", [(2, 0.7), (4, 1.0), (6, 1.1)])
+@pytest.mark.parametrize("seed", [123, 321, 2026])
+def test_reference_small_cases(oracle, backend_module, strategy, n, field, seed):
+    # reference: tests/test_backends_parity.py:91-128 (params 0.2 N(0,1), L = 2)
+    _check(oracle, backend_module, n, 2, field, strategy, seed, width=1, scale=0.2)
+
+
+@pytest.mark.parametrize("n,layers", [(8, 1), (8, 3), (10, 2), (11, 4), (12, 3)])
+def test_structured_whole_circuit_kernel(oracle, backend_module, n, layers):
+    _check(oracle, backend_module, n, layers, 0.9, "structured_adjoint", 1000 + n)
+
+
+@pytest.mark.parametrize("n,layers", [(13, 1), (13, 2), (14, 3), (16, 2), (21, 2), (22, 2)])
+def test_structured_tile_passes(oracle, backend_module, n, layers):
+    _check(oracle, backend_module, n, layers, 1.0, "structured_adjoint", 2000 + n)
+
+
+@pytest.mark.parametrize("width", [1, 2, 3, 4, 8])
+def test_structured_chunk_widths_agree(oracle, backend_module, width):
+    # reference: tests/test_backends_parity.py:163-195
+    _check(oracle, backend_module, 10, 2, 0.9, "structured_adjoint", 2030, width=width, scale=0.2)
+
+
+@pytest.mark.parametrize("n,layers", [(8, 2), (13, 2)])
+def test_gate_level_strategy(oracle, backend_module, n, layers):
+    _check(oracle, backend_module, n, layers, 0.9, "inverse_walk_cuQuantum", 2028 + n)
+
+
+@pytest.mark.parametrize("n,layers", [(3, 1), (5, 3), (6, 32), (7, 5), (8, 8), (8, 16)])
+def test_dense_scan(oracle, backend_module, n, layers):
+    # reference: tests/test_backends_parity.py:197-235 (6q x 32L, 8q x 8L)
+    _check(oracle, backend_module, n, layers, 1.0, "dense_scan", 2032 + n)
+
+
+def test_profile_reports_reference_stage_names(backend_module):
+    eng = backend_module.RingIsingCudaBackend(14, 2, 1.0, "structured_adjoint", 8, False)
+    out = eng.energy_and_grad(_params(5, 14, 2).reshape(-1), True, True)
+    names = set(out["timings_ms"])
+    for key in ("build_ops_host_ms", "zero_state_ms", "forward_apply_ms", "hamiltonian_ms",
+                "energy_dot_ms", "gradient_zero_ms", "backward_sweep_ms", "gradient_copy_ms"):
+        assert key in names
+    assert all(v >= 0.0 for v in out["timings_ms"].values())
+
+
+def test_forward_graph_benchmark(backend_module):
+    for n in (10, 14):
+        eng = backend_module.RingIsingCudaBackend(n, 2, 1.0, "structured_adjoint", 8, False)
+        res = eng.benchmark_structured_forward_graph(_params(6, n, 2).reshape(-1), 5)
+        assert res["normal_forward_ms"] > 0.0 and res["graph_forward_ms"] > 0.0
+        assert res["speedup"] > 0.0
+
+
+def test_results_are_bitwise_reproducible(backend_module):
+    params = _params(9, 16, 2).reshape(-1)
+    eng = backend_module.RingIsingCudaBackend(16, 2, 1.0, "structured_adjoint", 8, False)
+    a = eng.energy_and_grad(params)
+    b = eng.energy_and_grad(params)
+    assert a["energy"] == b["energy"]
+    assert np.array_equal(a["gradient"], b["gradient"])
+
+
+def test_final_state_matches_oracle(oracle, backend_module):
+    n, layers = 14, 3
+    params = _params(11, n, layers)
+    eng = backend_module.RingIsingCudaBackend(n, layers, 1.0, "structured_adjoint", 8, False)
+    eng.energy_and_grad(params.reshape(-1), False)
+    psi = np.asarray(eng.final_state())
+    ref = oracle.state(n, layers, params)
+    assert np.max(np.abs(psi - ref)) <= 1e-12
+
+
+@pytest.mark.parametrize("n,layers", [(10, 4), (20, 4)])
+def test_inverse_walk_drift_and_checkpoints(oracle, backend_module, n, layers):
+    params = _params(2024, n, layers)
+    eng = backend_module.RingIsingCudaBackend(n, layers, 1.0, "structured_adjoint", 8, False)
+    drift = eng.inverse_walk_drift(params.reshape(-1))
+    assert 0.0 <= drift < 1e-11
+    e_ref, g_ref = oracle.energy_and_grad(n, layers, 1.0, params)
+    eng.set_checkpoint_interval(2)
+    out = eng.energy_and_grad(params.reshape(-1))
+    assert abs(out["energy"] - e_ref) <= TOL
+    assert np.max(np.abs(out["gradient"] - g_ref)) <= TOL
+
+
+def test_module_level_energy_and_grad(oracle, backend_module):
+    n, layers = 6, 2
+    params = _params(7, n, layers)
+    out = backend_module.energy_and_grad(n, layers, 1.0, "structured_adjoint", params.reshape(-1))
+    e_ref, g_ref = oracle.energy_and_grad(n, layers, 1.0, params)
+    assert abs(out["energy"] - e_ref) <= TOL
+    assert np.max(np.abs(out["gradient"] - g_ref)) <= TOL
+    with pytest.raises(ValueError):
+        backend_module.energy_and_grad(n, layers, 1.0, "structured_adjoint", params)  # 3-D
