@@ -1,0 +1,208 @@
+#!/usr/bin/env python3
+"""Benchmark: structured_adjoint energy_and_grad, 30 qubits x 2 layers, complex128.
+
+One "step" is one energy_and_grad evaluation (forward sweep, lambda = H psi,
+energy, adjoint sweep, gradient) of the ring-Ising circuit on one MI355X, with
+parameters, energy and gradient resident in HBM (device-pointer entry point;
+no host transfer inside the timed region).  Parameters are U[0, 2pi) from
+numpy default_rng(5), J = h = 1.
+
+The printed JSON line carries
+  * value = evaluations per second (one GPU);
+  * roofline = the dominant kernel's algorithmic HBM bytes per launch over its
+    average launch duration, both measured with HIP events on the engine's
+    launch stream during the timed region, against the 8 TB/s HBM3E peak;
+  * cpu_baseline = the FP64 CPU oracle (gate-level adjoint, PennyLane gate
+    order, OpenMP) timed on this host's cores on a bounded smaller-n sample
+    and scaled to the 30-qubit workload by its 2^n * n cost.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import statistics
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+if str(ROOT) not in sys.path:
+    sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "energy_and_grad evals/s, % HBM roofline"
+TRAFFIC_FILE = ROOT / "profiles" / "hbm_traffic_structured_30q2l.json"
+
+
+def parse_args() -> argparse.Namespace:
+    p = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=5)
+    p.add_argument("--warmup", type=int, default=2)
+    p.add_argument("--qubits", type=int, default=30)
+    p.add_argument("--layers", type=int, default=2)
+    p.add_argument("--seed", type=int, default=5)
+    p.add_argument("--no-cpu-baseline", action="store_true")
+    return p.parse_args()
+
+
+def cpu_baseline(n_target: int, layers: int, budget_s: float = 20.0) -> dict:
+    """Time the oracle on a bounded sample and scale to n_target."""
+    from oracle import oracle as cpu_oracle
+
+    cpu_oracle.build()
+    cores = min(16, len(os.sched_getaffinity(0)))
+    cpu_oracle.set_num_threads(cores)
+    rng = np.random.default_rng(5)
+
+    def one_eval(n: int) -> float:
+        params = rng.uniform(0.0, 2.0 * math.pi, size=(layers, n, 2))
+        t0 = time.perf_counter()
+        cpu_oracle.energy_and_grad(n, layers, 1.0, params)
+        return time.perf_counter() - t0
+
+    n = 18
+    t = one_eval(n)
+    # grow n while 6 evaluations (1 warm-up + 5 timed) stay within the budget
+    while n < min(n_target, 26):
+        t_next = t * 2.0 * (n + 1) / n
+        if 6 * t_next > budget_s:
+            break
+        n += 1
+        t = one_eval(n)
+    samples = [one_eval(n) for _ in range(5)]
+    med = statistics.median(samples)
+    scale = (2.0 ** (n_target - n)) * (n_target / n)
+    return {
+        "value": 1.0 / (med * scale),
+        "unit": "evals/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": (f"oracle gate-level adjoint (FP64, OpenMP, {cores} threads) at {n} qubits x "
+                   f"{layers} layers: median {med * 1e3:.1f} ms/eval over 5 runs after 1 warm-up, "
+                   f"scaled to {n_target} qubits by 2^n * n (x{scale:.3g})"),
+    }
+
+
+def main() -> int:
+    args = parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the single-state "
+              "workload does not shard in this build.", file=sys.stderr)
+        return 2
+    import torch
+
+    from gpu_simulation_amd import native
+
+    ext = native.load_extension()
+    if not torch.cuda.is_available() or ext.device_count() < 1:
+        raise RuntimeError("no GPU visible")
+    n, layers = args.qubits, args.layers
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(args.seed)
+    host_params = rng.uniform(0.0, 2.0 * math.pi, size=(layers, n, 2)).reshape(-1)
+    params = torch.tensor(host_params, dtype=torch.float64, device=dev)
+    energy = torch.zeros(1, dtype=torch.float64, device=dev)
+    grad = torch.zeros(2 * n * layers, dtype=torch.float64, device=dev)
+    engine = ext.RingIsingCudaBackend(n, layers, 1.0, "structured_adjoint", 8, False)
+    stream = torch.cuda.current_stream(dev)
+    handle = stream.cuda_stream
+
+    def step() -> None:
+        engine.energy_and_grad_device(params.data_ptr(), energy.data_ptr(), grad.data_ptr(), handle)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    engine.reset_kernel_stats()
+    engine.set_kernel_timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    engine.set_kernel_timing(False)
+    stats = engine.kernel_stats()
+
+    e_val = float(energy.item())
+    if not (abs(e_val) <= n * 2.0 + 1e-9 and torch.isfinite(grad).all()):
+        raise RuntimeError(f"implausible result: energy {e_val}")
+
+    ms_per_step = elapsed * 1e3 / args.steps
+    kernels = []
+    total_bytes = 0.0
+    total_kernel_ms = 0.0
+    for k in stats:
+        if k["launches"] == 0:
+            continue
+        avg_ms = k["total_ms"] / k["launches"]
+        gbs = (k["total_bytes"] / 1e9) / (k["total_ms"] / 1e3) if k["total_ms"] > 0 else 0.0
+        kernels.append({"name": k["name"], "launches_per_step": k["launches"] / args.steps,
+                        "avg_ms": round(avg_ms, 4),
+                        "bytes_per_launch": k["total_bytes"] / k["launches"],
+                        "GBps": round(gbs, 1)})
+        total_bytes += k["total_bytes"]
+        total_kernel_ms += k["total_ms"]
+    dom = max(stats, key=lambda k: k["total_ms"])
+    dom_avg_s = dom["total_ms"] / dom["launches"] / 1e3
+    dom_bytes = dom["total_bytes"] / dom["launches"]
+    achieved = dom_bytes / dom_avg_s / 1e9
+    traffic = None
+    if TRAFFIC_FILE.exists():
+        try:
+            traffic = json.loads(TRAFFIC_FILE.read_text()).get(dom["name"])
+        except (ValueError, OSError):
+            traffic = None
+    roofline = {
+        "bound": "hbm",
+        "kernel": dom["name"],
+        "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": traffic,
+        "bytes_per_launch": dom_bytes,
+        "eval_achieved": round(total_bytes / args.steps / (ms_per_step / 1e3) / 1e9, 1),
+        "state_passes_per_eval": round(total_bytes / args.steps / (16.0 * 2 ** n), 2),
+    }
+    result = {
+        "metric": METRIC,
+        "value": round(1e3 / ms_per_step, 4),
+        "unit": "evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "complex128",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"structured_adjoint energy_and_grad, {n} qubits x {layers} layers, "
+                         f"complex128, params U[0,2pi) seed {args.seed}, J=h=1"),
+            "num_qubits": n,
+            "layers": layers,
+            "strategy": "structured_adjoint",
+            "parallelism": "single GPU",
+        },
+        "roofline": roofline,
+        "kernels": kernels,
+        "energy": e_val,
+    }
+    if not args.no_cpu_baseline:
+        result["cpu_baseline"] = cpu_baseline(n, layers)
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -197,6 +197,30 @@ class RingIsingCudaBackend {
         return out;
     }
 
+    void set_kernel_timing(bool enable) { check(rie_engine_set_kernel_timing(engine_, enable)); }
+
+    void reset_kernel_stats() { check(rie_engine_reset_kernel_stats(engine_)); }
+
+    py::list kernel_stats() {
+        size_t count = 0;
+        check(rie_engine_kernel_stat_count(engine_, &count));
+        py::list out;
+        for (size_t i = 0; i < count; ++i) {
+            const char *name = nullptr;
+            long launches = 0;
+            double ms = 0.0;
+            double bytes = 0.0;
+            check(rie_engine_kernel_stat(engine_, i, &name, &launches, &ms, &bytes));
+            py::dict d;
+            d["name"] = py::str(name);
+            d["launches"] = py::int_(launches);
+            d["total_ms"] = py::float_(ms);
+            d["total_bytes"] = py::float_(bytes);
+            out.append(std::move(d));
+        }
+        return out;
+    }
+
     size_t num_qubits() const { return num_qubits_; }
     size_t num_layers() const { return num_layers_; }
 
@@ -232,6 +256,9 @@ PYBIND11_MODULE(_cuda_backend, m) {
              py::arg("every_k_layers"))
         .def("inverse_walk_drift", &RingIsingCudaBackend::inverse_walk_drift, py::arg("params"))
         .def("final_state", &RingIsingCudaBackend::final_state)
+        .def("set_kernel_timing", &RingIsingCudaBackend::set_kernel_timing, py::arg("enable"))
+        .def("reset_kernel_stats", &RingIsingCudaBackend::reset_kernel_stats)
+        .def("kernel_stats", &RingIsingCudaBackend::kernel_stats)
         .def_property_readonly("num_qubits", &RingIsingCudaBackend::num_qubits)
         .def_property_readonly("num_layers", &RingIsingCudaBackend::num_layers);
 

@@ -224,6 +224,13 @@ Engine::~Engine() {
     device_free(d_epartials);
     device_free(d_drift);
     dense_free();
+    for (auto &p : kpending) {
+        kevent_pool.push_back(p.a);
+        kevent_pool.push_back(p.b);
+    }
+    for (auto ev : kevent_pool) {
+        (void)hipEventDestroy(ev);
+    }
     if (ev_a != nullptr) {
         (void)hipEventDestroy(ev_a);
     }
@@ -256,6 +263,59 @@ void Engine::stage_end(hipStream_t s, const char *name) {
     timings.emplace_back(name, (double)ms);
 }
 
+hipEvent_t Engine::kt_event() {
+    if (!kevent_pool.empty()) {
+        hipEvent_t e = kevent_pool.back();
+        kevent_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    check(hipEventCreate(&e), "hipEventCreate");
+    return e;
+}
+
+void Engine::kt_begin(hipStream_t s) {
+    if (!ktime) {
+        return;
+    }
+    kt_open = kt_event();
+    check(hipEventRecord(kt_open, s), "hipEventRecord");
+}
+
+void Engine::kt_end(hipStream_t s, const char *name, double bytes) {
+    if (!ktime) {
+        return;
+    }
+    hipEvent_t b = kt_event();
+    check(hipEventRecord(b, s), "hipEventRecord");
+    kpending.push_back({name, bytes, kt_open, b});
+    kt_open = nullptr;
+}
+
+void Engine::kt_flush() {
+    for (auto &p : kpending) {
+        check(hipEventSynchronize(p.b), "hipEventSynchronize");
+        float ms = 0.0f;
+        check(hipEventElapsedTime(&ms, p.a, p.b), "hipEventElapsedTime");
+        KernelStat *st_ = nullptr;
+        for (auto &k : kstats) {
+            if (k.name == p.name) {
+                st_ = &k;
+            }
+        }
+        if (st_ == nullptr) {
+            kstats.push_back({p.name, 0, 0.0, 0.0});
+            st_ = &kstats.back();
+        }
+        st_->launches += 1;
+        st_->total_ms += (double)ms;
+        st_->total_bytes += p.bytes;
+        kevent_pool.push_back(p.a);
+        kevent_pool.push_back(p.b);
+    }
+    kpending.clear();
+}
+
 // ---------------------------------------------------------------------------
 // Structured adjoint: tile-pass path (n > 12)
 // ---------------------------------------------------------------------------
@@ -264,16 +324,21 @@ void Engine::structured_forward(hipStream_t s, bool save_ckpts) {
     // Forward working buffers are st[0] and st[1]; boundary states that are
     // checkpointed are written straight into their checkpoint buffer by the
     // low (scatter) pass and read from there by the next layer's first pass.
+    const double S = (double)sizeof(double2) * (double)N;
+    kt_begin(s);
     launch_init_product_cnot(st[0], d_coef, n, s);
     check_launch("init_product_cnot");
+    kt_end(s, "init_product_cnot", S);
     const double2 *cur = st[0];
     int cur_buf = 0;  // -1 while cur is a checkpoint buffer
     for (int l = 1; l < L; ++l) {
         const WireCoef *lc = d_coef + (size_t)l * n;
         for (size_t p = 1; p < passes.size(); ++p) {
             double2 *out = (cur_buf >= 0) ? st[cur_buf] : st[0];
+            kt_begin(s);
             launch_fwd_pass(R, false, cur, out, lc, passes[p].g, passes[p].grid, s);
             check_launch("fwd_pass");
+            kt_end(s, "fwd_pass_high", 2 * S);
             if (cur_buf < 0) {
                 cur_buf = 0;
             }
@@ -291,8 +356,10 @@ void Engine::structured_forward(hipStream_t s, bool save_ckpts) {
             out_buf = (cur_buf == 0) ? 1 : 0;
             out = st[out_buf];
         }
+        kt_begin(s);
         launch_fwd_pass(R, true, cur, out, lc, passes[0].g, passes[0].grid, s);
         check_launch("fwd_pass scatter");
+        kt_end(s, "fwd_pass_low_cnot", 2 * S);
         cur = out;
         cur_buf = out_buf;
     }
@@ -319,10 +386,15 @@ void Engine::structured_hamiltonian(hipStream_t s, bool write_lam) {
     lam_idx = write_lam ? free_buffer({final_cur}) : -1;
     double2 *lam = write_lam ? st[lam_idx] : nullptr;
     size_t off = 0;
+    const double S = (double)sizeof(double2) * (double)N;
     for (size_t p = 0; p < passes.size(); ++p) {
-        launch_ham_pass(R, p == 0, write_lam && p > 0, write_lam, psi, lam, lam, passes[p].g, h,
+        const bool read_lam = write_lam && p > 0;
+        kt_begin(s);
+        launch_ham_pass(R, p == 0, read_lam, write_lam, psi, lam, lam, passes[p].g, h,
                         d_epartials + off, passes[p].grid, s);
         check_launch("ham_pass");
+        kt_end(s, p == 0 ? "ham_pass_low" : "ham_pass_high",
+               S * (1.0 + (write_lam ? 1.0 : 0.0) + (read_lam ? 1.0 : 0.0)));
         off += (size_t)passes[p].grid;
     }
     energy_partials = off;
@@ -342,23 +414,34 @@ void Engine::structured_backward(hipStream_t s, const double *scale, bool full_w
         const int nlam = free_buffer({cur, lam, ncur});
         const bool only_low = passes.size() == 1;
         const bool write_lam0 = !(last_layer && only_low && !full_walk);
+        const double S = (double)sizeof(double2) * (double)N;
+        kt_begin(s);
         launch_bwd_pass(R, true, analytic, !analytic, write_lam0, cur_in, st[lam],
                         analytic ? nullptr : st[ncur], st[nlam], lc, passes[0].g, 0, d_partials,
                         passes[0].grid, s);
         check_launch("bwd_pass gather");
+        kt_end(s, analytic ? "bwd_pass_low_cnot_layer0" : "bwd_pass_low_cnot",
+               S * ((analytic ? 1.0 : 4.0) + (analytic && write_lam0 ? 1.0 : 0.0)));
+        kt_begin(s);
         launch_reduce_grad(d_partials, passes[0].grid, T, passes[0].g, l, d_grad, scale, s);
         check_launch("reduce_grad");
+        kt_end(s, "reduce_grad", 0.0);
         cur = ncur;
         lam = nlam;
         for (size_t p = 1; p < passes.size(); ++p) {
             const bool final_pass = last_layer && (p + 1 == passes.size()) && !full_walk;
             const uint64_t processed = analytic ? wires_before_pass(p) : 0;
+            kt_begin(s);
             launch_bwd_pass(R, false, analytic, !analytic, !final_pass,
                             analytic ? nullptr : st[cur], st[lam], analytic ? nullptr : st[cur],
                             st[lam], lc, passes[p].g, processed, d_partials, passes[p].grid, s);
             check_launch("bwd_pass");
+            kt_end(s, analytic ? "bwd_pass_high_layer0" : "bwd_pass_high",
+                   S * ((analytic ? 1.0 : 4.0) + (analytic && !final_pass ? 1.0 : 0.0)));
+            kt_begin(s);
             launch_reduce_grad(d_partials, passes[p].grid, T, passes[p].g, l, d_grad, scale, s);
             check_launch("reduce_grad");
+            kt_end(s, "reduce_grad", 0.0);
         }
         if (l > 0) {
             // Restore the stored boundary state (after layer l-1) if there is one.
@@ -390,8 +473,10 @@ void Engine::small_run(hipStream_t s, bool want_grad, const double *scale, doubl
     job.drift = drift_out;
     job.gstate = nullptr;
     job.scale = scale;
+    kt_begin(s);
     launch_small(nullptr, job, 1, n, true, s);
     check_launch("small_circuit");
+    kt_end(s, "small_circuit", 0.0);
 }
 
 // ---------------------------------------------------------------------------
@@ -466,8 +551,10 @@ void Engine::forward_device(const double *params, double *energy, hipStream_t s,
     }
     structured_forward(s, ckpt_every > 0);
     structured_hamiltonian(s, keep_lam);
+    kt_begin(s);
     launch_reduce_sum(d_epartials, (int)energy_partials, energy, s);
     check_launch("energy reduce");
+    kt_end(s, "reduce_energy", 0.0);
     resident = keep_lam;
 }
 
@@ -903,6 +990,54 @@ rie_status rie_engine_inverse_walk_drift(rie_engine *engine, const double *param
             rie::fail(RIE_INVALID_ARGUMENT, "null pointer argument.");
         }
         *drift_out = as_engine(engine)->inverse_walk_drift(params, num_params);
+    });
+}
+
+rie_status rie_engine_set_kernel_timing(rie_engine *engine, int enable) {
+    return guard([&]() {
+        Engine *e = as_engine(engine);
+        e->kt_flush();
+        e->ktime = enable != 0;
+    });
+}
+
+rie_status rie_engine_reset_kernel_stats(rie_engine *engine) {
+    return guard([&]() {
+        Engine *e = as_engine(engine);
+        e->kt_flush();
+        e->kstats.clear();
+    });
+}
+
+rie_status rie_engine_kernel_stat_count(rie_engine *engine, size_t *count_out) {
+    return guard([&]() {
+        Engine *e = as_engine(engine);
+        if (count_out == nullptr) {
+            rie::fail(RIE_INVALID_ARGUMENT, "null pointer argument.");
+        }
+        e->kt_flush();
+        *count_out = e->kstats.size();
+    });
+}
+
+rie_status rie_engine_kernel_stat(rie_engine *engine, size_t index, const char **name_out,
+                                  long *launches_out, double *total_ms_out,
+                                  double *total_bytes_out) {
+    return guard([&]() {
+        Engine *e = as_engine(engine);
+        if (name_out == nullptr || launches_out == nullptr || total_ms_out == nullptr ||
+            total_bytes_out == nullptr) {
+            rie::fail(RIE_INVALID_ARGUMENT, "null pointer argument.");
+        }
+        e->kt_flush();
+        if (index >= e->kstats.size()) {
+            rie::fail(RIE_INVALID_ARGUMENT, "kernel stat index out of range.");
+        }
+        const auto &k = e->kstats[index];
+        *name_out = k.name.c_str();
+        *launches_out = k.launches;
+        *total_ms_out = k.total_ms;
+        *total_bytes_out = k.total_bytes;
     });
 }
 

@@ -85,6 +85,30 @@ struct Engine {
     int walked_cur = -1;
     bool resident = false;
 
+    // per-launch kernel timing (opt-in): HIP events around every launch on
+    // the launch stream, with the launch's algorithmic HBM bytes
+    struct KernelStat {
+        std::string name;
+        long launches = 0;
+        double total_ms = 0.0;
+        double total_bytes = 0.0;
+    };
+    struct PendingLaunch {
+        const char *name;
+        double bytes;
+        hipEvent_t a;
+        hipEvent_t b;
+    };
+    bool ktime = false;
+    std::vector<PendingLaunch> kpending;
+    std::vector<hipEvent_t> kevent_pool;
+    std::vector<KernelStat> kstats;
+    hipEvent_t kt_open = nullptr;
+    void kt_begin(hipStream_t s);
+    void kt_end(hipStream_t s, const char *name, double bytes);
+    void kt_flush();
+    hipEvent_t kt_event();
+
     // profiling
     bool profiling = false;
     hipEvent_t ev_a = nullptr;

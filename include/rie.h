@@ -153,6 +153,17 @@ rie_status rie_engine_set_checkpoint_interval(rie_engine *engine, size_t every_k
 rie_status rie_engine_inverse_walk_drift(rie_engine *engine, const double *params,
                                          size_t num_params, double *drift_out);
 
+/* Per-launch kernel timing (no counterpart in the reference; for benchmarks).
+ * When enabled, every kernel launch of the structured path is bracketed by
+ * HIP events on its launch stream and accumulated per kernel name together
+ * with the launch's algorithmic HBM bytes (state arrays read + written). */
+rie_status rie_engine_set_kernel_timing(rie_engine *engine, int enable);
+rie_status rie_engine_reset_kernel_stats(rie_engine *engine);
+rie_status rie_engine_kernel_stat_count(rie_engine *engine, size_t *count_out);
+rie_status rie_engine_kernel_stat(rie_engine *engine, size_t index, const char **name_out,
+                                  long *launches_out, double *total_ms_out,
+                                  double *total_bytes_out);
+
 /* Copy the final statevector of the last forward pass to host memory
  * (2^n complex128 values, interleaved re/im). */
 rie_status rie_engine_copy_state(rie_engine *engine, double *host_out, size_t num_doubles);
