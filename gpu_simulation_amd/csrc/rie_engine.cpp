@@ -168,6 +168,7 @@ Engine::Engine(size_t num_qubits, size_t num_layers, double field, const std::st
     check(hipStreamCreateWithFlags(&own_stream, hipStreamNonBlocking), "hipStreamCreate");
     check(hipEventCreate(&ev_a), "hipEventCreate");
     check(hipEventCreate(&ev_b), "hipEventCreate");
+    check(hipEventCreateWithFlags(&ev_last, hipEventDisableTiming), "hipEventCreate");
     build_plan();
 
     d_params = (double *)device_alloc(sizeof(double) * num_params, "alloc params");
@@ -237,6 +238,9 @@ Engine::~Engine() {
     if (ev_b != nullptr) {
         (void)hipEventDestroy(ev_b);
     }
+    if (ev_last != nullptr) {
+        (void)hipEventDestroy(ev_last);
+    }
     if (own_stream != nullptr) {
         (void)hipStreamDestroy(own_stream);
     }
@@ -261,6 +265,32 @@ void Engine::stage_end(hipStream_t s, const char *name) {
     float ms = 0.0f;
     check(hipEventElapsedTime(&ms, ev_a, ev_b), "hipEventElapsedTime");
     timings.emplace_back(name, (double)ms);
+}
+
+static bool is_capturing(hipStream_t s) {
+    hipStreamCaptureStatus status = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &status) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return status != hipStreamCaptureStatusNone;
+}
+
+// Successive calls on one engine may use different streams (its own for the
+// host-pointer API, the caller's for the device API); they share buffers, so
+// each call waits for the previous one.  Skipped inside stream capture.
+void Engine::order_begin(hipStream_t s) {
+    if (ev_last_valid && !is_capturing(s)) {
+        check(hipStreamWaitEvent(s, ev_last, 0), "hipStreamWaitEvent");
+    }
+}
+
+void Engine::order_end(hipStream_t s) {
+    if (is_capturing(s)) {
+        return;
+    }
+    check(hipEventRecord(ev_last, s), "hipEventRecord");
+    ev_last_valid = true;
 }
 
 hipEvent_t Engine::kt_event() {
@@ -607,6 +637,7 @@ void Engine::energy_and_grad_host(const double *params, size_t count, bool want_
         fail(RIE_INVALID_ARGUMENT, "null pointer argument.");
     }
     hipStream_t s = own_stream;
+    order_begin(s);
     timings.clear();
     profiling = profile;
     auto t0 = std::chrono::steady_clock::now();
@@ -675,6 +706,7 @@ void Engine::energy_and_grad_host(const double *params, size_t count, bool want_
                              s),
               "copy gradient");
     }
+    order_end(s);
     check(hipStreamSynchronize(s), "synchronize");
     stage_end(s, "gradient_copy_ms");
     profiling = false;
@@ -689,6 +721,7 @@ double Engine::inverse_walk_drift(const double *params, size_t count) {
         fail(RIE_INVALID_ARGUMENT, "inverse-walk drift requires structured_adjoint.");
     }
     hipStream_t s = own_stream;
+    order_begin(s);
     check(hipMemcpyAsync(d_params, params, sizeof(double) * num_params, hipMemcpyHostToDevice, s),
           "upload params");
     launch_coef(d_params, d_coef, n * L, s);
@@ -711,7 +744,9 @@ double Engine::inverse_walk_drift(const double *params, size_t count) {
         check(hipMemcpyAsync(&drift2, d_drift + 8192, sizeof(double), hipMemcpyDeviceToHost, s),
               "copy drift");
     }
+    order_end(s);
     check(hipStreamSynchronize(s), "synchronize");
+    resident = false;
     return std::sqrt(drift2);
 }
 
@@ -743,6 +778,7 @@ void Engine::benchmark_forward_graph(const double *params, size_t count, size_t 
         fail(RIE_INVALID_ARGUMENT, "repeats must be at least 1.");
     }
     hipStream_t s = own_stream;
+    order_begin(s);
     check(hipMemcpyAsync(d_params, params, sizeof(double) * num_params, hipMemcpyHostToDevice, s),
           "upload params");
     auto forward = [&]() {
@@ -778,6 +814,8 @@ void Engine::benchmark_forward_graph(const double *params, size_t count, size_t 
     *graph_ms = time_it([&]() { check(hipGraphLaunch(exec, s), "graph launch"); });
     (void)hipGraphExecDestroy(exec);
     (void)hipGraphDestroy(graph);
+    order_end(s);
+    resident = false;
 }
 
 void Engine::small_run_captured(hipStream_t s) {
@@ -790,6 +828,9 @@ void Engine::copy_state(double *host_out, size_t count) {
     }
     if (strategy != RIE_STRATEGY_STRUCTURED_ADJOINT || small) {
         fail(RIE_UNSUPPORTED, "copy_state requires structured_adjoint with num_qubits > 12.");
+    }
+    if (ev_last_valid) {
+        check(hipEventSynchronize(ev_last), "hipEventSynchronize");
     }
     check(hipMemcpy(host_out, final_state(), sizeof(double2) * N, hipMemcpyDeviceToHost),
           "copy state");
@@ -829,8 +870,11 @@ Engine *as_engine(rie_engine *e) {
     return reinterpret_cast<Engine *>(e);
 }
 
+// Device-path calls run on the caller's stream; NULL is HIP's default
+// (null) stream, which is also the handle torch reports for its default stream.
 hipStream_t stream_of(Engine *e, void *stream) {
-    return stream != nullptr ? static_cast<hipStream_t>(stream) : e->own_stream;
+    (void)e;
+    return static_cast<hipStream_t>(stream);
 }
 
 }  // namespace
@@ -951,7 +995,10 @@ rie_status rie_engine_forward_device(rie_engine *engine, const double *d_params,
         if (d_params == nullptr || d_energy == nullptr) {
             rie::fail(RIE_INVALID_ARGUMENT, "null pointer argument.");
         }
-        e->forward_device(d_params, d_energy, stream_of(e, stream), true);
+        hipStream_t s = stream_of(e, stream);
+        e->order_begin(s);
+        e->forward_device(d_params, d_energy, s, true);
+        e->order_end(s);
     });
 }
 
@@ -962,7 +1009,10 @@ rie_status rie_engine_backward_device(rie_engine *engine, const double *d_params
         if (d_params == nullptr || d_gradient == nullptr) {
             rie::fail(RIE_INVALID_ARGUMENT, "null pointer argument.");
         }
-        e->backward_device(d_params, d_scale, d_gradient, stream_of(e, stream));
+        hipStream_t s = stream_of(e, stream);
+        e->order_begin(s);
+        e->backward_device(d_params, d_scale, d_gradient, s);
+        e->order_end(s);
     });
 }
 
@@ -974,8 +1024,10 @@ rie_status rie_engine_energy_and_grad_device(rie_engine *engine, const double *d
             rie::fail(RIE_INVALID_ARGUMENT, "null pointer argument.");
         }
         hipStream_t s = stream_of(e, stream);
+        e->order_begin(s);
         e->forward_device(d_params, d_energy, s, true);
         e->backward_device(d_params, nullptr, d_gradient, s);
+        e->order_end(s);
     });
 }
 

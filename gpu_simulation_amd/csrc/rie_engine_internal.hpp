@@ -109,6 +109,12 @@ struct Engine {
     void kt_flush();
     hipEvent_t kt_event();
 
+    // cross-stream ordering of successive calls on one engine
+    hipEvent_t ev_last = nullptr;
+    bool ev_last_valid = false;
+    void order_begin(hipStream_t s);
+    void order_end(hipStream_t s);
+
     // profiling
     bool profiling = false;
     hipEvent_t ev_a = nullptr;

@@ -15,8 +15,9 @@
  * (reference: cpp/ising_cuda_kernels.cu:18-44, cpp/ising_cuda_bindings.cpp).
  *
  * No torch or HIP types appear in these signatures: streams are passed as
- * void* (a hipStream_t, NULL for the engine's own stream) and device buffers
- * as plain pointers.
+ * void* (a hipStream_t; NULL is HIP's default stream) and device buffers as
+ * plain pointers.  Host-pointer entry points run on a stream owned by the
+ * engine and return after the results are copied back.
  */
 #ifndef RIE_H
 #define RIE_H
@@ -119,7 +120,7 @@ int rie_engine_strategy(const rie_engine *engine);
  * These have no counterpart in the reference; they serve callers that keep
  * parameters on the GPU (the torch.autograd wrapper, captured optimizer
  * steps).  d_params, d_energy and d_gradient are device pointers; stream is
- * a hipStream_t or NULL for the engine's own stream.  The calls are
+ * the hipStream_t to issue on (NULL: the default stream).  The calls are
  * asynchronous with respect to the host. */
 
 /* Energy only; leaves the final state and lambda = H psi resident so that a
