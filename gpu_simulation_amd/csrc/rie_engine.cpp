@@ -121,16 +121,6 @@ void Engine::build_plan() {
     }
 }
 
-uint64_t Engine::wires_before_pass(size_t p) const {
-    uint64_t mask = 0;
-    for (size_t i = 0; i < p; ++i) {
-        for (int w = passes[i].first_wire; w < passes[i].end_wire; ++w) {
-            mask |= uint64_t{1} << w;
-        }
-    }
-    return mask;
-}
-
 Engine::Engine(size_t num_qubits, size_t num_layers, double field, const std::string &strat,
                size_t chunk_width, bool double_buffer_)
     : n((int)num_qubits), L((int)num_layers), h(field), strategy(parse_strategy(strat)),
@@ -190,9 +180,16 @@ Engine::Engine(size_t num_qubits, size_t num_layers, double field, const std::st
         for (const auto &p : passes) {
             eparts += (size_t)p.grid;
         }
-        partial_count = std::max(part, eparts);
+        // the layer-0 contraction writes 48 values per workgroup of the low pass
+        partial_count = std::max({part, eparts, (size_t)passes[0].grid * 48});
         d_partials = (double *)device_alloc(sizeof(double) * partial_count, "alloc partials");
         d_epartials = (double *)device_alloc(sizeof(double) * eparts, "alloc energy partials");
+        epartial_count = eparts;
+        const uint64_t tiles = passes[0].g.num_tiles;
+        d_tile_sums = (double2 *)device_alloc(sizeof(double2) * tiles, "alloc tile sums");
+        high_env_chunks = (int)std::min<uint64_t>(64, (tiles + 255) / 256);
+        partial2_count = 4 * (size_t)high_env_chunks * (size_t)std::max(1, n - T);
+        d_partials2 = (double *)device_alloc(sizeof(double) * partial2_count, "alloc partials");
     } else {
         // gate-level and dense strategies: psi, lambda
         for (int i = 0; i < 2; ++i) {
@@ -201,6 +198,7 @@ Engine::Engine(size_t num_qubits, size_t num_layers, double field, const std::st
         partial_count = (size_t)elementwise_grid(N);
         d_partials = (double *)device_alloc(sizeof(double) * partial_count, "alloc partials");
         d_epartials = (double *)device_alloc(sizeof(double) * partial_count, "alloc energy partials");
+        epartial_count = partial_count;
     }
     if (strategy == RIE_STRATEGY_DENSE_SCAN) {
         dense_init();
@@ -224,6 +222,8 @@ Engine::~Engine() {
     device_free(d_partials);
     device_free(d_epartials);
     device_free(d_drift);
+    device_free(d_tile_sums);
+    device_free(d_partials2);
     dense_free();
     for (auto &p : kpending) {
         kevent_pool.push_back(p.a);
@@ -398,6 +398,12 @@ void Engine::structured_forward(hipStream_t s, bool save_ckpts) {
 
 const double2 *Engine::final_state() const { return st[final_cur]; }
 
+void Engine::require_partials(size_t count) const {
+    if (count > partial_count) {
+        fail(RIE_DEVICE_ERROR, "internal: gradient partial buffer too small");
+    }
+}
+
 int Engine::free_buffer(std::initializer_list<int> used) const {
     for (int i = 0; i < 4; ++i) {
         bool taken = false;
@@ -417,6 +423,13 @@ void Engine::structured_hamiltonian(hipStream_t s, bool write_lam) {
     double2 *lam = write_lam ? st[lam_idx] : nullptr;
     size_t off = 0;
     const double S = (double)sizeof(double2) * (double)N;
+    size_t total = 0;
+    for (const auto &p : passes) {
+        total += (size_t)p.grid;
+    }
+    if (total > epartial_count) {
+        fail(RIE_DEVICE_ERROR, "internal: energy partial buffer too small");
+    }
     for (size_t p = 0; p < passes.size(); ++p) {
         const bool read_lam = write_lam && p > 0;
         kt_begin(s);
@@ -431,27 +444,32 @@ void Engine::structured_hamiltonian(hipStream_t s, bool write_lam) {
 }
 
 void Engine::structured_backward(hipStream_t s, const double *scale, bool full_walk) {
-    // Backward sweep: layer L-1 .. 0; in each layer the low pass (with the
-    // inverse ring CNOT on load) comes first, then the high passes in place.
-    int cur = final_cur;          // may be -1 (state in a checkpoint)
+    // Layers L-1 .. 1 (and 0 in a full walk) run the inverse walk in tile
+    // passes: first the low pass, which applies the inverse ring CNOT on load
+    // (out of place), then the high passes in place.  Layer 0 is a product
+    // state, so unless a full walk is asked for its gradients come from one
+    // contraction pass over lambda.
+    const double S = (double)sizeof(double2) * (double)N;
+    int cur = final_cur;  // -2 while the state lives in a checkpoint buffer
     const double2 *cur_in = final_state();
     int lam = lam_idx;
-    for (int l = L - 1; l >= 0; --l) {
+    const int last_walked = full_walk ? 0 : 1;
+    for (int l = L - 1; l >= last_walked; --l) {
         const WireCoef *lc = d_coef + (size_t)l * n;
-        const bool analytic = (l == 0) && !full_walk;
-        const bool last_layer = (l == 0);
-        const int ncur = analytic ? -1 : free_buffer({cur, lam});
+        // Is the state after un-rotating layer l read again?  Not when the
+        // next layer is the contracted layer 0, nor when a checkpoint holds it.
+        const bool ckpt_next = ckpt_every > 0 && l >= 2 && (l % (int)ckpt_every) == 0 &&
+                               (size_t)(l / ckpt_every) <= ckpts.size();
+        const bool need_cur_after = full_walk || (l >= 2 && !ckpt_next);
+        const int ncur = free_buffer({cur, lam});
         const int nlam = free_buffer({cur, lam, ncur});
         const bool only_low = passes.size() == 1;
-        const bool write_lam0 = !(last_layer && only_low && !full_walk);
-        const double S = (double)sizeof(double2) * (double)N;
+        require_partials((size_t)passes[0].grid * 2 * T);
         kt_begin(s);
-        launch_bwd_pass(R, true, analytic, !analytic, write_lam0, cur_in, st[lam],
-                        analytic ? nullptr : st[ncur], st[nlam], lc, passes[0].g, 0, d_partials,
-                        passes[0].grid, s);
+        launch_bwd_pass(R, true, only_low ? need_cur_after : true, true, cur_in, st[lam], st[ncur],
+                        st[nlam], lc, passes[0].g, d_partials, passes[0].grid, s);
         check_launch("bwd_pass gather");
-        kt_end(s, analytic ? "bwd_pass_low_cnot_layer0" : "bwd_pass_low_cnot",
-               S * ((analytic ? 1.0 : 4.0) + (analytic && write_lam0 ? 1.0 : 0.0)));
+        kt_end(s, "bwd_pass_low_cnot", S * (3.0 + ((only_low && !need_cur_after) ? 0.0 : 1.0)));
         kt_begin(s);
         launch_reduce_grad(d_partials, passes[0].grid, T, passes[0].g, l, d_grad, scale, s);
         check_launch("reduce_grad");
@@ -459,32 +477,46 @@ void Engine::structured_backward(hipStream_t s, const double *scale, bool full_w
         cur = ncur;
         lam = nlam;
         for (size_t p = 1; p < passes.size(); ++p) {
-            const bool final_pass = last_layer && (p + 1 == passes.size()) && !full_walk;
-            const uint64_t processed = analytic ? wires_before_pass(p) : 0;
+            const bool write_cur = (p + 1 < passes.size()) || need_cur_after;
+            require_partials((size_t)passes[p].grid * 2 * T);
             kt_begin(s);
-            launch_bwd_pass(R, false, analytic, !analytic, !final_pass,
-                            analytic ? nullptr : st[cur], st[lam], analytic ? nullptr : st[cur],
-                            st[lam], lc, passes[p].g, processed, d_partials, passes[p].grid, s);
+            launch_bwd_pass(R, false, write_cur, true, st[cur], st[lam], st[cur], st[lam], lc,
+                            passes[p].g, d_partials, passes[p].grid, s);
             check_launch("bwd_pass");
-            kt_end(s, analytic ? "bwd_pass_high_layer0" : "bwd_pass_high",
-                   S * ((analytic ? 1.0 : 4.0) + (analytic && !final_pass ? 1.0 : 0.0)));
+            kt_end(s, "bwd_pass_high", S * (3.0 + (write_cur ? 1.0 : 0.0)));
             kt_begin(s);
             launch_reduce_grad(d_partials, passes[p].grid, T, passes[p].g, l, d_grad, scale, s);
             check_launch("reduce_grad");
             kt_end(s, "reduce_grad", 0.0);
         }
-        if (l > 0) {
-            // Restore the stored boundary state (after layer l-1) if there is one.
-            if (!analytic && ckpt_every > 0 && (l % (int)ckpt_every) == 0 &&
-                (size_t)(l / ckpt_every) <= ckpts.size()) {
-                cur_in = ckpts[(size_t)l / ckpt_every - 1];
-                cur = -2;  // not one of the working buffers
-            } else {
-                cur_in = st[cur];
-            }
+        if (ckpt_next) {
+            cur_in = ckpts[(size_t)l / ckpt_every - 1];
+            cur = -2;
+        } else {
+            cur_in = st[cur];
         }
     }
     walked_cur = full_walk ? cur : -1;
+    if (!full_walk) {
+        const PassGeom &g0 = passes[0].g;
+        const int grid0 = passes[0].grid;
+        require_partials((size_t)grid0 * 48);
+        if ((size_t)high_env_chunks * 4 * (size_t)std::max(1, n - T) > partial2_count) {
+            fail(RIE_DEVICE_ERROR, "internal: layer-0 partial buffer too small");
+        }
+        kt_begin(s);
+        launch_layer0_contract(st[lam], d_coef, g0, d_tile_sums, d_partials, grid0, s);
+        check_launch("layer0_contract");
+        kt_end(s, "layer0_contract", S);
+        kt_begin(s);
+        launch_layer0_high_env(d_tile_sums, g0.num_tiles, d_coef, n, d_partials2, high_env_chunks,
+                               s);
+        check_launch("layer0_high_env");
+        launch_layer0_finish(d_partials, grid0, d_partials2, high_env_chunks, d_coef, n, d_grad,
+                             scale, s);
+        check_launch("layer0_finish");
+        kt_end(s, "layer0_reduce", 0.0);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -531,6 +563,9 @@ void Engine::gate_forward(hipStream_t s) {
 
 void Engine::gate_hamiltonian(hipStream_t s) {
     const int grid = elementwise_grid(N);
+    if ((size_t)grid > epartial_count) {
+        fail(RIE_DEVICE_ERROR, "internal: energy partial buffer too small");
+    }
     launch_ham_gather(st[0], st[1], n, h, d_epartials, grid, s);
     check_launch("ham_gather");
     energy_partials = (size_t)grid;
@@ -538,6 +573,7 @@ void Engine::gate_hamiltonian(hipStream_t s) {
 
 void Engine::gate_backward(hipStream_t s, const double *scale) {
     const int grid = elementwise_grid(N / 2);
+    require_partials((size_t)grid);
     for (int l = L - 1; l >= 0; --l) {
         for (int w = n - 1; w >= 0; --w) {
             launch_gate_cnot(st[0], n, w, (w + 1) % n, s);

@@ -73,7 +73,12 @@ struct Engine {
     double *d_partials = nullptr;
     double *d_epartials = nullptr;
     double *d_drift = nullptr;
-    size_t partial_count = 0;
+    double2 *d_tile_sums = nullptr;  // layer-0 contraction: one value per low tile
+    double *d_partials2 = nullptr;   // layer-0 contraction: high-wire partials
+    int high_env_chunks = 1;
+    size_t partial_count = 0;    // capacity of d_partials (doubles)
+    size_t epartial_count = 0;   // capacity of d_epartials
+    size_t partial2_count = 0;   // capacity of d_partials2
     size_t energy_partials = 0;
     std::vector<double2 *> ckpts;
     size_t ckpt_every = 0;
@@ -122,8 +127,8 @@ struct Engine {
     std::vector<std::pair<std::string, double>> timings;
 
     void build_plan();
-    uint64_t wires_before_pass(size_t p) const;
     const double2 *final_state() const;
+    void require_partials(size_t count) const;
     int free_buffer(std::initializer_list<int> used) const;
 
     void stage_begin(hipStream_t s);

@@ -300,60 +300,6 @@ void launch_fwd_pass(int R, bool scatter, const double2 *in, double2 *out,
 // Adjoint tile pass.
 // ---------------------------------------------------------------------------
 
-// Factor of wire w in the layer-0 state at the start of a pass: U_w|0> if w
-// is still rotated, |0> if an earlier pass already un-rotated it.
-__device__ __forceinline__ double2 layer0_factor(const WireCoef *__restrict__ lc0,
-                                                 uint64_t processed, int w, int bit) {
-    if ((processed >> w) & 1U) {
-        return make_double2(bit ? 0.0 : 1.0, 0.0);
-    }
-    const WireCoef k = lc0[w];
-    return bit ? u1_of(k) : u0_of(k);
-}
-
-template <int R, int M>
-__device__ __forceinline__ double2 analytic_thread_factor(const WireCoef *__restrict__ lc0,
-                                                          const PassGeom &G, uint64_t processed,
-                                                          uint32_t t) {
-    // Product over the tile bits taken from the thread index in mapping M.
-    double2 v = make_double2(1.0, 0.0);
-    const uint32_t o_t = omap<R, M>(t, 0);
-    constexpr int P0 = phase_first_bit<R, M>();
-    for (int j = 0; j < 3 * R; ++j) {
-        if (j >= P0 && j < P0 + R) {
-            continue;  // register bit
-        }
-        v = cmul(v, layer0_factor(lc0, processed, tile_wire(j, G), (o_t >> j) & 1));
-    }
-    return v;
-}
-
-template <int R, int M>
-__device__ __forceinline__ double2 analytic_reg_factor(const WireCoef *__restrict__ lc0,
-                                                       const PassGeom &G, uint64_t processed,
-                                                       int k) {
-    double2 v = make_double2(1.0, 0.0);
-    constexpr int P0 = phase_first_bit<R, M>();
-    for (int rb = 0; rb < R; ++rb) {
-        v = cmul(v, layer0_factor(lc0, processed, tile_wire(P0 + rb, G), (k >> rb) & 1));
-    }
-    return v;
-}
-
-__device__ __forceinline__ double2 analytic_outer_factor(const WireCoef *__restrict__ lc0,
-                                                         const PassGeom &G, uint64_t processed,
-                                                         uint64_t base, int T) {
-    double2 v = make_double2(1.0, 0.0);
-    const int hi_end = G.h0 + T - G.a;
-    for (int w = 0; w < G.n; ++w) {
-        const bool in_tile = (w < G.a) || (w >= G.h0 && w < hi_end);
-        if (!in_tile) {
-            v = cmul(v, layer0_factor(lc0, processed, w, (int)((base >> w) & 1U)));
-        }
-    }
-    return v;
-}
-
 template <int R>
 __device__ __forceinline__ void block_reduce_grad(double (&dt)[3 * R], double (&dp)[3 * R],
                                                   double *__restrict__ partials, const PassGeom &G) {
@@ -384,15 +330,13 @@ __device__ __forceinline__ void block_reduce_grad(double (&dt)[3 * R], double (&
     (void)G;
 }
 
-template <int R, bool kGather, bool kAnalytic, bool kWriteCur, bool kWriteLam>
+template <int R, bool kGather, bool kWriteCur, bool kWriteLam>
 __global__ __launch_bounds__(Tile<R>::THREADS) void bwd_pass_kernel(
     const double2 *cur_in, const double2 *lam_in, double2 *cur_out, double2 *lam_out,
-    const WireCoef *__restrict__ lc, PassGeom G, uint64_t processed,
-    double *__restrict__ partials) {
+    const WireCoef *__restrict__ lc, PassGeom G, double *__restrict__ partials) {
     using TL = Tile<R>;
     constexpr int T = TL::T;
     __shared__ double2 lds[TL::SLOTS];
-    __shared__ double2 reg_factor[kAnalytic ? TL::REGS : 1];
     const uint32_t t = threadIdx.x;
     double dt[T];
     double dp[T];
@@ -400,16 +344,6 @@ __global__ __launch_bounds__(Tile<R>::THREADS) void bwd_pass_kernel(
     for (int j = 0; j < T; ++j) {
         dt[j] = 0.0;
         dp[j] = 0.0;
-    }
-    // The first register mapping of the pass: B after a gather, A otherwise.
-    constexpr int M0 = kGather ? 1 : 0;
-    double2 thread_factor = make_double2(1.0, 0.0);
-    if constexpr (kAnalytic) {
-        thread_factor = analytic_thread_factor<R, M0>(lc, G, processed, t);
-        if (t < (uint32_t)TL::REGS) {
-            reg_factor[t] = analytic_reg_factor<R, M0>(lc, G, processed, (int)t);
-        }
-        __syncthreads();
     }
 
     const TileAddr ta = tile_addr<R>(t, G);
@@ -428,9 +362,7 @@ __global__ __launch_bounds__(Tile<R>::THREADS) void bwd_pass_kernel(
             for (int k = 0; k < TL::REGS; ++k) {
                 const uint64_t src = (k < TL::REGS / 2 ? run : run1) + ((uint64_t)k << (2 * R));
                 l[k] = lam_in[src];
-                if constexpr (!kAnalytic) {
-                    c[k] = cur_in[src];
-                }
+                c[k] = cur_in[src];
             }
             __syncthreads();
 #pragma unroll
@@ -442,17 +374,15 @@ __global__ __launch_bounds__(Tile<R>::THREADS) void bwd_pass_kernel(
             for (int k = 0; k < TL::REGS; ++k) {
                 l[k] = lds[lds_slot<R>(omap<R, 1>(t, k))];
             }
-            if constexpr (!kAnalytic) {
-                __syncthreads();
+            __syncthreads();
 #pragma unroll
-                for (int k = 0; k < TL::REGS; ++k) {
-                    lds[lds_slot<R>(ring_inv32(omap<R, 0>(t, k), T))] = c[k];
-                }
-                __syncthreads();
+            for (int k = 0; k < TL::REGS; ++k) {
+                lds[lds_slot<R>(ring_inv32(omap<R, 0>(t, k), T))] = c[k];
+            }
+            __syncthreads();
 #pragma unroll
-                for (int k = 0; k < TL::REGS; ++k) {
-                    c[k] = lds[lds_slot<R>(omap<R, 1>(t, k))];
-                }
+            for (int k = 0; k < TL::REGS; ++k) {
+                c[k] = lds[lds_slot<R>(omap<R, 1>(t, k))];
             }
         } else {
             const uint64_t row = base + ta.offA;
@@ -460,16 +390,7 @@ __global__ __launch_bounds__(Tile<R>::THREADS) void bwd_pass_kernel(
             for (int k = 0; k < TL::REGS; ++k) {
                 const uint64_t idx = row + k * ta.strideA;
                 l[k] = lam_in[idx];
-                if constexpr (!kAnalytic) {
-                    c[k] = cur_in[idx];
-                }
-            }
-        }
-        if constexpr (kAnalytic) {
-            const double2 f = cmul(analytic_outer_factor(lc, G, processed, base, T), thread_factor);
-#pragma unroll
-            for (int k = 0; k < TL::REGS; ++k) {
-                c[k] = cmul(f, reg_factor[k]);
+                c[k] = cur_in[idx];
             }
         }
 
@@ -517,41 +438,34 @@ __global__ __launch_bounds__(Tile<R>::THREADS) void bwd_pass_kernel(
 }
 
 template <int R>
-static void launch_bwd_pass_r(bool gather, bool analytic, bool write_cur, bool write_lam,
-                              const double2 *cur_in, const double2 *lam_in, double2 *cur_out,
-                              double2 *lam_out, const WireCoef *lc, const PassGeom &g,
-                              uint64_t processed, double *partials, int grid, hipStream_t s) {
+static void launch_bwd_pass_r(bool gather, bool write_cur, bool write_lam, const double2 *cur_in,
+                              const double2 *lam_in, double2 *cur_out, double2 *lam_out,
+                              const WireCoef *lc, const PassGeom &g, double *partials, int grid,
+                              hipStream_t s) {
     constexpr int TH = Tile<R>::THREADS;
-#define RIE_BWD(GA, AN, WC, WL)                                                           \
-    if (gather == GA && analytic == AN && write_cur == WC && write_lam == WL) {           \
-        bwd_pass_kernel<R, GA, AN, WC, WL><<<grid, TH, 0, s>>>(cur_in, lam_in, cur_out,   \
-                                                              lam_out, lc, g, processed,  \
-                                                              partials);                  \
-        return;                                                                           \
+#define RIE_BWD(GA, WC, WL)                                                                 \
+    if (gather == GA && write_cur == WC && write_lam == WL) {                               \
+        bwd_pass_kernel<R, GA, WC, WL><<<grid, TH, 0, s>>>(cur_in, lam_in, cur_out, lam_out, \
+                                                          lc, g, partials);                 \
+        return;                                                                             \
     }
-    RIE_BWD(true, false, true, true)
-    RIE_BWD(false, false, true, true)
-    RIE_BWD(true, true, false, true)
-    RIE_BWD(false, true, false, true)
-    RIE_BWD(true, true, false, false)
-    RIE_BWD(false, true, false, false)
-    RIE_BWD(true, false, false, true)
-    RIE_BWD(false, false, false, true)
-    RIE_BWD(true, false, false, false)
-    RIE_BWD(false, false, false, false)
+    RIE_BWD(true, true, true)
+    RIE_BWD(false, true, true)
+    RIE_BWD(true, false, true)
+    RIE_BWD(false, false, true)
 #undef RIE_BWD
 }
 
-void launch_bwd_pass(int R, bool gather, bool analytic, bool write_cur, bool write_lam,
-                     const double2 *cur_in, const double2 *lam_in, double2 *cur_out,
-                     double2 *lam_out, const WireCoef *layer_coef, const PassGeom &g,
-                     uint64_t processed_mask, double *partials, int grid, hipStream_t s) {
+void launch_bwd_pass(int R, bool gather, bool write_cur, bool write_lam, const double2 *cur_in,
+                     const double2 *lam_in, double2 *cur_out, double2 *lam_out,
+                     const WireCoef *layer_coef, const PassGeom &g, double *partials, int grid,
+                     hipStream_t s) {
     if (R == 4) {
-        launch_bwd_pass_r<4>(gather, analytic, write_cur, write_lam, cur_in, lam_in, cur_out,
-                             lam_out, layer_coef, g, processed_mask, partials, grid, s);
+        launch_bwd_pass_r<4>(gather, write_cur, write_lam, cur_in, lam_in, cur_out, lam_out,
+                             layer_coef, g, partials, grid, s);
     } else {
-        launch_bwd_pass_r<3>(gather, analytic, write_cur, write_lam, cur_in, lam_in, cur_out,
-                             lam_out, layer_coef, g, processed_mask, partials, grid, s);
+        launch_bwd_pass_r<3>(gather, write_cur, write_lam, cur_in, lam_in, cur_out, lam_out,
+                             layer_coef, g, partials, grid, s);
     }
 }
 
@@ -737,6 +651,316 @@ __global__ __launch_bounds__(256) void drift_kernel(const double2 *__restrict__ 
 
 void launch_drift(const double2 *state, uint64_t size, double *partials, int grid, hipStream_t s) {
     drift_kernel<<<grid, 256, 0, s>>>(state, size, partials);
+}
+
+// ---------------------------------------------------------------------------
+// Layer-0 gradients by contraction.
+//
+// Layer 0 acts on |0...0>, so the state right after its rotations is the
+// product state (x)_w u_w with u_w = U_w|0>, and every layer-0 derivative is
+// that product state with one factor replaced by d u_w.  With mu the adjoint
+// state at the same point (lambda after the inverse ring CNOT of layer 0),
+//   dE/dp_w = 2 Re sum_b E_w[b] (d u_w / dp)[b],
+//   E_w[b]  = sum_{i : i_w = b} conj(mu_i) prod_{v != w} u_v(i_v).
+// One read pass over lambda (gathered through P^-1 per low tile) produces
+// the environments E_w of the tile wires (accumulated with the tile's high
+// product B(g)) and the per-tile contraction S(g) of all tile wires; a small
+// second kernel contracts S with the high wires.  This replaces the layer-0
+// inverse walk (which would read and rewrite lambda once per tile pass).
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ double2 cconj_mul(double2 a, double2 b) {
+    // conj(a) * b
+    return make_double2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
+}
+
+__device__ __forceinline__ void cmac(double2 &acc, double2 a, double2 b) {
+    acc.x += a.x * b.x - a.y * b.y;
+    acc.y += a.x * b.y + a.y * b.x;
+}
+
+__device__ __forceinline__ double2 u_of(const WireCoef &k, int bit) {
+    return bit ? u1_of(k) : u0_of(k);
+}
+
+// R = 4 (T = 12) low tiles; 256 threads, thread t holds offsets t | k << 8.
+__global__ __launch_bounds__(256) void layer0_contract_kernel(
+    const double2 *__restrict__ lam, const WireCoef *__restrict__ lc0, PassGeom G,
+    double2 *__restrict__ tile_sums, double *__restrict__ partials) {
+    constexpr int R = 4;
+    using TL = Tile<R>;
+    constexpr int T = TL::T;
+    __shared__ double2 lds[TL::SLOTS];
+    __shared__ double2 w_base[16];      // prod over wires 8..11 of u(k bits)
+    __shared__ double2 w_env[4][16];    // same without wire 8 + j
+    __shared__ double2 red2[TL::WAVES];
+    __shared__ double red[TL::WAVES][48];
+    const uint32_t t = threadIdx.x;
+    const int lane = t & 63;
+    const int wave = t >> 6;
+    if (t < 16) {
+        double2 b = make_double2(1.0, 0.0);
+        for (int j = 0; j < 4; ++j) {
+            b = cmul(b, u_of(lc0[8 + j], (t >> j) & 1));
+        }
+        w_base[t] = b;
+        for (int e = 0; e < 4; ++e) {
+            double2 v = make_double2(1.0, 0.0);
+            for (int j = 0; j < 4; ++j) {
+                if (j != e) {
+                    v = cmul(v, u_of(lc0[8 + j], (t >> j) & 1));
+                }
+            }
+            w_env[e][t] = v;
+        }
+    }
+    // thread constants over offset bits 0..7 (wires 0..7)
+    double2 a_lo = make_double2(1.0, 0.0);
+    double2 a_skip[8];
+    for (int w = 0; w < 8; ++w) {
+        a_skip[w] = make_double2(1.0, 0.0);
+    }
+    for (int v = 0; v < 8; ++v) {
+        const double2 f = u_of(lc0[v], (t >> v) & 1);
+        a_lo = cmul(a_lo, f);
+#pragma unroll
+        for (int w = 0; w < 8; ++w) {
+            if (w != v) {
+                a_skip[w] = cmul(a_skip[w], f);
+            }
+        }
+    }
+    double2 acc_lo[8];     // E_w[t_w] partial for w < 8
+    double2 acc_reg[4][2]; // E_w[b] partial for w = 8..11
+#pragma unroll
+    for (int w = 0; w < 8; ++w) {
+        acc_lo[w] = make_double2(0.0, 0.0);
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        acc_reg[e][0] = make_double2(0.0, 0.0);
+        acc_reg[e][1] = make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+
+    for (uint64_t g = blockIdx.x; g < G.num_tiles; g += gridDim.x) {
+        const uint64_t base = g << T;
+        const uint64_t pbase = ring_fwd(base, G.n);
+        const uint64_t lo_flip = pbase & 1u;
+        const uint64_t run = (pbase & ~uint64_t{TL::SIZE - 1}) + (t ^ lo_flip);
+        const uint64_t run1 = run ^ G.hi_mask;
+        double2 m[16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            m[k] = lam[(k < 8 ? run : run1) + ((uint64_t)k << 8)];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            lds[lds_slot<R>(ring_inv32(omap<R, 0>(t, k), T))] = m[k];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            m[k] = lds[lds_slot<R>(omap<R, 0>(t, k))];
+        }
+        // contract the register bits (wires 8..11)
+        double2 pk = make_double2(0.0, 0.0);
+        double2 env[4][2];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            env[e][0] = make_double2(0.0, 0.0);
+            env[e][1] = make_double2(0.0, 0.0);
+        }
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const double2 cm = make_double2(m[k].x, -m[k].y);  // conj(mu)
+            cmac(pk, cm, w_base[k]);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                cmac(env[e][(k >> e) & 1], cm, w_env[e][k]);
+            }
+        }
+        // high-wire product of this tile
+        double2 bg = make_double2(1.0, 0.0);
+        for (int v = T; v < G.n; ++v) {
+            bg = cmul(bg, u_of(lc0[v], (int)((base >> v) & 1U)));
+        }
+        // per-tile contraction of all tile wires: S(g) = sum_t pk * a_lo
+        const double2 st = cmul(pk, a_lo);
+        const double sx = wave_sum(st.x);
+        const double sy = wave_sum(st.y);
+        if (lane == 0) {
+            red2[wave] = make_double2(sx, sy);
+        }
+        const double2 bpk = cmul(bg, pk);
+#pragma unroll
+        for (int w = 0; w < 8; ++w) {
+            cmac(acc_lo[w], bpk, a_skip[w]);
+        }
+        const double2 ba = cmul(bg, a_lo);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            cmac(acc_reg[e][0], ba, env[e][0]);
+            cmac(acc_reg[e][1], ba, env[e][1]);
+        }
+        __syncthreads();
+        if (t == 0) {
+            double2 tot = red2[0];
+            for (int w = 1; w < TL::WAVES; ++w) {
+                tot.x += red2[w].x;
+                tot.y += red2[w].y;
+            }
+            tile_sums[g] = tot;
+        }
+    }
+    // reduce the environments: 8 low wires x 2 entries + 4 register wires x 2
+    // entries, complex -> 48 doubles
+#pragma unroll
+    for (int w = 0; w < 8; ++w) {
+        const int b = (t >> w) & 1;
+        const double vals[4] = {b ? 0.0 : acc_lo[w].x, b ? 0.0 : acc_lo[w].y,
+                                b ? acc_lo[w].x : 0.0, b ? acc_lo[w].y : 0.0};
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const double r = wave_sum(vals[c]);
+            if (lane == 0) {
+                red[wave][4 * w + c] = r;
+            }
+        }
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const double vals[4] = {acc_reg[e][0].x, acc_reg[e][0].y, acc_reg[e][1].x,
+                                acc_reg[e][1].y};
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const double r = wave_sum(vals[c]);
+            if (lane == 0) {
+                red[wave][32 + 4 * e + c] = r;
+            }
+        }
+    }
+    __syncthreads();
+    if (t < 48) {
+        double acc = 0.0;
+        for (int w = 0; w < TL::WAVES; ++w) {
+            acc += red[w][t];
+        }
+        partials[(uint64_t)blockIdx.x * 48 + t] = acc;
+    }
+}
+
+// Environments of the high wires v in [12, n) from the per-tile sums S(g):
+// E_v[b] = sum_{g : bit v of (g << 12) = b} S(g) prod_{u >= 12, u != v} u_u(bit).
+// Block (v - 12, chunk) writes 4 doubles (E_v[0], E_v[1]) into partials.
+__global__ __launch_bounds__(256) void layer0_high_env_kernel(
+    const double2 *__restrict__ tile_sums, uint64_t num_tiles, const WireCoef *__restrict__ lc0,
+    int n, double *__restrict__ partials) {
+    constexpr int T = 12;
+    __shared__ double red[4][4];
+    const int v = T + blockIdx.y;
+    const int t = threadIdx.x;
+    double2 e0 = make_double2(0.0, 0.0);
+    double2 e1 = make_double2(0.0, 0.0);
+    const uint64_t stride = (uint64_t)gridDim.x * 256;
+    for (uint64_t g = (uint64_t)blockIdx.x * 256 + t; g < num_tiles; g += stride) {
+        double2 f = tile_sums[g];
+        for (int u = T; u < n; ++u) {
+            if (u != v) {
+                f = cmul(f, u_of(lc0[u], (int)((g >> (u - T)) & 1U)));
+            }
+        }
+        if ((g >> (v - T)) & 1U) {
+            e1.x += f.x;
+            e1.y += f.y;
+        } else {
+            e0.x += f.x;
+            e0.y += f.y;
+        }
+    }
+    const double vals[4] = {e0.x, e0.y, e1.x, e1.y};
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const double r = wave_sum(vals[c]);
+        if ((t & 63) == 0) {
+            red[t >> 6][c] = r;
+        }
+    }
+    __syncthreads();
+    if (t < 4) {
+        partials[((uint64_t)blockIdx.y * gridDim.x + blockIdx.x) * 4 + t] =
+            red[0][t] + red[1][t] + red[2][t] + red[3][t];
+    }
+}
+
+// Gradients of layer 0: d u / d theta = (-s/2 e-, c/2 e+),
+// d u / d phi = (-i/2 c e-, +i/2 s e+), e-+ = e^{-+i phi/2}.
+__global__ __launch_bounds__(64) void layer0_finish_kernel(
+    const double *__restrict__ low_partials, int low_blocks, const double *__restrict__ high_partials,
+    int high_chunks, const WireCoef *__restrict__ lc0, int n, double *__restrict__ grad,
+    const double *__restrict__ scale) {
+    const int w = blockIdx.x;  // wire
+    const int t = threadIdx.x;
+    double e[4] = {0.0, 0.0, 0.0, 0.0};
+    if (w < 12) {
+        const int col = (w < 8) ? 4 * w : 32 + 4 * (w - 8);
+        for (int b = t; b < low_blocks; b += 64) {
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                e[c] += low_partials[(uint64_t)b * 48 + col + c];
+            }
+        }
+    } else {
+        for (int b = t; b < high_chunks; b += 64) {
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                e[c] += high_partials[((uint64_t)(w - 12) * high_chunks + b) * 4 + c];
+            }
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        e[c] = wave_sum(e[c]);
+    }
+    if (t == 0) {
+        const WireCoef k = lc0[w];
+        const double2 em = make_double2(k.ch, -k.sh);
+        const double2 ep = make_double2(k.ch, k.sh);
+        const double2 dth0 = make_double2(-0.5 * k.s * em.x, -0.5 * k.s * em.y);
+        const double2 dth1 = make_double2(0.5 * k.c * ep.x, 0.5 * k.c * ep.y);
+        // -i/2 c e- and +i/2 s e+
+        const double2 dph0 = make_double2(0.5 * k.c * em.y, -0.5 * k.c * em.x);
+        const double2 dph1 = make_double2(-0.5 * k.s * ep.y, 0.5 * k.s * ep.x);
+        const double2 E0 = make_double2(e[0], e[1]);
+        const double2 E1 = make_double2(e[2], e[3]);
+        const double dth = 2.0 * (cmul(E0, dth0).x + cmul(E1, dth1).x);
+        const double dph = 2.0 * (cmul(E0, dph0).x + cmul(E1, dph1).x);
+        const double f = scale != nullptr ? *scale : 1.0;
+        grad[2 * w] = dth * f;
+        grad[2 * w + 1] = dph * f;
+    }
+}
+
+void launch_layer0_contract(const double2 *lam, const WireCoef *lc0, const PassGeom &g,
+                            double2 *tile_sums, double *partials, int grid, hipStream_t s) {
+    layer0_contract_kernel<<<grid, 256, 0, s>>>(lam, lc0, g, tile_sums, partials);
+}
+
+void launch_layer0_high_env(const double2 *tile_sums, uint64_t num_tiles, const WireCoef *lc0,
+                            int n, double *partials, int chunks, hipStream_t s) {
+    if (n <= 12) {
+        return;
+    }
+    dim3 grid(chunks, n - 12);
+    layer0_high_env_kernel<<<grid, 256, 0, s>>>(tile_sums, num_tiles, lc0, n, partials);
+}
+
+void launch_layer0_finish(const double *low_partials, int low_blocks, const double *high_partials,
+                          int high_chunks, const WireCoef *lc0, int n, double *grad,
+                          const double *scale, hipStream_t s) {
+    layer0_finish_kernel<<<n, 64, 0, s>>>(low_partials, low_blocks, high_partials, high_chunks,
+                                          lc0, n, grad, scale);
 }
 
 // ---------------------------------------------------------------------------

@@ -60,13 +60,12 @@ void launch_fwd_pass(int R, bool scatter, const double2 *in, double2 *out,
 
 // Adjoint tile pass: un-rotates the pass's wires on (cur, lam) and writes
 // per-workgroup gradient partials [grid][2T].  gather = apply the inverse
-// ring-CNOT on load (low pass, out of place).  analytic = layer 0, whose
-// state is the product state and need not be read (processed_mask lists the
-// layer-0 wires already un-rotated by earlier passes).
-void launch_bwd_pass(int R, bool gather, bool analytic, bool write_cur, bool write_lam,
-                     const double2 *cur_in, const double2 *lam_in, double2 *cur_out,
-                     double2 *lam_out, const WireCoef *layer_coef, const PassGeom &g,
-                     uint64_t processed_mask, double *partials, int grid, hipStream_t s);
+// ring-CNOT on load (low pass, out of place); write_cur / write_lam = store
+// the un-rotated state / adjoint state.
+void launch_bwd_pass(int R, bool gather, bool write_cur, bool write_lam, const double2 *cur_in,
+                     const double2 *lam_in, double2 *cur_out, double2 *lam_out,
+                     const WireCoef *layer_coef, const PassGeom &g, double *partials, int grid,
+                     hipStream_t s);
 
 // Hamiltonian tile pass: adds the X terms of the pass's wires (and the ZZ
 // diagonal if diag) of H psi, optionally into lam, and writes per-workgroup
@@ -74,6 +73,18 @@ void launch_bwd_pass(int R, bool gather, bool analytic, bool write_cur, bool wri
 void launch_ham_pass(int R, bool diag, bool read_lam, bool write_lam, const double2 *psi,
                      const double2 *lam_in, double2 *lam_out, const PassGeom &g, double field,
                      double *e_partials, int grid, hipStream_t s);
+
+// Layer-0 gradients by contracting the adjoint state with the layer-0 product
+// state: one read pass over lambda (gathered through the inverse ring CNOT
+// per low tile, R = 4), a reduction of the per-tile sums over the high wires,
+// and the final per-wire assembly into grad[0 .. 2n).
+void launch_layer0_contract(const double2 *lam, const WireCoef *lc0, const PassGeom &g,
+                            double2 *tile_sums, double *partials, int grid, hipStream_t s);
+void launch_layer0_high_env(const double2 *tile_sums, uint64_t num_tiles, const WireCoef *lc0,
+                            int n, double *partials, int chunks, hipStream_t s);
+void launch_layer0_finish(const double *low_partials, int low_blocks, const double *high_partials,
+                          int high_chunks, const WireCoef *lc0, int n, double *grad,
+                          const double *scale, hipStream_t s);
 
 // grad[(layer*n + wire)*2 + comp] = (sum over blocks) * scale for the pass's wires.
 void launch_reduce_grad(const double *partials, int num_blocks, int T, const PassGeom &g,
