@@ -49,6 +49,8 @@ def parse_args() -> argparse.Namespace:
     p.add_argument("--qubits", type=int, default=30)
     p.add_argument("--layers", type=int, default=2)
     p.add_argument("--seed", type=int, default=5)
+    p.add_argument("--schedule", choices=["auto", "stored", "inverse_walk"], default="auto",
+                   help="backward schedule (auto: stored when it fits in device memory)")
     p.add_argument("--no-cpu-baseline", action="store_true")
     return p.parse_args()
 
@@ -112,6 +114,8 @@ def main() -> int:
     energy = torch.zeros(1, dtype=torch.float64, device=dev)
     grad = torch.zeros(2 * n * layers, dtype=torch.float64, device=dev)
     engine = ext.RingIsingCudaBackend(n, layers, 1.0, "structured_adjoint", 8, False)
+    if args.schedule != "auto":
+        engine.set_schedule(args.schedule)
     stream = torch.cuda.current_stream(dev)
     handle = stream.cuda_stream
 
@@ -131,6 +135,22 @@ def main() -> int:
     elapsed = time.perf_counter() - t0
     engine.set_kernel_timing(False)
     stats = engine.kernel_stats()
+    stats_schedule = engine.schedule
+
+    # the other schedule, timed the same way, for reference
+    other = "inverse_walk" if engine.schedule == "stored" else "stored"
+    other_ms = None
+    try:
+        engine.set_schedule(other)
+        step()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+        torch.cuda.synchronize()
+        other_ms = (time.perf_counter() - t1) * 1e3 / args.steps
+    except (RuntimeError, MemoryError):
+        other_ms = None
 
     e_val = float(energy.item())
     if not (abs(e_val) <= n * 2.0 + 1e-9 and torch.isfinite(grad).all()):
@@ -192,8 +212,11 @@ def main() -> int:
             "num_qubits": n,
             "layers": layers,
             "strategy": "structured_adjoint",
+            "schedule": stats_schedule,
             "parallelism": "single GPU",
         },
+        "other_schedule": {"schedule": other,
+                           "ms_per_step": None if other_ms is None else round(other_ms, 3)},
         "roofline": roofline,
         "kernels": kernels,
         "energy": e_val,

@@ -2,10 +2,11 @@
 
 Forward runs the HIP forward sweep and ``lambda = H psi`` and returns the
 energy; the final state and lambda stay resident in the engine's HBM buffers.
-Backward runs the hand-written adjoint sweep on them: states are recomputed
-by inverse gates rather than stored (optionally restored at every-k-layer
-checkpoints, ``set_checkpoint_interval``), and the incoming gradient scales
-the parameter gradient on the device.  If another forward ran on the same
+Backward runs the hand-written adjoint sweep on them: by default states are
+recomputed by inverse gates rather than stored (optionally restored at
+every-k-layer checkpoints, ``checkpoint_every``); ``schedule="stored"`` keeps
+every forward pass output instead.  The incoming gradient scales the
+parameter gradient on the device.  If another forward ran on the same
 engine in between, backward first recomputes the forward for its own
 parameters.  Parameters, energy and gradient never leave the GPU; torch only
 provides the device memory and the current stream.
@@ -76,12 +77,17 @@ class RingIsingEnergy(torch.nn.Module):
     def __init__(self, num_qubits: int, layers: int, field: float = 1.0,
                  gradient_strategy: str = "structured_adjoint",
                  structured_rotation_chunk_width: int = 8,
-                 checkpoint_every: int = 0) -> None:
+                 checkpoint_every: int = 0, schedule: str = "inverse_walk") -> None:
         super().__init__()
         backend = RingIsingAdjointBackend(StandaloneBackendConfig(
             num_qubits=num_qubits, layers=layers, field=field,
             gradient_strategy=gradient_strategy,
             structured_rotation_chunk_width=structured_rotation_chunk_width))
+        if gradient_strategy == "structured_adjoint":
+            # "inverse_walk": states recomputed by inverse gates (optionally
+            # restored at every-k-layer checkpoints); "stored": every forward
+            # pass output is kept in HBM and read back.
+            backend.native.set_schedule(schedule)
         if checkpoint_every:
             backend.native.set_checkpoint_interval(checkpoint_every)
         self.handle = _EngineHandle(backend)

@@ -197,6 +197,22 @@ class RingIsingCudaBackend {
         return out;
     }
 
+    void set_schedule(const std::string &name) {
+        int sched;
+        if (name == "inverse_walk") {
+            sched = RIE_SCHEDULE_INVERSE_WALK;
+        } else if (name == "stored") {
+            sched = RIE_SCHEDULE_STORED;
+        } else {
+            throw py::value_error("schedule must be 'inverse_walk' or 'stored'");
+        }
+        check(rie_engine_set_schedule(engine_, sched));
+    }
+
+    std::string schedule() const {
+        return rie_engine_schedule(engine_) == RIE_SCHEDULE_STORED ? "stored" : "inverse_walk";
+    }
+
     void set_kernel_timing(bool enable) { check(rie_engine_set_kernel_timing(engine_, enable)); }
 
     void reset_kernel_stats() { check(rie_engine_reset_kernel_stats(engine_)); }
@@ -256,6 +272,8 @@ PYBIND11_MODULE(_cuda_backend, m) {
              py::arg("every_k_layers"))
         .def("inverse_walk_drift", &RingIsingCudaBackend::inverse_walk_drift, py::arg("params"))
         .def("final_state", &RingIsingCudaBackend::final_state)
+        .def("set_schedule", &RingIsingCudaBackend::set_schedule, py::arg("schedule"))
+        .def_property_readonly("schedule", &RingIsingCudaBackend::schedule)
         .def("set_kernel_timing", &RingIsingCudaBackend::set_kernel_timing, py::arg("enable"))
         .def("reset_kernel_stats", &RingIsingCudaBackend::reset_kernel_stats)
         .def("kernel_stats", &RingIsingCudaBackend::kernel_stats)

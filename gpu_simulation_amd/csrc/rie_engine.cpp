@@ -169,9 +169,15 @@ Engine::Engine(size_t num_qubits, size_t num_layers, double field, const std::st
     if (strategy == RIE_STRATEGY_STRUCTURED_ADJOINT && small) {
         // the whole circuit lives in one workgroup's LDS
     } else if (strategy == RIE_STRATEGY_STRUCTURED_ADJOINT) {
-        for (int i = 0; i < 4; ++i) {
-            st[i] = (double2 *)device_alloc(sizeof(double2) * N, "alloc statevector");
-        }
+        // The stored schedule keeps every forward pass output; use it when it
+        // fits comfortably in free device memory, else walk back by inverse gates.
+        size_t free_b = 0;
+        size_t total_b = 0;
+        check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+        const double need = (double)stored_buffer_count() * (double)sizeof(double2) * (double)N;
+        schedule = (L > 1 && need <= 0.85 * (double)free_b) ? RIE_SCHEDULE_STORED
+                                                            : RIE_SCHEDULE_INVERSE_WALK;
+        ensure_pool(schedule == RIE_SCHEDULE_STORED ? stored_buffer_count() : 4);
         size_t part = 0;
         for (const auto &p : passes) {
             part = std::max<size_t>(part, (size_t)p.grid * 2 * T);
@@ -192,9 +198,7 @@ Engine::Engine(size_t num_qubits, size_t num_layers, double field, const std::st
         d_partials2 = (double *)device_alloc(sizeof(double) * partial2_count, "alloc partials");
     } else {
         // gate-level and dense strategies: psi, lambda
-        for (int i = 0; i < 2; ++i) {
-            st[i] = (double2 *)device_alloc(sizeof(double2) * N, "alloc statevector");
-        }
+        ensure_pool(2);
         partial_count = (size_t)elementwise_grid(N);
         d_partials = (double *)device_alloc(sizeof(double) * partial_count, "alloc partials");
         d_epartials = (double *)device_alloc(sizeof(double) * partial_count, "alloc energy partials");
@@ -212,8 +216,8 @@ Engine::~Engine() {
     for (auto &c : ckpts) {
         device_free(c);
     }
-    for (int i = 0; i < 4; ++i) {
-        device_free(st[i]);
+    for (auto p : pool) {
+        device_free(p);
     }
     device_free(d_params);
     device_free(d_grad);
@@ -351,24 +355,59 @@ void Engine::kt_flush() {
 // ---------------------------------------------------------------------------
 
 void Engine::structured_forward(hipStream_t s, bool save_ckpts) {
-    // Forward working buffers are st[0] and st[1]; boundary states that are
-    // checkpointed are written straight into their checkpoint buffer by the
-    // low (scatter) pass and read from there by the next layer's first pass.
     const double S = (double)sizeof(double2) * (double)N;
-    kt_begin(s);
-    launch_init_product_cnot(st[0], d_coef, n, s);
-    check_launch("init_product_cnot");
-    kt_end(s, "init_product_cnot", S);
-    const double2 *cur = st[0];
-    int cur_buf = 0;  // -1 while cur is a checkpoint buffer
+    const int K = (int)passes.size() - 1;  // high passes per layer (>= 1 for n > 12)
+    if (L == 1) {
+        kt_begin(s);
+        launch_init_product_cnot(pool[0], d_coef, n, s);
+        check_launch("init_product_cnot");
+        kt_end(s, "init_product_cnot", S);
+        final_cur = 0;
+        final_ptr = pool[0];
+        return;
+    }
+    if (schedule == RIE_SCHEDULE_STORED) {
+        // Every pass writes its own buffer; layer 1's first pass generates the
+        // layer-0 state instead of reading it.
+        const double2 *in = nullptr;
+        for (int l = 1; l < L; ++l) {
+            const WireCoef *lc = d_coef + (size_t)l * n;
+            for (int j = 0; j < K; ++j) {
+                const bool gen = (l == 1 && j == 0);
+                kt_begin(s);
+                launch_fwd_pass(R, false, gen, in, fbuf(l, j), lc, d_coef, passes[1 + j].g,
+                                passes[1 + j].grid, s);
+                check_launch("fwd_pass");
+                kt_end(s, gen ? "fwd_pass_high_gen_init" : "fwd_pass_high", gen ? S : 2 * S);
+                in = fbuf(l, j);
+            }
+            kt_begin(s);
+            launch_fwd_pass(R, true, false, in, fbuf(l, K), lc, d_coef, passes[0].g,
+                            passes[0].grid, s);
+            check_launch("fwd_pass scatter");
+            kt_end(s, "fwd_pass_low_cnot", 2 * S);
+            in = fbuf(l, K);
+        }
+        final_cur = -1;
+        final_ptr = fbuf(L - 1, K);
+        return;
+    }
+    // Inverse-walk schedule: working buffers pool[0] and pool[1] (in place
+    // where possible); checkpointed boundary states are written straight into
+    // their checkpoint buffer by the low (scatter) pass and read from there
+    // by the next layer's first pass.
+    const double2 *cur = nullptr;
+    int cur_buf = -1;  // -1: generated (layer 1, first pass) or a checkpoint buffer
     for (int l = 1; l < L; ++l) {
         const WireCoef *lc = d_coef + (size_t)l * n;
-        for (size_t p = 1; p < passes.size(); ++p) {
-            double2 *out = (cur_buf >= 0) ? st[cur_buf] : st[0];
+        for (int j = 0; j < K; ++j) {
+            const bool gen = (l == 1 && j == 0);
+            double2 *out = (cur_buf >= 0) ? pool[cur_buf] : pool[0];
             kt_begin(s);
-            launch_fwd_pass(R, false, cur, out, lc, passes[p].g, passes[p].grid, s);
+            launch_fwd_pass(R, false, gen, cur, out, lc, d_coef, passes[1 + j].g,
+                            passes[1 + j].grid, s);
             check_launch("fwd_pass");
-            kt_end(s, "fwd_pass_high", 2 * S);
+            kt_end(s, gen ? "fwd_pass_high_gen_init" : "fwd_pass_high", gen ? S : 2 * S);
             if (cur_buf < 0) {
                 cur_buf = 0;
             }
@@ -384,19 +423,53 @@ void Engine::structured_forward(hipStream_t s, bool save_ckpts) {
             out_buf = -1;
         } else {
             out_buf = (cur_buf == 0) ? 1 : 0;
-            out = st[out_buf];
+            out = pool[out_buf];
         }
         kt_begin(s);
-        launch_fwd_pass(R, true, cur, out, lc, passes[0].g, passes[0].grid, s);
+        launch_fwd_pass(R, true, false, cur, out, lc, d_coef, passes[0].g, passes[0].grid, s);
         check_launch("fwd_pass scatter");
         kt_end(s, "fwd_pass_low_cnot", 2 * S);
         cur = out;
         cur_buf = out_buf;
     }
     final_cur = cur_buf;  // the last boundary (L) is never a checkpoint
+    final_ptr = pool[final_cur];
 }
 
-const double2 *Engine::final_state() const { return st[final_cur]; }
+const double2 *Engine::final_state() const { return final_ptr; }
+
+size_t Engine::stored_buffer_count() const {
+    const size_t per_layer = passes.empty() ? 1 : passes.size();
+    return (size_t)std::max(0, L - 1) * per_layer + 2;
+}
+
+double2 *Engine::fbuf(int l, int j) const {
+    return pool[(size_t)(l - 1) * passes.size() + (size_t)j];
+}
+
+double2 *Engine::lbuf(int i) const {
+    return pool[(size_t)(L - 1) * passes.size() + (size_t)i];
+}
+
+void Engine::ensure_pool(size_t count) {
+    while (pool.size() < count) {
+        pool.push_back((double2 *)device_alloc(sizeof(double2) * N, "alloc statevector"));
+    }
+}
+
+void Engine::set_schedule(int sched) {
+    if (sched != RIE_SCHEDULE_INVERSE_WALK && sched != RIE_SCHEDULE_STORED) {
+        fail(RIE_INVALID_ARGUMENT, "unknown schedule");
+    }
+    if (strategy != RIE_STRATEGY_STRUCTURED_ADJOINT || small) {
+        return;  // the whole-circuit kernel and other strategies have one schedule
+    }
+    if (sched == RIE_SCHEDULE_STORED && L > 1) {
+        ensure_pool(stored_buffer_count());
+    }
+    schedule = sched;
+    resident = false;
+}
 
 void Engine::require_partials(size_t count) const {
     if (count > partial_count) {
@@ -419,8 +492,16 @@ int Engine::free_buffer(std::initializer_list<int> used) const {
 
 void Engine::structured_hamiltonian(hipStream_t s, bool write_lam) {
     const double2 *psi = final_state();
-    lam_idx = write_lam ? free_buffer({final_cur}) : -1;
-    double2 *lam = write_lam ? st[lam_idx] : nullptr;
+    double2 *lam = nullptr;
+    if (write_lam) {
+        if (schedule == RIE_SCHEDULE_STORED && L > 1) {
+            lam_idx = -1;
+            lam = lbuf(0);
+        } else {
+            lam_idx = free_buffer({final_cur});
+            lam = pool[lam_idx];
+        }
+    }
     size_t off = 0;
     const double S = (double)sizeof(double2) * (double)N;
     size_t total = 0;
@@ -450,6 +531,10 @@ void Engine::structured_backward(hipStream_t s, const double *scale, bool full_w
     // state, so unless a full walk is asked for its gradients come from one
     // contraction pass over lambda.
     const double S = (double)sizeof(double2) * (double)N;
+    if (schedule == RIE_SCHEDULE_STORED && L > 1 && !full_walk) {
+        structured_backward_stored(s, scale);
+        return;
+    }
     int cur = final_cur;  // -2 while the state lives in a checkpoint buffer
     const double2 *cur_in = final_state();
     int lam = lam_idx;
@@ -466,8 +551,8 @@ void Engine::structured_backward(hipStream_t s, const double *scale, bool full_w
         const bool only_low = passes.size() == 1;
         require_partials((size_t)passes[0].grid * 2 * T);
         kt_begin(s);
-        launch_bwd_pass(R, true, only_low ? need_cur_after : true, true, cur_in, st[lam], st[ncur],
-                        st[nlam], lc, passes[0].g, d_partials, passes[0].grid, s);
+        launch_bwd_pass(R, true, only_low ? need_cur_after : true, true, cur_in, pool[lam], pool[ncur],
+                        pool[nlam], lc, passes[0].g, d_partials, passes[0].grid, s);
         check_launch("bwd_pass gather");
         kt_end(s, "bwd_pass_low_cnot", S * (3.0 + ((only_low && !need_cur_after) ? 0.0 : 1.0)));
         kt_begin(s);
@@ -480,7 +565,7 @@ void Engine::structured_backward(hipStream_t s, const double *scale, bool full_w
             const bool write_cur = (p + 1 < passes.size()) || need_cur_after;
             require_partials((size_t)passes[p].grid * 2 * T);
             kt_begin(s);
-            launch_bwd_pass(R, false, write_cur, true, st[cur], st[lam], st[cur], st[lam], lc,
+            launch_bwd_pass(R, false, write_cur, true, pool[cur], pool[lam], pool[cur], pool[lam], lc,
                             passes[p].g, d_partials, passes[p].grid, s);
             check_launch("bwd_pass");
             kt_end(s, "bwd_pass_high", S * (3.0 + (write_cur ? 1.0 : 0.0)));
@@ -493,30 +578,74 @@ void Engine::structured_backward(hipStream_t s, const double *scale, bool full_w
             cur_in = ckpts[(size_t)l / ckpt_every - 1];
             cur = -2;
         } else {
-            cur_in = st[cur];
+            cur_in = pool[cur];
         }
     }
     walked_cur = full_walk ? cur : -1;
     if (!full_walk) {
-        const PassGeom &g0 = passes[0].g;
-        const int grid0 = passes[0].grid;
-        require_partials((size_t)grid0 * 48);
-        if ((size_t)high_env_chunks * 4 * (size_t)std::max(1, n - T) > partial2_count) {
-            fail(RIE_DEVICE_ERROR, "internal: layer-0 partial buffer too small");
-        }
-        kt_begin(s);
-        launch_layer0_contract(st[lam], d_coef, g0, d_tile_sums, d_partials, grid0, s);
-        check_launch("layer0_contract");
-        kt_end(s, "layer0_contract", S);
-        kt_begin(s);
-        launch_layer0_high_env(d_tile_sums, g0.num_tiles, d_coef, n, d_partials2, high_env_chunks,
-                               s);
-        check_launch("layer0_high_env");
-        launch_layer0_finish(d_partials, grid0, d_partials2, high_env_chunks, d_coef, n, d_grad,
-                             scale, s);
-        check_launch("layer0_finish");
-        kt_end(s, "layer0_reduce", 0.0);
+        layer0_gradients(s, pool[lam], scale);
     }
+}
+
+void Engine::layer0_gradients(hipStream_t s, const double2 *lam, const double *scale) {
+    const double S = (double)sizeof(double2) * (double)N;
+    const PassGeom &g0 = passes[0].g;
+    const int grid0 = passes[0].grid;
+    require_partials((size_t)grid0 * 48);
+    if ((size_t)high_env_chunks * 4 * (size_t)std::max(1, n - T) > partial2_count) {
+        fail(RIE_DEVICE_ERROR, "internal: layer-0 partial buffer too small");
+    }
+    kt_begin(s);
+    launch_layer0_contract(lam, d_coef, g0, d_tile_sums, d_partials, grid0, s);
+    check_launch("layer0_contract");
+    kt_end(s, "layer0_contract", S);
+    kt_begin(s);
+    launch_layer0_high_env(d_tile_sums, g0.num_tiles, d_coef, n, d_partials2, high_env_chunks, s);
+    check_launch("layer0_high_env");
+    launch_layer0_finish(d_partials, grid0, d_partials2, high_env_chunks, d_coef, n, d_grad, scale,
+                         s);
+    check_launch("layer0_finish");
+    kt_end(s, "layer0_reduce", 0.0);
+}
+
+
+void Engine::structured_backward_stored(hipStream_t s, const double *scale) {
+    // Stored schedule: the forward kept every pass output, so the backward
+    // reads the state at each pass from there and only rewrites lambda.  The
+    // low pass (inverse ring CNOT on load) comes first; the high passes
+    // follow in reverse forward order.
+    const double S = (double)sizeof(double2) * (double)N;
+    const int K = (int)passes.size() - 1;
+    int a = 0;  // lambda lives in lbuf(a)
+    for (int l = L - 1; l >= 1; --l) {
+        const WireCoef *lc = d_coef + (size_t)l * n;
+        const int b = 1 - a;
+        require_partials((size_t)passes[0].grid * 2 * T);
+        kt_begin(s);
+        launch_bwd_pass(R, true, false, true, fbuf(l, K), lbuf(a), nullptr, lbuf(b), lc,
+                        passes[0].g, d_partials, passes[0].grid, s);
+        check_launch("bwd_pass gather");
+        kt_end(s, "bwd_pass_low_cnot", 3 * S);
+        kt_begin(s);
+        launch_reduce_grad(d_partials, passes[0].grid, T, passes[0].g, l, d_grad, scale, s);
+        check_launch("reduce_grad");
+        kt_end(s, "reduce_grad", 0.0);
+        for (int j = K - 1; j >= 0; --j) {
+            const PassPlan &pp = passes[1 + j];
+            require_partials((size_t)pp.grid * 2 * T);
+            kt_begin(s);
+            launch_bwd_pass(R, false, false, true, fbuf(l, j), lbuf(b), nullptr, lbuf(b), lc, pp.g,
+                            d_partials, pp.grid, s);
+            check_launch("bwd_pass");
+            kt_end(s, "bwd_pass_high", 3 * S);
+            kt_begin(s);
+            launch_reduce_grad(d_partials, pp.grid, T, pp.g, l, d_grad, scale, s);
+            check_launch("reduce_grad");
+            kt_end(s, "reduce_grad", 0.0);
+        }
+        a = b;
+    }
+    layer0_gradients(s, lbuf(a), scale);
 }
 
 // ---------------------------------------------------------------------------
@@ -546,16 +675,16 @@ void Engine::small_run(hipStream_t s, bool want_grad, const double *scale, doubl
 // ---------------------------------------------------------------------------
 
 void Engine::gate_forward(hipStream_t s) {
-    launch_zero_state(st[0], N, s);
+    launch_zero_state(pool[0], N, s);
     check_launch("zero_state");
     for (int l = 0; l < L; ++l) {
         for (int w = 0; w < n; ++w) {
             const WireCoef *k = d_coef + (size_t)l * n + w;
-            launch_gate_ry(st[0], n, w, k, false, s);
-            launch_gate_rz(st[0], n, w, k, false, s);
+            launch_gate_ry(pool[0], n, w, k, false, s);
+            launch_gate_rz(pool[0], n, w, k, false, s);
         }
         for (int w = 0; w < n; ++w) {
-            launch_gate_cnot(st[0], n, w, (w + 1) % n, s);
+            launch_gate_cnot(pool[0], n, w, (w + 1) % n, s);
         }
     }
     check_launch("gate forward");
@@ -566,7 +695,7 @@ void Engine::gate_hamiltonian(hipStream_t s) {
     if ((size_t)grid > epartial_count) {
         fail(RIE_DEVICE_ERROR, "internal: energy partial buffer too small");
     }
-    launch_ham_gather(st[0], st[1], n, h, d_epartials, grid, s);
+    launch_ham_gather(pool[0], pool[1], n, h, d_epartials, grid, s);
     check_launch("ham_gather");
     energy_partials = (size_t)grid;
 }
@@ -576,15 +705,15 @@ void Engine::gate_backward(hipStream_t s, const double *scale) {
     require_partials((size_t)grid);
     for (int l = L - 1; l >= 0; --l) {
         for (int w = n - 1; w >= 0; --w) {
-            launch_gate_cnot(st[0], n, w, (w + 1) % n, s);
-            launch_gate_cnot(st[1], n, w, (w + 1) % n, s);
+            launch_gate_cnot(pool[0], n, w, (w + 1) % n, s);
+            launch_gate_cnot(pool[1], n, w, (w + 1) % n, s);
         }
         for (int w = n - 1; w >= 0; --w) {
             const size_t base = ((size_t)l * n + w) * 2;
             const WireCoef *k = d_coef + (size_t)l * n + w;
-            launch_gate_rz_step(st[0], st[1], n, w, k, d_partials, grid, s);
+            launch_gate_rz_step(pool[0], pool[1], n, w, k, d_partials, grid, s);
             launch_reduce_to(d_partials, grid, d_grad + base + 1, scale, s);
-            launch_gate_ry_step(st[0], st[1], n, w, k, d_partials, grid, s);
+            launch_gate_ry_step(pool[0], pool[1], n, w, k, d_partials, grid, s);
             launch_reduce_to(d_partials, grid, d_grad + base, scale, s);
         }
     }
@@ -770,11 +899,14 @@ double Engine::inverse_walk_drift(const double *params, size_t count) {
         check(hipMemcpyAsync(&drift2, d_drift, sizeof(double), hipMemcpyDeviceToHost, s),
               "copy drift");
     } else {
+        const int saved = schedule;
+        schedule = RIE_SCHEDULE_INVERSE_WALK;
         structured_forward(s, ckpt_every > 0);
         structured_hamiltonian(s, true);
         structured_backward(s, nullptr, true);
+        schedule = saved;
         const int grid = elementwise_grid(N);
-        launch_drift(st[walked_cur], N, d_drift, grid, s);
+        launch_drift(pool[walked_cur], N, d_drift, grid, s);
         launch_reduce_sum(d_drift, grid, d_drift + 8192, s);
         check_launch("drift");
         check(hipMemcpyAsync(&drift2, d_drift + 8192, sizeof(double), hipMemcpyDeviceToHost, s),
@@ -1079,6 +1211,14 @@ rie_status rie_engine_inverse_walk_drift(rie_engine *engine, const double *param
         }
         *drift_out = as_engine(engine)->inverse_walk_drift(params, num_params);
     });
+}
+
+rie_status rie_engine_set_schedule(rie_engine *engine, int schedule) {
+    return guard([&]() { as_engine(engine)->set_schedule(schedule); });
+}
+
+int rie_engine_schedule(const rie_engine *engine) {
+    return engine == nullptr ? -1 : reinterpret_cast<const Engine *>(engine)->schedule;
 }
 
 rie_status rie_engine_set_kernel_timing(rie_engine *engine, int enable) {

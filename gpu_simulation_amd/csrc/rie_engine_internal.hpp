@@ -65,7 +65,8 @@ struct Engine {
     std::vector<PassPlan> passes;  // [0]: low pass (a = T); [1..]: high passes
 
     // device buffers
-    double2 *st[4] = {nullptr, nullptr, nullptr, nullptr};
+    std::vector<double2 *> pool;  // statevector buffers (16 * 2^n bytes each)
+    int schedule = RIE_SCHEDULE_INVERSE_WALK;
     double *d_params = nullptr;
     double *d_grad = nullptr;
     double *d_energy = nullptr;
@@ -86,6 +87,7 @@ struct Engine {
 
     // resident-state bookkeeping
     int final_cur = 0;
+    const double2 *final_ptr = nullptr;
     int lam_idx = -1;
     int walked_cur = -1;
     bool resident = false;
@@ -137,6 +139,13 @@ struct Engine {
     void structured_forward(hipStream_t s, bool save_ckpts);
     void structured_hamiltonian(hipStream_t s, bool write_lam);
     void structured_backward(hipStream_t s, const double *scale, bool full_walk);
+    void structured_backward_stored(hipStream_t s, const double *scale);
+    void layer0_gradients(hipStream_t s, const double2 *lam, const double *scale);
+    size_t stored_buffer_count() const;
+    double2 *fbuf(int l, int j) const;  // stored schedule: output of pass j of layer l
+    double2 *lbuf(int i) const;         // stored schedule: lambda buffers 0 and 1
+    void ensure_pool(size_t count);
+    void set_schedule(int sched);
     void small_run(hipStream_t s, bool want_grad, const double *scale, double *energy_out,
                    double *grad_out, double *drift_out);
     void small_run_captured(hipStream_t s);

@@ -135,21 +135,32 @@ __device__ __forceinline__ void phase_fwd(double2 (&v)[Tile<R>::REGS],
     }
 }
 
+// Per-wire gradient sums of one tile are reduced across the wave right away
+// and added by lane 0 to this wave's LDS slot (fixed order, no atomics), so
+// no per-thread accumulators stay live across phases and tiles.
 template <int R, int M>
 __device__ __forceinline__ void phase_bwd(double2 (&c)[Tile<R>::REGS], double2 (&l)[Tile<R>::REGS],
                                           const WireCoef *__restrict__ lc, const PassGeom &G,
-                                          double (&dt)[3 * R], double (&dp)[3 * R]) {
+                                          double *__restrict__ wave_acc, int lane) {
     constexpr int P0 = phase_first_bit<R, M>();
 #pragma unroll
     for (int rb = R - 1; rb >= 0; --rb) {
         const int j = P0 + rb;
         if (j >= G.rot_lo && j < G.rot_hi) {
             const WireCoef k = lc[tile_wire(j, G)];
+            double dt = 0.0;
+            double dp = 0.0;
 #pragma unroll
             for (int q = 0; q < Tile<R>::REGS; ++q) {
                 if (!(q & (1 << rb))) {
-                    bwd_pair(c[q], c[q | (1 << rb)], l[q], l[q | (1 << rb)], k, dt[j], dp[j]);
+                    bwd_pair(c[q], c[q | (1 << rb)], l[q], l[q | (1 << rb)], k, dt, dp);
                 }
+            }
+            dt = wave_sum(dt);
+            dp = wave_sum(dp);
+            if (lane == 0) {
+                wave_acc[2 * j] += dt;
+                wave_acc[2 * j + 1] += dp;
             }
         }
     }
@@ -224,20 +235,57 @@ void launch_init_product_cnot(double2 *out, const WireCoef *coef_layer0, int n, 
 // Forward tile pass.
 // ---------------------------------------------------------------------------
 
-template <int R, bool kScatter>
+// With kGenInit the pass does not read its input: the tile of the layer-0
+// state after its ring CNOT, chi(i) = prod_w u_w(bit w of P^-1(i)), is
+// generated from nibble tables of the layer-0 factors (lc0).
+constexpr int kInitNibbles = 9;  // covers n <= 36
+
+template <int R, bool kScatter, bool kGenInit>
 __global__ __launch_bounds__(Tile<R>::THREADS) void fwd_pass_kernel(
-    const double2 *in, double2 *out, const WireCoef *__restrict__ lc, PassGeom G) {
+    const double2 *in, double2 *out, const WireCoef *__restrict__ lc,
+    const WireCoef *__restrict__ lc0, PassGeom G) {
     using TL = Tile<R>;
     __shared__ double2 lds[TL::SLOTS];
+    __shared__ double2 nib[kGenInit ? kInitNibbles : 1][16];
     const uint32_t t = threadIdx.x;
     const TileAddr ta = tile_addr<R>(t, G);
+    if constexpr (kGenInit) {
+        for (uint32_t e = t; e < kInitNibbles * 16; e += TL::THREADS) {
+            const int q = (int)(e >> 4);
+            const int bits = (int)(e & 15);
+            double2 v = make_double2(1.0, 0.0);
+            for (int j = 0; j < 4; ++j) {
+                const int w = 4 * q + j;
+                if (w < G.n) {
+                    const WireCoef k = lc0[w];
+                    v = cmul(v, ((bits >> j) & 1) ? u1_of(k) : u0_of(k));
+                }
+            }
+            nib[q][bits] = v;
+        }
+        __syncthreads();
+    }
     for (uint64_t g = blockIdx.x; g < G.num_tiles; g += gridDim.x) {
         const uint64_t base = tile_base(g, G, TL::T);
         double2 v[TL::REGS];
-        const double2 *src = in + base + ta.offA;
+        if constexpr (kGenInit) {
+            const uint64_t row = base + ta.offA;
 #pragma unroll
-        for (int k = 0; k < TL::REGS; ++k) {
-            v[k] = src[k * ta.strideA];
+            for (int k = 0; k < TL::REGS; ++k) {
+                const uint64_t x = ring_inv(row + k * ta.strideA, G.n);
+                double2 p = nib[0][x & 15];
+#pragma unroll
+                for (int q = 1; q < kInitNibbles; ++q) {
+                    p = cmul(p, nib[q][(x >> (4 * q)) & 15]);
+                }
+                v[k] = p;
+            }
+        } else {
+            const double2 *src = in + base + ta.offA;
+#pragma unroll
+            for (int k = 0; k < TL::REGS; ++k) {
+                v[k] = src[k * ta.strideA];
+            }
         }
         phase_fwd<R, 0>(v, lc, G);
         exchange<R, 0, 1>(v, lds, t);
@@ -279,56 +327,37 @@ __global__ __launch_bounds__(Tile<R>::THREADS) void fwd_pass_kernel(
     }
 }
 
-void launch_fwd_pass(int R, bool scatter, const double2 *in, double2 *out,
-                     const WireCoef *layer_coef, const PassGeom &g, int grid, hipStream_t s) {
-    if (R == 4) {
+template <int R>
+static void launch_fwd_pass_r(bool scatter, bool gen_init, const double2 *in, double2 *out,
+                              const WireCoef *lc, const WireCoef *lc0, const PassGeom &g,
+                              int grid, hipStream_t s) {
+    constexpr int TH = Tile<R>::THREADS;
+    if (gen_init) {
         if (scatter) {
-            fwd_pass_kernel<4, true><<<grid, Tile<4>::THREADS, 0, s>>>(in, out, layer_coef, g);
+            fwd_pass_kernel<R, true, true><<<grid, TH, 0, s>>>(in, out, lc, lc0, g);
         } else {
-            fwd_pass_kernel<4, false><<<grid, Tile<4>::THREADS, 0, s>>>(in, out, layer_coef, g);
+            fwd_pass_kernel<R, false, true><<<grid, TH, 0, s>>>(in, out, lc, lc0, g);
         }
+    } else if (scatter) {
+        fwd_pass_kernel<R, true, false><<<grid, TH, 0, s>>>(in, out, lc, lc0, g);
     } else {
-        if (scatter) {
-            fwd_pass_kernel<3, true><<<grid, Tile<3>::THREADS, 0, s>>>(in, out, layer_coef, g);
-        } else {
-            fwd_pass_kernel<3, false><<<grid, Tile<3>::THREADS, 0, s>>>(in, out, layer_coef, g);
-        }
+        fwd_pass_kernel<R, false, false><<<grid, TH, 0, s>>>(in, out, lc, lc0, g);
+    }
+}
+
+void launch_fwd_pass(int R, bool scatter, bool gen_init, const double2 *in, double2 *out,
+                     const WireCoef *layer_coef, const WireCoef *layer0_coef, const PassGeom &g,
+                     int grid, hipStream_t s) {
+    if (R == 4) {
+        launch_fwd_pass_r<4>(scatter, gen_init, in, out, layer_coef, layer0_coef, g, grid, s);
+    } else {
+        launch_fwd_pass_r<3>(scatter, gen_init, in, out, layer_coef, layer0_coef, g, grid, s);
     }
 }
 
 // ---------------------------------------------------------------------------
 // Adjoint tile pass.
 // ---------------------------------------------------------------------------
-
-template <int R>
-__device__ __forceinline__ void block_reduce_grad(double (&dt)[3 * R], double (&dp)[3 * R],
-                                                  double *__restrict__ partials, const PassGeom &G) {
-    using TL = Tile<R>;
-    constexpr int T = TL::T;
-    __shared__ double red[TL::WAVES][2 * T];
-    const int t = threadIdx.x;
-    const int lane = t & (kWave - 1);
-    const int wave = t / kWave;
-#pragma unroll
-    for (int j = 0; j < T; ++j) {
-        const double a = wave_sum(dt[j]);
-        const double b = wave_sum(dp[j]);
-        if (lane == 0) {
-            red[wave][2 * j] = a;
-            red[wave][2 * j + 1] = b;
-        }
-    }
-    __syncthreads();
-    if (t < 2 * T) {
-        double acc = 0.0;
-#pragma unroll
-        for (int w = 0; w < TL::WAVES; ++w) {
-            acc += red[w][t];
-        }
-        partials[(uint64_t)blockIdx.x * (2 * T) + t] = acc;
-    }
-    (void)G;
-}
 
 template <int R, bool kGather, bool kWriteCur, bool kWriteLam>
 __global__ __launch_bounds__(Tile<R>::THREADS) void bwd_pass_kernel(
@@ -337,13 +366,12 @@ __global__ __launch_bounds__(Tile<R>::THREADS) void bwd_pass_kernel(
     using TL = Tile<R>;
     constexpr int T = TL::T;
     __shared__ double2 lds[TL::SLOTS];
+    __shared__ double acc[TL::WAVES][2 * T];
     const uint32_t t = threadIdx.x;
-    double dt[T];
-    double dp[T];
-#pragma unroll
-    for (int j = 0; j < T; ++j) {
-        dt[j] = 0.0;
-        dp[j] = 0.0;
+    const int lane = t & (kWave - 1);
+    double *wave_acc = acc[t / kWave];
+    if (lane < 2 * T) {
+        wave_acc[lane] = 0.0;
     }
 
     const TileAddr ta = tile_addr<R>(t, G);
@@ -395,13 +423,13 @@ __global__ __launch_bounds__(Tile<R>::THREADS) void bwd_pass_kernel(
         }
 
         if constexpr (kGather) {
-            phase_bwd<R, 1>(c, l, lc, G, dt, dp);
+            phase_bwd<R, 1>(c, l, lc, G, wave_acc, lane);
             exchange<R, 1, 2>(c, lds, t);
             exchange<R, 1, 2>(l, lds, t);
-            phase_bwd<R, 2>(c, l, lc, G, dt, dp);
+            phase_bwd<R, 2>(c, l, lc, G, wave_acc, lane);
             exchange<R, 2, 0>(c, lds, t);
             exchange<R, 2, 0>(l, lds, t);
-            phase_bwd<R, 0>(c, l, lc, G, dt, dp);
+            phase_bwd<R, 0>(c, l, lc, G, wave_acc, lane);
             const uint64_t row = base + ta.offA;
 #pragma unroll
             for (int k = 0; k < TL::REGS; ++k) {
@@ -414,13 +442,13 @@ __global__ __launch_bounds__(Tile<R>::THREADS) void bwd_pass_kernel(
                 }
             }
         } else {
-            phase_bwd<R, 0>(c, l, lc, G, dt, dp);
+            phase_bwd<R, 0>(c, l, lc, G, wave_acc, lane);
             exchange<R, 0, 1>(c, lds, t);
             exchange<R, 0, 1>(l, lds, t);
-            phase_bwd<R, 1>(c, l, lc, G, dt, dp);
+            phase_bwd<R, 1>(c, l, lc, G, wave_acc, lane);
             exchange<R, 1, 2>(c, lds, t);
             exchange<R, 1, 2>(l, lds, t);
-            phase_bwd<R, 2>(c, l, lc, G, dt, dp);
+            phase_bwd<R, 2>(c, l, lc, G, wave_acc, lane);
             const uint64_t row = base + ta.offC;
 #pragma unroll
             for (int k = 0; k < TL::REGS; ++k) {
@@ -434,7 +462,15 @@ __global__ __launch_bounds__(Tile<R>::THREADS) void bwd_pass_kernel(
             }
         }
     }
-    block_reduce_grad<R>(dt, dp, partials, G);
+    __syncthreads();
+    if (t < 2 * T) {
+        double total = 0.0;
+#pragma unroll
+        for (int w = 0; w < TL::WAVES; ++w) {
+            total += acc[w][t];
+        }
+        partials[(uint64_t)blockIdx.x * (2 * T) + t] = total;
+    }
 }
 
 template <int R>
@@ -488,9 +524,13 @@ __global__ __launch_bounds__(Tile<R>::THREADS) void ham_pass_kernel(
         const uint64_t base = tile_base(g, G, T);
         const uint64_t row = base + ta.offA;
         double2 v[TL::REGS];
+        double2 prev[kReadLam ? TL::REGS : 1];
 #pragma unroll
         for (int k = 0; k < TL::REGS; ++k) {
             v[k] = psi[row + k * ta.strideA];
+            if constexpr (kReadLam) {
+                prev[k] = lam_in[row + k * ta.strideA];
+            }
         }
         __syncthreads();
 #pragma unroll
@@ -518,9 +558,8 @@ __global__ __launch_bounds__(Tile<R>::THREADS) void ham_pass_kernel(
             if constexpr (kWriteLam) {
                 double2 out = contrib;
                 if constexpr (kReadLam) {
-                    const double2 prev = lam_in[idx];
-                    out.x += prev.x;
-                    out.y += prev.y;
+                    out.x += prev[k].x;
+                    out.y += prev[k].y;
                 }
                 lam_out[idx] = out;
             }
@@ -691,55 +730,31 @@ __global__ __launch_bounds__(256) void layer0_contract_kernel(
     using TL = Tile<R>;
     constexpr int T = TL::T;
     __shared__ double2 lds[TL::SLOTS];
-    __shared__ double2 w_base[16];      // prod over wires 8..11 of u(k bits)
-    __shared__ double2 w_env[4][16];    // same without wire 8 + j
     __shared__ double2 red2[TL::WAVES];
     __shared__ double red[TL::WAVES][48];
     const uint32_t t = threadIdx.x;
     const int lane = t & 63;
     const int wave = t >> 6;
-    if (t < 16) {
-        double2 b = make_double2(1.0, 0.0);
-        for (int j = 0; j < 4; ++j) {
-            b = cmul(b, u_of(lc0[8 + j], (t >> j) & 1));
-        }
-        w_base[t] = b;
-        for (int e = 0; e < 4; ++e) {
-            double2 v = make_double2(1.0, 0.0);
-            for (int j = 0; j < 4; ++j) {
-                if (j != e) {
-                    v = cmul(v, u_of(lc0[8 + j], (t >> j) & 1));
-                }
-            }
-            w_env[e][t] = v;
-        }
+    double2 uf[4][2];  // u_{8+j}(b) for the register wires
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const WireCoef kj = lc0[8 + j];
+        uf[j][0] = u0_of(kj);
+        uf[j][1] = u1_of(kj);
     }
-    // thread constants over offset bits 0..7 (wires 0..7)
+    // thread constant over offset bits 0..7 (wires 0..7)
     double2 a_lo = make_double2(1.0, 0.0);
-    double2 a_skip[8];
-    for (int w = 0; w < 8; ++w) {
-        a_skip[w] = make_double2(1.0, 0.0);
-    }
     for (int v = 0; v < 8; ++v) {
-        const double2 f = u_of(lc0[v], (t >> v) & 1);
-        a_lo = cmul(a_lo, f);
-#pragma unroll
-        for (int w = 0; w < 8; ++w) {
-            if (w != v) {
-                a_skip[w] = cmul(a_skip[w], f);
-            }
-        }
+        a_lo = cmul(a_lo, u_of(lc0[v], (t >> v) & 1));
     }
-    double2 acc_lo[8];     // E_w[t_w] partial for w < 8
-    double2 acc_reg[4][2]; // E_w[b] partial for w = 8..11
-#pragma unroll
-    for (int w = 0; w < 8; ++w) {
-        acc_lo[w] = make_double2(0.0, 0.0);
-    }
+    // Accumulated over this workgroup's tiles; the thread constants over
+    // wires 0..7 factor out and are applied once at the end.
+    double2 bpk_sum = make_double2(0.0, 0.0);   // sum_g B(g) pk(g)
+    double2 benv[4][2];                          // sum_g B(g) env_e[b](g)
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
-        acc_reg[e][0] = make_double2(0.0, 0.0);
-        acc_reg[e][1] = make_double2(0.0, 0.0);
+        benv[e][0] = make_double2(0.0, 0.0);
+        benv[e][1] = make_double2(0.0, 0.0);
     }
     __syncthreads();
 
@@ -764,45 +779,71 @@ __global__ __launch_bounds__(256) void layer0_contract_kernel(
         for (int k = 0; k < 16; ++k) {
             m[k] = lds[lds_slot<R>(omap<R, 0>(t, k))];
         }
-        // contract the register bits (wires 8..11)
-        double2 pk = make_double2(0.0, 0.0);
-        double2 env[4][2];
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            env[e][0] = make_double2(0.0, 0.0);
-            env[e][1] = make_double2(0.0, 0.0);
-        }
-#pragma unroll
-        for (int k = 0; k < 16; ++k) {
-            const double2 cm = make_double2(m[k].x, -m[k].y);  // conj(mu)
-            cmac(pk, cm, w_base[k]);
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                cmac(env[e][(k >> e) & 1], cm, w_env[e][k]);
-            }
-        }
         // high-wire product of this tile
         double2 bg = make_double2(1.0, 0.0);
         for (int v = T; v < G.n; ++v) {
             bg = cmul(bg, u_of(lc0[v], (int)((base >> v) & 1U)));
         }
+        // contract the register bits k0..k3 (wires 8..11) hierarchically:
+        // x[k] = conj(mu), A contracts k0, AB k0..k1, ABC k0..k2; the
+        // environments keep one bit uncontracted.
+        double2 x[16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            x[k] = make_double2(m[k].x, -m[k].y);
+        }
+        double2 A[8], F[8];
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {  // r = k1 + 2 k2 + 4 k3  /  k0 + 2 k2 + 4 k3
+            A[r] = cmul(x[2 * r], uf[0][0]);
+            cmac(A[r], x[2 * r + 1], uf[0][1]);
+            const int k0 = r & 1;
+            const int hi = r >> 1;  // k2 + 2 k3
+            F[r] = cmul(x[k0 + 4 * hi], uf[1][0]);
+            cmac(F[r], x[k0 + 2 + 4 * hi], uf[1][1]);
+        }
+        double2 AB[4], E[4], G[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            // AB[k2 + 2 k3]
+            AB[r] = cmul(A[2 * r], uf[1][0]);
+            cmac(AB[r], A[2 * r + 1], uf[1][1]);
+            // E[k1 + 2 k3] = contract k2 of A[k1, k2, k3]
+            const int k1 = r & 1;
+            const int k3 = r >> 1;
+            E[r] = cmul(A[k1 + 4 * k3], uf[2][0]);
+            cmac(E[r], A[k1 + 2 + 4 * k3], uf[2][1]);
+            // G[k0 + 2 k3] = contract k2 of F[k0, k2, k3]
+            G[r] = cmul(F[k1 + 4 * k3], uf[2][0]);
+            cmac(G[r], F[k1 + 2 + 4 * k3], uf[2][1]);
+        }
+        double2 ABC[2], env[4][2];
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            ABC[b] = cmul(AB[2 * b], uf[2][0]);          // contract k2 of AB[k2, k3=b]
+            cmac(ABC[b], AB[2 * b + 1], uf[2][1]);
+            env[3][b] = ABC[b];
+            env[2][b] = cmul(AB[b], uf[3][0]);           // contract k3 of AB[k2=b, k3]
+            cmac(env[2][b], AB[b + 2], uf[3][1]);
+            env[1][b] = cmul(E[b], uf[3][0]);            // contract k3 of E[k1=b, k3]
+            cmac(env[1][b], E[b + 2], uf[3][1]);
+            env[0][b] = cmul(G[b], uf[3][0]);            // contract k3 of G[k0=b, k3]
+            cmac(env[0][b], G[b + 2], uf[3][1]);
+        }
+        double2 pk = cmul(ABC[0], uf[3][0]);
+        cmac(pk, ABC[1], uf[3][1]);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            cmac(benv[e][0], bg, env[e][0]);
+            cmac(benv[e][1], bg, env[e][1]);
+        }
+        cmac(bpk_sum, bg, pk);
         // per-tile contraction of all tile wires: S(g) = sum_t pk * a_lo
         const double2 st = cmul(pk, a_lo);
         const double sx = wave_sum(st.x);
         const double sy = wave_sum(st.y);
         if (lane == 0) {
             red2[wave] = make_double2(sx, sy);
-        }
-        const double2 bpk = cmul(bg, pk);
-#pragma unroll
-        for (int w = 0; w < 8; ++w) {
-            cmac(acc_lo[w], bpk, a_skip[w]);
-        }
-        const double2 ba = cmul(bg, a_lo);
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            cmac(acc_reg[e][0], ba, env[e][0]);
-            cmac(acc_reg[e][1], ba, env[e][1]);
         }
         __syncthreads();
         if (t == 0) {
@@ -818,9 +859,16 @@ __global__ __launch_bounds__(256) void layer0_contract_kernel(
     // entries, complex -> 48 doubles
 #pragma unroll
     for (int w = 0; w < 8; ++w) {
+        double2 skip = make_double2(1.0, 0.0);  // prod_{v < 8, v != w} u_v(t_v)
+        for (int v = 0; v < 8; ++v) {
+            if (v != w) {
+                skip = cmul(skip, u_of(lc0[v], (t >> v) & 1));
+            }
+        }
+        const double2 val = cmul(bpk_sum, skip);
         const int b = (t >> w) & 1;
-        const double vals[4] = {b ? 0.0 : acc_lo[w].x, b ? 0.0 : acc_lo[w].y,
-                                b ? acc_lo[w].x : 0.0, b ? acc_lo[w].y : 0.0};
+        const double vals[4] = {b ? 0.0 : val.x, b ? 0.0 : val.y, b ? val.x : 0.0,
+                                b ? val.y : 0.0};
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
             const double r = wave_sum(vals[c]);
@@ -831,8 +879,9 @@ __global__ __launch_bounds__(256) void layer0_contract_kernel(
     }
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
-        const double vals[4] = {acc_reg[e][0].x, acc_reg[e][0].y, acc_reg[e][1].x,
-                                acc_reg[e][1].y};
+        const double2 v0 = cmul(a_lo, benv[e][0]);
+        const double2 v1 = cmul(a_lo, benv[e][1]);
+        const double vals[4] = {v0.x, v0.y, v1.x, v1.y};
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
             const double r = wave_sum(vals[c]);

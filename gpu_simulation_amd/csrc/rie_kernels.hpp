@@ -53,10 +53,13 @@ void launch_small(const SmallJob *d_jobs, const SmallJob &single, int num_jobs, 
 // Layer 0 as a product state followed by the ring-CNOT permutation.
 void launch_init_product_cnot(double2 *out, const WireCoef *coef_layer0, int n, hipStream_t s);
 
-// Forward tile pass (rotations on the pass's tile wires, in place) or, with
-// scatter, the low-tile pass followed by the ring-CNOT permutation into out.
-void launch_fwd_pass(int R, bool scatter, const double2 *in, double2 *out,
-                     const WireCoef *layer_coef, const PassGeom &g, int grid, hipStream_t s);
+// Forward tile pass (rotations on the pass's tile wires, in place or out of
+// place) or, with scatter, the low-tile pass followed by the ring-CNOT
+// permutation into out.  gen_init: the input is not read but generated as
+// the layer-0 product state after its ring CNOT (layer0_coef).
+void launch_fwd_pass(int R, bool scatter, bool gen_init, const double2 *in, double2 *out,
+                     const WireCoef *layer_coef, const WireCoef *layer0_coef, const PassGeom &g,
+                     int grid, hipStream_t s);
 
 // Adjoint tile pass: un-rotates the pass's wires on (cur, lam) and writes
 // per-workgroup gradient partials [grid][2T].  gather = apply the inverse

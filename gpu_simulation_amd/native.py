@@ -37,6 +37,8 @@ C_ABI_SYMBOLS = (
     "rie_engine_set_checkpoint_interval",
     "rie_engine_inverse_walk_drift",
     "rie_engine_copy_state",
+    "rie_engine_set_schedule",
+    "rie_engine_schedule",
     "rie_engine_set_kernel_timing",
     "rie_engine_reset_kernel_stats",
     "rie_engine_kernel_stat_count",

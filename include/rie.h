@@ -142,6 +142,21 @@ rie_status rie_engine_energy_and_grad_device(rie_engine *engine,
                                              double *d_energy, double *d_gradient,
                                              void *stream);
 
+/* How the backward sweep obtains the state at each pass (no counterpart in
+ * the reference, whose sweep always walks back by inverse gates):
+ *   RIE_SCHEDULE_INVERSE_WALK: recompute it by applying inverse gates
+ *     (4 statevector buffers, plus checkpoints);
+ *   RIE_SCHEDULE_STORED: keep every forward pass output and read it back
+ *     ((L-1)(K+1)+2 buffers, K = high passes per layer; each backward pass
+ *     then reads state and lambda and writes lambda only).
+ * A new engine uses STORED when it fits in 85% of free device memory. */
+typedef enum rie_schedule {
+    RIE_SCHEDULE_INVERSE_WALK = 0,
+    RIE_SCHEDULE_STORED = 1
+} rie_schedule;
+rie_status rie_engine_set_schedule(rie_engine *engine, int schedule);
+int rie_engine_schedule(const rie_engine *engine);
+
 /* Store the statevector at every k-th layer boundary during the forward pass
  * and restore it there during the backward sweep instead of continuing the
  * inverse walk (k = 0 disables).  Costs one extra state buffer per stored

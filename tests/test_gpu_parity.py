@@ -24,21 +24,27 @@ def _params(seed: int, n: int, layers: int, scale: float | None = None) -> np.nd
     return scale * rng.standard_normal((layers, n, 2))
 
 
-def _run(backend_module, n, layers, field, strategy, params, width=8, compute_gradient=True):
+def _run(backend_module, n, layers, field, strategy, params, width=8, compute_gradient=True,
+         schedule=None):
     eng = backend_module.RingIsingCudaBackend(n, layers, field, strategy, width, False)
+    if schedule is not None:
+        eng.set_schedule(schedule)
     out = eng.energy_and_grad(np.ascontiguousarray(params.reshape(-1)), compute_gradient)
     return out["energy"], np.asarray(out["gradient"])
 
 
-def _check(oracle, backend_module, n, layers, field, strategy, seed, width=8, scale=None):
+def _check(oracle, backend_module, n, layers, field, strategy, seed, width=8, scale=None,
+           schedule=None):
     params = _params(seed, n, layers, scale)
-    print(f"case strategy={strategy} n={n} L={layers} h={field} seed={seed}", flush=True)
+    print(f"case strategy={strategy} n={n} L={layers} h={field} seed={seed} schedule={schedule}",
+          flush=True)
     e_ref, g_ref = oracle.energy_and_grad(n, layers, field, params)
-    e, g = _run(backend_module, n, layers, field, strategy, params, width)
+    e, g = _run(backend_module, n, layers, field, strategy, params, width, schedule=schedule)
     assert abs(e - e_ref) <= TOL, (e, e_ref)
     assert g.shape == g_ref.shape
     assert np.max(np.abs(g - g_ref)) <= TOL, np.max(np.abs(g - g_ref))
-    e_only, g_only = _run(backend_module, n, layers, field, strategy, params, width, False)
+    e_only, g_only = _run(backend_module, n, layers, field, strategy, params, width, False,
+                          schedule=schedule)
     assert abs(e_only - e_ref) <= TOL
     assert g_only.size == 0
 
@@ -56,9 +62,23 @@ def test_structured_whole_circuit_kernel(oracle, backend_module, n, layers):
     _check(oracle, backend_module, n, layers, 0.9, "structured_adjoint", 1000 + n)
 
 
-@pytest.mark.parametrize("n,layers", [(13, 1), (13, 2), (14, 3), (16, 2), (21, 2), (22, 2)])
-def test_structured_tile_passes(oracle, backend_module, n, layers):
-    _check(oracle, backend_module, n, layers, 1.0, "structured_adjoint", 2000 + n)
+@pytest.mark.parametrize("schedule", ["stored", "inverse_walk"])
+@pytest.mark.parametrize("n,layers", [(13, 1), (13, 2), (14, 3), (16, 2), (21, 2), (22, 2),
+                                      (22, 4), (25, 2)])
+def test_structured_tile_passes(oracle, backend_module, n, layers, schedule):
+    _check(oracle, backend_module, n, layers, 1.0, "structured_adjoint", 2000 + n,
+           schedule=schedule)
+
+
+def test_schedule_selection(backend_module):
+    small = backend_module.RingIsingCudaBackend(10, 3, 1.0, "structured_adjoint", 8, False)
+    assert small.schedule == "inverse_walk"  # one fused kernel, nothing to store
+    eng = backend_module.RingIsingCudaBackend(20, 3, 1.0, "structured_adjoint", 8, False)
+    assert eng.schedule == "stored"  # fits easily
+    eng.set_schedule("inverse_walk")
+    assert eng.schedule == "inverse_walk"
+    with pytest.raises(ValueError):
+        eng.set_schedule("checkpoint")
 
 
 @pytest.mark.parametrize("width", [1, 2, 3, 4, 8])
@@ -122,6 +142,7 @@ def test_inverse_walk_drift_and_checkpoints(oracle, backend_module, n, layers):
     drift = eng.inverse_walk_drift(params.reshape(-1))
     assert 0.0 <= drift < 1e-11
     e_ref, g_ref = oracle.energy_and_grad(n, layers, 1.0, params)
+    eng.set_schedule("inverse_walk")  # checkpoints apply to the inverse walk
     eng.set_checkpoint_interval(2)
     out = eng.energy_and_grad(params.reshape(-1))
     assert abs(out["energy"] - e_ref) <= TOL
