@@ -52,6 +52,8 @@ def parse_args() -> argparse.Namespace:
     p.add_argument("--schedule", choices=["auto", "stored", "inverse_walk"], default="auto",
                    help="backward schedule (auto: stored when it fits in device memory)")
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--no-other-schedule", action="store_true",
+                   help="skip timing the other backward schedule")
     return p.parse_args()
 
 
@@ -141,6 +143,8 @@ def main() -> int:
     other = "inverse_walk" if engine.schedule == "stored" else "stored"
     other_ms = None
     try:
+        if args.no_other_schedule:
+            raise RuntimeError("skipped")
         engine.set_schedule(other)
         step()
         torch.cuda.synchronize()
@@ -149,7 +153,7 @@ def main() -> int:
             step()
         torch.cuda.synchronize()
         other_ms = (time.perf_counter() - t1) * 1e3 / args.steps
-    except (RuntimeError, MemoryError):
+    except (RuntimeError, MemoryError, ValueError):
         other_ms = None
 
     e_val = float(energy.item())
