@@ -37,7 +37,7 @@ if str(ROOT) not in sys.path:
     sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
-METRIC = "energy_and_grad evals/s, % HBM roofline"
+METRIC = "energy_and_grad evals/s, % HBM roofline; batched vs single-launch @|\u0394E|\u22641e-10"
 TRAFFIC_FILE = ROOT / "profiles" / "hbm_traffic_structured_30q2l.json"
 
 
@@ -54,6 +54,9 @@ def parse_args() -> argparse.Namespace:
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-other-schedule", action="store_true",
                    help="skip timing the other backward schedule")
+    p.add_argument("--no-batched", action="store_true",
+                   help="skip the batched uneven-mix measurement")
+    p.add_argument("--batch-count", type=int, default=2048)
     return p.parse_args()
 
 
@@ -92,6 +95,77 @@ def cpu_baseline(n_target: int, layers: int, budget_s: float = 20.0) -> dict:
         "sample": (f"oracle gate-level adjoint (FP64, OpenMP, {cores} threads) at {n} qubits x "
                    f"{layers} layers: median {med * 1e3:.1f} ms/eval over 5 runs after 1 warm-up, "
                    f"scaled to {n_target} qubits by 2^n * n (x{scale:.3g})"),
+    }
+
+
+def batched_mix(count: int, steps: int, torch) -> dict:
+    """Uneven mix of `count` circuits (n in 4..14, L in {8, 32, 128}, h in
+    U[0.2, 2]; n <= 8 on dense_scan): one persistent launch replayed from a
+    HIP graph vs one launch per circuit (same kernels) vs one engine call per
+    circuit (the reference's calling pattern), each an energy_and_grad of every
+    circuit."""
+    from gpu_simulation_amd import native
+    from gpu_simulation_amd.batch import RingIsingBatch, uneven_mix
+
+    ext = native.load_extension()
+    specs, params = uneven_mix(count, seed=3)
+    batch = RingIsingBatch(specs, dense_max_qubits=8)
+    batch.set_params(params)
+    t0 = time.perf_counter()
+    cfg = batch.autotune(repeats=2)
+    tune_s = time.perf_counter() - t0
+    stream = torch.cuda.current_stream().cuda_stream
+
+    def timed(fn, reps):
+        fn()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) / reps
+
+    batch.native.set_use_graph(True)
+    batched_s = timed(lambda: batch.native.run_device(True, stream), max(3, steps))
+    e_b, g_b = batch.results()
+    per_circuit_s = timed(lambda: batch.native.run_per_circuit(True, stream), 2)
+    e_p, g_p = batch.results()
+    engines = []
+    for s_ in specs:
+        strategy = "dense_scan" if s_.num_qubits <= 8 else "structured_adjoint"
+        engines.append(ext.RingIsingCudaBackend(s_.num_qubits, s_.layers, s_.field, strategy, 8,
+                                                False))
+    flat = [np.ascontiguousarray(p.reshape(-1)) for p in params]
+    outs = [None] * count
+
+    def engine_loop():
+        for i, eng in enumerate(engines):
+            outs[i] = eng.energy_and_grad(flat[i])
+
+    engine_s = timed(engine_loop, 1)
+    de = max(abs(float(outs[i]["energy"]) - float(e_b[i])) for i in range(count))
+    dg = max(float(np.max(np.abs(np.asarray(outs[i]["gradient"]) - g_b[i].reshape(-1))))
+             for i in range(count))
+    same = bool(np.array_equal(e_b, e_p) and all(np.array_equal(a, b) for a, b in zip(g_b, g_p)))
+    del engines
+    n_dense = sum(1 for s_ in specs if s_.num_qubits <= 8)
+    return {
+        "workload": (f"{count} circuits, n ~ U{{4..14}}, L ~ U{{8,32,128}}, h ~ U[0.2,2.0], "
+                     f"params U[0,2pi), rng default_rng([3, i]); {n_dense} on dense_scan (n <= 8), "
+                     f"{count - n_dense} on structured_adjoint"),
+        "batched_evals_per_s": round(count / batched_s, 1),
+        "batched_ms": round(batched_s * 1e3, 3),
+        "per_circuit_launch_evals_per_s": round(count / per_circuit_s, 1),
+        "per_circuit_launch_ms": round(per_circuit_s * 1e3, 3),
+        "engine_call_evals_per_s": round(count / engine_s, 1),
+        "engine_call_ms": round(engine_s * 1e3, 3),
+        "speedup_vs_per_circuit_launch": round(per_circuit_s / batched_s, 2),
+        "speedup_vs_engine_call": round(engine_s / batched_s, 2),
+        "max_abs_dE_vs_engine": de,
+        "max_abs_dgrad_vs_engine": dg,
+        "bitwise_equal_to_per_circuit_launch": same,
+        "autotune_s": round(tune_s, 2),
+        "config": {k: cfg[k] for k in ("threads", "grid", "dense_tile", "chunk_by_n")},
     }
 
 
@@ -225,6 +299,8 @@ def main() -> int:
         "kernels": kernels,
         "energy": e_val,
     }
+    if not args.no_batched:
+        result["batched"] = batched_mix(args.batch_count, args.steps, torch)
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(n, layers)
     print(json.dumps(result), flush=True)

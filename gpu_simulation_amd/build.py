@@ -40,7 +40,8 @@ HIP_FLAGS = [
     f"-I{INCLUDE}",
     f"-I{CSRC}",
 ]
-HIP_SOURCES = ["rie_kernels.hip", "rie_dense.hip", "rie_engine.cpp"]
+HIP_SOURCES = ["rie_kernels.hip", "rie_dense.hip", "rie_batch.hip", "rie_mlp.hip",
+               "rie_engine.cpp", "rie_batch.cpp", "rie_mlp.cpp"]
 
 
 def _run_parallel(commands: list[list[str]], verbose: bool) -> None:
@@ -68,7 +69,7 @@ def build(verbose: bool = False) -> None:
     objects = []
     commands = []
     for src in HIP_SOURCES:
-        obj = OBJ_DIR / (Path(src).stem + ".o")
+        obj = OBJ_DIR / (Path(src).name.replace(".", "_") + ".o")
         objects.append(str(obj))
         commands.append([HIPCC, *HIP_FLAGS, "-x", "hip", "-c", str(CSRC / src), "-o", str(obj)])
     _run_parallel(commands, verbose)

@@ -246,6 +246,241 @@ class RingIsingCudaBackend {
     size_t num_layers_;
 };
 
+// Many independent circuits evaluated together (C ABI rie_batch_*).
+class RingIsingBatch {
+  public:
+    RingIsingBatch(const std::vector<int> &num_qubits, const std::vector<int> &num_layers,
+                   const std::vector<double> &fields, int dense_max_qubits) {
+        if (num_qubits.size() != num_layers.size() || num_qubits.size() != fields.size()) {
+            throw py::value_error("num_qubits, layers and fields must have the same length.");
+        }
+        check(rie_batch_create(num_qubits.size(), num_qubits.data(), num_layers.data(),
+                               fields.data(), dense_max_qubits, &batch_));
+    }
+    ~RingIsingBatch() { rie_batch_destroy(batch_); }
+    RingIsingBatch(const RingIsingBatch &) = delete;
+    RingIsingBatch &operator=(const RingIsingBatch &) = delete;
+
+    size_t size() const { return rie_batch_size(batch_); }
+    size_t num_params() const { return rie_batch_num_params(batch_); }
+
+    py::array_t<uint64_t> param_offsets() const {
+        std::vector<size_t> off(size() + 1);
+        check(rie_batch_param_offsets(batch_, off.data(), off.size()));
+        py::array_t<uint64_t> out(off.size());
+        auto v = out.mutable_unchecked<1>();
+        for (size_t i = 0; i < off.size(); ++i) {
+            v(i) = off[i];
+        }
+        return out;
+    }
+
+    std::vector<std::string> strategies() const {
+        std::vector<int> st(size());
+        check(rie_batch_strategies(batch_, st.data(), st.size()));
+        std::vector<std::string> out;
+        for (int x : st) {
+            out.push_back(x == RIE_STRATEGY_DENSE_SCAN ? "dense_scan" : "structured_adjoint");
+        }
+        return out;
+    }
+
+    py::tuple energy_and_grad(const FlatArray &params, bool compute_gradient) {
+        const auto view = view_params(params);
+        if (params.ndim() != 1 || view.size != num_params()) {
+            throw py::value_error("params must be a flat array of num_params values.");
+        }
+        py::array_t<double> energies(size());
+        py::array_t<double> grad(compute_gradient ? num_params() : 0);
+        double *e = energies.mutable_data();
+        double *g = compute_gradient ? grad.mutable_data() : nullptr;
+        rie_status st;
+        {
+            py::gil_scoped_release release;
+            st = rie_batch_energy_and_grad(batch_, view.ptr, view.size, compute_gradient ? 1 : 0,
+                                           e, g);
+        }
+        check(st);
+        return py::make_tuple(energies, grad);
+    }
+
+    void set_params(const FlatArray &params) {
+        const auto view = view_params(params);
+        check(rie_batch_set_params(batch_, view.ptr, view.size));
+    }
+
+    py::array_t<double> get_params() {
+        py::array_t<double> out(num_params());
+        check(rie_batch_get_params(batch_, out.mutable_data(), num_params()));
+        return out;
+    }
+
+    py::tuple results() {
+        py::array_t<double> energies(size());
+        py::array_t<double> grad(num_params());
+        check(rie_batch_get_results(batch_, energies.mutable_data(), grad.mutable_data()));
+        return py::make_tuple(energies, grad);
+    }
+
+    py::dict device_buffers() {
+        double *p = nullptr;
+        double *e = nullptr;
+        double *g = nullptr;
+        check(rie_batch_device_buffers(batch_, &p, &e, &g));
+        py::dict d;
+        d["params"] = reinterpret_cast<uintptr_t>(p);
+        d["energies"] = reinterpret_cast<uintptr_t>(e);
+        d["gradient"] = reinterpret_cast<uintptr_t>(g);
+        return d;
+    }
+
+    void run_device(bool compute_gradient, uintptr_t stream) {
+        py::gil_scoped_release release;
+        check(rie_batch_run_device(batch_, compute_gradient ? 1 : 0,
+                                   reinterpret_cast<void *>(stream)));
+    }
+
+    void run_per_circuit(bool compute_gradient, uintptr_t stream) {
+        py::gil_scoped_release release;
+        check(rie_batch_run_per_circuit(batch_, compute_gradient ? 1 : 0,
+                                        reinterpret_cast<void *>(stream)));
+    }
+
+    void set_use_graph(bool use) { check(rie_batch_set_use_graph(batch_, use ? 1 : 0)); }
+
+    py::array_t<double> optimize(const std::string &optimizer, double lr, size_t steps,
+                                 bool use_graph, uintptr_t stream) {
+        int kind;
+        if (optimizer == "gd" || optimizer == "gradient_descent") {
+            kind = 0;
+        } else if (optimizer == "adam") {
+            kind = 1;
+        } else {
+            throw py::value_error("optimizer must be 'gd' or 'adam'.");
+        }
+        py::array_t<double> hist({steps, size()});
+        double *h = hist.mutable_data();
+        rie_status st;
+        {
+            py::gil_scoped_release release;
+            st = rie_batch_optimize(batch_, kind, lr, steps, use_graph ? 1 : 0, h,
+                                    reinterpret_cast<void *>(stream));
+        }
+        check(st);
+        return hist;
+    }
+
+    py::object autotune(size_t repeats) {
+        {
+            py::gil_scoped_release release;
+            check(rie_batch_autotune(batch_, repeats));
+        }
+        return config();
+    }
+
+    py::object config() const {
+        size_t need = 0;
+        check(rie_batch_config(batch_, nullptr, 0, &need));
+        std::string buf(need, '\0');
+        check(rie_batch_config(batch_, &buf[0], buf.size(), nullptr));
+        buf.resize(std::strlen(buf.c_str()));
+        return py::module_::import("json").attr("loads")(buf);
+    }
+
+    void set_config(int threads, py::object chunk_by_n, int dense_tile, int structured_grid) {
+        std::vector<int> chunks;
+        const int *cp = nullptr;
+        if (!chunk_by_n.is_none()) {
+            chunks = chunk_by_n.cast<std::vector<int>>();
+            cp = chunks.data();
+        }
+        check(rie_batch_set_config(batch_, threads, cp, chunks.size(), dense_tile,
+                                   structured_grid));
+    }
+
+  private:
+    rie_batch *batch_ = nullptr;
+};
+
+// FP32 warm-start perceptron (C ABI rie_mlp_*).
+class WarmStartMLP {
+  public:
+    using F32Array = py::array_t<float, py::array::c_style | py::array::forcecast>;
+
+    WarmStartMLP(int inputs, int hidden, int outputs, uint64_t seed)
+        : inputs_(inputs), outputs_(outputs) {
+        check(rie_mlp_create(inputs, hidden, outputs, seed, &mlp_));
+    }
+    ~WarmStartMLP() { rie_mlp_destroy(mlp_); }
+    WarmStartMLP(const WarmStartMLP &) = delete;
+    WarmStartMLP &operator=(const WarmStartMLP &) = delete;
+
+    size_t num_params() const { return rie_mlp_num_params(mlp_); }
+
+    py::array_t<float> get_params() {
+        py::array_t<float> out(num_params());
+        check(rie_mlp_get_params(mlp_, out.mutable_data(), num_params()));
+        return out;
+    }
+
+    void set_params(const F32Array &p) {
+        check(rie_mlp_set_params(mlp_, p.data(), (size_t)p.size()));
+    }
+
+    int rows_of(const F32Array &x, int width) const {
+        if (x.ndim() != 2 || x.shape(1) != width) {
+            throw py::value_error("expected a 2-D array with " + std::to_string(width) +
+                                  " columns.");
+        }
+        return (int)x.shape(0);
+    }
+
+    py::array_t<float> forward(const F32Array &x) {
+        const int rows = rows_of(x, inputs_);
+        py::array_t<float> y({(py::ssize_t)rows, (py::ssize_t)outputs_});
+        float *yp = y.mutable_data();
+        rie_status st;
+        {
+            py::gil_scoped_release release;
+            st = rie_mlp_forward(mlp_, x.data(), rows, yp);
+        }
+        check(st);
+        return y;
+    }
+
+    py::tuple loss_and_grad(const F32Array &x, const F32Array &t) {
+        const int rows = rows_of(x, inputs_);
+        if (rows_of(t, outputs_) != rows) {
+            throw py::value_error("inputs and targets need the same number of rows.");
+        }
+        float loss = 0.0f;
+        py::array_t<float> g(num_params());
+        check(rie_mlp_loss_and_grad(mlp_, x.data(), t.data(), rows, &loss, g.mutable_data()));
+        return py::make_tuple(loss, g);
+    }
+
+    py::array_t<float> train(const F32Array &x, const F32Array &t, int steps, float lr) {
+        const int rows = rows_of(x, inputs_);
+        if (rows_of(t, outputs_) != rows) {
+            throw py::value_error("inputs and targets need the same number of rows.");
+        }
+        py::array_t<float> hist(steps > 0 ? steps : 0);
+        float *hp = steps > 0 ? hist.mutable_data() : nullptr;
+        rie_status st;
+        {
+            py::gil_scoped_release release;
+            st = rie_mlp_train(mlp_, x.data(), t.data(), rows, steps, lr, hp);
+        }
+        check(st);
+        return hist;
+    }
+
+  private:
+    rie_mlp *mlp_ = nullptr;
+    int inputs_;
+    int outputs_;
+};
+
 }  // namespace
 
 PYBIND11_MODULE(_cuda_backend, m) {
@@ -303,6 +538,45 @@ PYBIND11_MODULE(_cuda_backend, m) {
         py::arg("gradient_strategy") = "structured_adjoint", py::arg("params"),
         py::arg("compute_gradient") = true, py::arg("profile") = false,
         py::arg("structured_rotation_chunk_width") = 8);
+
+    py::class_<RingIsingBatch>(m, "RingIsingBatch")
+        .def(py::init<const std::vector<int> &, const std::vector<int> &,
+                      const std::vector<double> &, int>(),
+             py::arg("num_qubits"), py::arg("num_layers"), py::arg("fields"),
+             py::arg("dense_max_qubits") = 8)
+        .def_property_readonly("size", &RingIsingBatch::size)
+        .def_property_readonly("num_params", &RingIsingBatch::num_params)
+        .def("param_offsets", &RingIsingBatch::param_offsets)
+        .def("strategies", &RingIsingBatch::strategies)
+        .def("energy_and_grad", &RingIsingBatch::energy_and_grad, py::arg("params"),
+             py::arg("compute_gradient") = true)
+        .def("set_params", &RingIsingBatch::set_params, py::arg("params"))
+        .def("get_params", &RingIsingBatch::get_params)
+        .def("results", &RingIsingBatch::results)
+        .def("device_buffers", &RingIsingBatch::device_buffers)
+        .def("run_device", &RingIsingBatch::run_device, py::arg("compute_gradient") = true,
+             py::arg("stream") = 0)
+        .def("run_per_circuit", &RingIsingBatch::run_per_circuit,
+             py::arg("compute_gradient") = true, py::arg("stream") = 0)
+        .def("set_use_graph", &RingIsingBatch::set_use_graph, py::arg("use_graph"))
+        .def("optimize", &RingIsingBatch::optimize, py::arg("optimizer"), py::arg("lr"),
+             py::arg("steps"), py::arg("use_graph") = true, py::arg("stream") = 0)
+        .def("autotune", &RingIsingBatch::autotune, py::arg("repeats") = 3)
+        .def("config", &RingIsingBatch::config)
+        .def("set_config", &RingIsingBatch::set_config, py::arg("threads"),
+             py::arg("chunk_by_n") = py::none(), py::arg("dense_tile") = 64,
+             py::arg("structured_grid") = 0);
+
+    py::class_<WarmStartMLP>(m, "WarmStartMLP")
+        .def(py::init<int, int, int, uint64_t>(), py::arg("inputs"), py::arg("hidden"),
+             py::arg("outputs"), py::arg("seed") = 0)
+        .def_property_readonly("num_params", &WarmStartMLP::num_params)
+        .def("get_params", &WarmStartMLP::get_params)
+        .def("set_params", &WarmStartMLP::set_params, py::arg("params"))
+        .def("forward", &WarmStartMLP::forward, py::arg("inputs"))
+        .def("loss_and_grad", &WarmStartMLP::loss_and_grad, py::arg("inputs"), py::arg("targets"))
+        .def("train", &WarmStartMLP::train, py::arg("inputs"), py::arg("targets"),
+             py::arg("steps"), py::arg("lr"));
 
     m.def("device_count", &rie_device_count);
     m.def("abi_version", &rie_abi_version);

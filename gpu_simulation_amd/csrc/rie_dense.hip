@@ -19,33 +19,48 @@
 //      dE/dtheta_{l,w} = Re<y_l | -i Y_w | x_l>
 //      dE/dphi_{l,w}   = Re<C^-1 y_{l+1} | -i Z_w | C^-1 x_{l+1}>
 //    with x_L = psi and y_L = H psi.
+//
+// Every kernel is driven by a job table (DenseJob) and a work table of
+// (job, first block) pairs, so one launch serves one circuit or a whole
+// batch of circuits of different sizes: the product-tree level j of every
+// circuit deep enough to have one is one GEMM launch per tile class.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <vector>
 
+#include "rie_dense.hpp"
 #include "rie_engine_internal.hpp"
 
 namespace rie {
 
 typedef double double4_t __attribute__((ext_vector_type(4)));
 
-struct DenseState {
-    int D = 0;         // padded dimension
-    int P = 0;         // leaves (power of two >= L)
-    int levels = 0;    // log2(P)
-    double *tree = nullptr;   // (2P - 1) nodes, each 2 * D * D doubles (re plane, im plane)
-    double2 *vec_a = nullptr; // P * 2 * D incoming vectors (ping)
-    double2 *vec_b = nullptr; // (pong)
-    double2 *head = nullptr;  // psi, lambda, eta: 3 * D
-    double *epart = nullptr;  // energy scratch
-    size_t node_doubles = 0;
-};
+// Entry of a launch's work table whose block range holds block b:
+// tab[k].x = job index, tab[k].y = first block of that job in the launch.
+__device__ __forceinline__ int find_item(const int2 *__restrict__ tab, int count, int b) {
+    int lo = 0;
+    int hi = count - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (tab[mid].y <= b) {
+            lo = mid;
+        } else {
+            hi = mid - 1;
+        }
+    }
+    return lo;
+}
+
+__device__ __forceinline__ size_t level_node(int P, int level) {
+    return (size_t)(2 * P - 2 * (P >> level));
+}
 
 // ---------------------------------------------------------------------------
 // Layer matrices: M_l[r][c] = R_l[P^-1(r)][c], R_l = (x)_w U_w with
 // U = RZ(phi) RY(theta) = [[c e-, -s e-], [s e+, c e+]], e-+ = e^{-+i phi/2}
 // (reference: fill_ryrz_local_u_entries_device, cpp/ising_cuda_dense_kernels.cu:43-55).
+// Leaves l >= L and the padding rows / columns are identity.
 // ---------------------------------------------------------------------------
 
 __device__ __forceinline__ double2 u_entry(const WireCoef &k, int rb, int cb) {
@@ -55,60 +70,74 @@ __device__ __forceinline__ double2 u_entry(const WireCoef &k, int rb, int cb) {
     return make_double2(mag * k.ch, mag * ph);
 }
 
-__global__ void dense_layers_kernel(const WireCoef *__restrict__ coef, int n, int L, int D,
-                                    double *__restrict__ tree, size_t node_doubles) {
-    const int leaf = blockIdx.x / D;
-    const int r = blockIdx.x % D;
-    const int c = threadIdx.x;
+__global__ __launch_bounds__(256) void dense_layers_kernel(const DenseJob *__restrict__ jobs,
+                                                           const int2 *__restrict__ tab,
+                                                           int count) {
+    const int k = find_item(tab, count, blockIdx.x);
+    const DenseJob &J = jobs[tab[k].x];
+    const int D = J.D;
+    const size_t DD = (size_t)D * D;
+    const size_t e = (size_t)(blockIdx.x - tab[k].y) * 256 + threadIdx.x;
+    if (e >= (size_t)J.P * DD) {
+        return;
+    }
+    const int leaf = (int)(e / DD);
+    const int r = (int)((e % DD) / D);
+    const int c = (int)(e % D);
+    const int n = J.n;
     const int d = 1 << n;
-    double *re = tree + (size_t)leaf * node_doubles;
-    double *im = re + (size_t)D * D;
     double2 v;
-    if (leaf >= L || r >= d || c >= d) {
+    if (leaf >= J.L || r >= d || c >= d) {
         v = make_double2((r == c) ? 1.0 : 0.0, 0.0);
     } else {
         const uint32_t src = ring_inv32((uint32_t)r, n);  // (C R)[r] = R[P^-1 r]
         v = make_double2(1.0, 0.0);
-        const WireCoef *lc = coef + (size_t)leaf * n;
+        const WireCoef *lc = J.coef + (size_t)leaf * n;
         for (int w = 0; w < n; ++w) {
             v = cmul(v, u_entry(lc[w], (src >> w) & 1, (c >> w) & 1));
         }
     }
+    double *re = J.tree + (size_t)leaf * J.node_doubles;
     re[(size_t)r * D + c] = v.x;
-    im[(size_t)r * D + c] = v.y;
+    re[DD + (size_t)r * D + c] = v.y;
 }
 
 // ---------------------------------------------------------------------------
-// Batched split-complex GEMM on FP64 MFMA: C_b = A_b * B_b, D x D row-major.
+// Product-tree level on FP64 MFMA: parent q = right child (2q+1) * left child
+// (2q), D x D row-major split-complex, one TILE x TILE output tile per block.
 // v_mfma_f64_16x16x4f64 lane layout: A[i][k] in lane i + 16k, B[k][j] in
 // lane j + 16k, D[i][j] in lane j + 16(i%4), element i/4 (one output row per
 // 16-lane group and register, unlike the FP32 16x16 forms).
 // ---------------------------------------------------------------------------
 
 template <int TILE>
-__global__ __launch_bounds__(256) void zgemm_mfma_kernel(const double *__restrict__ A,
-                                                         const double *__restrict__ B,
-                                                         double *__restrict__ C, int D,
-                                                         size_t stride_a, size_t stride_b,
-                                                         size_t stride_c) {
+__global__ __launch_bounds__(256) void zgemm_level_kernel(const DenseJob *__restrict__ jobs,
+                                                          const int2 *__restrict__ tab, int count,
+                                                          int level) {
     constexpr int KB = 16;
-    constexpr int SUB = TILE / 16;                    // 16x16 sub-tiles per side
-    constexpr int PER_WAVE = (SUB * SUB + 3) / 4;     // sub-tiles per wave
+    constexpr int SUB = TILE / 16;                 // 16x16 sub-tiles per side
+    constexpr int PER_WAVE = (SUB * SUB + 3) / 4;  // sub-tiles per wave
     __shared__ double as_r[TILE][KB + 1];
     __shared__ double as_i[TILE][KB + 1];
     __shared__ double bs_r[KB][TILE];
     __shared__ double bs_i[KB][TILE];
+    const int item = find_item(tab, count, blockIdx.x);
+    const DenseJob &J = jobs[tab[item].x];
+    const int D = J.D;
+    const int local = blockIdx.x - tab[item].y;
     const int tiles_per_side = D / TILE;
-    const int bt = blockIdx.x;
-    const int batch = blockIdx.y;
+    const int q = local / (tiles_per_side * tiles_per_side);
+    const int bt = local % (tiles_per_side * tiles_per_side);
     const int tr = (bt / tiles_per_side) * TILE;
     const int tc = (bt % tiles_per_side) * TILE;
     const size_t DD = (size_t)D * D;
-    const double *ar = A + batch * stride_a;
+    const size_t nd = J.node_doubles;
+    const double *left = J.tree + (level_node(J.P, level - 1) + 2 * (size_t)q) * nd;
+    const double *ar = left + nd;  // right child
     const double *ai = ar + DD;
-    const double *br = B + batch * stride_b;
+    const double *br = left;
     const double *bi = br + DD;
-    double *cr = C + batch * stride_c;
+    double *cr = J.tree + (level_node(J.P, level) + (size_t)q) * nd;
     double *ci = cr + DD;
     const int t = threadIdx.x;
     const int lane = t & 63;
@@ -172,36 +201,26 @@ __global__ __launch_bounds__(256) void zgemm_mfma_kernel(const double *__restric
     }
 }
 
-static void launch_zgemm(const double *A, const double *B, double *C, int D, int batch,
-                         size_t sa, size_t sb, size_t sc, hipStream_t s) {
-    if (D >= 64) {
-        dim3 grid((D / 64) * (D / 64), batch);
-        zgemm_mfma_kernel<64><<<grid, 256, 0, s>>>(A, B, C, D, sa, sb, sc);
-    } else if (D == 32) {
-        dim3 grid(1, batch);
-        zgemm_mfma_kernel<32><<<grid, 256, 0, s>>>(A, B, C, D, sa, sb, sc);
-    } else {
-        dim3 grid(1, batch);
-        zgemm_mfma_kernel<16><<<grid, 256, 0, s>>>(A, B, C, D, sa, sb, sc);
-    }
-}
-
 // ---------------------------------------------------------------------------
-// Root: psi = T e0, lambda = H psi, E = Re<psi|lambda>, eta = T^dagger lambda.
+// Root: psi = T e0, lambda = H psi, E = Re<psi|lambda>, eta = T^dagger lambda;
+// also seeds the down-sweep with the root pair [x = e0 | y = eta] in vec_a.
+// One block per circuit.
 // ---------------------------------------------------------------------------
 
-__global__ __launch_bounds__(256) void dense_head_kernel(const double *__restrict__ root, int n,
-                                                         int D, double field,
-                                                         double2 *__restrict__ head,
-                                                         double *__restrict__ energy) {
+__global__ __launch_bounds__(256) void dense_head_kernel(const DenseJob *__restrict__ jobs,
+                                                         const int2 *__restrict__ tab,
+                                                         double *energy_override) {
     __shared__ double2 psi[256];
     __shared__ double2 lam[256];
     __shared__ double red[4];
+    const DenseJob &J = jobs[tab[blockIdx.x].x];
     const int t = threadIdx.x;
+    const int n = J.n;
+    const int D = J.D;
     const int d = 1 << n;
     const size_t DD = (size_t)D * D;
-    const double *re = root;
-    const double *im = root + DD;
+    const double *re = J.tree + (size_t)(2 * J.P - 2) * J.node_doubles;
+    const double *im = re + DD;
     if (t < D) {
         psi[t] = (t < d) ? make_double2(re[(size_t)t * D], im[(size_t)t * D]) : make_double2(0, 0);
     }
@@ -219,7 +238,7 @@ __global__ __launch_bounds__(256) void dense_head_kernel(const double *__restric
                 fx += p.x;
                 fy += p.y;
             }
-            hv = make_double2(dg * v.x - field * fx, dg * v.y - field * fy);
+            hv = make_double2(dg * v.x - J.field * fx, dg * v.y - J.field * fy);
             e = v.x * hv.x + v.y * hv.y;
         }
         lam[t] = hv;
@@ -230,7 +249,8 @@ __global__ __launch_bounds__(256) void dense_head_kernel(const double *__restric
     }
     __syncthreads();
     if (t == 0) {
-        *energy = red[0] + red[1] + red[2] + red[3];
+        double *out = energy_override != nullptr ? energy_override : J.energy;
+        *out = red[0] + red[1] + red[2] + red[3];
     }
     if (t < D) {
         // eta_c = sum_r conj(T[r][c]) lambda_r
@@ -243,27 +263,35 @@ __global__ __launch_bounds__(256) void dense_head_kernel(const double *__restric
             ex += tr * l.x + ti * l.y;
             ey += tr * l.y - ti * l.x;
         }
-        head[t] = psi[t];
-        head[D + t] = lam[t];
-        head[2 * D + t] = make_double2(ex, ey);
+        const double2 eta = make_double2(ex, ey);
+        J.head[t] = psi[t];
+        J.head[D + t] = lam[t];
+        J.head[2 * D + t] = eta;
+        J.vec_a[t] = make_double2(t == 0 ? 1.0 : 0.0, 0.0);
+        J.vec_a[D + t] = eta;
     }
 }
 
-// One down-sweep level: parent node i (level j) hands (x, y) to its left
-// child and (N x, N y) to its right child, N = left child's product.
-__global__ __launch_bounds__(256) void dense_down_kernel(const double *__restrict__ child_level,
-                                                         size_t node_doubles, int D,
-                                                         const double2 *__restrict__ vin,
-                                                         double2 *__restrict__ vout,
-                                                         int parents, const double2 *__restrict__ seed) {
-    const int p = blockIdx.x;
+// One down-sweep level: parent p hands (x, y) to its left child and
+// (N x, N y) to its right child, N = the left child's product.  A circuit's
+// step at level j reads vec_a when (levels - j) is even, else vec_b.
+__global__ __launch_bounds__(256) void dense_down_kernel(const DenseJob *__restrict__ jobs,
+                                                         const int2 *__restrict__ tab, int count,
+                                                         int level) {
+    const int item = find_item(tab, count, blockIdx.x);
+    const DenseJob &J = jobs[tab[item].x];
+    const int p = blockIdx.x - tab[item].y;
     const int r = threadIdx.x;
-    if (p >= parents || r >= D) {
+    const int D = J.D;
+    if (r >= D) {
         return;
     }
-    const double2 *x = seed != nullptr ? seed : vin + (size_t)p * 2 * D;
-    const double2 *y = seed != nullptr ? seed + 2 * D : x + D;  // seed: [e0 | ... | eta]
-    const double *re = child_level + (size_t)(2 * p) * node_doubles;
+    const bool even = ((J.levels - level) & 1) == 0;
+    const double2 *vin = even ? J.vec_a : J.vec_b;
+    double2 *vout = even ? J.vec_b : J.vec_a;
+    const double2 *x = vin + (size_t)p * 2 * D;
+    const double2 *y = x + D;
+    const double *re = J.tree + (level_node(J.P, level - 1) + 2 * (size_t)p) * J.node_doubles;
     const double *im = re + (size_t)D * D;
     double2 nx = make_double2(0.0, 0.0);
     double2 ny = make_double2(0.0, 0.0);
@@ -285,31 +313,28 @@ __global__ __launch_bounds__(256) void dense_down_kernel(const double *__restric
     right[D + r] = ny;
 }
 
-// e0 and eta as a seed pair for the root: [e0 (D) | psi (D, unused) | eta (D)] is
-// laid out so that seed[0..D) = e0 and seed[2D..3D) = eta.
-__global__ void dense_seed_kernel(double2 *__restrict__ seed, const double2 *__restrict__ head,
-                                  int D) {
-    const int t = threadIdx.x;
-    if (t < D) {
-        seed[t] = make_double2(t == 0 ? 1.0 : 0.0, 0.0);
-        seed[D + t] = make_double2(0.0, 0.0);
-        seed[2 * D + t] = head[2 * D + t];
-    }
-}
-
-// Gradients of layer l from (x_l, y_l) and (x_{l+1}, y_{l+1}).
-__global__ __launch_bounds__(256) void dense_tail_kernel(const double2 *__restrict__ leaves,
-                                                         const double2 *__restrict__ head, int n,
-                                                         int L, int D, double *__restrict__ grad,
-                                                         const double *__restrict__ scale) {
+// Gradients of layer l from (x_l, y_l) and (x_{l+1}, y_{l+1}).  One block per
+// (circuit, layer).
+__global__ __launch_bounds__(256) void dense_tail_kernel(const DenseJob *__restrict__ jobs,
+                                                         const int2 *__restrict__ tab, int count,
+                                                         double *grad_override,
+                                                         const double *scale_override) {
     __shared__ double red[4][2 * 8];
-    const int l = blockIdx.x;
+    const int item = find_item(tab, count, blockIdx.x);
+    const DenseJob &J = jobs[tab[item].x];
+    const int l = blockIdx.x - tab[item].y;
     const int t = threadIdx.x;
+    const int n = J.n;
+    const int L = J.L;
+    const int D = J.D;
     const int d = 1 << n;
+    // the last down-sweep step (levels - 1) wrote vec_b when levels is odd
+    const double2 *leaves =
+        (J.levels == 0) ? J.vec_a : ((((J.levels - 1) & 1) == 0) ? J.vec_b : J.vec_a);
     const double2 *x = leaves + (size_t)l * 2 * D;
     const double2 *y = x + D;
-    const double2 *xn = (l + 1 < L) ? leaves + (size_t)(l + 1) * 2 * D : head;      // psi
-    const double2 *yn = (l + 1 < L) ? xn + D : head + D;                              // lambda
+    const double2 *xn = (l + 1 < L) ? leaves + (size_t)(l + 1) * 2 * D : J.head;  // psi
+    const double2 *yn = (l + 1 < L) ? xn + D : J.head + D;                          // lambda
     double dt[8];
     double dp[8];
     for (int w = 0; w < 8; ++w) {
@@ -327,8 +352,8 @@ __global__ __launch_bounds__(256) void dense_tail_kernel(const double2 *__restri
             const int bit = (t >> w) & 1;
             const double2 xp = x[t ^ (1 << w)];
             // (-iY x)[t] = -x[t^w] for bit 0, +x[t^w] for bit 1
-            const double re = yv.x * xp.x + yv.y * xp.y;
-            dt[w] = bit ? re : -re;
+            const double rr = yv.x * xp.x + yv.y * xp.y;
+            dt[w] = bit ? rr : -rr;
             dp[w] = bit ? -imba : imba;
         }
     }
@@ -344,97 +369,237 @@ __global__ __launch_bounds__(256) void dense_tail_kernel(const double2 *__restri
     }
     __syncthreads();
     if (t < 2 * n) {
-        const double f = scale != nullptr ? *scale : 1.0;
-        grad[(size_t)l * n * 2 + t] = (red[0][t] + red[1][t] + red[2][t] + red[3][t]) * f;
+        const double *sp = scale_override != nullptr ? scale_override : J.scale;
+        const double f = sp != nullptr ? *sp : 1.0;
+        double *g = grad_override != nullptr ? grad_override : J.grad;
+        g[(size_t)l * n * 2 + t] = (red[0][t] + red[1][t] + red[2][t] + red[3][t]) * f;
     }
 }
 
 // ---------------------------------------------------------------------------
-// Engine glue
+// Plans
+// ---------------------------------------------------------------------------
+
+namespace {
+
+struct Geometry {
+    int D;
+    int P;
+    int levels;
+    size_t nd;
+    size_t tree_bytes;
+    size_t vec_bytes;
+    size_t head_bytes;
+};
+
+Geometry geometry(int n, int L) {
+    Geometry g{};
+    g.D = std::max(1 << n, 16);
+    g.P = 1;
+    g.levels = 0;
+    while (g.P < L) {
+        g.P <<= 1;
+        g.levels++;
+    }
+    g.nd = (size_t)2 * g.D * g.D;
+    g.tree_bytes = sizeof(double) * g.nd * (2 * (size_t)g.P - 1);
+    g.vec_bytes = sizeof(double2) * (size_t)g.P * 2 * g.D;
+    g.head_bytes = sizeof(double2) * 3 * (size_t)g.D;
+    return g;
+}
+
+size_t align256(size_t b) { return (b + 255) & ~size_t{255}; }
+
+int tile_for(int D, int max_tile) {
+    if (D >= 64) {
+        return max_tile;
+    }
+    return D;  // 16 or 32
+}
+
+}  // namespace
+
+size_t DensePlan::bytes_for(int n, int L) {
+    const Geometry g = geometry(n, L);
+    return align256(g.tree_bytes) + 2 * align256(g.vec_bytes) + align256(g.head_bytes);
+}
+
+void DensePlan::release() {
+    device_free(d_jobs_);
+    device_free(d_tab_);
+    device_free(arena_);
+    d_jobs_ = nullptr;
+    d_tab_ = nullptr;
+    arena_ = nullptr;
+    arena_bytes_ = 0;
+    jobs_.clear();
+    gemm_.clear();
+    down_.clear();
+}
+
+DensePlan::~DensePlan() { release(); }
+
+void DensePlan::build(const std::vector<DenseSpec> &specs, int max_tile) {
+    release();
+    if (max_tile != 32 && max_tile != 64) {
+        fail(RIE_INVALID_ARGUMENT, "dense tile must be 32 or 64");
+    }
+    if (specs.empty()) {
+        return;
+    }
+    size_t total = 0;
+    for (const auto &sp : specs) {
+        if (sp.n < 1 || sp.n > 8 || sp.L < 1) {
+            fail(RIE_INVALID_ARGUMENT, "dense_scan requires 1 <= num_qubits <= 8 and layers >= 1.");
+        }
+        total += bytes_for(sp.n, sp.L);
+    }
+    arena_ = device_alloc(total, "alloc dense_scan buffers");
+    arena_bytes_ = total;
+    char *cursor = (char *)arena_;
+    int max_levels = 0;
+    for (const auto &sp : specs) {
+        const Geometry g = geometry(sp.n, sp.L);
+        DenseJob J{};
+        J.coef = sp.coef;
+        J.tree = (double *)cursor;
+        cursor += align256(g.tree_bytes);
+        J.vec_a = (double2 *)cursor;
+        cursor += align256(g.vec_bytes);
+        J.vec_b = (double2 *)cursor;
+        cursor += align256(g.vec_bytes);
+        J.head = (double2 *)cursor;
+        cursor += align256(g.head_bytes);
+        J.energy = sp.energy;
+        J.grad = sp.grad;
+        J.scale = sp.scale;
+        J.node_doubles = g.nd;
+        J.field = sp.field;
+        J.n = sp.n;
+        J.L = sp.L;
+        J.D = g.D;
+        J.P = g.P;
+        J.levels = g.levels;
+        jobs_.push_back(J);
+        max_levels = std::max(max_levels, g.levels);
+    }
+
+    std::vector<int2> tab;
+    auto add_launch = [&](Launch &la, auto &&blocks_of) {
+        la.tab_off = tab.size();
+        la.blocks = 0;
+        la.count = 0;
+        for (int j = 0; j < (int)jobs_.size(); ++j) {
+            const long b = blocks_of(jobs_[(size_t)j]);
+            if (b <= 0) {
+                continue;
+            }
+            if ((long)la.blocks + b > 0x7fffffffL) {
+                fail(RIE_INVALID_ARGUMENT, "dense_scan batch too large for one launch");
+            }
+            tab.push_back(make_int2(j, la.blocks));
+            la.blocks += (int)b;
+            la.count += 1;
+        }
+    };
+    add_launch(layers_, [](const DenseJob &J) {
+        return (long)(((size_t)J.P * J.D * J.D + 255) / 256);
+    });
+    for (int level = 1; level <= max_levels; ++level) {
+        for (int tile : {16, 32, 64}) {
+            Launch la;
+            la.level = level;
+            la.tile = tile;
+            add_launch(la, [&](const DenseJob &J) {
+                if (J.levels < level || tile_for(J.D, max_tile) != tile) {
+                    return 0L;
+                }
+                const long side = J.D / tile;
+                return (long)(J.P >> level) * side * side;
+            });
+            if (la.count > 0) {
+                gemm_.push_back(la);
+            }
+        }
+    }
+    add_launch(head_, [](const DenseJob &) { return 1L; });
+    for (int level = max_levels; level >= 1; --level) {
+        Launch la;
+        la.level = level;
+        add_launch(la, [&](const DenseJob &J) {
+            return J.levels >= level ? (long)(J.P >> level) : 0L;
+        });
+        if (la.count > 0) {
+            down_.push_back(la);
+        }
+    }
+    add_launch(tail_, [](const DenseJob &J) { return (long)J.L; });
+
+    d_jobs_ = (DenseJob *)device_alloc(sizeof(DenseJob) * jobs_.size(), "alloc dense jobs");
+    d_tab_ = (int2 *)device_alloc(sizeof(int2) * tab.size(), "alloc dense work tables");
+    check(hipMemcpy(d_jobs_, jobs_.data(), sizeof(DenseJob) * jobs_.size(),
+                    hipMemcpyHostToDevice),
+          "upload dense jobs");
+    check(hipMemcpy(d_tab_, tab.data(), sizeof(int2) * tab.size(), hipMemcpyHostToDevice),
+          "upload dense work tables");
+}
+
+void DensePlan::run(hipStream_t s, bool want_grad, double *energy_override, double *grad_override,
+                    const double *scale_override) const {
+    if (jobs_.empty()) {
+        return;
+    }
+    if ((energy_override != nullptr || grad_override != nullptr || scale_override != nullptr) &&
+        jobs_.size() != 1) {
+        fail(RIE_DEVICE_ERROR, "internal: dense overrides need a single-circuit plan");
+    }
+    dense_layers_kernel<<<layers_.blocks, 256, 0, s>>>(d_jobs_, d_tab_ + layers_.tab_off,
+                                                       layers_.count);
+    check_launch("dense_layers");
+    for (const Launch &la : gemm_) {
+        const int2 *tab = d_tab_ + la.tab_off;
+        if (la.tile == 64) {
+            zgemm_level_kernel<64><<<la.blocks, 256, 0, s>>>(d_jobs_, tab, la.count, la.level);
+        } else if (la.tile == 32) {
+            zgemm_level_kernel<32><<<la.blocks, 256, 0, s>>>(d_jobs_, tab, la.count, la.level);
+        } else {
+            zgemm_level_kernel<16><<<la.blocks, 256, 0, s>>>(d_jobs_, tab, la.count, la.level);
+        }
+        check_launch("zgemm_level");
+    }
+    dense_head_kernel<<<head_.blocks, 256, 0, s>>>(d_jobs_, d_tab_ + head_.tab_off, energy_override);
+    check_launch("dense_head");
+    if (!want_grad) {
+        return;
+    }
+    for (const Launch &la : down_) {
+        dense_down_kernel<<<la.blocks, 256, 0, s>>>(d_jobs_, d_tab_ + la.tab_off, la.count,
+                                                    la.level);
+        check_launch("dense_down");
+    }
+    dense_tail_kernel<<<tail_.blocks, 256, 0, s>>>(d_jobs_, d_tab_ + tail_.tab_off, tail_.count,
+                                                   grad_override, scale_override);
+    check_launch("dense_tail");
+}
+
+// ---------------------------------------------------------------------------
+// Engine glue (one circuit)
 // ---------------------------------------------------------------------------
 
 void Engine::dense_init() {
-    dense = new DenseState();
-    DenseState &ds = *dense;
-    ds.D = std::max(1 << n, 16);
-    ds.P = 1;
-    while (ds.P < L) {
-        ds.P <<= 1;
-    }
-    ds.levels = 0;
-    while ((1 << ds.levels) < ds.P) {
-        ds.levels++;
-    }
-    ds.node_doubles = (size_t)2 * ds.D * ds.D;
-    ds.tree = (double *)device_alloc(sizeof(double) * ds.node_doubles * (2 * (size_t)ds.P - 1),
-                                     "alloc dense tree");
-    ds.vec_a = (double2 *)device_alloc(sizeof(double2) * (size_t)ds.P * 2 * ds.D, "alloc dense vec");
-    ds.vec_b = (double2 *)device_alloc(sizeof(double2) * (size_t)ds.P * 2 * ds.D, "alloc dense vec");
-    ds.head = (double2 *)device_alloc(sizeof(double2) * 3 * ds.D, "alloc dense head");
+    dense = new DensePlan();
+    DenseSpec sp{n, L, h, d_coef, d_energy, d_grad, nullptr};
+    dense->build({sp}, 64);
 }
 
 void Engine::dense_free() {
-    if (dense == nullptr) {
-        return;
-    }
-    device_free(dense->tree);
-    device_free(dense->vec_a);
-    device_free(dense->vec_b);
-    device_free(dense->head);
     delete dense;
     dense = nullptr;
 }
 
 void Engine::dense_run(hipStream_t s, bool want_grad, const double *scale, double *energy_out,
                        double *grad_out) {
-    DenseState &ds = *dense;
-    const int D = ds.D;
-    // leaves: level 0 nodes [0, P)
-    dense_layers_kernel<<<ds.P * D, D, 0, s>>>(d_coef, n, L, D, ds.tree, ds.node_doubles);
-    check_launch("dense_layers");
-    // up-sweep: level j nodes follow level j-1; parent = right * left
-    size_t level_off = 0;
-    int count = ds.P;
-    std::vector<size_t> offsets;
-    offsets.push_back(0);
-    while (count > 1) {
-        const double *level = ds.tree + level_off * ds.node_doubles;
-        double *parent = ds.tree + (level_off + count) * ds.node_doubles;
-        launch_zgemm(level + ds.node_doubles, level, parent, D, count / 2, 2 * ds.node_doubles,
-                     2 * ds.node_doubles, ds.node_doubles, s);
-        check_launch("zgemm_mfma");
-        level_off += count;
-        count /= 2;
-        offsets.push_back(level_off);
-    }
-    const double *root = ds.tree + level_off * ds.node_doubles;
-    dense_head_kernel<<<1, 256, 0, s>>>(root, n, D, h, ds.head, energy_out);
-    check_launch("dense_head");
-    if (!want_grad) {
-        return;
-    }
-    // down-sweep from the root (level `levels`) to the leaves (level 0)
-    double2 *vin = ds.vec_a;
-    double2 *vout = ds.vec_b;
-    if (ds.levels == 0) {
-        dense_seed_kernel<<<1, 256, 0, s>>>(vout, ds.head, D);
-        // single leaf: x_0 = e0, y_0 = eta with layout [x | y]
-        hipMemcpyAsync(vout + D, ds.head + 2 * D, sizeof(double2) * D, hipMemcpyDeviceToDevice, s);
-        vin = vout;
-    } else {
-        dense_seed_kernel<<<1, 256, 0, s>>>(vin, ds.head, D);
-        check_launch("dense_seed");
-        for (int j = ds.levels; j >= 1; --j) {
-            const int parents = ds.P >> j;
-            const double *child_level = ds.tree + offsets[(size_t)j - 1] * ds.node_doubles;
-            dense_down_kernel<<<parents, 256, 0, s>>>(child_level, ds.node_doubles, D, vin, vout,
-                                                      parents, j == ds.levels ? vin : nullptr);
-            check_launch("dense_down");
-            std::swap(vin, vout);
-        }
-    }
-    dense_tail_kernel<<<L, 256, 0, s>>>(vin, ds.head, n, L, D, grad_out, scale);
-    check_launch("dense_tail");
+    dense->run(s, want_grad, energy_out, want_grad ? grad_out : nullptr, scale);
 }
 
 }  // namespace rie

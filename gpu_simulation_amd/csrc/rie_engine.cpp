@@ -81,7 +81,7 @@ static int pass_grid(uint64_t tiles, int max_grid) {
 
 void Engine::build_plan() {
     passes.clear();
-    if (n <= kSmallMaxLdsQubits) {
+    if (n <= kBatchMaxLdsQubits) {
         small = true;
         return;
     }
@@ -654,18 +654,18 @@ void Engine::structured_backward_stored(hipStream_t s, const double *scale) {
 
 void Engine::small_run(hipStream_t s, bool want_grad, const double *scale, double *energy_out,
                        double *grad_out, double *drift_out) {
-    SmallJob job{};
+    BatchJob job{};
     job.n = n;
     job.layers = L;
+    job.chunk = (int)std::min<size_t>(chunk_width, 4);
     job.field = h;
     job.coef = d_coef;
     job.energy = energy_out;
     job.grad = want_grad ? grad_out : nullptr;
     job.drift = drift_out;
-    job.gstate = nullptr;
     job.scale = scale;
     kt_begin(s);
-    launch_small(nullptr, job, 1, n, true, s);
+    launch_batch_structured(nullptr, job, nullptr, nullptr, 1, 256, n, nullptr, 0, s);
     check_launch("small_circuit");
     kt_end(s, "small_circuit", 0.0);
 }
@@ -1011,25 +1011,9 @@ void Engine::copy_state(double *host_out, size_t count) {
 // ===========================================================================
 
 using rie::Engine;
+using rie::guard;
 
 namespace {
-
-template <typename F>
-rie_status guard(F &&f) {
-    try {
-        f();
-        return RIE_OK;
-    } catch (const rie::Error &e) {
-        rie::g_last_error = e.what();
-        return e.status;
-    } catch (const std::bad_alloc &) {
-        rie::g_last_error = "host allocation failed";
-        return RIE_OUT_OF_MEMORY;
-    } catch (const std::exception &e) {
-        rie::g_last_error = e.what();
-        return RIE_DEVICE_ERROR;
-    }
-}
 
 Engine *as_engine(rie_engine *e) {
     if (e == nullptr) {

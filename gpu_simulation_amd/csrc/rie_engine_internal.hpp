@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <initializer_list>
+#include <new>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -28,6 +29,24 @@ int parse_strategy(const std::string &name);
 
 extern thread_local std::string g_last_error;
 
+// Runs f, mapping exceptions to an rie_status and the thread's last error.
+template <typename F>
+rie_status guard(F &&f) {
+    try {
+        f();
+        return RIE_OK;
+    } catch (const Error &e) {
+        g_last_error = e.what();
+        return e.status;
+    } catch (const std::bad_alloc &) {
+        g_last_error = "host allocation failed";
+        return RIE_OUT_OF_MEMORY;
+    } catch (const std::exception &e) {
+        g_last_error = e.what();
+        return RIE_DEVICE_ERROR;
+    }
+}
+
 struct PassPlan {
     PassGeom g;
     int grid;
@@ -35,7 +54,7 @@ struct PassPlan {
     int end_wire;
 };
 
-struct DenseState;  // dense_scan buffers (rie_dense.hip)
+class DensePlan;  // dense_scan buffers and launches (rie_dense.hpp)
 
 struct Engine {
     Engine(size_t num_qubits, size_t num_layers, double field, const std::string &strategy,
@@ -83,7 +102,7 @@ struct Engine {
     size_t energy_partials = 0;
     std::vector<double2 *> ckpts;
     size_t ckpt_every = 0;
-    DenseState *dense = nullptr;
+    DensePlan *dense = nullptr;
 
     // resident-state bookkeeping
     int final_cur = 0;

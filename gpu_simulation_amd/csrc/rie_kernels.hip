@@ -1013,228 +1013,6 @@ void launch_layer0_finish(const double *low_partials, int low_blocks, const doub
 }
 
 // ---------------------------------------------------------------------------
-// Whole-circuit kernel for small n: one workgroup per circuit, the state and
-// the adjoint state in LDS (n <= 12) or in a global workspace.
-// ---------------------------------------------------------------------------
-
-template <bool kLds>
-__global__ __launch_bounds__(kSmallThreads) void small_circuit_kernel(
-    const SmallJob *__restrict__ jobs, SmallJob single, int num_jobs) {
-    extern __shared__ double2 smem[];
-    __shared__ double red[kSmallThreads / kWave][2 * kSmallMaxQubits + 2];
-    if ((int)blockIdx.x >= num_jobs) {
-        return;
-    }
-    const SmallJob job = (jobs != nullptr) ? jobs[blockIdx.x] : single;
-    const int n = job.n;
-    const int L = job.layers;
-    const uint32_t N = 1u << n;
-    const int t = threadIdx.x;
-    const int lane = t & (kWave - 1);
-    const int wave = t / kWave;
-    double2 *cur = kLds ? smem : job.gstate;
-    double2 *lam = kLds ? smem + N : job.gstate + N;
-    double2 *tmp = kLds ? nullptr : job.gstate + 2 * N;  // permutation target (global variant)
-    const WireCoef *__restrict__ coef = job.coef;
-    constexpr int MAXR = (1 << kSmallMaxLdsQubits) / kSmallThreads;  // 16 amplitudes/thread
-
-    // Layer 0 as a product state, then the ring CNOT: cur[j] = prod u_w[bit_w(P^-1 j)].
-    for (uint32_t j = t; j < N; j += kSmallThreads) {
-        const uint32_t x = ring_inv32(j, n);
-        double2 v = make_double2(1.0, 0.0);
-        for (int w = 0; w < n; ++w) {
-            const WireCoef k = coef[w];
-            v = cmul(v, ((x >> w) & 1) ? u1_of(k) : u0_of(k));
-        }
-        cur[j] = v;
-    }
-    __syncthreads();
-
-    const uint32_t pairs = N >> 1;
-    for (int l = 1; l < L; ++l) {
-        for (int w = 0; w < n; ++w) {
-            const WireCoef k = coef[l * n + w];
-            for (uint32_t p = t; p < pairs; p += kSmallThreads) {
-                const uint32_t i0 = ((p >> w) << (w + 1)) | (p & ((1u << w) - 1u));
-                const uint32_t i1 = i0 | (1u << w);
-                double2 a0 = cur[i0];
-                double2 a1 = cur[i1];
-                fwd_pair(a0, a1, k);
-                cur[i0] = a0;
-                cur[i1] = a1;
-            }
-            __syncthreads();
-        }
-        if constexpr (kLds) {
-            double2 r[MAXR];
-#pragma unroll
-            for (int q = 0; q < MAXR; ++q) {
-                const uint32_t i = t + q * kSmallThreads;
-                if (i < N) {
-                    r[q] = cur[i];
-                }
-            }
-            __syncthreads();
-#pragma unroll
-            for (int q = 0; q < MAXR; ++q) {
-                const uint32_t i = t + q * kSmallThreads;
-                if (i < N) {
-                    cur[ring_fwd32(i, n)] = r[q];
-                }
-            }
-        } else {
-            for (uint32_t i = t; i < N; i += kSmallThreads) {
-                tmp[ring_fwd32(i, n)] = cur[i];
-            }
-            double2 *sw = cur;
-            cur = tmp;
-            tmp = sw;
-        }
-        __syncthreads();
-    }
-
-    // lam = H psi and the energy.
-    double e = 0.0;
-    for (uint32_t i = t; i < N; i += kSmallThreads) {
-        const double2 v = cur[i];
-        const double d = ham_diag(i, n);
-        double fx = 0.0;
-        double fy = 0.0;
-        for (int w = 0; w < n; ++w) {
-            const double2 p = cur[i ^ (1u << w)];
-            fx += p.x;
-            fy += p.y;
-        }
-        const double2 hv = make_double2(d * v.x - job.field * fx, d * v.y - job.field * fy);
-        lam[i] = hv;
-        e += v.x * hv.x + v.y * hv.y;
-    }
-    e = wave_sum(e);
-    if (lane == 0) {
-        red[wave][0] = e;
-    }
-    __syncthreads();
-    if (t == 0) {
-        double acc = 0.0;
-        for (int w = 0; w < kSmallThreads / kWave; ++w) {
-            acc += red[w][0];
-        }
-        *job.energy = acc;
-    }
-    __syncthreads();
-    if (job.grad == nullptr) {
-        return;
-    }
-    const double gscale = job.scale != nullptr ? *job.scale : 1.0;
-
-    for (int l = L - 1; l >= 0; --l) {
-        // Inverse ring CNOT on both states: new[i] = old[P(i)].
-        if constexpr (kLds) {
-            double2 rc[MAXR];
-            double2 rl[MAXR];
-#pragma unroll
-            for (int q = 0; q < MAXR; ++q) {
-                const uint32_t i = t + q * kSmallThreads;
-                if (i < N) {
-                    const uint32_t src = ring_fwd32(i, n);
-                    rc[q] = cur[src];
-                    rl[q] = lam[src];
-                }
-            }
-            __syncthreads();
-#pragma unroll
-            for (int q = 0; q < MAXR; ++q) {
-                const uint32_t i = t + q * kSmallThreads;
-                if (i < N) {
-                    cur[i] = rc[q];
-                    lam[i] = rl[q];
-                }
-            }
-        } else {
-            for (uint32_t i = t; i < N; i += kSmallThreads) {
-                tmp[i] = cur[ring_fwd32(i, n)];
-            }
-            __syncthreads();
-            double2 *sw = cur;
-            cur = tmp;
-            tmp = sw;
-            for (uint32_t i = t; i < N; i += kSmallThreads) {
-                tmp[i] = lam[ring_fwd32(i, n)];
-            }
-            sw = lam;
-            lam = tmp;
-            tmp = sw;
-        }
-        __syncthreads();
-        for (int w = n - 1; w >= 0; --w) {
-            const WireCoef k = coef[l * n + w];
-            double dt = 0.0;
-            double dp = 0.0;
-            for (uint32_t p = t; p < pairs; p += kSmallThreads) {
-                const uint32_t i0 = ((p >> w) << (w + 1)) | (p & ((1u << w) - 1u));
-                const uint32_t i1 = i0 | (1u << w);
-                double2 c0 = cur[i0];
-                double2 c1 = cur[i1];
-                double2 l0 = lam[i0];
-                double2 l1 = lam[i1];
-                bwd_pair(c0, c1, l0, l1, k, dt, dp);
-                cur[i0] = c0;
-                cur[i1] = c1;
-                lam[i0] = l0;
-                lam[i1] = l1;
-            }
-            dt = wave_sum(dt);
-            dp = wave_sum(dp);
-            if (lane == 0) {
-                red[wave][2 * w] = dt;
-                red[wave][2 * w + 1] = dp;
-            }
-            __syncthreads();
-        }
-        if (t < 2 * n) {
-            double acc = 0.0;
-            for (int w = 0; w < kSmallThreads / kWave; ++w) {
-                acc += red[w][t];
-            }
-            job.grad[(uint64_t)l * n * 2 + t] = acc * gscale;
-        }
-        __syncthreads();
-    }
-
-    if (job.drift != nullptr) {
-        double acc = 0.0;
-        for (uint32_t i = t; i < N; i += kSmallThreads) {
-            const double2 v = cur[i];
-            const double re = v.x - (i == 0 ? 1.0 : 0.0);
-            acc += re * re + v.y * v.y;
-        }
-        acc = wave_sum(acc);
-        if (lane == 0) {
-            red[wave][0] = acc;
-        }
-        __syncthreads();
-        if (t == 0) {
-            double s = 0.0;
-            for (int w = 0; w < kSmallThreads / kWave; ++w) {
-                s += red[w][0];
-            }
-            *job.drift = s;
-        }
-    }
-}
-
-void launch_small(const SmallJob *d_jobs, const SmallJob &single, int num_jobs, int max_n,
-                  bool lds, hipStream_t s) {
-    if (lds) {
-        const size_t bytes = sizeof(double2) * 2 * (size_t{1} << max_n);
-        small_circuit_kernel<true><<<num_jobs, kSmallThreads, bytes, s>>>(d_jobs, single,
-                                                                          num_jobs);
-    } else {
-        small_circuit_kernel<false><<<num_jobs, kSmallThreads, 0, s>>>(d_jobs, single, num_jobs);
-    }
-}
-
-// ---------------------------------------------------------------------------
 // Gate-level kernels (PennyLane's gate order: RY, RZ per wire, then n CNOTs).
 // ---------------------------------------------------------------------------
 

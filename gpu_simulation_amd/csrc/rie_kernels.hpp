@@ -25,30 +25,69 @@ struct PassGeom {
     uint64_t hi_mask;  // index bits [T, n): the ring-CNOT image of a low tile
 };
 
-// One independent circuit evaluated by a single workgroup (small n).
-struct SmallJob {
+// One independent circuit evaluated by a single workgroup (n <= 14).
+struct BatchJob {
     int n;
     int layers;
+    int chunk;             // wires per register chunk (1..4)
+    int reserved;
     double field;
     const WireCoef *coef;  // layers * n entries
     double *energy;        // 1 value
     double *grad;          // 2 * n * layers values, or nullptr for energy only
     double *drift;         // || walked state - |0> ||^2, or nullptr
-    double2 *gstate;       // 3 * 2^n amplitudes of global workspace (non-LDS variant)
     const double *scale;   // optional chain-rule factor for grad (nullptr = 1)
 };
 
-constexpr int kSmallThreads = 256;
-constexpr int kSmallMaxLdsQubits = 12;
-constexpr int kSmallMaxQubits = 16;  // global-workspace variant
+constexpr int kBatchMaxLdsQubits = 12;  // psi and lambda in LDS
+constexpr int kBatchMaxQubits = 14;     // global-scratch variant above that
+
+// Widest rotation chunk per workgroup size: a backward group keeps 2 * 2^K
+// amplitudes in registers, and the register budget per thread halves each
+// time the workgroup doubles.
+constexpr int batch_max_chunk(int threads) {
+    return threads >= 1024 ? 2 : (threads >= 512 ? 3 : 4);
+}
 
 // Coefficient table from device parameters: coef[i] from params[2i], params[2i+1].
 void launch_coef(const double *d_params, WireCoef *d_coef, int count, hipStream_t s);
 
-// Whole-circuit kernel, one workgroup per job (n <= 12 in LDS, else global).
-// With d_jobs == nullptr the single job passed by value is run.
-void launch_small(const SmallJob *d_jobs, const SmallJob &single, int num_jobs, int max_n,
-                  bool lds, hipStream_t s);
+// Persistent whole-circuit kernel: workgroup b runs the circuits
+// d_order[d_wg_begin[b] .. d_wg_begin[b+1]) in turn (without d_wg_begin:
+// circuit b of d_jobs, or `single` when d_jobs is null).  threads is 256,
+// 512 or 1024; max_lds_n sizes the LDS (largest n <= 12 in the launch, 0 if
+// none); circuits with n > 12 use scratch + b * scratch_stride (2 * 2^n
+// amplitudes).
+void launch_batch_structured(const BatchJob *d_jobs, const BatchJob &single, const int *d_order,
+                             const int *d_wg_begin, int grid, int threads, int max_lds_n,
+                             double2 *scratch, uint64_t scratch_stride, hipStream_t s);
+uint32_t batch_lds_state_amps(int max_lds_n);
+
+// Optimizer steps and helpers over packed arrays.
+void launch_gd_update(double *p, const double *g, uint64_t count, double lr, hipStream_t s);
+void launch_adam_update(double *p, const double *g, double *m, double *v, uint64_t count,
+                        double lr, double b1, double b2, double eps, const int *step,
+                        hipStream_t s);
+void launch_record_energies(const double *energies, int count, double *history, const int *step,
+                            int capacity, hipStream_t s);
+void launch_step_advance(int *step, hipStream_t s);
+void launch_fill(double *p, uint64_t count, double value, hipStream_t s);
+void launch_set_int(int *p, int value, hipStream_t s);
+
+// FP32 MLP kernels (learned warm start).  Epilogues of the GEMM:
+constexpr int kMlpEpiNone = 0;      // C = op(A) op(B)
+constexpr int kMlpEpiBias = 1;      // C = A B + bias
+constexpr int kMlpEpiBiasGelu = 2;  // C = Z = A B + bias, act = GELU(Z)
+constexpr int kMlpEpiDGelu = 3;     // C = (A B^T) * GELU'(zin)
+// ta: A is stored K x M (use A^T); tb: B is stored N x K (use B^T).
+// Supported: (ta, tb) = (0, 0) with None/Bias/BiasGelu, (0, 1) with None/DGelu, (1, 0) None.
+void launch_mlp_gemm(bool ta, bool tb, int epi, const float *A, const float *B, float *C, int M,
+                     int N, int K, const float *bias, float *act, const float *zin, hipStream_t s);
+void launch_mlp_colsum(const float *X, int rows, int cols, float *out, hipStream_t s);
+void launch_mlp_mse(const float *Y, const float *T, int count, float *dY, float *partials,
+                    int blocks, float *loss_out, hipStream_t s);
+void launch_mlp_adam(float *p, const float *g, float *m, float *v, int count, float lr, float b1,
+                     float b2, float eps, int t, hipStream_t s);
 
 // Layer 0 as a product state followed by the ring-CNOT permutation.
 void launch_init_product_cnot(double2 *out, const WireCoef *coef_layer0, int n, hipStream_t s);

@@ -43,6 +43,32 @@ C_ABI_SYMBOLS = (
     "rie_engine_reset_kernel_stats",
     "rie_engine_kernel_stat_count",
     "rie_engine_kernel_stat",
+    "rie_batch_create",
+    "rie_batch_destroy",
+    "rie_batch_size",
+    "rie_batch_num_params",
+    "rie_batch_param_offsets",
+    "rie_batch_strategies",
+    "rie_batch_device_buffers",
+    "rie_batch_set_params",
+    "rie_batch_get_params",
+    "rie_batch_get_results",
+    "rie_batch_energy_and_grad",
+    "rie_batch_run_device",
+    "rie_batch_set_use_graph",
+    "rie_batch_run_per_circuit",
+    "rie_batch_optimize",
+    "rie_batch_autotune",
+    "rie_batch_set_config",
+    "rie_batch_config",
+    "rie_mlp_create",
+    "rie_mlp_destroy",
+    "rie_mlp_num_params",
+    "rie_mlp_get_params",
+    "rie_mlp_set_params",
+    "rie_mlp_forward",
+    "rie_mlp_loss_and_grad",
+    "rie_mlp_train",
 )
 
 

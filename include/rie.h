@@ -184,6 +184,103 @@ rie_status rie_engine_kernel_stat(rie_engine *engine, size_t index, const char *
  * (2^n complex128 values, interleaved re/im). */
 rie_status rie_engine_copy_state(rie_engine *engine, double *host_out, size_t num_doubles);
 
+/* ---- Batches of independent circuits -----------------------------------
+ * No counterpart in the reference, which evaluates one circuit per
+ * RingIsingCudaBackend::energy_and_grad call (cpp/ising_cuda_backend.cu:
+ * 429-442).  A batch holds `count` circuits with their own qubit count
+ * (2..14), layer count and field; circuits with num_qubits <=
+ * dense_max_qubits (0..8) use dense_scan, the others structured_adjoint.
+ * All circuits are evaluated together: the structured ones in one persistent
+ * launch, the dense ones level by level in grouped FP64-MFMA GEMM launches,
+ * concurrently.  Parameters, energies and gradients live in device arrays
+ * owned by the batch; circuit i's 2*n_i*L_i parameters (and gradients) start
+ * at offsets[i] and follow the single-circuit layout. */
+typedef struct rie_batch rie_batch;
+
+rie_status rie_batch_create(size_t count, const int *num_qubits, const int *num_layers,
+                            const double *fields, int dense_max_qubits, rie_batch **batch_out);
+void rie_batch_destroy(rie_batch *batch);
+size_t rie_batch_size(const rie_batch *batch);
+size_t rie_batch_num_params(const rie_batch *batch);
+/* offsets_out receives count + 1 values (the last is the total). */
+rie_status rie_batch_param_offsets(const rie_batch *batch, size_t *offsets_out, size_t len);
+/* strategy_out receives count values (RIE_STRATEGY_*). */
+rie_status rie_batch_strategies(const rie_batch *batch, int *strategy_out, size_t len);
+/* Device addresses of the batch's parameter, energy and gradient arrays
+ * (any output may be NULL). */
+rie_status rie_batch_device_buffers(rie_batch *batch, double **d_params, double **d_energies,
+                                    double **d_gradient);
+rie_status rie_batch_set_params(rie_batch *batch, const double *params, size_t num_params);
+rie_status rie_batch_get_params(rie_batch *batch, double *params_out, size_t num_params);
+/* Energies (count values) and, if gradient_out is non-NULL, the packed
+ * gradient of the last evaluation, after synchronising the device. */
+rie_status rie_batch_get_results(rie_batch *batch, double *energies_out, double *gradient_out);
+
+/* Host pointers, synchronous: upload params, evaluate, download energies
+ * (count values) and, if compute_gradient, the packed gradient. */
+rie_status rie_batch_energy_and_grad(rie_batch *batch, const double *params, size_t num_params,
+                                     int compute_gradient, double *energies_out,
+                                     double *gradient_out);
+/* Evaluate on the batch's device arrays, asynchronously on stream (NULL: the
+ * default stream); replays a captured HIP graph when enabled. */
+rie_status rie_batch_run_device(rie_batch *batch, int compute_gradient, void *stream);
+rie_status rie_batch_set_use_graph(rie_batch *batch, int use_graph);
+/* The same evaluation issued as separate launches per circuit, one after the
+ * other (the one-launch-per-circuit comparison). */
+rie_status rie_batch_run_per_circuit(rie_batch *batch, int compute_gradient, void *stream);
+
+/* `steps` optimizer steps on every circuit at once, starting from the
+ * batch's current parameters: optimizer 0 = gradient descent p -= lr g
+ * (the reference's update, ring_ising/workflows/common.py:23-25), 1 = Adam
+ * (beta 0.9 / 0.999, eps 1e-8, fresh moments, as torch.optim.Adam).  With
+ * use_graph one step is captured in a HIP graph and replayed.  The energy of
+ * every circuit before each step is recorded; energy_history_out (host,
+ * steps * count values, may be NULL) receives it and the call then
+ * synchronises. */
+rie_status rie_batch_optimize(rie_batch *batch, int optimizer, double learning_rate, size_t steps,
+                              int use_graph, double *energy_history_out, void *stream);
+
+/* Autotuner: times the whole-circuit kernel per qubit count for every
+ * workgroup size (256/512/1024) and rotation-chunk width (1..4), the dense
+ * GEMM tile (32/64) and the share of CUs given to the structured launch,
+ * keeps the fastest, and rebuilds the schedule with the measured costs. */
+rie_status rie_batch_autotune(rie_batch *batch, size_t repeats);
+/* Manual configuration: threads 256/512/1024; chunk_by_n (15 values,
+ * indexed by num_qubits, each 1..4) or NULL to keep; dense_tile 32/64;
+ * structured_grid = workgroups of the persistent launch (0: one per CU). */
+rie_status rie_batch_set_config(rie_batch *batch, int threads, const int *chunk_by_n,
+                                size_t chunk_len, int dense_tile, int structured_grid);
+/* Current configuration and the last autotuning measurements as JSON. */
+rie_status rie_batch_config(const rie_batch *batch, char *json_out, size_t len,
+                            size_t *needed_out);
+
+/* ---- Warm-start MLP ------------------------------------------------------
+ * No counterpart in the reference.  An FP32 perceptron
+ * inputs -> GELU(hidden) -> GELU(hidden) -> outputs on hand-written HIP
+ * kernels, used to predict starting parameters of a VQE run from
+ * (h/J, n, L).  Parameters are packed W1 [in x hidden], b1, W2 [hidden x
+ * hidden], b2, W3 [hidden x out], b3 (row-major; y = x W + b), initialised
+ * U(-1/sqrt(fan_in), 1/sqrt(fan_in)) from seed.  Inputs and outputs are
+ * host arrays with one sample per row. */
+typedef struct rie_mlp rie_mlp;
+
+rie_status rie_mlp_create(int inputs, int hidden, int outputs, uint64_t seed, rie_mlp **mlp_out);
+void rie_mlp_destroy(rie_mlp *mlp);
+size_t rie_mlp_num_params(const rie_mlp *mlp);
+rie_status rie_mlp_get_params(rie_mlp *mlp, float *params_out, size_t count);
+rie_status rie_mlp_set_params(rie_mlp *mlp, const float *params, size_t count);
+/* outputs: rows x outputs values. */
+rie_status rie_mlp_forward(rie_mlp *mlp, const float *inputs, int rows, float *outputs);
+/* Mean squared error over all rows x outputs values and its gradient with
+ * respect to the packed parameters (grad_out may be NULL). */
+rie_status rie_mlp_loss_and_grad(rie_mlp *mlp, const float *inputs, const float *targets,
+                                 int rows, float *loss_out, float *grad_out);
+/* `steps` full-batch Adam steps (beta 0.9 / 0.999, eps 1e-8, fresh moments)
+ * on the mean squared error; loss_history_out (steps values, may be NULL)
+ * receives the loss before each step. */
+rie_status rie_mlp_train(rie_mlp *mlp, const float *inputs, const float *targets, int rows,
+                         int steps, float learning_rate, float *loss_history_out);
+
 #ifdef __cplusplus
 }
 #endif

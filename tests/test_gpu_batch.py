@@ -1,0 +1,172 @@
+"""GPU parity of the batched mode: many circuits of uneven size in one launch.
+
+Every circuit of a batch is checked against the FP64 CPU oracle at
+|dE|, |dgrad| <= 1e-10; the persistent launch, the per-circuit launches and a
+replayed HIP graph must agree bitwise (same kernels, same order of
+arithmetic); the autotuner may change the summation order but not the result
+beyond the tolerance; device-side gradient descent and Adam follow a CPU loop
+over oracle gradients.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from gpu_simulation_amd.batch import CircuitSpec, RingIsingBatch, uneven_mix
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10
+
+
+def _check_against_oracle(oracle, batch, params, energies, grads):
+    worst_e = worst_g = 0.0
+    for i, (s, p) in enumerate(zip(batch.specs, params)):
+        e_ref, g_ref = oracle.energy_and_grad(s.num_qubits, s.layers, s.field, p)
+        worst_e = max(worst_e, abs(energies[i] - e_ref))
+        worst_g = max(worst_g, float(np.max(np.abs(np.ravel(grads[i]) - np.ravel(g_ref)))))
+    assert worst_e <= TOL, worst_e
+    assert worst_g <= TOL, worst_g
+
+
+@pytest.mark.parametrize("dense_max", [8, 0])
+def test_uneven_batch_matches_oracle(oracle, dense_max):
+    specs, params = uneven_mix(48, seed=11, qubits=(2, 14), layer_choices=(1, 2, 3, 5))
+    print(f"case uneven batch dense_max={dense_max} count={len(specs)}", flush=True)
+    batch = RingIsingBatch(specs, dense_max_qubits=dense_max)
+    assert all((st == "dense_scan") == (s.num_qubits <= dense_max)
+               for st, s in zip(batch.strategies, specs))
+    energies, grads = batch.energy_and_grad(params)
+    _check_against_oracle(oracle, batch, params, energies, grads)
+    e_only, g_only = batch.energy_and_grad(params, compute_gradient=False)
+    assert g_only is None
+    assert np.max(np.abs(e_only - energies)) <= 1e-12
+
+
+def test_one_circuit_per_qubit_count_and_chunk_width(oracle, backend_module):
+    # every qubit count 2..14 with every rotation-chunk width the kernel has
+    specs = [CircuitSpec(n, 3, 0.7 + 0.05 * n) for n in range(2, 15)]
+    rng = np.random.default_rng(5)
+    params = [rng.uniform(0, 2 * np.pi, (s.layers, s.num_qubits, 2)) for s in specs]
+    batch = RingIsingBatch(specs, dense_max_qubits=0)
+    for threads, widths in [(256, (1, 2, 3, 4)), (512, (1, 2, 3)), (1024, (1, 2))]:
+        for k in widths:
+            print(f"case threads={threads} chunk={k}", flush=True)
+            batch.native.set_config(threads, [k] * 15, 64, 0)
+            energies, grads = batch.energy_and_grad(params)
+            _check_against_oracle(oracle, batch, params, energies, grads)
+
+
+def test_persistent_launch_equals_per_circuit_launches(oracle):
+    specs, params = uneven_mix(40, seed=12, qubits=(2, 14), layer_choices=(2, 4))
+    batch = RingIsingBatch(specs)
+    batch.set_params(params)
+    batch.native.run_device(True, 0)
+    e_batched, g_batched = batch.results()
+    batch.native.run_per_circuit(True, 0)
+    e_single, g_single = batch.results()
+    assert np.array_equal(e_batched, e_single)
+    for a, b in zip(g_batched, g_single):
+        assert np.array_equal(a, b)
+    _check_against_oracle(oracle, batch, params, e_batched, g_batched)
+
+
+def test_graph_replays_match_eager():
+    specs, params = uneven_mix(32, seed=13, qubits=(3, 13), layer_choices=(2, 3))
+    batch = RingIsingBatch(specs)
+    rng = np.random.default_rng(14)
+    other = [rng.uniform(0, 2 * np.pi, p.shape) for p in params]
+    eager_e, eager_g = batch.energy_and_grad(params)
+    eager_e2, _ = batch.energy_and_grad(other)
+    batch.native.set_use_graph(True)
+    for rep in range(3):
+        # eager host work and other parameters between the replays
+        e, g = batch.energy_and_grad(params)
+        assert np.array_equal(e, eager_e), rep
+        for a, b in zip(g, eager_g):
+            assert np.array_equal(a, b)
+        e2, _ = batch.energy_and_grad(other)
+        assert np.array_equal(e2, eager_e2), rep
+    batch.native.set_use_graph(False)
+
+
+def test_autotune_keeps_results(oracle):
+    specs, params = uneven_mix(64, seed=15, qubits=(4, 14), layer_choices=(2, 6))
+    batch = RingIsingBatch(specs)
+    before_e, _ = batch.energy_and_grad(params)
+    cfg = batch.autotune(repeats=1)
+    print("autotune", {k: cfg[k] for k in ("threads", "grid", "dense_tile", "chunk_by_n")},
+          flush=True)
+    assert cfg["measured"] is True
+    assert cfg["threads"] in (256, 512, 1024)
+    energies, grads = batch.energy_and_grad(params)
+    assert np.max(np.abs(energies - before_e)) <= 1e-12
+    _check_against_oracle(oracle, batch, params, energies, grads)
+
+
+def _oracle_gd(oracle, spec, p0, lr, steps):
+    p = p0.copy()
+    hist = []
+    for _ in range(steps):
+        e, g = oracle.energy_and_grad(spec.num_qubits, spec.layers, spec.field, p)
+        hist.append(e)
+        p = p - lr * np.reshape(g, p.shape)
+    return np.asarray(hist), p
+
+
+def _oracle_adam(oracle, spec, p0, lr, steps, b1=0.9, b2=0.999, eps=1e-8):
+    # torch.optim.Adam's arithmetic (no weight decay, no amsgrad)
+    p = p0.copy()
+    m = np.zeros_like(p)
+    v = np.zeros_like(p)
+    hist = []
+    for t in range(1, steps + 1):
+        e, g = oracle.energy_and_grad(spec.num_qubits, spec.layers, spec.field, p)
+        g = np.reshape(g, p.shape)
+        hist.append(e)
+        m = b1 * m + (1 - b1) * g
+        v = b2 * v + (1 - b2) * g * g
+        denom = np.sqrt(v) / np.sqrt(1 - b2 ** t) + eps
+        p = p - (lr / (1 - b1 ** t)) * (m / denom)
+    return np.asarray(hist), p
+
+
+@pytest.mark.parametrize("optimizer,lr", [("gd", 0.1), ("adam", 0.05)])
+def test_device_optimizer_steps_follow_cpu_loop(oracle, optimizer, lr):
+    specs, params = uneven_mix(12, seed=16, qubits=(3, 13), layer_choices=(2, 3))
+    steps = 6
+    batch = RingIsingBatch(specs)
+    ref = [(_oracle_gd if optimizer == "gd" else _oracle_adam)(oracle, s, p, lr, steps)
+           for s, p in zip(specs, params)]
+    results = {}
+    for use_graph in (True, False):
+        batch.set_params(params)
+        hist = batch.optimize(optimizer, lr, steps, use_graph=use_graph)
+        results[use_graph] = (hist, batch.get_params())
+        assert hist.shape == (steps, len(specs))
+        for i, (h_ref, p_ref) in enumerate(ref):
+            assert np.max(np.abs(hist[:, i] - h_ref)) <= 1e-9, (use_graph, i)
+            assert np.max(np.abs(np.ravel(results[use_graph][1][i]) - np.ravel(p_ref))) <= 1e-9, (
+                use_graph, i)
+    assert np.array_equal(results[True][0], results[False][0])
+
+
+def test_plain_method_12q_3l_descent(oracle, backend_module):
+    # one energy_and_grad, then 20 gradient-descent steps at lr 0.1
+    n, layers, steps, lr = 12, 3, 20, 0.1
+    p0 = np.random.default_rng(0).uniform(0, 2 * np.pi, (layers, n, 2))
+    spec = CircuitSpec(n, layers, 1.0)
+    h_ref, p_ref = _oracle_gd(oracle, spec, p0, lr, steps)
+    eng = backend_module.RingIsingCudaBackend(n, layers, 1.0, "structured_adjoint", 8, False)
+    p = p0.reshape(-1).copy()
+    for k in range(steps):
+        out = eng.energy_and_grad(p)
+        assert abs(out["energy"] - h_ref[k]) <= 1e-9, k
+        p = p - lr * np.asarray(out["gradient"])
+    assert np.max(np.abs(p - p_ref.reshape(-1))) <= 1e-9
+    batch = RingIsingBatch([spec])
+    batch.set_params([p0])
+    hist = batch.optimize("gd", lr, steps)
+    assert np.max(np.abs(hist[:, 0] - h_ref)) <= 1e-9
+    assert np.max(np.abs(np.ravel(batch.get_params()[0]) - np.ravel(p_ref))) <= 1e-9

@@ -1,0 +1,63 @@
+"""Warm-start MLP: the FP32 HIP kernels against the FP64 numpy oracle, and the
+gain of the learned warm start over the random initialisation."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle import mlp_oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _net(backend_module, inputs, hidden, outputs, seed):
+    return backend_module.WarmStartMLP(inputs, hidden, outputs, seed)
+
+
+def test_forward_and_gradient_match_fp64_oracle(backend_module):
+    inputs, hidden, outputs, rows = 3, 256, 80, 96
+    net = _net(backend_module, inputs, hidden, outputs, 7)
+    rng = np.random.default_rng(8)
+    x = rng.uniform(-1, 2, (rows, inputs)).astype(np.float32)
+    t = rng.standard_normal((rows, outputs)).astype(np.float32)
+    p = np.asarray(net.get_params(), dtype=np.float64)
+    y = np.asarray(net.forward(x))
+    y_ref, _ = mlp_oracle.forward(p, x, inputs, hidden, outputs)
+    assert np.max(np.abs(y - y_ref)) <= 1e-5 * max(1.0, np.max(np.abs(y_ref)))
+    loss, g = net.loss_and_grad(x, t)
+    loss_ref, g_ref = mlp_oracle.loss_and_grad(p, x, t, inputs, hidden, outputs)
+    assert abs(loss - loss_ref) <= 1e-5 * loss_ref
+    g = np.asarray(g, dtype=np.float64)
+    print("max |dg| / max |g|", np.max(np.abs(g - g_ref)) / np.max(np.abs(g_ref)), flush=True)
+    assert np.max(np.abs(g - g_ref)) <= 1e-4 * np.max(np.abs(g_ref))
+
+
+def test_training_steps_follow_oracle_adam(backend_module):
+    inputs, hidden, outputs, rows, steps, lr = 3, 64, 12, 128, 8, 1e-3
+    net = _net(backend_module, inputs, hidden, outputs, 9)
+    rng = np.random.default_rng(10)
+    x = rng.uniform(-1, 2, (rows, inputs)).astype(np.float32)
+    t = rng.standard_normal((rows, outputs)).astype(np.float32)
+    p0 = np.asarray(net.get_params(), dtype=np.float64)
+    losses = np.asarray(net.train(x, t, steps, lr))
+    p_ref, losses_ref = mlp_oracle.adam(p0, x, t, inputs, hidden, outputs, steps, lr)
+    assert np.max(np.abs(losses - losses_ref)) <= 1e-4 * losses_ref[0]
+    p = np.asarray(net.get_params(), dtype=np.float64)
+    # Adam moves each weight by about lr per step; FP32 rounding stays far below that
+    assert np.max(np.abs(p - p_ref)) <= 1e-2 * lr
+
+
+def test_learned_warm_start_beats_random_init():
+    from gpu_simulation_amd import warm_start as ws
+
+    n, layers, steps, lr = 6, 3, 200, 0.05
+    data = ws.generate_training_data(n, layers, count=512, steps=steps, lr=lr, seed=4)
+    model = ws.WarmStartModel(n, layers, hidden=256, seed=0)
+    losses = model.fit(data.fields, data.params, steps=1500, lr=1e-3)
+    print("branch", data.branch, data.branch_mean_energies, "loss", losses[0], losses[-1],
+          flush=True)
+    assert losses[-1] < 0.1 * losses[0]
+    res = ws.evaluate(model, np.random.default_rng(44).uniform(0.2, 2.0, 64), steps, lr, 1e-4)
+    print({k: v for k, v in res.items() if not isinstance(v, np.ndarray)}, flush=True)
+    assert res["median_steps_warm"] < res["median_steps_plain"]

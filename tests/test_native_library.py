@@ -10,6 +10,7 @@ import ctypes
 import re
 from pathlib import Path
 
+import numpy as np
 import pytest
 
 from gpu_simulation_amd import native
@@ -55,3 +56,31 @@ def test_extension_maps_errors_like_the_reference():
     if ext.device_count() == 0:
         with pytest.raises(RuntimeError):
             ext.RingIsingCudaBackend(4, 2, 1.0, "structured_adjoint", 8, False)
+
+
+def test_batch_validates_arguments_before_touching_the_device():
+    ext = native.load_extension()
+    with pytest.raises(ValueError):
+        ext.RingIsingBatch([4, 15], [2, 2], [1.0, 1.0])  # 15 qubits: beyond the batch kernel
+    with pytest.raises(ValueError):
+        ext.RingIsingBatch([4, 5], [2, 0], [1.0, 1.0])  # zero layers
+    with pytest.raises(ValueError):
+        ext.RingIsingBatch([4, 5], [2], [1.0, 1.0])  # ragged descriptions
+    with pytest.raises(ValueError):
+        ext.RingIsingBatch([], [], [])  # empty batch
+    with pytest.raises(ValueError):
+        ext.RingIsingBatch([4], [2], [1.0], 9)  # dense_scan stops at 8 qubits
+    if ext.device_count() == 0:
+        with pytest.raises(RuntimeError):
+            ext.RingIsingBatch([4, 5], [2, 2], [1.0, 1.0])
+
+
+def test_uneven_mix_is_reproducible():
+    from gpu_simulation_amd.batch import uneven_mix
+
+    a_specs, a_params = uneven_mix(16, seed=3)
+    b_specs, b_params = uneven_mix(16, seed=3)
+    assert a_specs == b_specs
+    assert all(np.array_equal(x, y) for x, y in zip(a_params, b_params))
+    assert all(4 <= s.num_qubits <= 14 and s.layers in (8, 32, 128) and 0.2 <= s.field <= 2.0
+               for s in a_specs)
