@@ -82,3 +82,29 @@ def test_backend_profile_and_alias(oracle):
     assert "backward_sweep_ms" in out["timings_ms"]
     with pytest.raises(ValueError):
         backend.energy(params[:, :12, :])
+
+
+def test_cli_runs_the_standalone_workflow(oracle):
+    from gpu_simulation_amd.ring_ising.cli import main
+
+    result = main(["--qubits", "10", "--layers", "2", "--steps", "3", "--seed", "7"])
+    assert result.final_params.shape == (2, 10, 2)
+    e_ref, _ = oracle.energy_and_grad(10, 2, 1.0, result.final_params)
+    assert abs(result.final_energy - e_ref) <= 1e-10
+
+
+def test_stage_profiling_harness_writes_reference_columns(tmp_path):
+    import csv
+
+    from gpu_simulation_amd import profiling
+
+    out = tmp_path / "stages.csv"
+    rows = profiling.main(["--cases", "6x2", "14x2", "--repeats", "2", "--structured-widths", "2",
+                           "--kernels", "--csv-out", str(out)])
+    assert rows and out.exists()
+    with out.open() as handle:
+        read = list(csv.DictReader(handle))
+    assert list(read[0].keys()) == profiling.CSV_FIELDS
+    modes = {r["mode"] for r in read}
+    assert {"inverse_walk_cuQuantum", "structured_w2", "dense_scan"} <= modes
+    assert any(r["stage"].startswith("kernel:") for r in read)
