@@ -265,9 +265,22 @@ __global__ __launch_bounds__(Tile<R>::THREADS) void fwd_pass_kernel(
         }
         __syncthreads();
     }
+    // The next tile's loads are issued before this tile's phases, so they
+    // stay in flight across the LDS exchanges (whose barriers wait on LDS
+    // traffic only).
+    double2 v[TL::REGS];
+    double2 nv[TL::REGS];
+    if constexpr (!kGenInit) {
+        if ((uint64_t)blockIdx.x < G.num_tiles) {
+            const double2 *src = in + tile_base(blockIdx.x, G, TL::T) + ta.offA;
+#pragma unroll
+            for (int k = 0; k < TL::REGS; ++k) {
+                nv[k] = src[k * ta.strideA];
+            }
+        }
+    }
     for (uint64_t g = blockIdx.x; g < G.num_tiles; g += gridDim.x) {
         const uint64_t base = tile_base(g, G, TL::T);
-        double2 v[TL::REGS];
         if constexpr (kGenInit) {
             const uint64_t row = base + ta.offA;
 #pragma unroll
@@ -281,10 +294,17 @@ __global__ __launch_bounds__(Tile<R>::THREADS) void fwd_pass_kernel(
                 v[k] = p;
             }
         } else {
-            const double2 *src = in + base + ta.offA;
 #pragma unroll
             for (int k = 0; k < TL::REGS; ++k) {
-                v[k] = src[k * ta.strideA];
+                v[k] = nv[k];
+            }
+            const uint64_t gn = g + gridDim.x;
+            if (gn < G.num_tiles) {
+                const double2 *src = in + tile_base(gn, G, TL::T) + ta.offA;
+#pragma unroll
+                for (int k = 0; k < TL::REGS; ++k) {
+                    nv[k] = src[k * ta.strideA];
+                }
             }
         }
         phase_fwd<R, 0>(v, lc, G);
@@ -520,6 +540,18 @@ __global__ __launch_bounds__(Tile<R>::THREADS) void ham_pass_kernel(
     const uint32_t t = threadIdx.x;
     const TileAddr ta = tile_addr<R>(t, G);
     double e = 0.0;
+    // psi of the next tile is loaded while this tile is processed; flips of
+    // the register-resident bits [2R, 3R) come from registers, the others
+    // from the LDS copy of the tile.
+    double2 nv[TL::REGS];
+    if ((uint64_t)blockIdx.x < G.num_tiles) {
+        const uint64_t row0 = tile_base(blockIdx.x, G, T) + ta.offA;
+#pragma unroll
+        for (int k = 0; k < TL::REGS; ++k) {
+            nv[k] = psi[row0 + k * ta.strideA];
+        }
+    }
+    const int lds_hi = G.rot_hi < 2 * R ? G.rot_hi : 2 * R;
     for (uint64_t g = blockIdx.x; g < G.num_tiles; g += gridDim.x) {
         const uint64_t base = tile_base(g, G, T);
         const uint64_t row = base + ta.offA;
@@ -527,9 +559,17 @@ __global__ __launch_bounds__(Tile<R>::THREADS) void ham_pass_kernel(
         double2 prev[kReadLam ? TL::REGS : 1];
 #pragma unroll
         for (int k = 0; k < TL::REGS; ++k) {
-            v[k] = psi[row + k * ta.strideA];
+            v[k] = nv[k];
             if constexpr (kReadLam) {
                 prev[k] = lam_in[row + k * ta.strideA];
+            }
+        }
+        const uint64_t gn = g + gridDim.x;
+        if (gn < G.num_tiles) {
+            const uint64_t rown = tile_base(gn, G, T) + ta.offA;
+#pragma unroll
+            for (int k = 0; k < TL::REGS; ++k) {
+                nv[k] = psi[rown + k * ta.strideA];
             }
         }
         __syncthreads();
@@ -543,10 +583,19 @@ __global__ __launch_bounds__(Tile<R>::THREADS) void ham_pass_kernel(
             const uint32_t o = omap<R, 0>(t, k);
             const uint64_t idx = row + k * ta.strideA;
             double2 flips = make_double2(0.0, 0.0);
-            for (int j = G.rot_lo; j < G.rot_hi; ++j) {
+            for (int j = G.rot_lo; j < lds_hi; ++j) {
                 const double2 p = lds[lds_slot<R>(o ^ (1u << j))];
                 flips.x += p.x;
                 flips.y += p.y;
+            }
+#pragma unroll
+            for (int rb = 0; rb < R; ++rb) {
+                const int j = 2 * R + rb;
+                if (j >= G.rot_lo && j < G.rot_hi) {
+                    const double2 p = v[k ^ (1 << rb)];
+                    flips.x += p.x;
+                    flips.y += p.y;
+                }
             }
             double2 contrib = make_double2(-field * flips.x, -field * flips.y);
             if constexpr (kDiag) {
