@@ -11,9 +11,10 @@
 //    M_l = C R_l (a row permutation of R_l), so the tree has L leaves;
 //  * pads every matrix to D = max(2^n, 16) with an identity block so all
 //    products run on v_mfma_f64_16x16x4f64 tiles;
-//  * multiplies split-complex (re / im planes) matrices with four real MFMAs
-//    per k-step (Cr += Ar Br - Ai Bi, Ci += Ar Bi + Ai Br), 64x64 output
-//    tiles staged through LDS;
+//  * multiplies split-complex (re / im planes) matrices on FP64 MFMA: 64x64
+//    output tiles (D >= 64) with three real products per complex one and the
+//    next k-block prefetched into registers; 16x16 / 32x32 tiles (D <= 32)
+//    with four real products per k-step;
 //  * obtains every gradient from the prefix vectors x_l = M_{l-1}..M_0 e0 and
 //    y_l = M_{l-1}..M_0 T^dagger H psi:
 //      dE/dtheta_{l,w} = Re<y_l | -i Y_w | x_l>
@@ -70,36 +71,81 @@ __device__ __forceinline__ double2 u_entry(const WireCoef &k, int rb, int cb) {
     return make_double2(mag * k.ch, mag * ph);
 }
 
+// Each block fills kLayerChunk consecutive entries of one circuit's P * D * D
+// leaf matrices (D and D * D are powers of two: indices by shifts and masks).
+// A row of R_l is the tensor product of the rows U_w[src_w][.] of the wire
+// gates, so for n >= 6 every row's entries are one product of two 16-entry
+// tables (wires 0..3 and 4..n-1) built in LDS; smaller circuits multiply the
+// n factors directly.
+constexpr int kLayerChunk = 4096;
+
 __global__ __launch_bounds__(256) void dense_layers_kernel(const DenseJob *__restrict__ jobs,
                                                            const int2 *__restrict__ tab,
                                                            int count) {
-    const int k = find_item(tab, count, blockIdx.x);
-    const DenseJob &J = jobs[tab[k].x];
+    __shared__ double2 tables[64][2][16];  // rows of the chunk (D >= 64: at most 64)
+    const int item = find_item(tab, count, blockIdx.x);
+    const DenseJob &J = jobs[tab[item].x];
     const int D = J.D;
+    const int logD = J.log_d;
+    const uint32_t maskD = (uint32_t)D - 1u;
     const size_t DD = (size_t)D * D;
-    const size_t e = (size_t)(blockIdx.x - tab[k].y) * 256 + threadIdx.x;
-    if (e >= (size_t)J.P * DD) {
-        return;
-    }
-    const int leaf = (int)(e / DD);
-    const int r = (int)((e % DD) / D);
-    const int c = (int)(e % D);
+    const size_t total = (size_t)J.P * DD;
     const int n = J.n;
     const int d = 1 << n;
-    double2 v;
-    if (leaf >= J.L || r >= d || c >= d) {
-        v = make_double2((r == c) ? 1.0 : 0.0, 0.0);
-    } else {
-        const uint32_t src = ring_inv32((uint32_t)r, n);  // (C R)[r] = R[P^-1 r]
-        v = make_double2(1.0, 0.0);
-        const WireCoef *lc = J.coef + (size_t)leaf * n;
-        for (int w = 0; w < n; ++w) {
-            v = cmul(v, u_entry(lc[w], (src >> w) & 1, (c >> w) & 1));
+    const int t = threadIdx.x;
+    const size_t begin = (size_t)(blockIdx.x - tab[item].y) * kLayerChunk;
+    const bool factored = D >= 64;
+    const size_t row0 = begin >> logD;  // first global row (leaf * D + r) of the chunk
+    if (factored) {
+        const int rows = kLayerChunk >> logD;
+        for (int q = t; q < rows * 32; q += 256) {
+            const int lr = q >> 5;
+            const int half = (q >> 4) & 1;
+            const int idx = q & 15;
+            const size_t grow = row0 + lr;
+            const int leaf = (int)(grow >> logD);
+            const int r = (int)(grow & maskD);
+            double2 v = make_double2(1.0, 0.0);
+            if (leaf < J.L && r < d) {
+                const uint32_t src = ring_inv32((uint32_t)r, n);
+                const WireCoef *lc = J.coef + (size_t)leaf * n;
+                const int w0 = half ? 4 : 0;
+                const int w1 = half ? n : 4;
+                for (int w = w0; w < w1; ++w) {
+                    v = cmul(v, u_entry(lc[w], (src >> w) & 1, (idx >> (w - w0)) & 1));
+                }
+            }
+            tables[lr][half][idx] = v;
         }
+        __syncthreads();
     }
-    double *re = J.tree + (size_t)leaf * J.node_doubles;
-    re[(size_t)r * D + c] = v.x;
-    re[DD + (size_t)r * D + c] = v.y;
+    for (int q = t; q < kLayerChunk; q += 256) {
+        const size_t e = begin + q;
+        if (e >= total) {
+            break;
+        }
+        const int leaf = (int)(e >> (2 * logD));
+        const uint32_t rc = (uint32_t)(e & (DD - 1));
+        const int r = (int)(rc >> logD);
+        const int c = (int)(rc & maskD);
+        double2 v;
+        if (leaf >= J.L || r >= d || c >= d) {
+            v = make_double2((r == c) ? 1.0 : 0.0, 0.0);
+        } else if (factored) {
+            const int lr = q >> logD;
+            v = cmul(tables[lr][0][c & 15], tables[lr][1][c >> 4]);
+        } else {
+            const uint32_t src = ring_inv32((uint32_t)r, n);  // (C R)[r] = R[P^-1 r]
+            v = make_double2(1.0, 0.0);
+            const WireCoef *lc = J.coef + (size_t)leaf * n;
+            for (int w = 0; w < n; ++w) {
+                v = cmul(v, u_entry(lc[w], (src >> w) & 1, (c >> w) & 1));
+            }
+        }
+        double *re = J.tree + (size_t)leaf * J.node_doubles;
+        re[rc] = v.x;
+        re[DD + rc] = v.y;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -201,6 +247,136 @@ __global__ __launch_bounds__(256) void zgemm_level_kernel(const DenseJob *__rest
     }
 }
 
+// 64 x 64 output tiles for D >= 64: each of the four waves computes a 32 x 32
+// quadrant (2 x 2 MFMA sub-tiles, A and B fragments shared across them) with
+// three real products per complex one (P1 = Ar Br, P2 = Ai Bi,
+// P3 = (Ar + Ai)(Br + Bi); Cr = P1 - P2, Ci = P3 - P1 - P2), and the next
+// 16-wide k-block is loaded into registers while the current one is
+// multiplied from LDS.
+__global__ __launch_bounds__(256) void zgemm64_level_kernel(const DenseJob *__restrict__ jobs,
+                                                            const int2 *__restrict__ tab,
+                                                            int count, int level) {
+    constexpr int TILE = 64;
+    constexpr int KB = 16;
+    __shared__ double as[3][TILE][KB + 1];  // re, im, re + im
+    __shared__ double bs[3][KB][TILE + 2];
+    const int item = find_item(tab, count, blockIdx.x);
+    const DenseJob &J = jobs[tab[item].x];
+    const int D = J.D;
+    const int local = blockIdx.x - tab[item].y;
+    const int tiles_per_side = D / TILE;
+    const int q = local / (tiles_per_side * tiles_per_side);
+    const int bt = local % (tiles_per_side * tiles_per_side);
+    const int tr = (bt / tiles_per_side) * TILE;
+    const int tc = (bt % tiles_per_side) * TILE;
+    const size_t DD = (size_t)D * D;
+    const size_t nd = J.node_doubles;
+    const double *left = J.tree + (level_node(J.P, level - 1) + 2 * (size_t)q) * nd;
+    const double *ar = left + nd;  // right child
+    const double *ai = ar + DD;
+    const double *br = left;
+    const double *bi = br + DD;
+    double *cr = J.tree + (level_node(J.P, level) + (size_t)q) * nd;
+    double *ci = cr + DD;
+    const int t = threadIdx.x;
+    const int lane = t & 63;
+    const int wave = t >> 6;
+    const int wr = (wave >> 1) * 32;  // quadrant of this wave
+    const int wc = (wave & 1) * 32;
+    // global -> register staging: A rows t/4, cols 4(t%4)..+3; B rows t/16, cols 4(t%16)..+3
+    const int a_row = t >> 2;
+    const int a_col = (t & 3) * 4;
+    const int b_row = t >> 4;
+    const int b_col = (t & 15) * 4;
+    double2 ra[4];
+    double2 rb[4];
+    auto load_block = [&](int k0) {
+        const size_t ao = (size_t)(tr + a_row) * D + k0 + a_col;
+        const size_t bo = (size_t)(k0 + b_row) * D + tc + b_col;
+        ra[0] = *reinterpret_cast<const double2 *>(ar + ao);
+        ra[1] = *reinterpret_cast<const double2 *>(ar + ao + 2);
+        ra[2] = *reinterpret_cast<const double2 *>(ai + ao);
+        ra[3] = *reinterpret_cast<const double2 *>(ai + ao + 2);
+        rb[0] = *reinterpret_cast<const double2 *>(br + bo);
+        rb[1] = *reinterpret_cast<const double2 *>(br + bo + 2);
+        rb[2] = *reinterpret_cast<const double2 *>(bi + bo);
+        rb[3] = *reinterpret_cast<const double2 *>(bi + bo + 2);
+    };
+    double4_t p1[2][2];
+    double4_t p2[2][2];
+    double4_t p3[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            p1[i][j] = (double4_t){0.0, 0.0, 0.0, 0.0};
+            p2[i][j] = (double4_t){0.0, 0.0, 0.0, 0.0};
+            p3[i][j] = (double4_t){0.0, 0.0, 0.0, 0.0};
+        }
+    }
+    load_block(0);
+    for (int k0 = 0; k0 < D; k0 += KB) {
+        __syncthreads();
+        {
+            const double a_re[4] = {ra[0].x, ra[0].y, ra[1].x, ra[1].y};
+            const double a_im[4] = {ra[2].x, ra[2].y, ra[3].x, ra[3].y};
+            const double b_re[4] = {rb[0].x, rb[0].y, rb[1].x, rb[1].y};
+            const double b_im[4] = {rb[2].x, rb[2].y, rb[3].x, rb[3].y};
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                as[0][a_row][a_col + e] = a_re[e];
+                as[1][a_row][a_col + e] = a_im[e];
+                as[2][a_row][a_col + e] = a_re[e] + a_im[e];
+                bs[0][b_row][b_col + e] = b_re[e];
+                bs[1][b_row][b_col + e] = b_im[e];
+                bs[2][b_row][b_col + e] = b_re[e] + b_im[e];
+            }
+        }
+        __syncthreads();
+        if (k0 + KB < D) {
+            load_block(k0 + KB);  // in flight during the products below
+        }
+#pragma unroll 2
+        for (int kk = 0; kk < KB; kk += 4) {
+            double fa[3][2];
+            double fb[3][2];
+#pragma unroll
+            for (int p = 0; p < 3; ++p) {
+#pragma unroll
+                for (int i = 0; i < 2; ++i) {
+                    fa[p][i] = as[p][wr + 16 * i + (lane & 15)][kk + (lane >> 4)];
+                    fb[p][i] = bs[p][kk + (lane >> 4)][wc + 16 * i + (lane & 15)];
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    p1[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[0][i], fb[0][j], p1[i][j],
+                                                                     0, 0, 0);
+                    p2[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[1][i], fb[1][j], p2[i][j],
+                                                                     0, 0, 0);
+                    p3[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[2][i], fb[2][j], p3[i][j],
+                                                                     0, 0, 0);
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int col = tc + wc + 16 * j + (lane & 15);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = tr + wr + 16 * i + 4 * r + (lane >> 4);
+                cr[(size_t)row * D + col] = p1[i][j][r] - p2[i][j][r];
+                ci[(size_t)row * D + col] = p3[i][j][r] - p1[i][j][r] - p2[i][j][r];
+            }
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Root: psi = T e0, lambda = H psi, E = Re<psi|lambda>, eta = T^dagger lambda;
 // also seeds the down-sweep with the root pair [x = e0 | y = eta] in vec_a.
@@ -274,43 +450,93 @@ __global__ __launch_bounds__(256) void dense_head_kernel(const DenseJob *__restr
 
 // One down-sweep level: parent p hands (x, y) to its left child and
 // (N x, N y) to its right child, N = the left child's product.  A circuit's
-// step at level j reads vec_a when (levels - j) is even, else vec_b.
+// step at level j reads vec_a when (levels - j) is even, else vec_b.  One
+// block per parent.  N is row-major; 16-column strips of it are staged
+// through LDS with coalesced loads, and thread r accumulates row r.
 __global__ __launch_bounds__(256) void dense_down_kernel(const DenseJob *__restrict__ jobs,
                                                          const int2 *__restrict__ tab, int count,
                                                          int level) {
+    constexpr int CB = 16;
+    __shared__ double mre[256][CB + 1];
+    __shared__ double mim[256][CB + 1];
+    __shared__ double2 xs[256];
+    __shared__ double2 ys[256];
     const int item = find_item(tab, count, blockIdx.x);
     const DenseJob &J = jobs[tab[item].x];
-    const int p = blockIdx.x - tab[item].y;
-    const int r = threadIdx.x;
     const int D = J.D;
-    if (r >= D) {
-        return;
-    }
+    const int t = threadIdx.x;
     const bool even = ((J.levels - level) & 1) == 0;
     const double2 *vin = even ? J.vec_a : J.vec_b;
     double2 *vout = even ? J.vec_b : J.vec_a;
-    const double2 *x = vin + (size_t)p * 2 * D;
-    const double2 *y = x + D;
-    const double *re = J.tree + (level_node(J.P, level - 1) + 2 * (size_t)p) * J.node_doubles;
-    const double *im = re + (size_t)D * D;
-    double2 nx = make_double2(0.0, 0.0);
-    double2 ny = make_double2(0.0, 0.0);
-    for (int c = 0; c < D; ++c) {
-        const double mr = re[(size_t)r * D + c];
-        const double mi = im[(size_t)r * D + c];
-        const double2 xv = x[c];
-        const double2 yv = y[c];
-        nx.x += mr * xv.x - mi * xv.y;
-        nx.y += mr * xv.y + mi * xv.x;
-        ny.x += mr * yv.x - mi * yv.y;
-        ny.y += mr * yv.y + mi * yv.x;
+    const size_t DD = (size_t)D * D;
+    {
+        const int p = blockIdx.x - tab[item].y;
+        const double2 *x = vin + (size_t)p * 2 * D;
+        const double2 *y = x + D;
+        const double *re = J.tree + (level_node(J.P, level - 1) + 2 * (size_t)p) * J.node_doubles;
+        const double *im = re + DD;
+        if (t < D) {
+            xs[t] = x[t];
+            ys[t] = y[t];
+        }
+        double2 nx = make_double2(0.0, 0.0);
+        double2 ny = make_double2(0.0, 0.0);
+        // strip c0 of N: D rows x CB columns, at most 16 entries per thread
+        // and plane, loaded into registers (all loads in flight at once; the
+        // next strip's while this one is multiplied)
+        constexpr int PER = 256 * CB / 256;
+        const int strip = D * CB;
+        double vr[PER];
+        double vi[PER];
+        auto load_strip = [&](int c0) {
+#pragma unroll
+            for (int i = 0; i < PER; ++i) {
+                const int e = t + 256 * i;
+                if (e < strip) {
+                    const size_t g = (size_t)(e / CB) * D + c0 + (e % CB);
+                    vr[i] = re[g];
+                    vi[i] = im[g];
+                }
+            }
+        };
+        load_strip(0);
+        for (int c0 = 0; c0 < D; c0 += CB) {
+            __syncthreads();
+#pragma unroll
+            for (int i = 0; i < PER; ++i) {
+                const int e = t + 256 * i;
+                if (e < strip) {
+                    mre[e / CB][e % CB] = vr[i];
+                    mim[e / CB][e % CB] = vi[i];
+                }
+            }
+            __syncthreads();
+            if (c0 + CB < D) {
+                load_strip(c0 + CB);
+            }
+            if (t < D) {
+#pragma unroll
+                for (int c = 0; c < CB; ++c) {
+                    const double mr = mre[t][c];
+                    const double mi = mim[t][c];
+                    const double2 xv = xs[c0 + c];
+                    const double2 yv = ys[c0 + c];
+                    nx.x += mr * xv.x - mi * xv.y;
+                    nx.y += mr * xv.y + mi * xv.x;
+                    ny.x += mr * yv.x - mi * yv.y;
+                    ny.y += mr * yv.y + mi * yv.x;
+                }
+            }
+        }
+        if (t < D) {
+            double2 *left = vout + (size_t)(2 * p) * 2 * D;
+            double2 *right = vout + (size_t)(2 * p + 1) * 2 * D;
+            left[t] = xs[t];
+            left[D + t] = ys[t];
+            right[t] = nx;
+            right[D + t] = ny;
+        }
     }
-    double2 *left = vout + (size_t)(2 * p) * 2 * D;
-    double2 *right = vout + (size_t)(2 * p + 1) * 2 * D;
-    left[r] = x[r];
-    left[D + r] = y[r];
-    right[r] = nx;
-    right[D + r] = ny;
 }
 
 // Gradients of layer l from (x_l, y_l) and (x_{l+1}, y_{l+1}).  One block per
@@ -447,6 +673,9 @@ void DensePlan::build(const std::vector<DenseSpec> &specs, int max_tile) {
     if (specs.empty()) {
         return;
     }
+    if (specs.size() > 65535) {
+        fail(RIE_INVALID_ARGUMENT, "at most 65535 dense_scan circuits per batch");
+    }
     size_t total = 0;
     for (const auto &sp : specs) {
         if (sp.n < 1 || sp.n > 8 || sp.L < 1) {
@@ -478,6 +707,10 @@ void DensePlan::build(const std::vector<DenseSpec> &specs, int max_tile) {
         J.n = sp.n;
         J.L = sp.L;
         J.D = g.D;
+        J.log_d = 0;
+        while ((1 << J.log_d) < g.D) {
+            J.log_d++;
+        }
         J.P = g.P;
         J.levels = g.levels;
         jobs_.push_back(J);
@@ -503,7 +736,7 @@ void DensePlan::build(const std::vector<DenseSpec> &specs, int max_tile) {
         }
     };
     add_launch(layers_, [](const DenseJob &J) {
-        return (long)(((size_t)J.P * J.D * J.D + 255) / 256);
+        return (long)(((size_t)J.P * J.D * J.D + kLayerChunk - 1) / kLayerChunk);
     });
     for (int level = 1; level <= max_levels; ++level) {
         for (int tile : {16, 32, 64}) {
@@ -559,7 +792,7 @@ void DensePlan::run(hipStream_t s, bool want_grad, double *energy_override, doub
     for (const Launch &la : gemm_) {
         const int2 *tab = d_tab_ + la.tab_off;
         if (la.tile == 64) {
-            zgemm_level_kernel<64><<<la.blocks, 256, 0, s>>>(d_jobs_, tab, la.count, la.level);
+            zgemm64_level_kernel<<<la.blocks, 256, 0, s>>>(d_jobs_, tab, la.count, la.level);
         } else if (la.tile == 32) {
             zgemm_level_kernel<32><<<la.blocks, 256, 0, s>>>(d_jobs_, tab, la.count, la.level);
         } else {

@@ -26,6 +26,7 @@ struct DenseJob {
     int n;
     int L;
     int D;       // max(2^n, 16)
+    int log_d;   // log2(D)
     int P;       // power of two >= L
     int levels;  // log2(P)
 };
@@ -64,8 +65,8 @@ class DensePlan {
     struct Launch {
         int level = 0;
         int tile = 0;
-        int blocks = 0;
-        int count = 0;
+        int blocks = 0;  // total blocks
+        int count = 0;   // circuits in the launch
         size_t tab_off = 0;
     };
     std::vector<DenseJob> jobs_;
