@@ -52,6 +52,8 @@ def parse_args() -> argparse.Namespace:
     p.add_argument("--schedule", choices=["auto", "stored", "inverse_walk"], default="auto",
                    help="backward schedule (auto: stored when it fits in device memory)")
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--tile-shapes", default="auto",
+                   help="register shape per pass kind fwd,ham,bwd (each 3 or 4) or auto")
     p.add_argument("--no-other-schedule", action="store_true",
                    help="skip timing the other backward schedule")
     p.add_argument("--no-batched", action="store_true",
@@ -212,6 +214,8 @@ def main() -> int:
     engine = ext.RingIsingCudaBackend(n, layers, 1.0, "structured_adjoint", 8, False)
     if args.schedule != "auto":
         engine.set_schedule(args.schedule)
+    if args.tile_shapes != "auto":
+        engine.set_tile_shapes(*[int(x) for x in args.tile_shapes.split(",")])
     stream = torch.cuda.current_stream(dev)
     handle = stream.cuda_stream
 
@@ -311,6 +315,7 @@ def main() -> int:
             "layers": layers,
             "strategy": "structured_adjoint",
             "schedule": stats_schedule,
+            "tile_shapes": list(engine.tile_shapes),
             "parallelism": "single GPU",
         },
         "other_schedule": {"schedule": other,

@@ -237,6 +237,16 @@ class RingIsingCudaBackend {
         return out;
     }
 
+    void set_tile_shapes(int r_fwd, int r_ham, int r_bwd) {
+        check(rie_engine_set_tile_shapes(engine_, r_fwd, r_ham, r_bwd));
+    }
+
+    std::vector<int> tile_shapes() const {
+        std::vector<int> out(3);
+        check(rie_engine_tile_shapes(engine_, out.data()));
+        return out;
+    }
+
     size_t num_qubits() const { return num_qubits_; }
     size_t num_layers() const { return num_layers_; }
 
@@ -512,6 +522,9 @@ PYBIND11_MODULE(_cuda_backend, m) {
         .def("set_kernel_timing", &RingIsingCudaBackend::set_kernel_timing, py::arg("enable"))
         .def("reset_kernel_stats", &RingIsingCudaBackend::reset_kernel_stats)
         .def("kernel_stats", &RingIsingCudaBackend::kernel_stats)
+        .def("set_tile_shapes", &RingIsingCudaBackend::set_tile_shapes, py::arg("r_fwd"),
+             py::arg("r_ham"), py::arg("r_bwd"))
+        .def_property_readonly("tile_shapes", &RingIsingCudaBackend::tile_shapes)
         .def_property_readonly("num_qubits", &RingIsingCudaBackend::num_qubits)
         .def_property_readonly("num_layers", &RingIsingCudaBackend::num_layers);
 

@@ -86,8 +86,7 @@ void Engine::build_plan() {
         return;
     }
     small = false;
-    R = 4;
-    T = 3 * R;
+    T = 12;  // tile bits of every pass (both register shapes)
     const uint64_t tiles = uint64_t{1} << (n - T);
     const uint64_t full = (n >= 64) ? ~uint64_t{0} : ((uint64_t{1} << n) - 1);
     const uint64_t tile_mask = (uint64_t{1} << T) - 1;
@@ -375,14 +374,14 @@ void Engine::structured_forward(hipStream_t s, bool save_ckpts) {
             for (int j = 0; j < K; ++j) {
                 const bool gen = (l == 1 && j == 0);
                 kt_begin(s);
-                launch_fwd_pass(R, false, gen, in, fbuf(l, j), lc, d_coef, passes[1 + j].g,
+                launch_fwd_pass(tile_r_fwd, false, gen, in, fbuf(l, j), lc, d_coef, passes[1 + j].g,
                                 passes[1 + j].grid, s);
                 check_launch("fwd_pass");
                 kt_end(s, gen ? "fwd_pass_high_gen_init" : "fwd_pass_high", gen ? S : 2 * S);
                 in = fbuf(l, j);
             }
             kt_begin(s);
-            launch_fwd_pass(R, true, false, in, fbuf(l, K), lc, d_coef, passes[0].g,
+            launch_fwd_pass(tile_r_fwd, true, false, in, fbuf(l, K), lc, d_coef, passes[0].g,
                             passes[0].grid, s);
             check_launch("fwd_pass scatter");
             kt_end(s, "fwd_pass_low_cnot", 2 * S);
@@ -404,7 +403,7 @@ void Engine::structured_forward(hipStream_t s, bool save_ckpts) {
             const bool gen = (l == 1 && j == 0);
             double2 *out = (cur_buf >= 0) ? pool[cur_buf] : pool[0];
             kt_begin(s);
-            launch_fwd_pass(R, false, gen, cur, out, lc, d_coef, passes[1 + j].g,
+            launch_fwd_pass(tile_r_fwd, false, gen, cur, out, lc, d_coef, passes[1 + j].g,
                             passes[1 + j].grid, s);
             check_launch("fwd_pass");
             kt_end(s, gen ? "fwd_pass_high_gen_init" : "fwd_pass_high", gen ? S : 2 * S);
@@ -426,7 +425,7 @@ void Engine::structured_forward(hipStream_t s, bool save_ckpts) {
             out = pool[out_buf];
         }
         kt_begin(s);
-        launch_fwd_pass(R, true, false, cur, out, lc, d_coef, passes[0].g, passes[0].grid, s);
+        launch_fwd_pass(tile_r_fwd, true, false, cur, out, lc, d_coef, passes[0].g, passes[0].grid, s);
         check_launch("fwd_pass scatter");
         kt_end(s, "fwd_pass_low_cnot", 2 * S);
         cur = out;
@@ -514,7 +513,7 @@ void Engine::structured_hamiltonian(hipStream_t s, bool write_lam) {
     for (size_t p = 0; p < passes.size(); ++p) {
         const bool read_lam = write_lam && p > 0;
         kt_begin(s);
-        launch_ham_pass(R, p == 0, read_lam, write_lam, psi, lam, lam, passes[p].g, h,
+        launch_ham_pass(tile_r_ham, p == 0, read_lam, write_lam, psi, lam, lam, passes[p].g, h,
                         d_epartials + off, passes[p].grid, s);
         check_launch("ham_pass");
         kt_end(s, p == 0 ? "ham_pass_low" : "ham_pass_high",
@@ -551,7 +550,7 @@ void Engine::structured_backward(hipStream_t s, const double *scale, bool full_w
         const bool only_low = passes.size() == 1;
         require_partials((size_t)passes[0].grid * 2 * T);
         kt_begin(s);
-        launch_bwd_pass(R, true, only_low ? need_cur_after : true, true, cur_in, pool[lam], pool[ncur],
+        launch_bwd_pass(tile_r_bwd, true, only_low ? need_cur_after : true, true, cur_in, pool[lam], pool[ncur],
                         pool[nlam], lc, passes[0].g, d_partials, passes[0].grid, s);
         check_launch("bwd_pass gather");
         kt_end(s, "bwd_pass_low_cnot", S * (3.0 + ((only_low && !need_cur_after) ? 0.0 : 1.0)));
@@ -565,7 +564,7 @@ void Engine::structured_backward(hipStream_t s, const double *scale, bool full_w
             const bool write_cur = (p + 1 < passes.size()) || need_cur_after;
             require_partials((size_t)passes[p].grid * 2 * T);
             kt_begin(s);
-            launch_bwd_pass(R, false, write_cur, true, pool[cur], pool[lam], pool[cur], pool[lam], lc,
+            launch_bwd_pass(tile_r_bwd, false, write_cur, true, pool[cur], pool[lam], pool[cur], pool[lam], lc,
                             passes[p].g, d_partials, passes[p].grid, s);
             check_launch("bwd_pass");
             kt_end(s, "bwd_pass_high", S * (3.0 + (write_cur ? 1.0 : 0.0)));
@@ -622,7 +621,7 @@ void Engine::structured_backward_stored(hipStream_t s, const double *scale) {
         const int b = 1 - a;
         require_partials((size_t)passes[0].grid * 2 * T);
         kt_begin(s);
-        launch_bwd_pass(R, true, false, true, fbuf(l, K), lbuf(a), nullptr, lbuf(b), lc,
+        launch_bwd_pass(tile_r_bwd, true, false, true, fbuf(l, K), lbuf(a), nullptr, lbuf(b), lc,
                         passes[0].g, d_partials, passes[0].grid, s);
         check_launch("bwd_pass gather");
         kt_end(s, "bwd_pass_low_cnot", 3 * S);
@@ -634,7 +633,7 @@ void Engine::structured_backward_stored(hipStream_t s, const double *scale) {
             const PassPlan &pp = passes[1 + j];
             require_partials((size_t)pp.grid * 2 * T);
             kt_begin(s);
-            launch_bwd_pass(R, false, false, true, fbuf(l, j), lbuf(b), nullptr, lbuf(b), lc, pp.g,
+            launch_bwd_pass(tile_r_bwd, false, false, true, fbuf(l, j), lbuf(b), nullptr, lbuf(b), lc, pp.g,
                             d_partials, pp.grid, s);
             check_launch("bwd_pass");
             kt_end(s, "bwd_pass_high", 3 * S);
@@ -1180,6 +1179,32 @@ rie_status rie_engine_energy_and_grad_device(rie_engine *engine, const double *d
         e->forward_device(d_params, d_energy, s, true);
         e->backward_device(d_params, nullptr, d_gradient, s);
         e->order_end(s);
+    });
+}
+
+rie_status rie_engine_set_tile_shapes(rie_engine *engine, int r_fwd, int r_ham, int r_bwd) {
+    return guard([&]() {
+        Engine *e = as_engine(engine);
+        for (int r : {r_fwd, r_ham, r_bwd}) {
+            if (r != 3 && r != 4) {
+                rie::fail(RIE_INVALID_ARGUMENT, "tile shapes must be 3 or 4.");
+            }
+        }
+        e->tile_r_fwd = r_fwd;
+        e->tile_r_ham = r_ham;
+        e->tile_r_bwd = r_bwd;
+    });
+}
+
+rie_status rie_engine_tile_shapes(const rie_engine *engine, int *shapes_out3) {
+    return guard([&]() {
+        if (engine == nullptr || shapes_out3 == nullptr) {
+            rie::fail(RIE_INVALID_ARGUMENT, "null pointer argument.");
+        }
+        const Engine *e = reinterpret_cast<const Engine *>(engine);
+        shapes_out3[0] = e->tile_r_fwd;
+        shapes_out3[1] = e->tile_r_ham;
+        shapes_out3[2] = e->tile_r_bwd;
     });
 }
 

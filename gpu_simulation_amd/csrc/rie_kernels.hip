@@ -2,9 +2,10 @@
 //
 // Design (see the header of each kernel):
 //  * Large states stream through HBM in tile passes.  A workgroup loads a
-//    tile of 2^T amplitudes (T = 3R) into registers, 2^R per thread, applies
+//    tile of 2^T amplitudes (T = 12) into registers, 2^R per thread, applies
 //    the rotations of the R tile bits held in registers, and exchanges the
-//    tile through LDS twice so every tile bit is register-resident once.
+//    tile through LDS between its T / R register mappings so every tile bit
+//    is register-resident once.
 //    Global loads/stores are 16-byte-per-lane and contiguous across lanes.
 //  * The ring-CNOT layer is a GF(2)-linear index permutation.  The image of a
 //    low tile (index bits [0, T)) is two contiguous runs, so the permutation
@@ -23,34 +24,36 @@ namespace rie {
 // Tile helpers
 // ---------------------------------------------------------------------------
 
+// A tile is 2^12 amplitudes (T = 12) held as 2^R registers by 2^(T - R)
+// threads, visited in P = T / R register mappings:
+//   R = 4: 256 threads x 16 registers, P = 3;
+//   R = 3: 512 threads x 8 registers, P = 4 (half the registers per thread,
+//          so twice the waves per SIMD, at one more LDS exchange per pass).
 template <int R>
 struct Tile {
-    static constexpr int T = 3 * R;
+    static constexpr int T = 12;
+    static constexpr int P = T / R;
     static constexpr uint32_t SIZE = 1u << T;
-    static constexpr int THREADS = 1 << (2 * R);
+    static constexpr int THREADS = 1 << (T - R);
     static constexpr int REGS = 1 << R;
     static constexpr int WAVES = THREADS / kWave;
     static constexpr uint32_t SLOTS = SIZE + (SIZE >> R);
+    static_assert(P * R == T, "tile bits must split into whole mappings");
 };
 
-// Register/thread mappings of a tile offset o in [0, 2^T):
-//   M = 0 (A): registers hold offset bits [2R, 3R), threads bits [0, 2R)
-//   M = 1 (B): registers hold bits [0, R),  threads bits [R, 3R)
-//   M = 2 (C): registers hold bits [R, 2R), threads bits [0, R) and [2R, 3R)
+// Register/thread mappings of a tile offset o in [0, 2^T): mapping M holds
+// offset bits [lo, lo + R) in registers, lo = T - R for M = 0 (threads hold
+// the contiguous low bits: loads), lo = (M - 1) R for M >= 1; the threads hold
+// the other bits in order.  Mapping P - 1 stores.
 template <int R, int M>
-__device__ __forceinline__ uint32_t omap(uint32_t t, int k) {
-    if constexpr (M == 0) {
-        return t | ((uint32_t)k << (2 * R));
-    } else if constexpr (M == 1) {
-        return (uint32_t)k | (t << R);
-    } else {
-        return (t & ((1u << R) - 1u)) | ((uint32_t)k << R) | ((t >> R) << (2 * R));
-    }
+constexpr int phase_first_bit() {
+    return M == 0 ? Tile<R>::T - R : (M - 1) * R;
 }
 
 template <int R, int M>
-constexpr int phase_first_bit() {
-    return M == 0 ? 2 * R : (M == 1 ? 0 : R);
+__device__ __forceinline__ uint32_t omap(uint32_t t, int k) {
+    constexpr int lo = phase_first_bit<R, M>();
+    return ((t >> lo) << (lo + R)) | ((uint32_t)k << lo) | (t & ((1u << lo) - 1u));
 }
 
 // LDS slot of offset o: one pad amplitude every 2^R keeps the strided
@@ -77,9 +80,10 @@ __device__ __forceinline__ int tile_wire(int j, const PassGeom &G) {
     return j < G.a ? j : G.h0 + (j - G.a);
 }
 
-// Per-thread global index offsets of mappings A and C and the uniform index
-// step of their register index k (k covers offset bits [2R, 3R) in A and
-// [R, 2R) in C, which lie entirely in the low or the high part of the tile).
+// Per-thread global index offsets of the load mapping (0, "A") and the store
+// mapping (P - 1, "C") and the uniform index step of their register index k
+// (k covers offset bits [T - R, T) in A and [(P - 2) R, (P - 1) R) in C, which
+// lie entirely in the low or the high part of the tile).
 struct TileAddr {
     uint64_t offA;
     uint64_t offC;
@@ -90,14 +94,16 @@ struct TileAddr {
 template <int R>
 __device__ __forceinline__ TileAddr tile_addr(uint32_t t, const PassGeom &G) {
     TileAddr ta;
+    constexpr int T = Tile<R>::T;
+    constexpr int lo_c = phase_first_bit<R, Tile<R>::P - 1>();
     ta.offA = tile_index(0, omap<R, 0>(t, 0), G);
-    ta.offC = tile_index(0, omap<R, 2>(t, 0), G);
-    if (G.a >= 3 * R) {
-        ta.strideA = uint64_t{1} << (2 * R);
-        ta.strideC = uint64_t{1} << R;
+    ta.offC = tile_index(0, omap<R, Tile<R>::P - 1>(t, 0), G);
+    if (G.a >= T) {
+        ta.strideA = uint64_t{1} << (T - R);
+        ta.strideC = uint64_t{1} << lo_c;
     } else {
-        ta.strideA = uint64_t{1} << (G.h0 + 2 * R - G.a);
-        ta.strideC = uint64_t{1} << (G.h0 + R - G.a);
+        ta.strideA = uint64_t{1} << (G.h0 + T - R - G.a);
+        ta.strideC = uint64_t{1} << (G.h0 + lo_c - G.a);
     }
     return ta;
 }
@@ -308,10 +314,23 @@ __global__ __launch_bounds__(Tile<R>::THREADS) void fwd_pass_kernel(
             }
         }
         phase_fwd<R, 0>(v, lc, G);
-        exchange<R, 0, 1>(v, lds, t);
-        phase_fwd<R, 1>(v, lc, G);
-        exchange<R, 1, 2>(v, lds, t);
-        phase_fwd<R, 2>(v, lc, G);
+        if constexpr (TL::P == 3) {
+            exchange<R, 0, 1>(v, lds, t);
+            phase_fwd<R, 1>(v, lc, G);
+            exchange<R, 1, 2>(v, lds, t);
+            phase_fwd<R, 2>(v, lc, G);
+        } else {
+            if (G.rot_lo < R) {  // the low tile: offset bits [0, R) rotate too
+                exchange<R, 0, 1>(v, lds, t);
+                phase_fwd<R, 1>(v, lc, G);
+                exchange<R, 1, 2>(v, lds, t);
+            } else {
+                exchange<R, 0, 2>(v, lds, t);
+            }
+            phase_fwd<R, 2>(v, lc, G);
+            exchange<R, 2, 3>(v, lds, t);
+            phase_fwd<R, 3>(v, lc, G);
+        }
         if constexpr (!kScatter) {
             double2 *dst = out + base + ta.offC;
 #pragma unroll
@@ -324,10 +343,10 @@ __global__ __launch_bounds__(Tile<R>::THREADS) void fwd_pass_kernel(
             __syncthreads();
 #pragma unroll
             for (int k = 0; k < TL::REGS; ++k) {
-                lds[lds_slot<R>(ring_fwd32(omap<R, 2>(t, k), TL::T))] = v[k];
+                lds[lds_slot<R>(ring_fwd32(omap<R, TL::P - 1>(t, k), TL::T))] = v[k];
             }
             __syncthreads();
-            // q = t | k << 2R is contiguous in the image run; P(base) has zero
+            // q = t | k << (T - R) is contiguous in the image run; P(base) has zero
             // bits [1, T) and its bit 0 only swaps neighbours inside the run.
             const uint64_t pbase = ring_fwd(base, G.n);
             const uint64_t lo_flip = pbase & 1u;
@@ -338,9 +357,9 @@ __global__ __launch_bounds__(Tile<R>::THREADS) void fwd_pass_kernel(
             for (int k = 0; k < TL::REGS; ++k) {
                 const double2 val = lds[lds_slot<R>(omap<R, 0>(t, k))];
                 if (k < TL::REGS / 2) {
-                    dst0[(uint64_t)k << (2 * R)] = val;
+                    dst0[(uint64_t)k << (TL::T - R)] = val;
                 } else {
-                    dst1[(uint64_t)k << (2 * R)] = val;
+                    dst1[(uint64_t)k << (TL::T - R)] = val;
                 }
             }
         }
@@ -379,8 +398,15 @@ void launch_fwd_pass(int R, bool scatter, bool gen_init, const double2 *in, doub
 // Adjoint tile pass.
 // ---------------------------------------------------------------------------
 
+// Waves per SIMD to budget registers for: two 512-thread workgroups per CU
+// (four waves per SIMD) for the R = 3 shape, whose LDS allows two.
+template <int R>
+constexpr int tile_min_blocks() {
+    return R == 3 ? 4 : 1;
+}
+
 template <int R, bool kGather, bool kWriteCur, bool kWriteLam>
-__global__ __launch_bounds__(Tile<R>::THREADS) void bwd_pass_kernel(
+__global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_pass_kernel(
     const double2 *cur_in, const double2 *lam_in, double2 *cur_out, double2 *lam_out,
     const WireCoef *__restrict__ lc, PassGeom G, double *__restrict__ partials) {
     using TL = Tile<R>;
@@ -408,7 +434,7 @@ __global__ __launch_bounds__(Tile<R>::THREADS) void bwd_pass_kernel(
             const uint64_t run1 = run ^ G.hi_mask;
 #pragma unroll
             for (int k = 0; k < TL::REGS; ++k) {
-                const uint64_t src = (k < TL::REGS / 2 ? run : run1) + ((uint64_t)k << (2 * R));
+                const uint64_t src = (k < TL::REGS / 2 ? run : run1) + ((uint64_t)k << (T - R));
                 l[k] = lam_in[src];
                 c[k] = cur_in[src];
             }
@@ -447,8 +473,16 @@ __global__ __launch_bounds__(Tile<R>::THREADS) void bwd_pass_kernel(
             exchange<R, 1, 2>(c, lds, t);
             exchange<R, 1, 2>(l, lds, t);
             phase_bwd<R, 2>(c, l, lc, G, wave_acc, lane);
-            exchange<R, 2, 0>(c, lds, t);
-            exchange<R, 2, 0>(l, lds, t);
+            if constexpr (TL::P == 4) {
+                exchange<R, 2, 3>(c, lds, t);
+                exchange<R, 2, 3>(l, lds, t);
+                phase_bwd<R, 3>(c, l, lc, G, wave_acc, lane);
+                exchange<R, 3, 0>(c, lds, t);
+                exchange<R, 3, 0>(l, lds, t);
+            } else {
+                exchange<R, 2, 0>(c, lds, t);
+                exchange<R, 2, 0>(l, lds, t);
+            }
             phase_bwd<R, 0>(c, l, lc, G, wave_acc, lane);
             const uint64_t row = base + ta.offA;
 #pragma unroll
@@ -463,12 +497,29 @@ __global__ __launch_bounds__(Tile<R>::THREADS) void bwd_pass_kernel(
             }
         } else {
             phase_bwd<R, 0>(c, l, lc, G, wave_acc, lane);
-            exchange<R, 0, 1>(c, lds, t);
-            exchange<R, 0, 1>(l, lds, t);
-            phase_bwd<R, 1>(c, l, lc, G, wave_acc, lane);
-            exchange<R, 1, 2>(c, lds, t);
-            exchange<R, 1, 2>(l, lds, t);
-            phase_bwd<R, 2>(c, l, lc, G, wave_acc, lane);
+            if constexpr (TL::P == 3) {
+                exchange<R, 0, 1>(c, lds, t);
+                exchange<R, 0, 1>(l, lds, t);
+                phase_bwd<R, 1>(c, l, lc, G, wave_acc, lane);
+                exchange<R, 1, 2>(c, lds, t);
+                exchange<R, 1, 2>(l, lds, t);
+                phase_bwd<R, 2>(c, l, lc, G, wave_acc, lane);
+            } else {
+                if (G.rot_lo < R) {
+                    exchange<R, 0, 1>(c, lds, t);
+                    exchange<R, 0, 1>(l, lds, t);
+                    phase_bwd<R, 1>(c, l, lc, G, wave_acc, lane);
+                    exchange<R, 1, 2>(c, lds, t);
+                    exchange<R, 1, 2>(l, lds, t);
+                } else {
+                    exchange<R, 0, 2>(c, lds, t);
+                    exchange<R, 0, 2>(l, lds, t);
+                }
+                phase_bwd<R, 2>(c, l, lc, G, wave_acc, lane);
+                exchange<R, 2, 3>(c, lds, t);
+                exchange<R, 2, 3>(l, lds, t);
+                phase_bwd<R, 3>(c, l, lc, G, wave_acc, lane);
+            }
             const uint64_t row = base + ta.offC;
 #pragma unroll
             for (int k = 0; k < TL::REGS; ++k) {
@@ -530,7 +581,7 @@ void launch_bwd_pass(int R, bool gather, bool write_cur, bool write_lam, const d
 // ---------------------------------------------------------------------------
 
 template <int R, bool kDiag, bool kReadLam, bool kWriteLam>
-__global__ __launch_bounds__(Tile<R>::THREADS) void ham_pass_kernel(
+__global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void ham_pass_kernel(
     const double2 *__restrict__ psi, const double2 *lam_in, double2 *lam_out, PassGeom G,
     double field, double *__restrict__ e_partials) {
     using TL = Tile<R>;
@@ -541,7 +592,7 @@ __global__ __launch_bounds__(Tile<R>::THREADS) void ham_pass_kernel(
     const TileAddr ta = tile_addr<R>(t, G);
     double e = 0.0;
     // psi of the next tile is loaded while this tile is processed; flips of
-    // the register-resident bits [2R, 3R) come from registers, the others
+    // the register-resident bits [T - R, T) come from registers, the others
     // from the LDS copy of the tile.
     double2 nv[TL::REGS];
     if ((uint64_t)blockIdx.x < G.num_tiles) {
@@ -551,7 +602,7 @@ __global__ __launch_bounds__(Tile<R>::THREADS) void ham_pass_kernel(
             nv[k] = psi[row0 + k * ta.strideA];
         }
     }
-    const int lds_hi = G.rot_hi < 2 * R ? G.rot_hi : 2 * R;
+    const int lds_hi = G.rot_hi < T - R ? G.rot_hi : T - R;
     for (uint64_t g = blockIdx.x; g < G.num_tiles; g += gridDim.x) {
         const uint64_t base = tile_base(g, G, T);
         const uint64_t row = base + ta.offA;
@@ -590,7 +641,7 @@ __global__ __launch_bounds__(Tile<R>::THREADS) void ham_pass_kernel(
             }
 #pragma unroll
             for (int rb = 0; rb < R; ++rb) {
-                const int j = 2 * R + rb;
+                const int j = T - R + rb;
                 if (j >= G.rot_lo && j < G.rot_hi) {
                     const double2 p = v[k ^ (1 << rb)];
                     flips.x += p.x;

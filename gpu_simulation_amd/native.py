@@ -43,6 +43,8 @@ C_ABI_SYMBOLS = (
     "rie_engine_reset_kernel_stats",
     "rie_engine_kernel_stat_count",
     "rie_engine_kernel_stat",
+    "rie_engine_set_tile_shapes",
+    "rie_engine_tile_shapes",
     "rie_batch_create",
     "rie_batch_destroy",
     "rie_batch_size",

@@ -157,6 +157,14 @@ typedef enum rie_schedule {
 rie_status rie_engine_set_schedule(rie_engine *engine, int schedule);
 int rie_engine_schedule(const rie_engine *engine);
 
+/* Register shape of the tile passes (no counterpart in the reference): each
+ * of r_fwd, r_ham, r_bwd is 4 (256 threads x 16 registers per 4096-amplitude
+ * tile, three register mappings) or 3 (512 threads x 8 registers, four
+ * mappings: twice the waves per SIMD, one more LDS exchange).  Results agree
+ * to rounding.  Defaults 3, 3, 4. */
+rie_status rie_engine_set_tile_shapes(rie_engine *engine, int r_fwd, int r_ham, int r_bwd);
+rie_status rie_engine_tile_shapes(const rie_engine *engine, int *shapes_out3);
+
 /* Store the statevector at every k-th layer boundary during the forward pass
  * and restore it there during the backward sweep instead of continuing the
  * inverse walk (k = 0 disables).  Costs one extra state buffer per stored

@@ -70,6 +70,24 @@ def test_structured_tile_passes(oracle, backend_module, n, layers, schedule):
            schedule=schedule)
 
 
+@pytest.mark.parametrize("shapes", [(3, 3, 3), (4, 3, 3), (3, 4, 4)])
+@pytest.mark.parametrize("schedule", ["stored", "inverse_walk"])
+@pytest.mark.parametrize("n,layers", [(13, 2), (16, 3), (22, 2), (25, 2)])
+def test_tile_register_shapes(oracle, backend_module, n, layers, schedule, shapes):
+    params = _params(2100 + n, n, layers)
+    print(f"case shapes={shapes} n={n} L={layers} schedule={schedule}", flush=True)
+    e_ref, g_ref = oracle.energy_and_grad(n, layers, 1.0, params)
+    eng = backend_module.RingIsingCudaBackend(n, layers, 1.0, "structured_adjoint", 8, False)
+    eng.set_schedule(schedule)
+    eng.set_tile_shapes(*shapes)
+    assert list(eng.tile_shapes) == list(shapes)
+    out = eng.energy_and_grad(params.reshape(-1))
+    assert abs(out["energy"] - e_ref) <= TOL
+    assert np.max(np.abs(np.asarray(out["gradient"]) - g_ref)) <= TOL
+    with pytest.raises(ValueError):
+        eng.set_tile_shapes(5, 4, 4)
+
+
 def test_schedule_selection(backend_module):
     small = backend_module.RingIsingCudaBackend(10, 3, 1.0, "structured_adjoint", 8, False)
     assert small.schedule == "inverse_walk"  # one fused kernel, nothing to store
