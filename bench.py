@@ -52,6 +52,8 @@ def parse_args() -> argparse.Namespace:
     p.add_argument("--schedule", choices=["auto", "stored", "inverse_walk"], default="auto",
                    help="backward schedule (auto: stored when it fits in device memory)")
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--no-autotune", action="store_true",
+                   help="keep the default tile shapes and schedule instead of autotuning")
     p.add_argument("--tile-shapes", default="auto",
                    help="register shape per pass kind fwd,ham,bwd (each 3 or 4) or auto")
     p.add_argument("--no-other-schedule", action="store_true",
@@ -212,6 +214,11 @@ def main() -> int:
     energy = torch.zeros(1, dtype=torch.float64, device=dev)
     grad = torch.zeros(2 * n * layers, dtype=torch.float64, device=dev)
     engine = ext.RingIsingCudaBackend(n, layers, 1.0, "structured_adjoint", 8, False)
+    tune = None
+    if not args.no_autotune and args.schedule == "auto" and args.tile_shapes == "auto":
+        t_tune = time.perf_counter()
+        tune = engine.autotune(np.ascontiguousarray(host_params), 1)
+        tune["seconds"] = round(time.perf_counter() - t_tune, 2)
     if args.schedule != "auto":
         engine.set_schedule(args.schedule)
     if args.tile_shapes != "auto":
@@ -318,6 +325,7 @@ def main() -> int:
             "tile_shapes": list(engine.tile_shapes),
             "parallelism": "single GPU",
         },
+        "autotune": tune,
         "other_schedule": {"schedule": other,
                            "ms_per_step": None if other_ms is None else round(other_ms, 3)},
         "roofline": roofline,

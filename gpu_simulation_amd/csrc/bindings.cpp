@@ -241,6 +241,22 @@ class RingIsingCudaBackend {
         check(rie_engine_set_tile_shapes(engine_, r_fwd, r_ham, r_bwd));
     }
 
+    py::object autotune(const FlatArray &params, size_t repeats) {
+        validate_params_shape(num_qubits_, num_layers_, params);
+        const auto view = view_params(params);
+        size_t need = 0;
+        std::string buf(4096, '\0');
+        rie_status st;
+        {
+            py::gil_scoped_release release;
+            st = rie_engine_autotune(engine_, view.ptr, view.size, repeats, &buf[0], buf.size(),
+                                     &need);
+        }
+        check(st);
+        buf.resize(std::strlen(buf.c_str()));
+        return py::module_::import("json").attr("loads")(buf);
+    }
+
     std::vector<int> tile_shapes() const {
         std::vector<int> out(3);
         check(rie_engine_tile_shapes(engine_, out.data()));
@@ -525,6 +541,8 @@ PYBIND11_MODULE(_cuda_backend, m) {
         .def("set_tile_shapes", &RingIsingCudaBackend::set_tile_shapes, py::arg("r_fwd"),
              py::arg("r_ham"), py::arg("r_bwd"))
         .def_property_readonly("tile_shapes", &RingIsingCudaBackend::tile_shapes)
+        .def("autotune", &RingIsingCudaBackend::autotune, py::arg("params"),
+             py::arg("repeats") = 2)
         .def_property_readonly("num_qubits", &RingIsingCudaBackend::num_qubits)
         .def_property_readonly("num_layers", &RingIsingCudaBackend::num_layers);
 

@@ -932,6 +932,93 @@ void Engine::set_checkpoint_interval(size_t every) {
     }
 }
 
+// Times energy_and_grad for every tile register shape per pass kind (3 or 4
+// each) and both backward schedules (when the stored one fits), keeps the
+// fastest and returns a JSON report.  Replaces fixed per-qubit-count kernel
+// choices (reference: structured_register_threads_per_block,
+// cpp/ising_cuda_rotation_kernels.cu:562-575).
+std::string Engine::autotune(const double *params, size_t count, size_t repeats) {
+    if (count != num_params) {
+        fail(RIE_INVALID_ARGUMENT, "num_params does not match the circuit.");
+    }
+    if (repeats == 0) {
+        fail(RIE_INVALID_ARGUMENT, "repeats must be at least 1.");
+    }
+    if (strategy != RIE_STRATEGY_STRUCTURED_ADJOINT || small) {
+        return "{\"tuned\": false}";
+    }
+    hipStream_t s = own_stream;
+    order_begin(s);
+    check(hipMemcpyAsync(d_params, params, sizeof(double) * num_params, hipMemcpyHostToDevice, s),
+          "upload params");
+    std::vector<int> schedules{schedule};
+    if (L > 1) {
+        size_t free_b = 0;
+        size_t total_b = 0;
+        check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+        const double extra = (double)stored_buffer_count() - (double)pool.size();
+        const bool stored_fits =
+            extra <= 0.0 || extra * (double)sizeof(double2) * (double)N <= 0.85 * (double)free_b;
+        schedules = {RIE_SCHEDULE_INVERSE_WALK};
+        if (stored_fits) {
+            schedules.push_back(RIE_SCHEDULE_STORED);
+        }
+    }
+    std::string rep = "{\"tuned\": true, \"ms\": {";
+    double best = -1.0;
+    int best_cfg[4] = {schedule, tile_r_fwd, tile_r_ham, tile_r_bwd};
+    bool first = true;
+    for (int sched : schedules) {
+        set_schedule(sched);
+        for (int rf : {4, 3}) {
+            for (int rh : {4, 3}) {
+                for (int rb : {4, 3}) {
+                    tile_r_fwd = rf;
+                    tile_r_ham = rh;
+                    tile_r_bwd = rb;
+                    forward_device(d_params, d_energy, s, true);
+                    backward_device(d_params, nullptr, d_grad, s);
+                    check(hipEventRecord(ev_a, s), "event");
+                    for (size_t r = 0; r < repeats; ++r) {
+                        forward_device(d_params, d_energy, s, true);
+                        backward_device(d_params, nullptr, d_grad, s);
+                    }
+                    check(hipEventRecord(ev_b, s), "event");
+                    check(hipEventSynchronize(ev_b), "event sync");
+                    float ms = 0.0f;
+                    check(hipEventElapsedTime(&ms, ev_a, ev_b), "elapsed");
+                    const double avg = (double)ms / (double)repeats;
+                    char buf[96];
+                    std::snprintf(buf, sizeof(buf), "%s\"%s/%d%d%d\": %.3f", first ? "" : ", ",
+                                  sched == RIE_SCHEDULE_STORED ? "stored" : "inverse_walk", rf,
+                                  rh, rb, avg);
+                    rep += buf;
+                    first = false;
+                    if (best < 0.0 || avg < best) {
+                        best = avg;
+                        best_cfg[0] = sched;
+                        best_cfg[1] = rf;
+                        best_cfg[2] = rh;
+                        best_cfg[3] = rb;
+                    }
+                }
+            }
+        }
+    }
+    set_schedule(best_cfg[0]);
+    tile_r_fwd = best_cfg[1];
+    tile_r_ham = best_cfg[2];
+    tile_r_bwd = best_cfg[3];
+    order_end(s);
+    char tail[160];
+    std::snprintf(tail, sizeof(tail),
+                  "}, \"best_ms\": %.3f, \"schedule\": \"%s\", \"tile_shapes\": [%d, %d, %d]}", best,
+                  best_cfg[0] == RIE_SCHEDULE_STORED ? "stored" : "inverse_walk", best_cfg[1],
+                  best_cfg[2], best_cfg[3]);
+    rep += tail;
+    return rep;
+}
+
 void Engine::benchmark_forward_graph(const double *params, size_t count, size_t repeats,
                                      double *normal_ms, double *graph_ms) {
     // reference: benchmark_structured_forward_graph (cpp/ising_cuda_backend.cu:444-541)
@@ -1179,6 +1266,25 @@ rie_status rie_engine_energy_and_grad_device(rie_engine *engine, const double *d
         e->forward_device(d_params, d_energy, s, true);
         e->backward_device(d_params, nullptr, d_gradient, s);
         e->order_end(s);
+    });
+}
+
+rie_status rie_engine_autotune(rie_engine *engine, const double *params, size_t num_params,
+                              size_t repeats, char *report_out, size_t report_len,
+                              size_t *report_needed) {
+    return guard([&]() {
+        if (params == nullptr) {
+            rie::fail(RIE_INVALID_ARGUMENT, "null pointer argument.");
+        }
+        const std::string rep = as_engine(engine)->autotune(params, num_params, repeats);
+        if (report_needed != nullptr) {
+            *report_needed = rep.size() + 1;
+        }
+        if (report_out != nullptr && report_len > 0) {
+            const size_t k = std::min(report_len - 1, rep.size());
+            std::memcpy(report_out, rep.data(), k);
+            report_out[k] = '\0';
+        }
     });
 }
 

@@ -192,6 +192,7 @@ struct Engine {
     void benchmark_forward_graph(const double *params, size_t count, size_t repeats,
                                  double *normal_ms, double *graph_ms);
     void copy_state(double *host_out, size_t count);
+    std::string autotune(const double *params, size_t count, size_t repeats);
 };
 
 }  // namespace rie

@@ -45,6 +45,7 @@ C_ABI_SYMBOLS = (
     "rie_engine_kernel_stat",
     "rie_engine_set_tile_shapes",
     "rie_engine_tile_shapes",
+    "rie_engine_autotune",
     "rie_batch_create",
     "rie_batch_destroy",
     "rie_batch_size",

@@ -165,6 +165,17 @@ int rie_engine_schedule(const rie_engine *engine);
 rie_status rie_engine_set_tile_shapes(rie_engine *engine, int r_fwd, int r_ham, int r_bwd);
 rie_status rie_engine_tile_shapes(const rie_engine *engine, int *shapes_out3);
 
+/* Autotuner for the tile-pass path: times energy_and_grad on `params` (host,
+ * num_params values) `repeats` times for each tile shape combination and each
+ * backward schedule that fits in memory, keeps the fastest, and writes a JSON
+ * report (truncated to report_len; report_needed receives the full size).
+ * Engines on the whole-circuit path (n <= 12) or other strategies report
+ * {"tuned": false}.  Replaces the reference's fixed qubit-count heuristics
+ * (cpp/ising_cuda_rotation_kernels.cu:562-575). */
+rie_status rie_engine_autotune(rie_engine *engine, const double *params, size_t num_params,
+                              size_t repeats, char *report_out, size_t report_len,
+                              size_t *report_needed);
+
 /* Store the statevector at every k-th layer boundary during the forward pass
  * and restore it there during the backward sweep instead of continuing the
  * inverse walk (k = 0 disables).  Costs one extra state buffer per stored

@@ -88,6 +88,24 @@ def test_tile_register_shapes(oracle, backend_module, n, layers, schedule, shape
         eng.set_tile_shapes(5, 4, 4)
 
 
+def test_engine_autotune_keeps_results(oracle, backend_module):
+    n, layers = 18, 3
+    params = _params(2200, n, layers)
+    eng = backend_module.RingIsingCudaBackend(n, layers, 1.0, "structured_adjoint", 8, False)
+    rep = eng.autotune(params.reshape(-1), 1)
+    print("engine autotune", rep, flush=True)
+    assert rep["tuned"] is True
+    assert len(rep["ms"]) in (8, 16)
+    assert list(eng.tile_shapes) == rep["tile_shapes"]
+    assert eng.schedule == rep["schedule"]
+    e_ref, g_ref = oracle.energy_and_grad(n, layers, 1.0, params)
+    out = eng.energy_and_grad(params.reshape(-1))
+    assert abs(out["energy"] - e_ref) <= TOL
+    assert np.max(np.abs(np.asarray(out["gradient"]) - g_ref)) <= TOL
+    small = backend_module.RingIsingCudaBackend(8, 2, 1.0, "structured_adjoint", 8, False)
+    assert small.autotune(_params(1, 8, 2).reshape(-1), 1) == {"tuned": False}
+
+
 def test_schedule_selection(backend_module):
     small = backend_module.RingIsingCudaBackend(10, 3, 1.0, "structured_adjoint", 8, False)
     assert small.schedule == "inverse_walk"  # one fused kernel, nothing to store
