@@ -113,3 +113,48 @@ class RingIsingBatch:
 
     def config(self) -> dict:
         return self._b.config()
+
+
+def main(argv=None) -> None:
+    """Evaluate an uneven mix a few times (a fixed workload for profilers):
+
+        python -m gpu_simulation_amd.batch --count 2048 --evals 3 [--dense-only]
+    """
+    import argparse
+    import time
+
+    p = argparse.ArgumentParser(description="evaluate an uneven batch of circuits")
+    p.add_argument("--count", type=int, default=2048)
+    p.add_argument("--seed", type=int, default=3)
+    p.add_argument("--evals", type=int, default=3)
+    p.add_argument("--dense-only", action="store_true", help="keep only the n <= 8 circuits")
+    p.add_argument("--threads", type=int, default=256)
+    p.add_argument("--chunks", default="3,3,3,3,3,3,3,3,3,2,2,3,4,4,4",
+                   help="rotation chunk width per qubit count 0..14")
+    p.add_argument("--dense-tile", type=int, default=64)
+    p.add_argument("--grid", type=int, default=128, help="structured persistent workgroups")
+    p.add_argument("--autotune", action="store_true", help="autotune instead of the fixed config")
+    args = p.parse_args(argv)
+    specs, params = uneven_mix(args.count, seed=args.seed)
+    if args.dense_only:
+        keep = [i for i, s in enumerate(specs) if s.num_qubits <= 8]
+        specs, params = [specs[i] for i in keep], [params[i] for i in keep]
+    batch = RingIsingBatch(specs)
+    batch.set_params(params)
+    if args.autotune:
+        batch.autotune(repeats=2)
+    else:
+        batch.native.set_config(args.threads, [int(c) for c in args.chunks.split(",")],
+                                args.dense_tile, 0 if args.dense_only else args.grid)
+    batch.results()  # synchronise
+    t0 = time.perf_counter()
+    for _ in range(args.evals):
+        batch.native.run_device(True, 0)
+    energies, _ = batch.results()
+    dt = (time.perf_counter() - t0) / max(args.evals, 1)
+    print(f"{len(specs)} circuits, {dt * 1e3:.3f} ms per evaluation of all, "
+          f"mean energy {float(np.mean(energies)):.6f}", flush=True)
+
+
+if __name__ == "__main__":
+    main()
