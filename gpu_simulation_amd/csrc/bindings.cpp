@@ -213,6 +213,12 @@ class RingIsingCudaBackend {
         return rie_engine_schedule(engine_) == RIE_SCHEDULE_STORED ? "stored" : "inverse_walk";
     }
 
+    void set_fused_hamiltonian(int enabled) {
+        check(rie_engine_set_fused_hamiltonian(engine_, enabled));
+    }
+
+    bool fused_hamiltonian() const { return rie_engine_fused_hamiltonian(engine_) == 1; }
+
     void set_kernel_timing(bool enable) { check(rie_engine_set_kernel_timing(engine_, enable)); }
 
     void reset_kernel_stats() { check(rie_engine_reset_kernel_stats(engine_)); }
@@ -534,6 +540,9 @@ PYBIND11_MODULE(_cuda_backend, m) {
         .def("inverse_walk_drift", &RingIsingCudaBackend::inverse_walk_drift, py::arg("params"))
         .def("final_state", &RingIsingCudaBackend::final_state)
         .def("set_schedule", &RingIsingCudaBackend::set_schedule, py::arg("schedule"))
+        .def("set_fused_hamiltonian", &RingIsingCudaBackend::set_fused_hamiltonian,
+             py::arg("enabled"))
+        .def_property_readonly("fused_hamiltonian", &RingIsingCudaBackend::fused_hamiltonian)
         .def_property_readonly("schedule", &RingIsingCudaBackend::schedule)
         .def("set_kernel_timing", &RingIsingCudaBackend::set_kernel_timing, py::arg("enable"))
         .def("reset_kernel_stats", &RingIsingCudaBackend::reset_kernel_stats)

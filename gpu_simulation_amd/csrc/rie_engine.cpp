@@ -118,6 +118,42 @@ void Engine::build_plan() {
         passes.push_back({hg, pass_grid(tiles, max_grid), g0, g0 + bg});
         g0 += bg;
     }
+    // Fused Hamiltonian.  In the pre-CNOT frame of the last layer, X_w of H is
+    // X_w X_{w+1} for w < n - 1 (X_{n-1} touches wires 0, 1 and n - 1).  The
+    // pairs inside the low tile above wire c0 and inside each high group are
+    // added by the adjoint passes; the crossing wires (11, the last wire below
+    // each further high group, n - 1) and wires [0, c0) go to one psi-frame
+    // pass over a scattered tile together with the ZZ diagonal.
+    std::vector<int> sc{T - 1};
+    for (size_t p = 2; p < passes.size(); ++p) {
+        sc.push_back(passes[p].first_wire - 1);
+    }
+    sc.push_back(n - 1);
+    std::sort(sc.begin(), sc.end());
+    sc.erase(std::unique(sc.begin(), sc.end()), sc.end());
+    fuse_ready = sc.size() <= 4 && n > T;
+    if (fuse_ready) {
+        PassGeom hf{};
+        hf.n = n;
+        hf.sc_n = (int)sc.size();
+        hf.a = T - hf.sc_n;
+        hf.h0 = 0;
+        for (int i = 0; i < hf.sc_n; ++i) {
+            hf.sc[i] = sc[(size_t)i];
+        }
+        hf.rot_lo = 0;
+        hf.rot_hi = T;
+        hf.num_tiles = tiles;
+        hf.hi_mask = 0;
+        ham_fused_g = hf;
+        ham_fused_grid = pass_grid(tiles, max_grid);
+        passes[0].g.pair_lo = hf.a;
+        passes[0].g.pair_hi = T;
+        for (size_t p = 1; p < passes.size(); ++p) {
+            passes[p].g.pair_lo = passes[p].g.rot_lo;
+            passes[p].g.pair_hi = passes[p].g.rot_hi;
+        }
+    }
 }
 
 Engine::Engine(size_t num_qubits, size_t num_layers, double field, const std::string &strat,
@@ -185,6 +221,9 @@ Engine::Engine(size_t num_qubits, size_t num_layers, double field, const std::st
         for (const auto &p : passes) {
             eparts += (size_t)p.grid;
         }
+        // the fused evaluation: the scattered pass plus every adjoint pass of
+        // the last layer
+        eparts += (size_t)ham_fused_grid;
         // the layer-0 contraction writes 48 values per workgroup of the low pass
         partial_count = std::max({part, eparts, (size_t)passes[0].grid * 48});
         d_partials = (double *)device_alloc(sizeof(double) * partial_count, "alloc partials");
@@ -523,6 +562,94 @@ void Engine::structured_hamiltonian(hipStream_t s, bool write_lam) {
     energy_partials = off;
 }
 
+void Engine::launch_bwd(int l, bool gather, bool write_cur, const double2 *cur_in,
+                        const double2 *lam_in, double2 *cur_out, double2 *lam_out, size_t p,
+                        hipStream_t s) {
+    const WireCoef *lc = d_coef + (size_t)l * n;
+    const PassPlan &pp = passes[p];
+    if (l != fuse_layer) {
+        launch_bwd_pass(tile_r_bwd, gather, write_cur, true, cur_in, lam_in, cur_out, lam_out, lc,
+                        pp.g, d_partials, pp.grid, s);
+        return;
+    }
+    if (fuse_eoff + (size_t)pp.grid > epartial_count) {
+        fail(RIE_DEVICE_ERROR, "internal: energy partial buffer too small");
+    }
+    launch_bwd_pass(tile_r_bwd, gather, write_cur, true, cur_in, lam_in, cur_out, lam_out, lc,
+                    pp.g, d_partials, pp.grid, s, true, h, d_epartials + fuse_eoff);
+    fuse_eoff += (size_t)pp.grid;
+}
+
+bool Engine::fused_eval() const {
+    return fuse_enabled && fuse_ready && strategy == RIE_STRATEGY_STRUCTURED_ADJOINT && !small &&
+           L >= 2;
+}
+
+void Engine::structured_hamiltonian_fused(hipStream_t s) {
+    // lambda goes where the unfused Hamiltonian puts it
+    double2 *lam = nullptr;
+    if (schedule == RIE_SCHEDULE_STORED) {
+        lam_idx = -1;
+        lam = lbuf(0);
+    } else {
+        lam_idx = free_buffer({final_cur});
+        lam = pool[lam_idx];
+    }
+    if ((size_t)ham_fused_grid > epartial_count) {
+        fail(RIE_DEVICE_ERROR, "internal: energy partial buffer too small");
+    }
+    const double S = (double)sizeof(double2) * (double)N;
+    kt_begin(s);
+    launch_ham_pass_scattered(tile_r_ham, final_state(), lam, ham_fused_g, h, d_epartials,
+                              ham_fused_grid, s);
+    check_launch("ham_pass scattered");
+    kt_end(s, "ham_pass_fused", 2.0 * S);
+    fuse_eoff = (size_t)ham_fused_grid;
+}
+
+// Energy and gradient with the fused Hamiltonian: the energy partials of the
+// scattered pass and of the last layer's adjoint passes are reduced after the
+// backward sweep.
+void Engine::structured_eval_fused(hipStream_t s, const double *scale, double *energy_out,
+                                   bool staged) {
+    if (staged) {
+        stage_begin(s);
+    }
+    structured_forward(s, ckpt_every > 0);
+    if (staged) {
+        stage_end(s, "forward_apply_ms");
+        stage_begin(s);
+    }
+    structured_hamiltonian_fused(s);
+    if (staged) {
+        stage_end(s, "hamiltonian_ms");
+        stage_begin(s);
+        stage_end(s, "gradient_zero_ms");  // gradients are written, not accumulated
+        stage_begin(s);
+    }
+    fuse_layer = L - 1;
+    try {
+        structured_backward(s, scale, false);
+    } catch (...) {
+        fuse_layer = -1;
+        throw;
+    }
+    fuse_layer = -1;
+    if (staged) {
+        stage_end(s, "backward_sweep_ms");
+        stage_begin(s);
+    }
+    energy_partials = fuse_eoff;
+    kt_begin(s);
+    launch_reduce_sum(d_epartials, (int)energy_partials, energy_out, s);
+    check_launch("energy reduce");
+    kt_end(s, "reduce_energy", 0.0);
+    if (staged) {
+        stage_end(s, "energy_dot_ms");
+    }
+    resident = false;
+}
+
 void Engine::structured_backward(hipStream_t s, const double *scale, bool full_walk) {
     // Layers L-1 .. 1 (and 0 in a full walk) run the inverse walk in tile
     // passes: first the low pass, which applies the inverse ring CNOT on load
@@ -539,7 +666,6 @@ void Engine::structured_backward(hipStream_t s, const double *scale, bool full_w
     int lam = lam_idx;
     const int last_walked = full_walk ? 0 : 1;
     for (int l = L - 1; l >= last_walked; --l) {
-        const WireCoef *lc = d_coef + (size_t)l * n;
         // Is the state after un-rotating layer l read again?  Not when the
         // next layer is the contracted layer 0, nor when a checkpoint holds it.
         const bool ckpt_next = ckpt_every > 0 && l >= 2 && (l % (int)ckpt_every) == 0 &&
@@ -550,8 +676,8 @@ void Engine::structured_backward(hipStream_t s, const double *scale, bool full_w
         const bool only_low = passes.size() == 1;
         require_partials((size_t)passes[0].grid * 2 * T);
         kt_begin(s);
-        launch_bwd_pass(tile_r_bwd, true, only_low ? need_cur_after : true, true, cur_in, pool[lam], pool[ncur],
-                        pool[nlam], lc, passes[0].g, d_partials, passes[0].grid, s);
+        launch_bwd(l, true, only_low ? need_cur_after : true, cur_in, pool[lam], pool[ncur],
+                   pool[nlam], 0, s);
         check_launch("bwd_pass gather");
         kt_end(s, "bwd_pass_low_cnot", S * (3.0 + ((only_low && !need_cur_after) ? 0.0 : 1.0)));
         kt_begin(s);
@@ -564,8 +690,7 @@ void Engine::structured_backward(hipStream_t s, const double *scale, bool full_w
             const bool write_cur = (p + 1 < passes.size()) || need_cur_after;
             require_partials((size_t)passes[p].grid * 2 * T);
             kt_begin(s);
-            launch_bwd_pass(tile_r_bwd, false, write_cur, true, pool[cur], pool[lam], pool[cur], pool[lam], lc,
-                            passes[p].g, d_partials, passes[p].grid, s);
+            launch_bwd(l, false, write_cur, pool[cur], pool[lam], pool[cur], pool[lam], p, s);
             check_launch("bwd_pass");
             kt_end(s, "bwd_pass_high", S * (3.0 + (write_cur ? 1.0 : 0.0)));
             kt_begin(s);
@@ -617,12 +742,10 @@ void Engine::structured_backward_stored(hipStream_t s, const double *scale) {
     const int K = (int)passes.size() - 1;
     int a = 0;  // lambda lives in lbuf(a)
     for (int l = L - 1; l >= 1; --l) {
-        const WireCoef *lc = d_coef + (size_t)l * n;
         const int b = 1 - a;
         require_partials((size_t)passes[0].grid * 2 * T);
         kt_begin(s);
-        launch_bwd_pass(tile_r_bwd, true, false, true, fbuf(l, K), lbuf(a), nullptr, lbuf(b), lc,
-                        passes[0].g, d_partials, passes[0].grid, s);
+        launch_bwd(l, true, false, fbuf(l, K), lbuf(a), nullptr, lbuf(b), 0, s);
         check_launch("bwd_pass gather");
         kt_end(s, "bwd_pass_low_cnot", 3 * S);
         kt_begin(s);
@@ -633,8 +756,7 @@ void Engine::structured_backward_stored(hipStream_t s, const double *scale) {
             const PassPlan &pp = passes[1 + j];
             require_partials((size_t)pp.grid * 2 * T);
             kt_begin(s);
-            launch_bwd_pass(tile_r_bwd, false, false, true, fbuf(l, j), lbuf(b), nullptr, lbuf(b), lc, pp.g,
-                            d_partials, pp.grid, s);
+            launch_bwd(l, false, false, fbuf(l, j), lbuf(b), nullptr, lbuf(b), (size_t)(1 + j), s);
             check_launch("bwd_pass");
             kt_end(s, "bwd_pass_high", 3 * S);
             kt_begin(s);
@@ -826,6 +948,12 @@ void Engine::energy_and_grad_host(const double *params, size_t count, bool want_
         stage_begin(s);
         small_run(s, want_grad, nullptr, d_energy, want_grad ? d_grad : nullptr, nullptr);
         stage_end(s, want_grad ? "fused_circuit_ms" : "forward_apply_ms");
+    } else if (want_grad && fused_eval()) {
+        stage_begin(s);
+        launch_coef(d_params, d_coef, n * L, s);
+        check_launch("coef");
+        stage_end(s, "zero_state_ms");
+        structured_eval_fused(s, nullptr, d_energy, true);
     } else {
         const bool gate = strategy == RIE_STRATEGY_INVERSE_WALK_GATE;
         stage_begin(s);
@@ -1263,8 +1391,15 @@ rie_status rie_engine_energy_and_grad_device(rie_engine *engine, const double *d
         }
         hipStream_t s = stream_of(e, stream);
         e->order_begin(s);
-        e->forward_device(d_params, d_energy, s, true);
-        e->backward_device(d_params, nullptr, d_gradient, s);
+        if (e->fused_eval()) {
+            rie::launch_coef(d_params, e->d_coef, e->n * e->L, s);
+            rie::check_launch("coef");
+            e->structured_eval_fused(s, nullptr, d_energy, false);
+            e->copy_grad_if_needed(d_gradient, s);
+        } else {
+            e->forward_device(d_params, d_energy, s, true);
+            e->backward_device(d_params, nullptr, d_gradient, s);
+        }
         e->order_end(s);
     });
 }
@@ -1334,6 +1469,21 @@ rie_status rie_engine_set_schedule(rie_engine *engine, int schedule) {
 
 int rie_engine_schedule(const rie_engine *engine) {
     return engine == nullptr ? -1 : reinterpret_cast<const Engine *>(engine)->schedule;
+}
+
+rie_status rie_engine_set_fused_hamiltonian(rie_engine *engine, int enabled) {
+    return guard([&]() {
+        Engine *e = as_engine(engine);
+        if (enabled != 0 && enabled != 1) {
+            rie::fail(RIE_INVALID_ARGUMENT, "enabled must be 0 or 1.");
+        }
+        e->fuse_enabled = enabled == 1;
+        e->resident = false;
+    });
+}
+
+int rie_engine_fused_hamiltonian(const rie_engine *engine) {
+    return engine == nullptr ? -1 : (reinterpret_cast<const Engine *>(engine)->fused_eval() ? 1 : 0);
 }
 
 rie_status rie_engine_set_kernel_timing(rie_engine *engine, int enable) {

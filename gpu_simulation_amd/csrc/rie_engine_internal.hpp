@@ -88,6 +88,16 @@ struct Engine {
     int tile_r_bwd = 4;
     int T = 12;
     std::vector<PassPlan> passes;  // [0]: low pass (a = T); [1..]: high passes
+    // Fused Hamiltonian of gradient evaluations (L >= 2): one psi-frame pass
+    // over a scattered tile (the ZZ diagonal and the X terms of its wires);
+    // every other X term is an adjacent-wire pair in the pre-CNOT frame of the
+    // last layer, added by that layer's adjoint passes (PassGeom pair ranges).
+    PassGeom ham_fused_g{};
+    int ham_fused_grid = 0;
+    bool fuse_ready = false;   // the plan supports it
+    bool fuse_enabled = true;  // caller's switch
+    int fuse_layer = -1;       // layer whose adjoint passes add pair terms now
+    size_t fuse_eoff = 0;      // next energy-partial slot of the fused passes
 
     // device buffers
     std::vector<double2 *> pool;  // statevector buffers (16 * 2^n bytes each)
@@ -163,6 +173,13 @@ struct Engine {
 
     void structured_forward(hipStream_t s, bool save_ckpts);
     void structured_hamiltonian(hipStream_t s, bool write_lam);
+    bool fused_eval() const;
+    void structured_hamiltonian_fused(hipStream_t s);
+    void structured_eval_fused(hipStream_t s, const double *scale, double *energy_out,
+                               bool staged);
+    void launch_bwd(int l, bool gather, bool write_cur, const double2 *cur_in,
+                    const double2 *lam_in, double2 *cur_out, double2 *lam_out, size_t p,
+                    hipStream_t s);
     void structured_backward(hipStream_t s, const double *scale, bool full_walk);
     void structured_backward_stored(hipStream_t s, const double *scale);
     void layer0_gradients(hipStream_t s, const double2 *lam, const double *scale);

@@ -80,6 +80,26 @@ __device__ __forceinline__ int tile_wire(int j, const PassGeom &G) {
     return j < G.a ? j : G.h0 + (j - G.a);
 }
 
+// Scattered tiles (sc_n > 0): offset bits [0, a) are index bits [0, a), offset
+// bit a + i is index bit sc[i]; the tile number fills the other index bits in
+// order.
+__device__ __forceinline__ uint64_t scat_index(uint32_t o, const PassGeom &G) {
+    uint64_t r = o & ((1u << G.a) - 1u);
+    for (int i = 0; i < G.sc_n; ++i) {
+        r |= (uint64_t)((o >> (G.a + i)) & 1u) << G.sc[i];
+    }
+    return r;
+}
+
+__device__ __forceinline__ uint64_t scat_base(uint64_t g, const PassGeom &G) {
+    uint64_t b = g << G.a;
+    for (int i = 0; i < G.sc_n; ++i) {  // open a zero bit at each sc[i], ascending
+        const int p = G.sc[i];
+        b = (b & ((uint64_t{1} << p) - 1)) | ((b >> p) << (p + 1));
+    }
+    return b;
+}
+
 // Per-thread global index offsets of the load mapping (0, "A") and the store
 // mapping (P - 1, "C") and the uniform index step of their register index k
 // (k covers offset bits [T - R, T) in A and [(P - 2) R, (P - 1) R) in C, which
@@ -405,20 +425,62 @@ constexpr int tile_min_blocks() {
     return R == 3 ? 4 : 1;
 }
 
-template <int R, bool kGather, bool kWriteCur, bool kWriteLam>
+// Transverse-field terms of the adjacent tile-bit pairs [pair_lo, pair_hi):
+// lam += -field * X_j X_{j+1} cur for every pair, with cur in mapping M and
+// the whole tile of cur in LDS at offset order; e accumulates
+// Re(conj(cur) * added).  Pairs inside the register bits of M read registers.
+template <int R, int M>
+__device__ __forceinline__ void add_pair_terms(const double2 (&c)[Tile<R>::REGS],
+                                               double2 (&l)[Tile<R>::REGS], const double2 *lds,
+                                               uint32_t t, const PassGeom &G, double field,
+                                               double &e) {
+    constexpr int lo = phase_first_bit<R, M>();
+    // one pair at a time keeps a single partner amplitude per register live
+    for (int j = G.pair_lo; j + 1 < G.pair_hi; ++j) {
+        if (j >= lo && j + 1 < lo + R) {
+            continue;  // both bits in registers: below
+        }
+        const uint32_t m = 3u << j;
+#pragma unroll
+        for (int k = 0; k < Tile<R>::REGS; ++k) {
+            const double2 p = lds[lds_slot<R>(omap<R, M>(t, k) ^ m)];
+            l[k].x -= field * p.x;
+            l[k].y -= field * p.y;
+            e -= field * (c[k].x * p.x + c[k].y * p.y);
+        }
+    }
+#pragma unroll
+    for (int rb = 0; rb + 1 < R; ++rb) {
+        const int j = lo + rb;
+        if (j >= G.pair_lo && j + 1 < G.pair_hi) {
+#pragma unroll
+            for (int k = 0; k < Tile<R>::REGS; ++k) {
+                const double2 p = c[k ^ (3 << rb)];
+                l[k].x -= field * p.x;
+                l[k].y -= field * p.y;
+                e -= field * (c[k].x * p.x + c[k].y * p.y);
+            }
+        }
+    }
+}
+
+template <int R, bool kGather, bool kWriteCur, bool kWriteLam, bool kHam>
 __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_pass_kernel(
     const double2 *cur_in, const double2 *lam_in, double2 *cur_out, double2 *lam_out,
-    const WireCoef *__restrict__ lc, PassGeom G, double *__restrict__ partials) {
+    const WireCoef *__restrict__ lc, PassGeom G, double *__restrict__ partials, double field,
+    double *__restrict__ e_partials) {
     using TL = Tile<R>;
     constexpr int T = TL::T;
     __shared__ double2 lds[TL::SLOTS];
     __shared__ double acc[TL::WAVES][2 * T];
+    __shared__ double ered[kHam ? TL::WAVES : 1];
     const uint32_t t = threadIdx.x;
     const int lane = t & (kWave - 1);
     double *wave_acc = acc[t / kWave];
     if (lane < 2 * T) {
         wave_acc[lane] = 0.0;
     }
+    double e = 0.0;
 
     const TileAddr ta = tile_addr<R>(t, G);
     for (uint64_t g = blockIdx.x; g < G.num_tiles; g += gridDim.x) {
@@ -458,6 +520,10 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_pa
             for (int k = 0; k < TL::REGS; ++k) {
                 c[k] = lds[lds_slot<R>(omap<R, 1>(t, k))];
             }
+            if constexpr (kHam) {
+                // the LDS still holds the gathered tile of cur in offset order
+                add_pair_terms<R, 1>(c, l, lds, t, G, field, e);
+            }
         } else {
             const uint64_t row = base + ta.offA;
 #pragma unroll
@@ -465,6 +531,15 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_pa
                 const uint64_t idx = row + k * ta.strideA;
                 l[k] = lam_in[idx];
                 c[k] = cur_in[idx];
+            }
+            if constexpr (kHam) {
+                __syncthreads();
+#pragma unroll
+                for (int k = 0; k < TL::REGS; ++k) {
+                    lds[lds_slot<R>(omap<R, 0>(t, k))] = c[k];
+                }
+                __syncthreads();
+                add_pair_terms<R, 0>(c, l, lds, t, G, field, e);
             }
         }
 
@@ -533,6 +608,12 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_pa
             }
         }
     }
+    if constexpr (kHam) {
+        const double ws = wave_sum(e);
+        if (lane == 0) {
+            ered[t / kWave] = ws;
+        }
+    }
     __syncthreads();
     if (t < 2 * T) {
         double total = 0.0;
@@ -542,37 +623,50 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_pa
         }
         partials[(uint64_t)blockIdx.x * (2 * T) + t] = total;
     }
+    if constexpr (kHam) {
+        if (t == 0) {
+            double total = 0.0;
+            for (int w = 0; w < TL::WAVES; ++w) {
+                total += ered[w];
+            }
+            e_partials[blockIdx.x] = total;
+        }
+    }
 }
 
 template <int R>
 static void launch_bwd_pass_r(bool gather, bool write_cur, bool write_lam, const double2 *cur_in,
                               const double2 *lam_in, double2 *cur_out, double2 *lam_out,
                               const WireCoef *lc, const PassGeom &g, double *partials, int grid,
-                              hipStream_t s) {
+                              hipStream_t s, bool ham, double field, double *e_partials) {
     constexpr int TH = Tile<R>::THREADS;
-#define RIE_BWD(GA, WC, WL)                                                                 \
-    if (gather == GA && write_cur == WC && write_lam == WL) {                               \
-        bwd_pass_kernel<R, GA, WC, WL><<<grid, TH, 0, s>>>(cur_in, lam_in, cur_out, lam_out, \
-                                                          lc, g, partials);                 \
-        return;                                                                             \
+#define RIE_BWD(GA, WC, WL, HM)                                                              \
+    if (gather == GA && write_cur == WC && write_lam == WL && ham == HM) {                   \
+        bwd_pass_kernel<R, GA, WC, WL, HM><<<grid, TH, 0, s>>>(                              \
+            cur_in, lam_in, cur_out, lam_out, lc, g, partials, field, e_partials);           \
+        return;                                                                              \
     }
-    RIE_BWD(true, true, true)
-    RIE_BWD(false, true, true)
-    RIE_BWD(true, false, true)
-    RIE_BWD(false, false, true)
+    RIE_BWD(true, true, true, false)
+    RIE_BWD(false, true, true, false)
+    RIE_BWD(true, false, true, false)
+    RIE_BWD(false, false, true, false)
+    RIE_BWD(true, true, true, true)
+    RIE_BWD(false, true, true, true)
+    RIE_BWD(true, false, true, true)
+    RIE_BWD(false, false, true, true)
 #undef RIE_BWD
 }
 
 void launch_bwd_pass(int R, bool gather, bool write_cur, bool write_lam, const double2 *cur_in,
                      const double2 *lam_in, double2 *cur_out, double2 *lam_out,
                      const WireCoef *layer_coef, const PassGeom &g, double *partials, int grid,
-                     hipStream_t s) {
+                     hipStream_t s, bool ham_pairs, double field, double *e_partials) {
     if (R == 4) {
         launch_bwd_pass_r<4>(gather, write_cur, write_lam, cur_in, lam_in, cur_out, lam_out,
-                             layer_coef, g, partials, grid, s);
+                             layer_coef, g, partials, grid, s, ham_pairs, field, e_partials);
     } else {
         launch_bwd_pass_r<3>(gather, write_cur, write_lam, cur_in, lam_in, cur_out, lam_out,
-                             layer_coef, g, partials, grid, s);
+                             layer_coef, g, partials, grid, s, ham_pairs, field, e_partials);
     }
 }
 
@@ -580,7 +674,7 @@ void launch_bwd_pass(int R, bool gather, bool write_cur, bool write_lam, const d
 // Hamiltonian tile pass.
 // ---------------------------------------------------------------------------
 
-template <int R, bool kDiag, bool kReadLam, bool kWriteLam>
+template <int R, bool kDiag, bool kReadLam, bool kWriteLam, bool kScat>
 __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void ham_pass_kernel(
     const double2 *__restrict__ psi, const double2 *lam_in, double2 *lam_out, PassGeom G,
     double field, double *__restrict__ e_partials) {
@@ -589,22 +683,49 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void ham_pa
     __shared__ double2 lds[TL::SLOTS];
     __shared__ double red[TL::WAVES];
     const uint32_t t = threadIdx.x;
-    const TileAddr ta = tile_addr<R>(t, G);
+    // Index of register k of the load mapping relative to the thread's row:
+    // a uniform stride for contiguous high bits, a deposit of k's bits into
+    // the scattered positions otherwise (the offset bits are disjoint).
+    TileAddr ta{};
+    uint64_t koff[kScat ? TL::REGS : 1];
+    if constexpr (kScat) {
+        ta.offA = scat_index(omap<R, 0>(t, 0), G);
+#pragma unroll
+        for (int k = 0; k < TL::REGS; ++k) {
+            koff[k] = scat_index((uint32_t)k << (T - R), G);
+        }
+    } else {
+        ta = tile_addr<R>(t, G);
+    }
+    auto eidx = [&](uint64_t row, int k) -> uint64_t {
+        if constexpr (kScat) {
+            return row + koff[k];
+        } else {
+            return row + k * ta.strideA;
+        }
+    };
+    auto tbase = [&](uint64_t g) -> uint64_t {
+        if constexpr (kScat) {
+            return scat_base(g, G);
+        } else {
+            return tile_base(g, G, T);
+        }
+    };
     double e = 0.0;
     // psi of the next tile is loaded while this tile is processed; flips of
     // the register-resident bits [T - R, T) come from registers, the others
     // from the LDS copy of the tile.
     double2 nv[TL::REGS];
     if ((uint64_t)blockIdx.x < G.num_tiles) {
-        const uint64_t row0 = tile_base(blockIdx.x, G, T) + ta.offA;
+        const uint64_t row0 = tbase(blockIdx.x) + ta.offA;
 #pragma unroll
         for (int k = 0; k < TL::REGS; ++k) {
-            nv[k] = psi[row0 + k * ta.strideA];
+            nv[k] = psi[eidx(row0, k)];
         }
     }
     const int lds_hi = G.rot_hi < T - R ? G.rot_hi : T - R;
     for (uint64_t g = blockIdx.x; g < G.num_tiles; g += gridDim.x) {
-        const uint64_t base = tile_base(g, G, T);
+        const uint64_t base = tbase(g);
         const uint64_t row = base + ta.offA;
         double2 v[TL::REGS];
         double2 prev[kReadLam ? TL::REGS : 1];
@@ -612,15 +733,15 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void ham_pa
         for (int k = 0; k < TL::REGS; ++k) {
             v[k] = nv[k];
             if constexpr (kReadLam) {
-                prev[k] = lam_in[row + k * ta.strideA];
+                prev[k] = lam_in[eidx(row, k)];
             }
         }
         const uint64_t gn = g + gridDim.x;
         if (gn < G.num_tiles) {
-            const uint64_t rown = tile_base(gn, G, T) + ta.offA;
+            const uint64_t rown = tbase(gn) + ta.offA;
 #pragma unroll
             for (int k = 0; k < TL::REGS; ++k) {
-                nv[k] = psi[rown + k * ta.strideA];
+                nv[k] = psi[eidx(rown, k)];
             }
         }
         __syncthreads();
@@ -632,7 +753,7 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void ham_pa
 #pragma unroll
         for (int k = 0; k < TL::REGS; ++k) {
             const uint32_t o = omap<R, 0>(t, k);
-            const uint64_t idx = row + k * ta.strideA;
+            const uint64_t idx = eidx(row, k);
             double2 flips = make_double2(0.0, 0.0);
             for (int j = G.rot_lo; j < lds_hi; ++j) {
                 const double2 p = lds[lds_slot<R>(o ^ (1u << j))];
@@ -686,8 +807,8 @@ static void launch_ham_pass_r(bool diag, bool read_lam, bool write_lam, const do
     constexpr int TH = Tile<R>::THREADS;
 #define RIE_HAM(DG, RL, WL)                                                                  \
     if (diag == DG && read_lam == RL && write_lam == WL) {                                   \
-        ham_pass_kernel<R, DG, RL, WL><<<grid, TH, 0, s>>>(psi, lam_in, lam_out, g, field,   \
-                                                           e_partials);                      \
+        ham_pass_kernel<R, DG, RL, WL, false><<<grid, TH, 0, s>>>(psi, lam_in, lam_out, g,   \
+                                                                  field, e_partials);        \
         return;                                                                              \
     }
     RIE_HAM(true, false, true)
@@ -695,6 +816,17 @@ static void launch_ham_pass_r(bool diag, bool read_lam, bool write_lam, const do
     RIE_HAM(true, false, false)
     RIE_HAM(false, false, false)
 #undef RIE_HAM
+}
+
+void launch_ham_pass_scattered(int R, const double2 *psi, double2 *lam_out, const PassGeom &g,
+                               double field, double *e_partials, int grid, hipStream_t s) {
+    if (R == 4) {
+        ham_pass_kernel<4, true, false, true, true>
+            <<<grid, Tile<4>::THREADS, 0, s>>>(psi, nullptr, lam_out, g, field, e_partials);
+    } else {
+        ham_pass_kernel<3, true, false, true, true>
+            <<<grid, Tile<3>::THREADS, 0, s>>>(psi, nullptr, lam_out, g, field, e_partials);
+    }
 }
 
 void launch_ham_pass(int R, bool diag, bool read_lam, bool write_lam, const double2 *psi,

@@ -23,6 +23,15 @@ struct PassGeom {
     int rot_hi;
     uint64_t num_tiles;
     uint64_t hi_mask;  // index bits [T, n): the ring-CNOT image of a low tile
+    // Adjacent tile-bit pairs (j, j + 1), pair_lo <= j < pair_hi - 1, whose
+    // transverse-field terms a fused adjoint pass adds before un-rotating.
+    int pair_lo;
+    int pair_hi;
+    // Scattered tile (Hamiltonian pass only): when sc_n > 0, offset bits
+    // [a, a + sc_n) are the index bits sc[0] < sc[1] < ... (all >= a), and h0
+    // is unused.
+    int sc_n;
+    int sc[4];
 };
 
 // One independent circuit evaluated by a single workgroup (n <= 14).
@@ -104,10 +113,15 @@ void launch_fwd_pass(int R, bool scatter, bool gen_init, const double2 *in, doub
 // per-workgroup gradient partials [grid][2T].  gather = apply the inverse
 // ring-CNOT on load (low pass, out of place); write_cur / write_lam = store
 // the un-rotated state / adjoint state.
+// ham_pairs: before un-rotating, add the transverse-field terms of the
+// tile-bit pairs [g.pair_lo, g.pair_hi) to lam (-field * X_j X_{j+1} cur) and
+// write per-workgroup energy partials of those terms into e_partials.  In the
+// pre-CNOT frame of the last layer these pairs are the X terms X_j of H.
 void launch_bwd_pass(int R, bool gather, bool write_cur, bool write_lam, const double2 *cur_in,
                      const double2 *lam_in, double2 *cur_out, double2 *lam_out,
                      const WireCoef *layer_coef, const PassGeom &g, double *partials, int grid,
-                     hipStream_t s);
+                     hipStream_t s, bool ham_pairs = false, double field = 0.0,
+                     double *e_partials = nullptr);
 
 // Hamiltonian tile pass: adds the X terms of the pass's wires (and the ZZ
 // diagonal if diag) of H psi, optionally into lam, and writes per-workgroup
@@ -115,6 +129,10 @@ void launch_bwd_pass(int R, bool gather, bool write_cur, bool write_lam, const d
 void launch_ham_pass(int R, bool diag, bool read_lam, bool write_lam, const double2 *psi,
                      const double2 *lam_in, double2 *lam_out, const PassGeom &g, double field,
                      double *e_partials, int grid, hipStream_t s);
+// The same with a scattered tile (g.sc_n > 0): the ZZ diagonal and the X terms
+// of all 12 tile wires, written into lam.
+void launch_ham_pass_scattered(int R, const double2 *psi, double2 *lam_out, const PassGeom &g,
+                               double field, double *e_partials, int grid, hipStream_t s);
 
 // Layer-0 gradients by contracting the adjoint state with the layer-0 product
 // state: one read pass over lambda (gathered through the inverse ring CNOT

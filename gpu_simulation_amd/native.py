@@ -39,6 +39,8 @@ C_ABI_SYMBOLS = (
     "rie_engine_copy_state",
     "rie_engine_set_schedule",
     "rie_engine_schedule",
+    "rie_engine_set_fused_hamiltonian",
+    "rie_engine_fused_hamiltonian",
     "rie_engine_set_kernel_timing",
     "rie_engine_reset_kernel_stats",
     "rie_engine_kernel_stat_count",
@@ -81,12 +83,28 @@ def ensure_built() -> None:
         _build.build()
 
 
+def _share_hip_runtime() -> None:
+    """Load PyTorch's HIP runtime first when PyTorch is installed.
+
+    PyTorch ships its own ``libamdhip64`` with the same soname as the ROCm one
+    ``librie.so`` links; the process keeps whichever is loaded first for both.
+    Loading PyTorch first keeps its runtime, so tensors it allocates and the
+    engine's kernels share one HIP runtime and PyTorch's device stays usable.
+    """
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
 def load_library() -> ctypes.CDLL:
     ensure_built()
+    _share_hip_runtime()
     return ctypes.CDLL(str(LIB_PATH))
 
 
 def load_extension():
-    """Import the pybind11 module ``_cuda_backend`` built from this tree."""
+    """Import the pybind11 module ``_cuda_backend`` built from this tree (its
+    package imports PyTorch first, as ``load_library`` does)."""
     ensure_built()
     return importlib.import_module("gpu_simulation_amd.standalone_backend._cuda_backend")

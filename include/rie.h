@@ -157,6 +157,17 @@ typedef enum rie_schedule {
 rie_status rie_engine_set_schedule(rie_engine *engine, int schedule);
 int rie_engine_schedule(const rie_engine *engine);
 
+/* Fused Hamiltonian for gradient evaluations of the tile-pass path with at
+ * least two layers (no counterpart in the reference; on by default): one
+ * Hamiltonian pass over a scattered tile (ZZ diagonal and the X terms of 12
+ * wires) replaces the per-group Hamiltonian passes, and the remaining X terms
+ * are added, as adjacent-wire pairs of the pre-CNOT frame, by the last
+ * layer's adjoint passes (cpp/ising_cuda_kernels.cu:313-358 computes all of
+ * lambda = H psi in one kernel).  Energies and gradients agree with the
+ * unfused evaluation to rounding.  enabled: 1 on, 0 off. */
+rie_status rie_engine_set_fused_hamiltonian(rie_engine *engine, int enabled);
+int rie_engine_fused_hamiltonian(const rie_engine *engine);
+
 /* Register shape of the tile passes (no counterpart in the reference): each
  * of r_fwd, r_ham, r_bwd is 4 (256 threads x 16 registers per 4096-amplitude
  * tile, three register mappings) or 3 (512 threads x 8 registers, four

@@ -88,6 +88,59 @@ def test_tile_register_shapes(oracle, backend_module, n, layers, schedule, shape
         eng.set_tile_shapes(5, 4, 4)
 
 
+@pytest.mark.parametrize("schedule", ["stored", "inverse_walk"])
+@pytest.mark.parametrize("n,layers", [(13, 2), (14, 3), (16, 2), (21, 2), (22, 2), (22, 4),
+                                      (25, 2)])
+def test_fused_hamiltonian(oracle, backend_module, n, layers, schedule):
+    # the fused evaluation (one scattered Hamiltonian pass, the other X terms
+    # added by the last layer's adjoint passes) against the oracle and against
+    # the per-group Hamiltonian passes; n = 13 / 21 / 22 / 25 give one narrow,
+    # one full, two even and two uneven high groups
+    params = _params(2300 + n, n, layers)
+    print(f"case fused n={n} L={layers} schedule={schedule}", flush=True)
+    e_ref, g_ref = oracle.energy_and_grad(n, layers, 0.8, params)
+    eng = backend_module.RingIsingCudaBackend(n, layers, 0.8, "structured_adjoint", 8, False)
+    eng.set_schedule(schedule)
+    assert eng.fused_hamiltonian is True
+    fused = eng.energy_and_grad(params.reshape(-1))
+    eng.set_fused_hamiltonian(False)
+    assert eng.fused_hamiltonian is False
+    plain = eng.energy_and_grad(params.reshape(-1))
+    for out in (fused, plain):
+        assert abs(out["energy"] - e_ref) <= TOL
+        assert np.max(np.abs(np.asarray(out["gradient"]) - g_ref)) <= TOL
+    assert abs(fused["energy"] - plain["energy"]) <= 1e-12
+    assert np.max(np.abs(np.asarray(fused["gradient"]) - np.asarray(plain["gradient"]))) <= 1e-12
+    eng.set_fused_hamiltonian(True)
+    again = eng.energy_and_grad(params.reshape(-1))
+    assert again["energy"] == fused["energy"]
+    assert np.array_equal(np.asarray(again["gradient"]), np.asarray(fused["gradient"]))
+
+
+def test_fused_hamiltonian_device_path_and_scope(oracle, backend_module):
+    import torch
+    n, layers = 18, 3
+    params = _params(2400, n, layers)
+    e_ref, g_ref = oracle.energy_and_grad(n, layers, 1.0, params)
+    eng = backend_module.RingIsingCudaBackend(n, layers, 1.0, "structured_adjoint", 8, False)
+    p = torch.tensor(params.reshape(-1), dtype=torch.float64, device="cuda")
+    e = torch.empty(1, dtype=torch.float64, device="cuda")
+    g = torch.empty_like(p)
+    stream = torch.cuda.current_stream().cuda_stream
+    eng.energy_and_grad_device(p.data_ptr(), e.data_ptr(), g.data_ptr(), stream)
+    torch.cuda.synchronize()
+    assert abs(float(e.item()) - e_ref) <= TOL
+    assert np.max(np.abs(g.cpu().numpy() - g_ref)) <= TOL
+    # one layer (product state + contraction) and the whole-circuit kernel keep
+    # their own evaluation
+    assert backend_module.RingIsingCudaBackend(16, 1, 1.0, "structured_adjoint", 8,
+                                               False).fused_hamiltonian is False
+    assert backend_module.RingIsingCudaBackend(10, 3, 1.0, "structured_adjoint", 8,
+                                               False).fused_hamiltonian is False
+    with pytest.raises(ValueError):
+        eng.set_fused_hamiltonian(2)
+
+
 def test_engine_autotune_keeps_results(oracle, backend_module):
     n, layers = 18, 3
     params = _params(2200, n, layers)
