@@ -58,6 +58,8 @@ def parse_args() -> argparse.Namespace:
                    help="register shape per pass kind fwd,ham,bwd (each 3 or 4) or auto")
     p.add_argument("--no-other-schedule", action="store_true",
                    help="skip timing the other backward schedule")
+    p.add_argument("--no-fused-hamiltonian", action="store_true",
+                   help="per-group Hamiltonian passes instead of the fused Hamiltonian")
     p.add_argument("--no-batched", action="store_true",
                    help="skip the batched uneven-mix measurement")
     p.add_argument("--batch-count", type=int, default=2048)
@@ -214,6 +216,8 @@ def main() -> int:
     energy = torch.zeros(1, dtype=torch.float64, device=dev)
     grad = torch.zeros(2 * n * layers, dtype=torch.float64, device=dev)
     engine = ext.RingIsingCudaBackend(n, layers, 1.0, "structured_adjoint", 8, False)
+    if args.no_fused_hamiltonian:
+        engine.set_fused_hamiltonian(0)
     tune = None
     if not args.no_autotune and args.schedule == "auto" and args.tile_shapes == "auto":
         t_tune = time.perf_counter()
@@ -323,6 +327,7 @@ def main() -> int:
             "strategy": "structured_adjoint",
             "schedule": stats_schedule,
             "tile_shapes": list(engine.tile_shapes),
+            "fused_hamiltonian": bool(engine.fused_hamiltonian),
             "parallelism": "single GPU",
         },
         "autotune": tune,

@@ -131,6 +131,8 @@ void Engine::build_plan() {
     sc.push_back(n - 1);
     std::sort(sc.begin(), sc.end());
     sc.erase(std::unique(sc.begin(), sc.end()), sc.end());
+    // at most four crossing wires: the low pass's pairs then start at tile
+    // bit T - 4 or above, which its kernel relies on
     fuse_ready = sc.size() <= 4 && n > T;
     if (fuse_ready) {
         PassGeom hf{};

@@ -422,46 +422,76 @@ void launch_fwd_pass(int R, bool scatter, bool gen_init, const double2 *in, doub
 // (four waves per SIMD) for the R = 3 shape, whose LDS allows two.
 template <int R>
 constexpr int tile_min_blocks() {
-    return R == 3 ? 4 : 1;
+    return R == 3 ? 4 : 2;
 }
 
-// Transverse-field terms of the adjacent tile-bit pairs [pair_lo, pair_hi):
-// lam += -field * X_j X_{j+1} cur for every pair, with cur in mapping M and
-// the whole tile of cur in LDS at offset order; e accumulates
-// Re(conj(cur) * added).  Pairs inside the register bits of M read registers.
+// Register window of mapping M as a mask of tile offset bits.
+template <int R, int M>
+constexpr uint32_t window_mask() {
+    return ((1u << R) - 1u) << phase_first_bit<R, M>();
+}
+
+// Transverse-field terms of the adjacent tile-bit pairs (j, j + 1),
+// pair_lo <= j < pair_hi - 1: lam += -field * X_j X_{j+1} cur, and e gets
+// their energy Re(conj(cur) * term).  Called before un-rotating the window of
+// mapping M, it adds the pairs that touch the window and no bit of `done`
+// (the windows un-rotated earlier in the pass, whose pairs were added then):
+// each pair is added once, while both of its bits are still rotated, so it
+// commutes with everything un-rotated before.  Pairs inside the window read
+// registers; pairs with one bit outside read the LDS copy of the tile, which
+// must hold cur at this point in offset order.
+// The low (gather) pass's pairs start at tile bit T - 4 or above (the
+// scattered Hamiltonian tile takes at most four crossing wires), so windows
+// wholly below bit T - 5 have none.
+template <int R, int M>
+constexpr bool gather_pairs_reach() {
+    return phase_first_bit<R, M>() + R - 1 >= Tile<R>::T - 4;
+}
+
 template <int R, int M>
 __device__ __forceinline__ void add_pair_terms(const double2 (&c)[Tile<R>::REGS],
                                                double2 (&l)[Tile<R>::REGS], const double2 *lds,
-                                               uint32_t t, const PassGeom &G, double field,
-                                               double &e) {
+                                               uint32_t t, const PassGeom &G, uint32_t done,
+                                               double field, double &e) {
+    constexpr int T = Tile<R>::T;
     constexpr int lo = phase_first_bit<R, M>();
-    // one pair at a time keeps a single partner amplitude per register live
-    for (int j = G.pair_lo; j + 1 < G.pair_hi; ++j) {
-        if (j >= lo && j + 1 < lo + R) {
-            continue;  // both bits in registers: below
+    double es = 0.0;
+    // the two pairs across the window's edges: one bit in registers, the
+    // partner from LDS
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {
+        const int j = side == 0 ? lo - 1 : lo + R - 1;
+        if (j < 0 || j + 1 >= T) {
+            continue;
         }
         const uint32_t m = 3u << j;
+        if (j >= G.pair_lo && j + 1 < G.pair_hi && (m & done) == 0) {
 #pragma unroll
-        for (int k = 0; k < Tile<R>::REGS; ++k) {
-            const double2 p = lds[lds_slot<R>(omap<R, M>(t, k) ^ m)];
-            l[k].x -= field * p.x;
-            l[k].y -= field * p.y;
-            e -= field * (c[k].x * p.x + c[k].y * p.y);
+            for (int k = 0; k < Tile<R>::REGS; ++k) {
+                const double2 p = lds[lds_slot<R>(omap<R, M>(t, k) ^ m)];
+                l[k].x -= field * p.x;
+                l[k].y -= field * p.y;
+                es += c[k].x * p.x + c[k].y * p.y;
+            }
         }
     }
+    double er = 0.0;  // pairs inside the window: each unordered pair once, doubled
 #pragma unroll
     for (int rb = 0; rb + 1 < R; ++rb) {
         const int j = lo + rb;
-        if (j >= G.pair_lo && j + 1 < G.pair_hi) {
+        if (j >= G.pair_lo && j + 1 < G.pair_hi && ((3u << j) & done) == 0) {
 #pragma unroll
             for (int k = 0; k < Tile<R>::REGS; ++k) {
                 const double2 p = c[k ^ (3 << rb)];
                 l[k].x -= field * p.x;
                 l[k].y -= field * p.y;
-                e -= field * (c[k].x * p.x + c[k].y * p.y);
+                if ((k & (1 << rb)) == 0) {
+                    er += c[k].x * p.x + c[k].y * p.y;
+                }
             }
         }
     }
+    e -= field * (es + 2.0 * er);
 }
 
 template <int R, bool kGather, bool kWriteCur, bool kWriteLam, bool kHam>
@@ -520,9 +550,9 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_pa
             for (int k = 0; k < TL::REGS; ++k) {
                 c[k] = lds[lds_slot<R>(omap<R, 1>(t, k))];
             }
-            if constexpr (kHam) {
+            if constexpr (kHam && gather_pairs_reach<R, 1>()) {
                 // the LDS still holds the gathered tile of cur in offset order
-                add_pair_terms<R, 1>(c, l, lds, t, G, field, e);
+                add_pair_terms<R, 1>(c, l, lds, t, G, 0u, field, e);
             }
         } else {
             const uint64_t row = base + ta.offA;
@@ -533,30 +563,50 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_pa
                 c[k] = cur_in[idx];
             }
             if constexpr (kHam) {
-                __syncthreads();
+                // only the pair straddling the load window's lower edge needs
+                // a copy of the tile in LDS
+                if (G.pair_lo <= T - R - 1 && G.pair_hi >= T - R + 1) {
+                    __syncthreads();
 #pragma unroll
-                for (int k = 0; k < TL::REGS; ++k) {
-                    lds[lds_slot<R>(omap<R, 0>(t, k))] = c[k];
+                    for (int k = 0; k < TL::REGS; ++k) {
+                        lds[lds_slot<R>(omap<R, 0>(t, k))] = c[k];
+                    }
+                    __syncthreads();
                 }
-                __syncthreads();
-                add_pair_terms<R, 0>(c, l, lds, t, G, field, e);
+                add_pair_terms<R, 0>(c, l, lds, t, G, 0u, field, e);
             }
         }
 
+        // lam is exchanged before cur, so the LDS holds cur for the pair terms
         if constexpr (kGather) {
+            constexpr uint32_t w1 = window_mask<R, 1>();
+            constexpr uint32_t w2 = window_mask<R, 2>();
             phase_bwd<R, 1>(c, l, lc, G, wave_acc, lane);
-            exchange<R, 1, 2>(c, lds, t);
             exchange<R, 1, 2>(l, lds, t);
+            exchange<R, 1, 2>(c, lds, t);
+            if constexpr (kHam && gather_pairs_reach<R, 2>()) {
+                add_pair_terms<R, 2>(c, l, lds, t, G, w1, field, e);
+            }
             phase_bwd<R, 2>(c, l, lc, G, wave_acc, lane);
             if constexpr (TL::P == 4) {
-                exchange<R, 2, 3>(c, lds, t);
                 exchange<R, 2, 3>(l, lds, t);
+                exchange<R, 2, 3>(c, lds, t);
+                if constexpr (kHam && gather_pairs_reach<R, 3>()) {
+                    add_pair_terms<R, 3>(c, l, lds, t, G, w1 | w2, field, e);
+                }
                 phase_bwd<R, 3>(c, l, lc, G, wave_acc, lane);
-                exchange<R, 3, 0>(c, lds, t);
                 exchange<R, 3, 0>(l, lds, t);
+                exchange<R, 3, 0>(c, lds, t);
+                if constexpr (kHam) {
+                    add_pair_terms<R, 0>(c, l, lds, t, G, w1 | w2 | window_mask<R, 3>(), field,
+                                         e);
+                }
             } else {
-                exchange<R, 2, 0>(c, lds, t);
                 exchange<R, 2, 0>(l, lds, t);
+                exchange<R, 2, 0>(c, lds, t);
+                if constexpr (kHam) {
+                    add_pair_terms<R, 0>(c, l, lds, t, G, w1 | w2, field, e);
+                }
             }
             phase_bwd<R, 0>(c, l, lc, G, wave_acc, lane);
             const uint64_t row = base + ta.offA;
@@ -571,28 +621,47 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_pa
                 }
             }
         } else {
+            constexpr uint32_t w0 = window_mask<R, 0>();
             phase_bwd<R, 0>(c, l, lc, G, wave_acc, lane);
             if constexpr (TL::P == 3) {
-                exchange<R, 0, 1>(c, lds, t);
                 exchange<R, 0, 1>(l, lds, t);
+                exchange<R, 0, 1>(c, lds, t);
+                if constexpr (kHam) {
+                    add_pair_terms<R, 1>(c, l, lds, t, G, w0, field, e);
+                }
                 phase_bwd<R, 1>(c, l, lc, G, wave_acc, lane);
-                exchange<R, 1, 2>(c, lds, t);
                 exchange<R, 1, 2>(l, lds, t);
-                phase_bwd<R, 2>(c, l, lc, G, wave_acc, lane);
-            } else {
-                if (G.rot_lo < R) {
-                    exchange<R, 0, 1>(c, lds, t);
-                    exchange<R, 0, 1>(l, lds, t);
-                    phase_bwd<R, 1>(c, l, lc, G, wave_acc, lane);
-                    exchange<R, 1, 2>(c, lds, t);
-                    exchange<R, 1, 2>(l, lds, t);
-                } else {
-                    exchange<R, 0, 2>(c, lds, t);
-                    exchange<R, 0, 2>(l, lds, t);
+                exchange<R, 1, 2>(c, lds, t);
+                if constexpr (kHam) {
+                    add_pair_terms<R, 2>(c, l, lds, t, G, w0 | window_mask<R, 1>(), field, e);
                 }
                 phase_bwd<R, 2>(c, l, lc, G, wave_acc, lane);
-                exchange<R, 2, 3>(c, lds, t);
+            } else {
+                uint32_t done = w0;
+                if (G.rot_lo < R) {
+                    exchange<R, 0, 1>(l, lds, t);
+                    exchange<R, 0, 1>(c, lds, t);
+                    if constexpr (kHam) {
+                        add_pair_terms<R, 1>(c, l, lds, t, G, done, field, e);
+                    }
+                    phase_bwd<R, 1>(c, l, lc, G, wave_acc, lane);
+                    done |= window_mask<R, 1>();
+                    exchange<R, 1, 2>(l, lds, t);
+                    exchange<R, 1, 2>(c, lds, t);
+                } else {
+                    exchange<R, 0, 2>(l, lds, t);
+                    exchange<R, 0, 2>(c, lds, t);
+                }
+                if constexpr (kHam) {
+                    add_pair_terms<R, 2>(c, l, lds, t, G, done, field, e);
+                }
+                phase_bwd<R, 2>(c, l, lc, G, wave_acc, lane);
+                done |= window_mask<R, 2>();
                 exchange<R, 2, 3>(l, lds, t);
+                exchange<R, 2, 3>(c, lds, t);
+                if constexpr (kHam) {
+                    add_pair_terms<R, 3>(c, l, lds, t, G, done, field, e);
+                }
                 phase_bwd<R, 3>(c, l, lc, G, wave_acc, lane);
             }
             const uint64_t row = base + ta.offC;
