@@ -118,6 +118,26 @@ __device__ __forceinline__ void bwd_pair(double2 &c0, double2 &c1, double2 &l0, 
     dt += (l1.x * c0.x + l1.y * c0.y) - (l0.x * c1.x + l0.y * c1.y);
 }
 
+// RY(theta) / RY(theta)^dagger alone on a pair (the RZ factors of a whole
+// register window are applied as one diagonal phase per amplitude).
+__device__ __forceinline__ void ry_pair(double2 &a0, double2 &a1, const WireCoef &k) {
+    const double b0x = k.c * a0.x - k.s * a1.x;
+    const double b0y = k.c * a0.y - k.s * a1.y;
+    a1.x = k.s * a0.x + k.c * a1.x;
+    a1.y = k.s * a0.y + k.c * a1.y;
+    a0.x = b0x;
+    a0.y = b0y;
+}
+
+__device__ __forceinline__ void ry_inv_pair(double2 &a0, double2 &a1, const WireCoef &k) {
+    const double b0x = k.c * a0.x + k.s * a1.x;
+    const double b0y = k.c * a0.y + k.s * a1.y;
+    a1.x = k.c * a1.x - k.s * a0.x;
+    a1.y = k.c * a1.y - k.s * a0.y;
+    a0.x = b0x;
+    a0.y = b0y;
+}
+
 // Components of U|0> for one wire: u0 = c e^{-i phi/2}, u1 = s e^{+i phi/2}
 // (reference: cpp/ising_cuda_rotation_kernels.cu:686-703).
 __device__ __forceinline__ double2 u0_of(const WireCoef &k) {

@@ -142,9 +142,42 @@ __device__ __forceinline__ void exchange(double2 (&v)[Tile<R>::REGS], double2 *l
     }
 }
 
+// RZ factors of the register windows.  RZ on one wire commutes with RY on
+// the others, so a window's rotations RZ RY (per wire) are all its RY
+// followed by one diagonal phase per register: tab[M * 2^R + k] is the
+// product over the window's rotated bits of e^{-i phi/2} (bit of k clear) or
+// e^{+i phi/2} (set).  The adjoint applies the conjugate first.  Built once
+// per workgroup (the table does not depend on the tile).
+template <int R>
+__device__ __forceinline__ void build_rz_tables(double2 *tab, const WireCoef *__restrict__ lc,
+                                                const PassGeom &G, uint32_t t) {
+    using TL = Tile<R>;
+    for (uint32_t i = t; i < (uint32_t)(TL::P * TL::REGS); i += TL::THREADS) {
+        const int M = (int)(i / TL::REGS);
+        const int k = (int)(i % TL::REGS);
+        const int lo = M == 0 ? TL::T - R : (M - 1) * R;
+        double2 q = make_double2(1.0, 0.0);
+        for (int rb = 0; rb < R; ++rb) {
+            const int j = lo + rb;
+            if (j >= G.rot_lo && j < G.rot_hi) {
+                const WireCoef w = lc[tile_wire(j, G)];
+                q = cmul(q, make_double2(w.ch, ((k >> rb) & 1) ? w.sh : -w.sh));
+            }
+        }
+        tab[i] = q;
+    }
+}
+
+template <int R, int M>
+__device__ __forceinline__ bool window_rotates(const PassGeom &G) {
+    constexpr int P0 = phase_first_bit<R, M>();
+    return G.rot_lo < P0 + R && G.rot_hi > P0;
+}
+
 template <int R, int M>
 __device__ __forceinline__ void phase_fwd(double2 (&v)[Tile<R>::REGS],
-                                          const WireCoef *__restrict__ lc, const PassGeom &G) {
+                                          const WireCoef *__restrict__ lc, const PassGeom &G,
+                                          const double2 *rz_tab) {
     constexpr int P0 = phase_first_bit<R, M>();
 #pragma unroll
     for (int rb = 0; rb < R; ++rb) {
@@ -154,9 +187,15 @@ __device__ __forceinline__ void phase_fwd(double2 (&v)[Tile<R>::REGS],
 #pragma unroll
             for (int q = 0; q < Tile<R>::REGS; ++q) {
                 if (!(q & (1 << rb))) {
-                    fwd_pair(v[q], v[q | (1 << rb)], k);
+                    ry_pair(v[q], v[q | (1 << rb)], k);
                 }
             }
+        }
+    }
+    if (window_rotates<R, M>(G)) {
+#pragma unroll
+        for (int q = 0; q < Tile<R>::REGS; ++q) {
+            v[q] = cmul(v[q], rz_tab[M * Tile<R>::REGS + q]);
         }
     }
 }
@@ -164,11 +203,25 @@ __device__ __forceinline__ void phase_fwd(double2 (&v)[Tile<R>::REGS],
 // Per-wire gradient sums of one tile are reduced across the wave right away
 // and added by lane 0 to this wave's LDS slot (fixed order, no atomics), so
 // no per-thread accumulators stay live across phases and tiles.
+// The window's RZ^dagger as one phase per register comes first; then per
+// wire dE/dphi (phase-invariant: Im(l0* c0) - Im(l1* c1)), RY^dagger on both
+// states, and dE/dtheta = Re(l1* c0) - Re(l0* c1) just before the wire's
+// rotations (the quantities of bwd_pair).
 template <int R, int M>
 __device__ __forceinline__ void phase_bwd(double2 (&c)[Tile<R>::REGS], double2 (&l)[Tile<R>::REGS],
                                           const WireCoef *__restrict__ lc, const PassGeom &G,
-                                          double *__restrict__ wave_acc, int lane) {
+                                          double *__restrict__ wave_acc, int lane,
+                                          const double2 *rz_tab) {
     constexpr int P0 = phase_first_bit<R, M>();
+    if (window_rotates<R, M>(G)) {
+#pragma unroll
+        for (int q = 0; q < Tile<R>::REGS; ++q) {
+            const double2 z = rz_tab[M * Tile<R>::REGS + q];
+            const double2 zc = make_double2(z.x, -z.y);
+            c[q] = cmul(c[q], zc);
+            l[q] = cmul(l[q], zc);
+        }
+    }
 #pragma unroll
     for (int rb = R - 1; rb >= 0; --rb) {
         const int j = P0 + rb;
@@ -179,7 +232,14 @@ __device__ __forceinline__ void phase_bwd(double2 (&c)[Tile<R>::REGS], double2 (
 #pragma unroll
             for (int q = 0; q < Tile<R>::REGS; ++q) {
                 if (!(q & (1 << rb))) {
-                    bwd_pair(c[q], c[q | (1 << rb)], l[q], l[q | (1 << rb)], k, dt, dp);
+                    double2 &c0 = c[q];
+                    double2 &c1 = c[q | (1 << rb)];
+                    double2 &l0 = l[q];
+                    double2 &l1 = l[q | (1 << rb)];
+                    dp += (l0.x * c0.y - l0.y * c0.x) - (l1.x * c1.y - l1.y * c1.x);
+                    ry_inv_pair(c0, c1, k);
+                    ry_inv_pair(l0, l1, k);
+                    dt += (l1.x * c0.x + l1.y * c0.y) - (l0.x * c1.x + l0.y * c1.y);
                 }
             }
             dt = wave_sum(dt);
@@ -273,8 +333,13 @@ __global__ __launch_bounds__(Tile<R>::THREADS) void fwd_pass_kernel(
     using TL = Tile<R>;
     __shared__ double2 lds[TL::SLOTS];
     __shared__ double2 nib[kGenInit ? kInitNibbles : 1][16];
+    __shared__ double2 rz_tab[TL::P * TL::REGS];
     const uint32_t t = threadIdx.x;
     const TileAddr ta = tile_addr<R>(t, G);
+    build_rz_tables<R>(rz_tab, lc, G, t);
+    if constexpr (!kGenInit) {
+        __syncthreads();  // gen_init syncs after its nibble tables
+    }
     if constexpr (kGenInit) {
         for (uint32_t e = t; e < kInitNibbles * 16; e += TL::THREADS) {
             const int q = (int)(e >> 4);
@@ -333,23 +398,23 @@ __global__ __launch_bounds__(Tile<R>::THREADS) void fwd_pass_kernel(
                 }
             }
         }
-        phase_fwd<R, 0>(v, lc, G);
+        phase_fwd<R, 0>(v, lc, G, rz_tab);
         if constexpr (TL::P == 3) {
             exchange<R, 0, 1>(v, lds, t);
-            phase_fwd<R, 1>(v, lc, G);
+            phase_fwd<R, 1>(v, lc, G, rz_tab);
             exchange<R, 1, 2>(v, lds, t);
-            phase_fwd<R, 2>(v, lc, G);
+            phase_fwd<R, 2>(v, lc, G, rz_tab);
         } else {
             if (G.rot_lo < R) {  // the low tile: offset bits [0, R) rotate too
                 exchange<R, 0, 1>(v, lds, t);
-                phase_fwd<R, 1>(v, lc, G);
+                phase_fwd<R, 1>(v, lc, G, rz_tab);
                 exchange<R, 1, 2>(v, lds, t);
             } else {
                 exchange<R, 0, 2>(v, lds, t);
             }
-            phase_fwd<R, 2>(v, lc, G);
+            phase_fwd<R, 2>(v, lc, G, rz_tab);
             exchange<R, 2, 3>(v, lds, t);
-            phase_fwd<R, 3>(v, lc, G);
+            phase_fwd<R, 3>(v, lc, G, rz_tab);
         }
         if constexpr (!kScatter) {
             double2 *dst = out + base + ta.offC;
@@ -504,6 +569,7 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_pa
     __shared__ double2 lds[TL::SLOTS];
     __shared__ double acc[TL::WAVES][2 * T];
     __shared__ double ered[kHam ? TL::WAVES : 1];
+    __shared__ double2 rz_tab[TL::P * TL::REGS];
     const uint32_t t = threadIdx.x;
     const int lane = t & (kWave - 1);
     double *wave_acc = acc[t / kWave];
@@ -511,6 +577,8 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_pa
         wave_acc[lane] = 0.0;
     }
     double e = 0.0;
+    build_rz_tables<R>(rz_tab, lc, G, t);
+    __syncthreads();
 
     const TileAddr ta = tile_addr<R>(t, G);
     for (uint64_t g = blockIdx.x; g < G.num_tiles; g += gridDim.x) {
@@ -581,20 +649,20 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_pa
         if constexpr (kGather) {
             constexpr uint32_t w1 = window_mask<R, 1>();
             constexpr uint32_t w2 = window_mask<R, 2>();
-            phase_bwd<R, 1>(c, l, lc, G, wave_acc, lane);
+            phase_bwd<R, 1>(c, l, lc, G, wave_acc, lane, rz_tab);
             exchange<R, 1, 2>(l, lds, t);
             exchange<R, 1, 2>(c, lds, t);
             if constexpr (kHam && gather_pairs_reach<R, 2>()) {
                 add_pair_terms<R, 2>(c, l, lds, t, G, w1, field, e);
             }
-            phase_bwd<R, 2>(c, l, lc, G, wave_acc, lane);
+            phase_bwd<R, 2>(c, l, lc, G, wave_acc, lane, rz_tab);
             if constexpr (TL::P == 4) {
                 exchange<R, 2, 3>(l, lds, t);
                 exchange<R, 2, 3>(c, lds, t);
                 if constexpr (kHam && gather_pairs_reach<R, 3>()) {
                     add_pair_terms<R, 3>(c, l, lds, t, G, w1 | w2, field, e);
                 }
-                phase_bwd<R, 3>(c, l, lc, G, wave_acc, lane);
+                phase_bwd<R, 3>(c, l, lc, G, wave_acc, lane, rz_tab);
                 exchange<R, 3, 0>(l, lds, t);
                 exchange<R, 3, 0>(c, lds, t);
                 if constexpr (kHam) {
@@ -608,7 +676,7 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_pa
                     add_pair_terms<R, 0>(c, l, lds, t, G, w1 | w2, field, e);
                 }
             }
-            phase_bwd<R, 0>(c, l, lc, G, wave_acc, lane);
+            phase_bwd<R, 0>(c, l, lc, G, wave_acc, lane, rz_tab);
             const uint64_t row = base + ta.offA;
 #pragma unroll
             for (int k = 0; k < TL::REGS; ++k) {
@@ -622,20 +690,20 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_pa
             }
         } else {
             constexpr uint32_t w0 = window_mask<R, 0>();
-            phase_bwd<R, 0>(c, l, lc, G, wave_acc, lane);
+            phase_bwd<R, 0>(c, l, lc, G, wave_acc, lane, rz_tab);
             if constexpr (TL::P == 3) {
                 exchange<R, 0, 1>(l, lds, t);
                 exchange<R, 0, 1>(c, lds, t);
                 if constexpr (kHam) {
                     add_pair_terms<R, 1>(c, l, lds, t, G, w0, field, e);
                 }
-                phase_bwd<R, 1>(c, l, lc, G, wave_acc, lane);
+                phase_bwd<R, 1>(c, l, lc, G, wave_acc, lane, rz_tab);
                 exchange<R, 1, 2>(l, lds, t);
                 exchange<R, 1, 2>(c, lds, t);
                 if constexpr (kHam) {
                     add_pair_terms<R, 2>(c, l, lds, t, G, w0 | window_mask<R, 1>(), field, e);
                 }
-                phase_bwd<R, 2>(c, l, lc, G, wave_acc, lane);
+                phase_bwd<R, 2>(c, l, lc, G, wave_acc, lane, rz_tab);
             } else {
                 uint32_t done = w0;
                 if (G.rot_lo < R) {
@@ -644,7 +712,7 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_pa
                     if constexpr (kHam) {
                         add_pair_terms<R, 1>(c, l, lds, t, G, done, field, e);
                     }
-                    phase_bwd<R, 1>(c, l, lc, G, wave_acc, lane);
+                    phase_bwd<R, 1>(c, l, lc, G, wave_acc, lane, rz_tab);
                     done |= window_mask<R, 1>();
                     exchange<R, 1, 2>(l, lds, t);
                     exchange<R, 1, 2>(c, lds, t);
@@ -655,14 +723,14 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_pa
                 if constexpr (kHam) {
                     add_pair_terms<R, 2>(c, l, lds, t, G, done, field, e);
                 }
-                phase_bwd<R, 2>(c, l, lc, G, wave_acc, lane);
+                phase_bwd<R, 2>(c, l, lc, G, wave_acc, lane, rz_tab);
                 done |= window_mask<R, 2>();
                 exchange<R, 2, 3>(l, lds, t);
                 exchange<R, 2, 3>(c, lds, t);
                 if constexpr (kHam) {
                     add_pair_terms<R, 3>(c, l, lds, t, G, done, field, e);
                 }
-                phase_bwd<R, 3>(c, l, lc, G, wave_acc, lane);
+                phase_bwd<R, 3>(c, l, lc, G, wave_acc, lane, rz_tab);
             }
             const uint64_t row = base + ta.offC;
 #pragma unroll
