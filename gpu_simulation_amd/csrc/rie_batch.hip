@@ -145,9 +145,33 @@ __device__ __forceinline__ uint32_t member_offset(const ChunkGeom<K> &cg, int o)
     return off;
 }
 
+// RZ factors of one layer's chunks (RZ on one wire commutes with RY on the
+// others, so a chunk's rotations are its RY followed by one diagonal phase):
+// rzt[c * 2^K + o] = prod over the chunk's wires j of e^{-i phi/2} (bit j of
+// o clear) or e^{+i phi/2} (set), chunk c = wires [c K, c K + K).  Callers
+// synchronize before the first chunk reads it.
+__device__ __forceinline__ void build_chunk_rz(double2 *rzt, int n, int K, const WireCoef *lc) {
+    const int A = 1 << K;
+    const int chunks = (n + K - 1) / K;
+    for (int i = threadIdx.x; i < chunks * A; i += blockDim.x) {
+        const int w0 = (i / A) * K;
+        const int o = i % A;
+        const int kw = K < n - w0 ? K : n - w0;
+        double2 q = make_double2(1.0, 0.0);
+        for (int j = 0; j < kw; ++j) {
+            const WireCoef w = lc[w0 + j];
+            q = cmul(q, make_double2(w.ch, ((o >> j) & 1) ? w.sh : -w.sh));
+        }
+        rzt[i] = q;
+    }
+}
+
+// ceil(n / K) * 2^K <= 64 for n <= 14, K <= 4
+constexpr int kBatchRzEntries = 64;
+
 template <int K, int kThreads, bool kLds>
 __device__ __forceinline__ void chunk_fwd(double2 *st, int n, int w0, const WireCoef *lc,
-                                          const FrameTables &ft, int fb) {
+                                          const FrameTables &ft, int fb, const double2 *rz) {
     constexpr int A = 1 << K;
     const ChunkGeom<K> cg = chunk_geom<K>(ft, fb, w0);
     WireCoef k[K];
@@ -168,13 +192,13 @@ __device__ __forceinline__ void chunk_fwd(double2 *st, int n, int w0, const Wire
 #pragma unroll
             for (int o = 0; o < A; ++o) {
                 if (((o >> j) & 1) == 0) {
-                    fwd_pair(a[o], a[o | (1 << j)], k[j]);
+                    ry_pair(a[o], a[o | (1 << j)], k[j]);
                 }
             }
         }
 #pragma unroll
         for (int o = 0; o < A; ++o) {
-            st[slot<kLds>(base ^ member_offset<K>(cg, o))] = a[o];
+            st[slot<kLds>(base ^ member_offset<K>(cg, o))] = cmul(a[o], rz[o]);
         }
     }
 }
@@ -184,7 +208,7 @@ __device__ __forceinline__ void chunk_fwd(double2 *st, int n, int w0, const Wire
 template <int K, int kThreads, bool kLds>
 __device__ __forceinline__ void chunk_bwd(double2 *cur, double2 *lam, int n, int w0,
                                           const WireCoef *lc, const FrameTables &ft, int fb,
-                                          double (*red)[kRedCols]) {
+                                          double (*red)[kRedCols], const double2 *rz) {
     constexpr int A = 1 << K;
     const ChunkGeom<K> cg = chunk_geom<K>(ft, fb, w0);
     WireCoef k[K];
@@ -204,15 +228,26 @@ __device__ __forceinline__ void chunk_bwd(double2 *cur, double2 *lam, int n, int
 #pragma unroll
         for (int o = 0; o < A; ++o) {
             const uint32_t i = slot<kLds>(base ^ member_offset<K>(cg, o));
-            c[o] = cur[i];
-            l[o] = lam[i];
+            const double2 z = rz[o];
+            const double2 zc = make_double2(z.x, -z.y);
+            c[o] = cmul(cur[i], zc);
+            l[o] = cmul(lam[i], zc);
         }
+        // the chunk's RZ^dagger is applied; per wire dE/dphi (phase-invariant),
+        // RY^dagger on both states, then dE/dtheta (the quantities of bwd_pair)
 #pragma unroll
         for (int j = K - 1; j >= 0; --j) {
 #pragma unroll
             for (int o = 0; o < A; ++o) {
                 if (((o >> j) & 1) == 0) {
-                    bwd_pair(c[o], c[o | (1 << j)], l[o], l[o | (1 << j)], k[j], dt[j], dp[j]);
+                    double2 &c0 = c[o];
+                    double2 &c1 = c[o | (1 << j)];
+                    double2 &l0 = l[o];
+                    double2 &l1 = l[o | (1 << j)];
+                    dp[j] += (l0.x * c0.y - l0.y * c0.x) - (l1.x * c1.y - l1.y * c1.x);
+                    ry_inv_pair(c0, c1, k[j]);
+                    ry_inv_pair(l0, l1, k[j]);
+                    dt[j] += (l1.x * c0.x + l1.y * c0.y) - (l0.x * c1.x + l0.y * c1.y);
                 }
             }
         }
@@ -239,19 +274,19 @@ __device__ __forceinline__ void chunk_bwd(double2 *cur, double2 *lam, int n, int
 template <int kThreads, bool kLds>
 __device__ __forceinline__ void chunk_fwd_any(int K, double2 *st, int n, int w0,
                                               const WireCoef *lc, const FrameTables &ft,
-                                              int fb) {
+                                              int fb, const double2 *rz) {
     switch (K) {
         case 1:
-            chunk_fwd<1, kThreads, kLds>(st, n, w0, lc, ft, fb);
+            chunk_fwd<1, kThreads, kLds>(st, n, w0, lc, ft, fb, rz);
             break;
         case 2:
-            chunk_fwd<clamp_chunk<kThreads>(2), kThreads, kLds>(st, n, w0, lc, ft, fb);
+            chunk_fwd<clamp_chunk<kThreads>(2), kThreads, kLds>(st, n, w0, lc, ft, fb, rz);
             break;
         case 3:
-            chunk_fwd<clamp_chunk<kThreads>(3), kThreads, kLds>(st, n, w0, lc, ft, fb);
+            chunk_fwd<clamp_chunk<kThreads>(3), kThreads, kLds>(st, n, w0, lc, ft, fb, rz);
             break;
         default:
-            chunk_fwd<max_chunk<kThreads>(), kThreads, kLds>(st, n, w0, lc, ft, fb);
+            chunk_fwd<max_chunk<kThreads>(), kThreads, kLds>(st, n, w0, lc, ft, fb, rz);
             break;
     }
 }
@@ -259,26 +294,26 @@ __device__ __forceinline__ void chunk_fwd_any(int K, double2 *st, int n, int w0,
 template <int kThreads, bool kLds>
 __device__ __forceinline__ void chunk_bwd_any(int K, double2 *cur, double2 *lam, int n, int w0,
                                               const WireCoef *lc, const FrameTables &ft, int fb,
-                                              double (*red)[kRedCols]) {
+                                              double (*red)[kRedCols], const double2 *rz) {
     switch (K) {
         case 1:
-            chunk_bwd<1, kThreads, kLds>(cur, lam, n, w0, lc, ft, fb, red);
+            chunk_bwd<1, kThreads, kLds>(cur, lam, n, w0, lc, ft, fb, red, rz);
             break;
         case 2:
-            chunk_bwd<clamp_chunk<kThreads>(2), kThreads, kLds>(cur, lam, n, w0, lc, ft, fb, red);
+            chunk_bwd<clamp_chunk<kThreads>(2), kThreads, kLds>(cur, lam, n, w0, lc, ft, fb, red, rz);
             break;
         case 3:
-            chunk_bwd<clamp_chunk<kThreads>(3), kThreads, kLds>(cur, lam, n, w0, lc, ft, fb, red);
+            chunk_bwd<clamp_chunk<kThreads>(3), kThreads, kLds>(cur, lam, n, w0, lc, ft, fb, red, rz);
             break;
         default:
-            chunk_bwd<max_chunk<kThreads>(), kThreads, kLds>(cur, lam, n, w0, lc, ft, fb, red);
+            chunk_bwd<max_chunk<kThreads>(), kThreads, kLds>(cur, lam, n, w0, lc, ft, fb, red, rz);
             break;
     }
 }
 
 template <int kThreads, bool kLds>
 __device__ __forceinline__ void run_job(const BatchJob &job, double2 *cur, double2 *lam,
-                                        FrameTables &ft, double (*red)[kRedCols]) {
+                                        FrameTables &ft, double (*red)[kRedCols], double2 *rzt) {
     const int n = job.n;
     const int L = job.layers;
     const uint32_t N = 1u << n;
@@ -320,8 +355,11 @@ __device__ __forceinline__ void run_job(const BatchJob &job, double2 *cur, doubl
 
     for (int l = 1; l < L; ++l) {
         const WireCoef *lc = coef + (size_t)l * n;
+        build_chunk_rz(rzt, n, K, lc);
+        __syncthreads();
         for (int w0 = 0; w0 < n; w0 += K) {
-            chunk_fwd_any<kThreads, kLds>(K < n - w0 ? K : n - w0, cur, n, w0, lc, ft, fb);
+            chunk_fwd_any<kThreads, kLds>(K < n - w0 ? K : n - w0, cur, n, w0, lc, ft, fb,
+                                          rzt + (w0 / K) * (1 << K));
             __syncthreads();
         }
         frame_step(ft, n, fb, true);
@@ -368,9 +406,11 @@ __device__ __forceinline__ void run_job(const BatchJob &job, double2 *cur, doubl
     for (int l = L - 1; l >= 0; --l) {
         frame_step(ft, n, fb, false);  // inverse ring CNOT on both states
         const WireCoef *lc = coef + (size_t)l * n;
+        build_chunk_rz(rzt, n, K, lc);
+        __syncthreads();
         for (int w0 = 0; w0 < n; w0 += K) {
             chunk_bwd_any<kThreads, kLds>(K < n - w0 ? K : n - w0, cur, lam, n, w0, lc, ft, fb,
-                                          red);
+                                          red, rzt + (w0 / K) * (1 << K));
             __syncthreads();
         }
         if (t < 2 * n) {
@@ -417,6 +457,7 @@ __global__ __launch_bounds__(kThreads) void batch_structured_kernel(
     extern __shared__ double2 smem[];
     __shared__ double red[kThreads / kWave][kRedCols];
     __shared__ FrameTables ft;
+    __shared__ double2 rzt[kBatchRzEntries];
     const int b = blockIdx.x;
     const int q0 = wg_begin != nullptr ? wg_begin[b] : b;
     const int q1 = wg_begin != nullptr ? wg_begin[b + 1] : b + 1;
@@ -424,11 +465,11 @@ __global__ __launch_bounds__(kThreads) void batch_structured_kernel(
         const int idx = order != nullptr ? order[q] : q;
         const BatchJob job = jobs != nullptr ? jobs[idx] : single;
         if (job.n <= kBatchMaxLdsQubits) {
-            run_job<kThreads, true>(job, smem, smem + lds_state, ft, red);
+            run_job<kThreads, true>(job, smem, smem + lds_state, ft, red, rzt);
         } else {
             const uint64_t N = uint64_t{1} << job.n;
             double2 *gs = scratch + (uint64_t)b * scratch_stride;
-            run_job<kThreads, false>(job, gs, gs + N, ft, red);
+            run_job<kThreads, false>(job, gs, gs + N, ft, red, rzt);
         }
         __syncthreads();
     }
