@@ -905,6 +905,20 @@ void Engine::backward_device(const double *params, const double *scale, double *
     resident = false;
 }
 
+// Energy and gradient from device-resident params: the fused evaluation
+// when it applies, else the forward and backward halves.
+void Engine::eval_device(const double *params, double *energy, double *grad, hipStream_t s) {
+    if (fused_eval()) {
+        launch_coef(params, d_coef, n * L, s);
+        check_launch("coef");
+        structured_eval_fused(s, nullptr, energy, false);
+        copy_grad_if_needed(grad, s);
+        return;
+    }
+    forward_device(params, energy, s, true);
+    backward_device(params, nullptr, grad, s);
+}
+
 void Engine::copy_grad_if_needed(double *grad, hipStream_t s) {
     if (grad != d_grad) {
         check(hipMemcpyAsync(grad, d_grad, sizeof(double) * num_params, hipMemcpyDeviceToDevice, s),
@@ -1106,12 +1120,10 @@ std::string Engine::autotune(const double *params, size_t count, size_t repeats)
                     tile_r_fwd = rf;
                     tile_r_ham = rh;
                     tile_r_bwd = rb;
-                    forward_device(d_params, d_energy, s, true);
-                    backward_device(d_params, nullptr, d_grad, s);
+                    eval_device(d_params, d_energy, d_grad, s);
                     check(hipEventRecord(ev_a, s), "event");
                     for (size_t r = 0; r < repeats; ++r) {
-                        forward_device(d_params, d_energy, s, true);
-                        backward_device(d_params, nullptr, d_grad, s);
+                        eval_device(d_params, d_energy, d_grad, s);
                     }
                     check(hipEventRecord(ev_b, s), "event");
                     check(hipEventSynchronize(ev_b), "event sync");
@@ -1393,15 +1405,7 @@ rie_status rie_engine_energy_and_grad_device(rie_engine *engine, const double *d
         }
         hipStream_t s = stream_of(e, stream);
         e->order_begin(s);
-        if (e->fused_eval()) {
-            rie::launch_coef(d_params, e->d_coef, e->n * e->L, s);
-            rie::check_launch("coef");
-            e->structured_eval_fused(s, nullptr, d_energy, false);
-            e->copy_grad_if_needed(d_gradient, s);
-        } else {
-            e->forward_device(d_params, d_energy, s, true);
-            e->backward_device(d_params, nullptr, d_gradient, s);
-        }
+        e->eval_device(d_params, d_energy, d_gradient, s);
         e->order_end(s);
     });
 }

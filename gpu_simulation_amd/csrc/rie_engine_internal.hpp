@@ -202,6 +202,7 @@ struct Engine {
     void forward_device(const double *params, double *energy, hipStream_t s, bool keep_lam);
     void backward_device(const double *params, const double *scale, double *grad, hipStream_t s);
     void copy_grad_if_needed(double *grad, hipStream_t s);
+    void eval_device(const double *params, double *energy, double *grad, hipStream_t s);
     void energy_and_grad_host(const double *params, size_t count, bool want_grad, bool profile,
                               double *energy_out, double *grad_out);
     double inverse_walk_drift(const double *params, size_t count);
