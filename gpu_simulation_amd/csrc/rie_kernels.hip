@@ -370,17 +370,46 @@ __global__ __launch_bounds__(Tile<R>::THREADS) void fwd_pass_kernel(
             }
         }
     }
+    // Generation: x = P^-1(index) is GF(2)-linear and the base, thread and
+    // register parts of an index are disjoint, so x = P^-1(base) ^ xt ^ xk[k].
+    // Nibbles of x that no tile bit reaches are the same for the whole tile:
+    // their product is formed once per tile.
+    uint32_t vary = 0;  // bit q: nibble q of x depends on the tile offset
+    uint64_t xt = 0;
+    uint64_t xk[kGenInit ? TL::REGS : 1];
+    if constexpr (kGenInit) {
+        for (int b = 0; b < TL::T; ++b) {
+            const uint64_t vb = ring_inv(tile_index(0, 1u << b, G), G.n);
+            for (int q = 0; q < kInitNibbles; ++q) {
+                vary |= ((vb >> (4 * q)) & 15u) != 0 ? (1u << q) : 0u;
+            }
+        }
+        xt = ring_inv(ta.offA, G.n);
+#pragma unroll
+        for (int k = 0; k < TL::REGS; ++k) {
+            xk[k] = ring_inv((uint64_t)k * ta.strideA, G.n);
+        }
+    }
     for (uint64_t g = blockIdx.x; g < G.num_tiles; g += gridDim.x) {
         const uint64_t base = tile_base(g, G, TL::T);
         if constexpr (kGenInit) {
-            const uint64_t row = base + ta.offA;
+            const uint64_t xb = ring_inv(base, G.n) ^ xt;
+            double2 cb = make_double2(1.0, 0.0);
+#pragma unroll
+            for (int q = 0; q < kInitNibbles; ++q) {
+                if (((vary >> q) & 1u) == 0) {
+                    cb = cmul(cb, nib[q][(xb >> (4 * q)) & 15]);
+                }
+            }
 #pragma unroll
             for (int k = 0; k < TL::REGS; ++k) {
-                const uint64_t x = ring_inv(row + k * ta.strideA, G.n);
-                double2 p = nib[0][x & 15];
+                const uint64_t x = xb ^ xk[k];
+                double2 p = cb;
 #pragma unroll
-                for (int q = 1; q < kInitNibbles; ++q) {
-                    p = cmul(p, nib[q][(x >> (4 * q)) & 15]);
+                for (int q = 0; q < kInitNibbles; ++q) {
+                    if ((vary >> q) & 1u) {
+                        p = cmul(p, nib[q][(x >> (4 * q)) & 15]);
+                    }
                 }
                 v[k] = p;
             }

@@ -117,6 +117,23 @@ def test_fused_hamiltonian(oracle, backend_module, n, layers, schedule):
     assert np.array_equal(np.asarray(again["gradient"]), np.asarray(fused["gradient"]))
 
 
+def test_fused_hamiltonian_three_high_groups(backend_module):
+    # 31 qubits: three high groups and four crossing wires (the scattered tile
+    # holds index bits 0..7, 11, 18, 24, 30), beyond the oracle's reach; the
+    # fused evaluation must equal the per-group Hamiltonian passes
+    n, layers = 31, 2
+    params = _params(2500, n, layers).reshape(-1)
+    print(f"case fused n={n} L={layers}", flush=True)
+    eng = backend_module.RingIsingCudaBackend(n, layers, 1.0, "structured_adjoint", 8, False)
+    assert eng.fused_hamiltonian is True
+    fused = eng.energy_and_grad(params)
+    eng.set_fused_hamiltonian(0)
+    plain = eng.energy_and_grad(params)
+    assert abs(fused["energy"] - plain["energy"]) <= 1e-10
+    assert np.max(np.abs(np.asarray(fused["gradient"]) - np.asarray(plain["gradient"]))) <= 1e-10
+    assert abs(fused["energy"]) <= 2 * n
+
+
 def test_fused_hamiltonian_device_path_and_scope(oracle, backend_module):
     import torch
     n, layers = 18, 3
