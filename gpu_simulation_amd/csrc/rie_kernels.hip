@@ -203,16 +203,27 @@ __device__ __forceinline__ void phase_fwd(double2 (&v)[Tile<R>::REGS],
 // Per-wire gradient sums of one tile are reduced across the wave right away
 // and added by lane 0 to this wave's LDS slot (fixed order, no atomics), so
 // no per-thread accumulators stay live across phases and tiles.
-// The window's RZ^dagger as one phase per register comes first; then per
-// wire dE/dphi (phase-invariant: Im(l0* c0) - Im(l1* c1)), RY^dagger on both
-// states, and dE/dtheta = Re(l1* c0) - Re(l0* c1) just before the wire's
-// rotations (the quantities of bwd_pair).
-template <int R, int M>
+// The window's RZ^dagger as one phase per register comes first.  dE/dphi of
+// a wire, sum of Im(l* c) signed by the wire's bit (Im(l0* c0) - Im(l1* c1)
+// per pair), does not change under the phases or under RY^dagger of the
+// window's other wires, so one sweep over the registers forms it for every
+// wire of the window.  Then per wire RY^dagger on both states and
+// dE/dtheta = Re(l1* c0) - Re(l0* c1) just before the wire's rotations (the
+// quantities of bwd_pair).
+// kSweepDp: form dE/dphi of all the window's wires in the phase sweep (fewer
+// FP64 operations for windows of several rotated wires, four more live
+// registers); otherwise per pair next to RY^dagger.
+template <int R, int M, bool kSweepDp>
 __device__ __forceinline__ void phase_bwd(double2 (&c)[Tile<R>::REGS], double2 (&l)[Tile<R>::REGS],
                                           const WireCoef *__restrict__ lc, const PassGeom &G,
                                           double *__restrict__ wave_acc, int lane,
                                           const double2 *rz_tab) {
     constexpr int P0 = phase_first_bit<R, M>();
+    double dps[kSweepDp ? R : 1];
+#pragma unroll
+    for (int rb = 0; rb < (kSweepDp ? R : 1); ++rb) {
+        dps[rb] = 0.0;
+    }
     if (window_rotates<R, M>(G)) {
 #pragma unroll
         for (int q = 0; q < Tile<R>::REGS; ++q) {
@@ -220,6 +231,13 @@ __device__ __forceinline__ void phase_bwd(double2 (&c)[Tile<R>::REGS], double2 (
             const double2 zc = make_double2(z.x, -z.y);
             c[q] = cmul(c[q], zc);
             l[q] = cmul(l[q], zc);
+            if constexpr (kSweepDp) {
+                const double im = l[q].x * c[q].y - l[q].y * c[q].x;
+#pragma unroll
+                for (int rb = 0; rb < R; ++rb) {
+                    dps[rb] += ((q >> rb) & 1) ? -im : im;
+                }
+            }
         }
     }
 #pragma unroll
@@ -236,11 +254,16 @@ __device__ __forceinline__ void phase_bwd(double2 (&c)[Tile<R>::REGS], double2 (
                     double2 &c1 = c[q | (1 << rb)];
                     double2 &l0 = l[q];
                     double2 &l1 = l[q | (1 << rb)];
-                    dp += (l0.x * c0.y - l0.y * c0.x) - (l1.x * c1.y - l1.y * c1.x);
+                    if constexpr (!kSweepDp) {
+                        dp += (l0.x * c0.y - l0.y * c0.x) - (l1.x * c1.y - l1.y * c1.x);
+                    }
                     ry_inv_pair(c0, c1, k);
                     ry_inv_pair(l0, l1, k);
                     dt += (l1.x * c0.x + l1.y * c0.y) - (l0.x * c1.x + l0.y * c1.y);
                 }
+            }
+            if constexpr (kSweepDp) {
+                dp = dps[rb];
             }
             dt = wave_sum(dt);
             dp = wave_sum(dp);
@@ -678,20 +701,20 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_pa
         if constexpr (kGather) {
             constexpr uint32_t w1 = window_mask<R, 1>();
             constexpr uint32_t w2 = window_mask<R, 2>();
-            phase_bwd<R, 1>(c, l, lc, G, wave_acc, lane, rz_tab);
+            phase_bwd<R, 1, kGather>(c, l, lc, G, wave_acc, lane, rz_tab);
             exchange<R, 1, 2>(l, lds, t);
             exchange<R, 1, 2>(c, lds, t);
             if constexpr (kHam && gather_pairs_reach<R, 2>()) {
                 add_pair_terms<R, 2>(c, l, lds, t, G, w1, field, e);
             }
-            phase_bwd<R, 2>(c, l, lc, G, wave_acc, lane, rz_tab);
+            phase_bwd<R, 2, kGather>(c, l, lc, G, wave_acc, lane, rz_tab);
             if constexpr (TL::P == 4) {
                 exchange<R, 2, 3>(l, lds, t);
                 exchange<R, 2, 3>(c, lds, t);
                 if constexpr (kHam && gather_pairs_reach<R, 3>()) {
                     add_pair_terms<R, 3>(c, l, lds, t, G, w1 | w2, field, e);
                 }
-                phase_bwd<R, 3>(c, l, lc, G, wave_acc, lane, rz_tab);
+                phase_bwd<R, 3, kGather>(c, l, lc, G, wave_acc, lane, rz_tab);
                 exchange<R, 3, 0>(l, lds, t);
                 exchange<R, 3, 0>(c, lds, t);
                 if constexpr (kHam) {
@@ -705,7 +728,7 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_pa
                     add_pair_terms<R, 0>(c, l, lds, t, G, w1 | w2, field, e);
                 }
             }
-            phase_bwd<R, 0>(c, l, lc, G, wave_acc, lane, rz_tab);
+            phase_bwd<R, 0, kGather>(c, l, lc, G, wave_acc, lane, rz_tab);
             const uint64_t row = base + ta.offA;
 #pragma unroll
             for (int k = 0; k < TL::REGS; ++k) {
@@ -719,20 +742,20 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_pa
             }
         } else {
             constexpr uint32_t w0 = window_mask<R, 0>();
-            phase_bwd<R, 0>(c, l, lc, G, wave_acc, lane, rz_tab);
+            phase_bwd<R, 0, kGather>(c, l, lc, G, wave_acc, lane, rz_tab);
             if constexpr (TL::P == 3) {
                 exchange<R, 0, 1>(l, lds, t);
                 exchange<R, 0, 1>(c, lds, t);
                 if constexpr (kHam) {
                     add_pair_terms<R, 1>(c, l, lds, t, G, w0, field, e);
                 }
-                phase_bwd<R, 1>(c, l, lc, G, wave_acc, lane, rz_tab);
+                phase_bwd<R, 1, kGather>(c, l, lc, G, wave_acc, lane, rz_tab);
                 exchange<R, 1, 2>(l, lds, t);
                 exchange<R, 1, 2>(c, lds, t);
                 if constexpr (kHam) {
                     add_pair_terms<R, 2>(c, l, lds, t, G, w0 | window_mask<R, 1>(), field, e);
                 }
-                phase_bwd<R, 2>(c, l, lc, G, wave_acc, lane, rz_tab);
+                phase_bwd<R, 2, kGather>(c, l, lc, G, wave_acc, lane, rz_tab);
             } else {
                 uint32_t done = w0;
                 if (G.rot_lo < R) {
@@ -741,7 +764,7 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_pa
                     if constexpr (kHam) {
                         add_pair_terms<R, 1>(c, l, lds, t, G, done, field, e);
                     }
-                    phase_bwd<R, 1>(c, l, lc, G, wave_acc, lane, rz_tab);
+                    phase_bwd<R, 1, kGather>(c, l, lc, G, wave_acc, lane, rz_tab);
                     done |= window_mask<R, 1>();
                     exchange<R, 1, 2>(l, lds, t);
                     exchange<R, 1, 2>(c, lds, t);
@@ -752,14 +775,14 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_pa
                 if constexpr (kHam) {
                     add_pair_terms<R, 2>(c, l, lds, t, G, done, field, e);
                 }
-                phase_bwd<R, 2>(c, l, lc, G, wave_acc, lane, rz_tab);
+                phase_bwd<R, 2, kGather>(c, l, lc, G, wave_acc, lane, rz_tab);
                 done |= window_mask<R, 2>();
                 exchange<R, 2, 3>(l, lds, t);
                 exchange<R, 2, 3>(c, lds, t);
                 if constexpr (kHam) {
                     add_pair_terms<R, 3>(c, l, lds, t, G, done, field, e);
                 }
-                phase_bwd<R, 3>(c, l, lc, G, wave_acc, lane, rz_tab);
+                phase_bwd<R, 3, kGather>(c, l, lc, G, wave_acc, lane, rz_tab);
             }
             const uint64_t row = base + ta.offC;
 #pragma unroll
