@@ -55,7 +55,8 @@ def parse_args() -> argparse.Namespace:
     p.add_argument("--no-autotune", action="store_true",
                    help="keep the default tile shapes and schedule instead of autotuning")
     p.add_argument("--tile-shapes", default="auto",
-                   help="register shape per pass kind fwd,ham,bwd (each 3 or 4) or auto")
+                   help="register shapes (3 or 4): fwd,ham,bwd or per pass kind "
+                        "gen,fwd_high,fwd_low,ham,bwd_low,bwd_high, or auto")
     p.add_argument("--no-other-schedule", action="store_true",
                    help="skip timing the other backward schedule")
     p.add_argument("--no-fused-hamiltonian", action="store_true",
@@ -226,7 +227,11 @@ def main() -> int:
     if args.schedule != "auto":
         engine.set_schedule(args.schedule)
     if args.tile_shapes != "auto":
-        engine.set_tile_shapes(*[int(x) for x in args.tile_shapes.split(",")])
+        shapes = [int(x) for x in args.tile_shapes.split(",")]
+        if len(shapes) == 3:
+            engine.set_tile_shapes(*shapes)
+        else:
+            engine.set_pass_shapes(shapes)
     stream = torch.cuda.current_stream(dev)
     handle = stream.cuda_stream
 
@@ -327,6 +332,7 @@ def main() -> int:
             "strategy": "structured_adjoint",
             "schedule": stats_schedule,
             "tile_shapes": list(engine.tile_shapes),
+            "pass_shapes": list(engine.pass_shapes),
             "fused_hamiltonian": bool(engine.fused_hamiltonian),
             "parallelism": "single GPU",
         },

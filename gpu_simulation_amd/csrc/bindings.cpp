@@ -269,6 +269,16 @@ class RingIsingCudaBackend {
         return out;
     }
 
+    void set_pass_shapes(const std::vector<int> &shapes) {
+        check(rie_engine_set_pass_shapes(engine_, shapes.data(), shapes.size()));
+    }
+
+    std::vector<int> pass_shapes() const {
+        std::vector<int> out(RIE_PASS_KINDS);
+        check(rie_engine_pass_shapes(engine_, out.data(), out.size()));
+        return out;
+    }
+
     size_t num_qubits() const { return num_qubits_; }
     size_t num_layers() const { return num_layers_; }
 
@@ -550,6 +560,8 @@ PYBIND11_MODULE(_cuda_backend, m) {
         .def("set_tile_shapes", &RingIsingCudaBackend::set_tile_shapes, py::arg("r_fwd"),
              py::arg("r_ham"), py::arg("r_bwd"))
         .def_property_readonly("tile_shapes", &RingIsingCudaBackend::tile_shapes)
+        .def("set_pass_shapes", &RingIsingCudaBackend::set_pass_shapes, py::arg("shapes"))
+        .def_property_readonly("pass_shapes", &RingIsingCudaBackend::pass_shapes)
         .def("autotune", &RingIsingCudaBackend::autotune, py::arg("params"),
              py::arg("repeats") = 2)
         .def_property_readonly("num_qubits", &RingIsingCudaBackend::num_qubits)

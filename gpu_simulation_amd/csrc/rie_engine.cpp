@@ -415,14 +415,16 @@ void Engine::structured_forward(hipStream_t s, bool save_ckpts) {
             for (int j = 0; j < K; ++j) {
                 const bool gen = (l == 1 && j == 0);
                 kt_begin(s);
-                launch_fwd_pass(tile_r_fwd, false, gen, in, fbuf(l, j), lc, d_coef, passes[1 + j].g,
+                launch_fwd_pass(pass_r[gen ? RIE_PASS_FWD_GEN : RIE_PASS_FWD_HIGH], false, gen, in,
+                                fbuf(l, j), lc, d_coef, passes[1 + j].g,
                                 passes[1 + j].grid, s);
                 check_launch("fwd_pass");
                 kt_end(s, gen ? "fwd_pass_high_gen_init" : "fwd_pass_high", gen ? S : 2 * S);
                 in = fbuf(l, j);
             }
             kt_begin(s);
-            launch_fwd_pass(tile_r_fwd, true, false, in, fbuf(l, K), lc, d_coef, passes[0].g,
+            launch_fwd_pass(pass_r[RIE_PASS_FWD_LOW], true, false, in, fbuf(l, K), lc, d_coef,
+                            passes[0].g,
                             passes[0].grid, s);
             check_launch("fwd_pass scatter");
             kt_end(s, "fwd_pass_low_cnot", 2 * S);
@@ -444,7 +446,8 @@ void Engine::structured_forward(hipStream_t s, bool save_ckpts) {
             const bool gen = (l == 1 && j == 0);
             double2 *out = (cur_buf >= 0) ? pool[cur_buf] : pool[0];
             kt_begin(s);
-            launch_fwd_pass(tile_r_fwd, false, gen, cur, out, lc, d_coef, passes[1 + j].g,
+            launch_fwd_pass(pass_r[gen ? RIE_PASS_FWD_GEN : RIE_PASS_FWD_HIGH], false, gen, cur,
+                            out, lc, d_coef, passes[1 + j].g,
                             passes[1 + j].grid, s);
             check_launch("fwd_pass");
             kt_end(s, gen ? "fwd_pass_high_gen_init" : "fwd_pass_high", gen ? S : 2 * S);
@@ -466,7 +469,8 @@ void Engine::structured_forward(hipStream_t s, bool save_ckpts) {
             out = pool[out_buf];
         }
         kt_begin(s);
-        launch_fwd_pass(tile_r_fwd, true, false, cur, out, lc, d_coef, passes[0].g, passes[0].grid, s);
+        launch_fwd_pass(pass_r[RIE_PASS_FWD_LOW], true, false, cur, out, lc, d_coef, passes[0].g,
+                        passes[0].grid, s);
         check_launch("fwd_pass scatter");
         kt_end(s, "fwd_pass_low_cnot", 2 * S);
         cur = out;
@@ -554,7 +558,8 @@ void Engine::structured_hamiltonian(hipStream_t s, bool write_lam) {
     for (size_t p = 0; p < passes.size(); ++p) {
         const bool read_lam = write_lam && p > 0;
         kt_begin(s);
-        launch_ham_pass(tile_r_ham, p == 0, read_lam, write_lam, psi, lam, lam, passes[p].g, h,
+        launch_ham_pass(pass_r[RIE_PASS_HAM], p == 0, read_lam, write_lam, psi, lam, lam,
+                        passes[p].g, h,
                         d_epartials + off, passes[p].grid, s);
         check_launch("ham_pass");
         kt_end(s, p == 0 ? "ham_pass_low" : "ham_pass_high",
@@ -570,14 +575,16 @@ void Engine::launch_bwd(int l, bool gather, bool write_cur, const double2 *cur_i
     const WireCoef *lc = d_coef + (size_t)l * n;
     const PassPlan &pp = passes[p];
     if (l != fuse_layer) {
-        launch_bwd_pass(tile_r_bwd, gather, write_cur, true, cur_in, lam_in, cur_out, lam_out, lc,
+        launch_bwd_pass(pass_r[gather ? RIE_PASS_BWD_LOW : RIE_PASS_BWD_HIGH], gather, write_cur,
+                        true, cur_in, lam_in, cur_out, lam_out, lc,
                         pp.g, d_partials, pp.grid, s);
         return;
     }
     if (fuse_eoff + (size_t)pp.grid > epartial_count) {
         fail(RIE_DEVICE_ERROR, "internal: energy partial buffer too small");
     }
-    launch_bwd_pass(tile_r_bwd, gather, write_cur, true, cur_in, lam_in, cur_out, lam_out, lc,
+    launch_bwd_pass(pass_r[gather ? RIE_PASS_BWD_LOW : RIE_PASS_BWD_HIGH], gather, write_cur, true,
+                    cur_in, lam_in, cur_out, lam_out, lc,
                     pp.g, d_partials, pp.grid, s, true, h, d_epartials + fuse_eoff);
     fuse_eoff += (size_t)pp.grid;
 }
@@ -602,7 +609,7 @@ void Engine::structured_hamiltonian_fused(hipStream_t s) {
     }
     const double S = (double)sizeof(double2) * (double)N;
     kt_begin(s);
-    launch_ham_pass_scattered(tile_r_ham, final_state(), lam, ham_fused_g, h, d_epartials,
+    launch_ham_pass_scattered(pass_r[RIE_PASS_HAM], final_state(), lam, ham_fused_g, h, d_epartials,
                               ham_fused_grid, s);
     check_launch("ham_pass scattered");
     kt_end(s, "ham_pass_fused", 2.0 * S);
@@ -1076,11 +1083,38 @@ void Engine::set_checkpoint_interval(size_t every) {
     }
 }
 
-// Times energy_and_grad for every tile register shape per pass kind (3 or 4
-// each) and both backward schedules (when the stored one fits), keeps the
-// fastest and returns a JSON report.  Replaces fixed per-qubit-count kernel
-// choices (reference: structured_register_threads_per_block,
+// Tile register shapes and backward schedule for the caller's parameters.
+// For each schedule that fits, energy_and_grad runs with every pass kind at
+// shape 4 and then at shape 3 under per-launch kernel timing; each pass kind
+// takes the shape whose kernels ran faster (the kinds are separate kernels),
+// and that mixed choice is timed as well.  The fastest timed configuration
+// wins.  Replaces fixed per-qubit-count kernel choices (reference:
+// structured_register_threads_per_block,
 // cpp/ising_cuda_rotation_kernels.cu:562-575).
+namespace {
+int pass_kind_of(const std::string &name) {
+    if (name == "fwd_pass_high_gen_init") {
+        return RIE_PASS_FWD_GEN;
+    }
+    if (name == "fwd_pass_high") {
+        return RIE_PASS_FWD_HIGH;
+    }
+    if (name == "fwd_pass_low_cnot") {
+        return RIE_PASS_FWD_LOW;
+    }
+    if (name.rfind("ham_pass", 0) == 0) {
+        return RIE_PASS_HAM;
+    }
+    if (name == "bwd_pass_low_cnot") {
+        return RIE_PASS_BWD_LOW;
+    }
+    if (name == "bwd_pass_high") {
+        return RIE_PASS_BWD_HIGH;
+    }
+    return -1;
+}
+}  // namespace
+
 std::string Engine::autotune(const double *params, size_t count, size_t repeats) {
     if (count != num_params) {
         fail(RIE_INVALID_ARGUMENT, "num_params does not match the circuit.");
@@ -1108,55 +1142,101 @@ std::string Engine::autotune(const double *params, size_t count, size_t repeats)
             schedules.push_back(RIE_SCHEDULE_STORED);
         }
     }
-    std::string rep = "{\"tuned\": true, \"ms\": {";
-    double best = -1.0;
-    int best_cfg[4] = {schedule, tile_r_fwd, tile_r_ham, tile_r_bwd};
-    bool first = true;
-    for (int sched : schedules) {
-        set_schedule(sched);
-        for (int rf : {4, 3}) {
-            for (int rh : {4, 3}) {
-                for (int rb : {4, 3}) {
-                    tile_r_fwd = rf;
-                    tile_r_ham = rh;
-                    tile_r_bwd = rb;
-                    eval_device(d_params, d_energy, d_grad, s);
-                    check(hipEventRecord(ev_a, s), "event");
-                    for (size_t r = 0; r < repeats; ++r) {
-                        eval_device(d_params, d_energy, d_grad, s);
-                    }
-                    check(hipEventRecord(ev_b, s), "event");
-                    check(hipEventSynchronize(ev_b), "event sync");
-                    float ms = 0.0f;
-                    check(hipEventElapsedTime(&ms, ev_a, ev_b), "elapsed");
-                    const double avg = (double)ms / (double)repeats;
-                    char buf[96];
-                    std::snprintf(buf, sizeof(buf), "%s\"%s/%d%d%d\": %.3f", first ? "" : ", ",
-                                  sched == RIE_SCHEDULE_STORED ? "stored" : "inverse_walk", rf,
-                                  rh, rb, avg);
-                    rep += buf;
-                    first = false;
-                    if (best < 0.0 || avg < best) {
-                        best = avg;
-                        best_cfg[0] = sched;
-                        best_cfg[1] = rf;
-                        best_cfg[2] = rh;
-                        best_cfg[3] = rb;
-                    }
+    const bool saved_ktime = ktime;
+    kt_flush();
+    // wall time of `repeats` evaluations; with per_kind, the kernel time per
+    // pass kind as well
+    auto time_config = [&](double *per_kind) -> double {
+        eval_device(d_params, d_energy, d_grad, s);
+        if (per_kind != nullptr) {
+            kt_flush();
+            kstats.clear();
+            ktime = true;
+        }
+        check(hipEventRecord(ev_a, s), "event");
+        for (size_t r = 0; r < repeats; ++r) {
+            eval_device(d_params, d_energy, d_grad, s);
+        }
+        check(hipEventRecord(ev_b, s), "event");
+        check(hipEventSynchronize(ev_b), "event sync");
+        float ms = 0.0f;
+        check(hipEventElapsedTime(&ms, ev_a, ev_b), "elapsed");
+        if (per_kind != nullptr) {
+            kt_flush();
+            ktime = false;
+            for (int k = 0; k < RIE_PASS_KINDS; ++k) {
+                per_kind[k] = 0.0;
+            }
+            for (const auto &st : kstats) {
+                const int k = pass_kind_of(st.name);
+                if (k >= 0) {
+                    per_kind[k] += st.total_ms;
                 }
             }
+            kstats.clear();
         }
+        return (double)ms / (double)repeats;
+    };
+    auto shapes_tag = [](const int *r) {
+        std::string t;
+        for (int k = 0; k < RIE_PASS_KINDS; ++k) {
+            t += (char)('0' + r[k]);
+        }
+        return t;
+    };
+    std::string rep = "{\"tuned\": true, \"ms\": {";
+    double best = -1.0;
+    int best_sched = schedule;
+    int best_r[RIE_PASS_KINDS];
+    std::copy(pass_r, pass_r + RIE_PASS_KINDS, best_r);
+    bool first = true;
+    auto record = [&](int sched, const int *r, double ms) {
+        char buf[96];
+        std::snprintf(buf, sizeof(buf), "%s\"%s/%s\": %.3f", first ? "" : ", ",
+                      sched == RIE_SCHEDULE_STORED ? "stored" : "inverse_walk",
+                      shapes_tag(r).c_str(), ms);
+        rep += buf;
+        first = false;
+        if (best < 0.0 || ms < best) {
+            best = ms;
+            best_sched = sched;
+            std::copy(r, r + RIE_PASS_KINDS, best_r);
+        }
+    };
+    try {
+        for (int sched : schedules) {
+            set_schedule(sched);
+            double kind_ms[2][RIE_PASS_KINDS];
+            for (int u = 0; u < 2; ++u) {
+                std::fill(pass_r, pass_r + RIE_PASS_KINDS, u == 0 ? 4 : 3);
+                record(sched, pass_r, time_config(kind_ms[u]));
+            }
+            int mixed[RIE_PASS_KINDS];
+            bool differs = false;
+            for (int k = 0; k < RIE_PASS_KINDS; ++k) {
+                mixed[k] = kind_ms[1][k] < kind_ms[0][k] ? 3 : 4;
+                differs = differs || mixed[k] != mixed[0];
+            }
+            if (differs) {
+                std::copy(mixed, mixed + RIE_PASS_KINDS, pass_r);
+                record(sched, pass_r, time_config(nullptr));
+            }
+        }
+    } catch (...) {
+        ktime = saved_ktime;
+        throw;
     }
-    set_schedule(best_cfg[0]);
-    tile_r_fwd = best_cfg[1];
-    tile_r_ham = best_cfg[2];
-    tile_r_bwd = best_cfg[3];
+    ktime = saved_ktime;
+    set_schedule(best_sched);
+    std::copy(best_r, best_r + RIE_PASS_KINDS, pass_r);
     order_end(s);
-    char tail[160];
+    char tail[256];
     std::snprintf(tail, sizeof(tail),
-                  "}, \"best_ms\": %.3f, \"schedule\": \"%s\", \"tile_shapes\": [%d, %d, %d]}", best,
-                  best_cfg[0] == RIE_SCHEDULE_STORED ? "stored" : "inverse_walk", best_cfg[1],
-                  best_cfg[2], best_cfg[3]);
+                  "}, \"best_ms\": %.3f, \"schedule\": \"%s\", \"tile_shapes\": [%d, %d, %d], "
+                  "\"pass_shapes\": [%d, %d, %d, %d, %d, %d]}",
+                  best, best_sched == RIE_SCHEDULE_STORED ? "stored" : "inverse_walk",
+                  pass_r[RIE_PASS_FWD_HIGH], pass_r[RIE_PASS_HAM], pass_r[RIE_PASS_BWD_HIGH],
+                  pass_r[0], pass_r[1], pass_r[2], pass_r[3], pass_r[4], pass_r[5]);
     rep += tail;
     return rep;
 }
@@ -1437,9 +1517,12 @@ rie_status rie_engine_set_tile_shapes(rie_engine *engine, int r_fwd, int r_ham, 
                 rie::fail(RIE_INVALID_ARGUMENT, "tile shapes must be 3 or 4.");
             }
         }
-        e->tile_r_fwd = r_fwd;
-        e->tile_r_ham = r_ham;
-        e->tile_r_bwd = r_bwd;
+        e->pass_r[RIE_PASS_FWD_GEN] = r_fwd;
+        e->pass_r[RIE_PASS_FWD_HIGH] = r_fwd;
+        e->pass_r[RIE_PASS_FWD_LOW] = r_fwd;
+        e->pass_r[RIE_PASS_HAM] = r_ham;
+        e->pass_r[RIE_PASS_BWD_LOW] = r_bwd;
+        e->pass_r[RIE_PASS_BWD_HIGH] = r_bwd;
     });
 }
 
@@ -1449,9 +1532,40 @@ rie_status rie_engine_tile_shapes(const rie_engine *engine, int *shapes_out3) {
             rie::fail(RIE_INVALID_ARGUMENT, "null pointer argument.");
         }
         const Engine *e = reinterpret_cast<const Engine *>(engine);
-        shapes_out3[0] = e->tile_r_fwd;
-        shapes_out3[1] = e->tile_r_ham;
-        shapes_out3[2] = e->tile_r_bwd;
+        shapes_out3[0] = e->pass_r[RIE_PASS_FWD_HIGH];
+        shapes_out3[1] = e->pass_r[RIE_PASS_HAM];
+        shapes_out3[2] = e->pass_r[RIE_PASS_BWD_HIGH];
+    });
+}
+
+rie_status rie_engine_set_pass_shapes(rie_engine *engine, const int *shapes, size_t count) {
+    return guard([&]() {
+        Engine *e = as_engine(engine);
+        if (shapes == nullptr) {
+            rie::fail(RIE_INVALID_ARGUMENT, "null pointer argument.");
+        }
+        if (count != (size_t)RIE_PASS_KINDS) {
+            rie::fail(RIE_INVALID_ARGUMENT, "count must be RIE_PASS_KINDS.");
+        }
+        for (size_t k = 0; k < count; ++k) {
+            if (shapes[k] != 3 && shapes[k] != 4) {
+                rie::fail(RIE_INVALID_ARGUMENT, "tile shapes must be 3 or 4.");
+            }
+        }
+        std::copy(shapes, shapes + count, e->pass_r);
+    });
+}
+
+rie_status rie_engine_pass_shapes(const rie_engine *engine, int *shapes_out, size_t count) {
+    return guard([&]() {
+        if (engine == nullptr || shapes_out == nullptr) {
+            rie::fail(RIE_INVALID_ARGUMENT, "null pointer argument.");
+        }
+        if (count != (size_t)RIE_PASS_KINDS) {
+            rie::fail(RIE_INVALID_ARGUMENT, "count must be RIE_PASS_KINDS.");
+        }
+        const Engine *e = reinterpret_cast<const Engine *>(engine);
+        std::copy(e->pass_r, e->pass_r + count, shapes_out);
     });
 }
 

@@ -40,6 +40,8 @@ C_ABI_SYMBOLS = (
     "rie_engine_set_schedule",
     "rie_engine_schedule",
     "rie_engine_set_fused_hamiltonian",
+    "rie_engine_set_pass_shapes",
+    "rie_engine_pass_shapes",
     "rie_engine_fused_hamiltonian",
     "rie_engine_set_kernel_timing",
     "rie_engine_reset_kernel_stats",

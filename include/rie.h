@@ -176,6 +176,24 @@ int rie_engine_fused_hamiltonian(const rie_engine *engine);
 rie_status rie_engine_set_tile_shapes(rie_engine *engine, int r_fwd, int r_ham, int r_bwd);
 rie_status rie_engine_tile_shapes(const rie_engine *engine, int *shapes_out3);
 
+/* The same per pass kind (no counterpart in the reference): shapes[i] is the
+ * register shape (3 or 4) of kind i of rie_pass_kind, count = RIE_PASS_KINDS.
+ * rie_engine_set_tile_shapes sets the three forward kinds to r_fwd, the
+ * Hamiltonian kind to r_ham and the two backward kinds to r_bwd;
+ * rie_engine_tile_shapes reports the high-pass forward, Hamiltonian and
+ * high-pass backward kinds. */
+typedef enum rie_pass_kind {
+    RIE_PASS_FWD_GEN = 0,   /* first forward pass of layer 1 (layer 0 generated) */
+    RIE_PASS_FWD_HIGH = 1,  /* forward, high wires */
+    RIE_PASS_FWD_LOW = 2,   /* forward, low wires + ring CNOT */
+    RIE_PASS_HAM = 3,       /* Hamiltonian passes */
+    RIE_PASS_BWD_LOW = 4,   /* adjoint, inverse ring CNOT + low wires */
+    RIE_PASS_BWD_HIGH = 5,  /* adjoint, high wires */
+    RIE_PASS_KINDS = 6
+} rie_pass_kind;
+rie_status rie_engine_set_pass_shapes(rie_engine *engine, const int *shapes, size_t count);
+rie_status rie_engine_pass_shapes(const rie_engine *engine, int *shapes_out, size_t count);
+
 /* Autotuner for the tile-pass path: times energy_and_grad on `params` (host,
  * num_params values) `repeats` times for each tile shape combination and each
  * backward schedule that fits in memory, keeps the fastest, and writes a JSON

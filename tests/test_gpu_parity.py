@@ -86,6 +86,15 @@ def test_tile_register_shapes(oracle, backend_module, n, layers, schedule, shape
     assert np.max(np.abs(np.asarray(out["gradient"]) - g_ref)) <= TOL
     with pytest.raises(ValueError):
         eng.set_tile_shapes(5, 4, 4)
+    # per pass kind: gen, fwd high, fwd low, H, bwd low, bwd high
+    mixed = [3, 4, 3, 4, 4, 3]
+    eng.set_pass_shapes(mixed)
+    assert list(eng.pass_shapes) == mixed
+    out = eng.energy_and_grad(params.reshape(-1))
+    assert abs(out["energy"] - e_ref) <= TOL
+    assert np.max(np.abs(np.asarray(out["gradient"]) - g_ref)) <= TOL
+    with pytest.raises(ValueError):
+        eng.set_pass_shapes([3, 4, 3])
 
 
 @pytest.mark.parametrize("schedule", ["stored", "inverse_walk"])
@@ -165,8 +174,11 @@ def test_engine_autotune_keeps_results(oracle, backend_module):
     rep = eng.autotune(params.reshape(-1), 1)
     print("engine autotune", rep, flush=True)
     assert rep["tuned"] is True
-    assert len(rep["ms"]) in (8, 16)
+    # per schedule: all kinds at shape 4, all at 3, and the per-kind choice
+    # when it differs from both
+    assert 4 <= len(rep["ms"]) <= 6
     assert list(eng.tile_shapes) == rep["tile_shapes"]
+    assert list(eng.pass_shapes) == rep["pass_shapes"]
     assert eng.schedule == rep["schedule"]
     e_ref, g_ref = oracle.energy_and_grad(n, layers, 1.0, params)
     out = eng.energy_and_grad(params.reshape(-1))
