@@ -983,13 +983,13 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_la
             constexpr uint32_t w1 = window_mask<R, 1>();
             constexpr uint32_t w2 = window_mask<R, 2>();
             grad_window<R, 1>(c, l, lc, G, wave_acc, lane, rz_tab);
-            exchange<R, 1, 2>(l, lds, tl);
-            exchange<R, 1, 2>(c, lds, tl);
-            if constexpr (kHam && gather_pairs_reach<R, 2>()) {
-                add_pair_terms<R, 2>(c, l, lds, tl, G, w1, field, e);
-            }
-            grad_window<R, 2>(c, l, lc, G, wave_acc, lane, rz_tab);
             if constexpr (TL::P == 4) {
+                exchange<R, 1, 2>(l, lds, tl);
+                exchange<R, 1, 2>(c, lds, tl);
+                if constexpr (kHam && gather_pairs_reach<R, 2>()) {
+                    add_pair_terms<R, 2>(c, l, lds, tl, G, w1, field, e);
+                }
+                grad_window<R, 2>(c, l, lc, G, wave_acc, lane, rz_tab);
                 exchange<R, 2, 3>(l, lds, tl);
                 exchange<R, 2, 3>(c, lds, tl);
                 if constexpr (kHam && gather_pairs_reach<R, 3>()) {
@@ -1012,18 +1012,26 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_la
                 ry_inv_window<R, 3>(l, tile_coef, G);
                 store_lam<R, 3, true>(l, lam_out, base, ta, tl);
             } else {
-                exchange<R, 2, 0>(l, lds, tl);
-                exchange<R, 2, 0>(c, lds, tl);
+                // windows 1, 0, 2, then lam back through 1 to the contiguous
+                // mapping 0 for the store
+                exchange<R, 1, 0>(l, lds, tl);
+                exchange<R, 1, 0>(c, lds, tl);
                 if constexpr (kHam) {
-                    add_pair_terms<R, 0>(c, l, lds, tl, G, w1 | w2, field, e);
+                    add_pair_terms<R, 0>(c, l, lds, tl, G, w1, field, e);
                 }
                 grad_window<R, 0>(c, l, lc, G, wave_acc, lane, rz_tab);
-                ry_inv_window<R, 0>(l, tile_coef, G);
-                exchange<R, 0, 1>(l, lds, tl);
-                ry_inv_window<R, 1>(l, tile_coef, G);
-                exchange<R, 1, 2>(l, lds, tl);
+                exchange<R, 0, 2>(l, lds, tl);
+                exchange<R, 0, 2>(c, lds, tl);
+                if constexpr (kHam && gather_pairs_reach<R, 2>()) {
+                    add_pair_terms<R, 2>(c, l, lds, tl, G, w1 | window_mask<R, 0>(), field, e);
+                }
+                grad_window<R, 2>(c, l, lc, G, wave_acc, lane, rz_tab);
                 ry_inv_window<R, 2>(l, tile_coef, G);
-                store_lam<R, 2, true>(l, lam_out, base, ta, tl);
+                exchange<R, 2, 1>(l, lds, tl);
+                ry_inv_window<R, 1>(l, tile_coef, G);
+                exchange<R, 1, 0>(l, lds, tl);
+                ry_inv_window<R, 0>(l, tile_coef, G);
+                store_lam<R, 0, true>(l, lam_out, base, ta, tl);
             }
         } else {
             constexpr uint32_t w0 = window_mask<R, 0>();
@@ -1132,18 +1140,18 @@ static void launch_bwd_pass_r(bool gather, bool write_cur, bool write_lam, const
                               const WireCoef *lc, const PassGeom &g, double *partials, int grid,
                               hipStream_t s, bool ham, double field, double *e_partials) {
     constexpr int TH = Tile<R>::THREADS;
-    // A high pass that does not rewrite the state takes the lambda-only sweep
-    // (measured at 30 qubits: 16.1 -> 15.2 ms); the low pass keeps the one
-    // sweep, whose store mapping is the contiguous one.
-    if (!write_cur && write_lam && !gather) {
-        if (ham) {
-            bwd_lam_kernel<R, false, true><<<grid, TH, 0, s>>>(cur_in, lam_in, lam_out, lc, g,
-                                                              partials, field, e_partials);
-        } else {
-            bwd_lam_kernel<R, false, false><<<grid, TH, 0, s>>>(cur_in, lam_in, lam_out, lc, g,
-                                                               partials, field, e_partials);
-        }
-        return;
+    if (!write_cur && write_lam) {  // the state is not rewritten: lambda-only sweep
+#define RIE_BWD_LAM(GA, HM)                                                                  \
+    if (gather == GA && ham == HM) {                                                         \
+        bwd_lam_kernel<R, GA, HM><<<grid, TH, 0, s>>>(cur_in, lam_in, lam_out, lc, g,        \
+                                                      partials, field, e_partials);           \
+        return;                                                                              \
+    }
+        RIE_BWD_LAM(true, false)
+        RIE_BWD_LAM(false, false)
+        RIE_BWD_LAM(true, true)
+        RIE_BWD_LAM(false, true)
+#undef RIE_BWD_LAM
     }
 #define RIE_BWD(GA, WC, WL, HM)                                                              \
     if (gather == GA && write_cur == WC && write_lam == WL && ham == HM) {                   \
