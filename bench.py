@@ -56,7 +56,7 @@ def parse_args() -> argparse.Namespace:
                    help="keep the default tile shapes and schedule instead of autotuning")
     p.add_argument("--tile-shapes", default="auto",
                    help="register shapes (3 or 4): fwd,ham,bwd or per pass kind "
-                        "gen,fwd_high,fwd_low,ham,bwd_low,bwd_high, or auto")
+                        "gen,fwd_high,fwd_low,ham,bwd_low,bwd_high,layer0, or auto")
     p.add_argument("--no-other-schedule", action="store_true",
                    help="skip timing the other backward schedule")
     p.add_argument("--no-fused-hamiltonian", action="store_true",

@@ -729,7 +729,8 @@ void Engine::layer0_gradients(hipStream_t s, const double2 *lam, const double *s
         fail(RIE_DEVICE_ERROR, "internal: layer-0 partial buffer too small");
     }
     kt_begin(s);
-    launch_layer0_contract(lam, d_coef, g0, d_tile_sums, d_partials, grid0, s);
+    launch_layer0_contract(pass_r[RIE_PASS_LAYER0], lam, d_coef, g0, d_tile_sums, d_partials,
+                           grid0, s);
     check_launch("layer0_contract");
     kt_end(s, "layer0_contract", S);
     kt_begin(s);
@@ -1111,6 +1112,9 @@ int pass_kind_of(const std::string &name) {
     if (name == "bwd_pass_high") {
         return RIE_PASS_BWD_HIGH;
     }
+    if (name == "layer0_contract") {
+        return RIE_PASS_LAYER0;
+    }
     return -1;
 }
 }  // namespace
@@ -1233,10 +1237,10 @@ std::string Engine::autotune(const double *params, size_t count, size_t repeats)
     char tail[256];
     std::snprintf(tail, sizeof(tail),
                   "}, \"best_ms\": %.3f, \"schedule\": \"%s\", \"tile_shapes\": [%d, %d, %d], "
-                  "\"pass_shapes\": [%d, %d, %d, %d, %d, %d]}",
+                  "\"pass_shapes\": [%d, %d, %d, %d, %d, %d, %d]}",
                   best, best_sched == RIE_SCHEDULE_STORED ? "stored" : "inverse_walk",
                   pass_r[RIE_PASS_FWD_HIGH], pass_r[RIE_PASS_HAM], pass_r[RIE_PASS_BWD_HIGH],
-                  pass_r[0], pass_r[1], pass_r[2], pass_r[3], pass_r[4], pass_r[5]);
+                  pass_r[0], pass_r[1], pass_r[2], pass_r[3], pass_r[4], pass_r[5], pass_r[6]);
     rep += tail;
     return rep;
 }

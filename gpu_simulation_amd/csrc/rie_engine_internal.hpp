@@ -83,7 +83,7 @@ struct Engine {
     // registers, 3 = 512 threads x 8 registers (both 4096-amplitude tiles)
     // (defaults measured on MI355X at 30 qubits: the 512-thread shape is
     // faster for the forward and H passes, the 256-thread one for backward)
-    int pass_r[RIE_PASS_KINDS] = {3, 3, 3, 3, 4, 4};
+    int pass_r[RIE_PASS_KINDS] = {3, 3, 3, 3, 4, 4, 4};
     int T = 12;
     std::vector<PassPlan> passes;  // [0]: low pass (a = T); [1..]: high passes
     // Fused Hamiltonian of gradient evaluations (L >= 2): one psi-frame pass

@@ -344,13 +344,20 @@ void launch_init_product_cnot(double2 *out, const WireCoef *coef_layer0, int n, 
 // Forward tile pass.
 // ---------------------------------------------------------------------------
 
+// Waves per SIMD to budget registers for: two 512-thread workgroups per CU
+// (four waves per SIMD) for the R = 3 shape, whose LDS allows two.
+template <int R>
+constexpr int tile_min_blocks() {
+    return R == 3 ? 4 : 2;
+}
+
 // With kGenInit the pass does not read its input: the tile of the layer-0
 // state after its ring CNOT, chi(i) = prod_w u_w(bit w of P^-1(i)), is
 // generated from nibble tables of the layer-0 factors (lc0).
 constexpr int kInitNibbles = 9;  // covers n <= 36
 
 template <int R, bool kScatter, bool kGenInit>
-__global__ __launch_bounds__(Tile<R>::THREADS) void fwd_pass_kernel(
+__global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void fwd_pass_kernel(
     const double2 *in, double2 *out, const WireCoef *__restrict__ lc,
     const WireCoef *__restrict__ lc0, PassGeom G) {
     using TL = Tile<R>;
@@ -535,12 +542,6 @@ void launch_fwd_pass(int R, bool scatter, bool gen_init, const double2 *in, doub
 // Adjoint tile pass.
 // ---------------------------------------------------------------------------
 
-// Waves per SIMD to budget registers for: two 512-thread workgroups per CU
-// (four waves per SIMD) for the R = 3 shape, whose LDS allows two.
-template <int R>
-constexpr int tile_min_blocks() {
-    return R == 3 ? 4 : 2;
-}
 
 // Register window of mapping M as a mask of tile offset bits.
 template <int R, int M>
@@ -1238,6 +1239,9 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void ham_pa
     }
     const int lds_hi = G.rot_hi < T - R ? G.rot_hi : T - R;
     for (uint64_t g = blockIdx.x; g < G.num_tiles; g += gridDim.x) {
+        // per-tile opaque thread index: LDS slots are formed where used
+        uint32_t tl = t;
+        asm volatile("" : "+v"(tl));
         const uint64_t base = tbase(g);
         const uint64_t row = base + ta.offA;
         double2 v[TL::REGS];
@@ -1260,12 +1264,12 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void ham_pa
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < TL::REGS; ++k) {
-            lds[lds_slot<R>(omap<R, 0>(t, k))] = v[k];
+            lds[lds_slot<R>(omap<R, 0>(tl, k))] = v[k];
         }
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < TL::REGS; ++k) {
-            const uint32_t o = omap<R, 0>(t, k);
+            const uint32_t o = omap<R, 0>(tl, k);
             const uint64_t idx = eidx(row, k);
             double2 flips = make_double2(0.0, 0.0);
             for (int j = G.rot_lo; j < lds_hi; ++j) {
@@ -1463,6 +1467,16 @@ __device__ __forceinline__ void cmac(double2 &acc, double2 a, double2 b) {
     acc.y += a.x * b.y + a.y * b.x;
 }
 
+// conj(a) * b and acc += conj(a) * b
+__device__ __forceinline__ double2 cmul_ca(double2 a, double2 b) {
+    return make_double2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
+}
+
+__device__ __forceinline__ void cmac_ca(double2 &acc, double2 a, double2 b) {
+    acc.x += a.x * b.x + a.y * b.y;
+    acc.y += a.x * b.y - a.y * b.x;
+}
+
 __device__ __forceinline__ double2 u_of(const WireCoef &k, int bit) {
     return bit ? u1_of(k) : u0_of(k);
 }
@@ -1504,10 +1518,14 @@ __global__ __launch_bounds__(256) void layer0_contract_kernel(
     __syncthreads();
 
     for (uint64_t g = blockIdx.x; g < G.num_tiles; g += gridDim.x) {
+        // per-tile opaque thread index: the LDS slots are formed where used,
+        // not held (or spilled) across the loop
+        uint32_t tl = t;
+        asm volatile("" : "+v"(tl));
         const uint64_t base = g << T;
         const uint64_t pbase = ring_fwd(base, G.n);
         const uint64_t lo_flip = pbase & 1u;
-        const uint64_t run = (pbase & ~uint64_t{TL::SIZE - 1}) + (t ^ lo_flip);
+        const uint64_t run = (pbase & ~uint64_t{TL::SIZE - 1}) + (tl ^ lo_flip);
         const uint64_t run1 = run ^ G.hi_mask;
         double2 m[16];
 #pragma unroll
@@ -1517,35 +1535,30 @@ __global__ __launch_bounds__(256) void layer0_contract_kernel(
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < 16; ++k) {
-            lds[lds_slot<R>(ring_inv32(omap<R, 0>(t, k), T))] = m[k];
+            lds[lds_slot<R>(ring_inv32(omap<R, 0>(tl, k), T))] = m[k];
         }
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < 16; ++k) {
-            m[k] = lds[lds_slot<R>(omap<R, 0>(t, k))];
+            m[k] = lds[lds_slot<R>(omap<R, 0>(tl, k))];
         }
         // high-wire product of this tile
         double2 bg = make_double2(1.0, 0.0);
         for (int v = T; v < G.n; ++v) {
             bg = cmul(bg, u_of(lc0[v], (int)((base >> v) & 1U)));
         }
-        // contract the register bits k0..k3 (wires 8..11) hierarchically:
-        // x[k] = conj(mu), A contracts k0, AB k0..k1, ABC k0..k2; the
+        // contract the register bits k0..k3 (wires 8..11) hierarchically
+        // against conj(mu): A contracts k0, AB k0..k1, ABC k0..k2; the
         // environments keep one bit uncontracted.
-        double2 x[16];
-#pragma unroll
-        for (int k = 0; k < 16; ++k) {
-            x[k] = make_double2(m[k].x, -m[k].y);
-        }
         double2 A[8], F[8];
 #pragma unroll
         for (int r = 0; r < 8; ++r) {  // r = k1 + 2 k2 + 4 k3  /  k0 + 2 k2 + 4 k3
-            A[r] = cmul(x[2 * r], uf[0][0]);
-            cmac(A[r], x[2 * r + 1], uf[0][1]);
+            A[r] = cmul_ca(m[2 * r], uf[0][0]);
+            cmac_ca(A[r], m[2 * r + 1], uf[0][1]);
             const int k0 = r & 1;
             const int hi = r >> 1;  // k2 + 2 k3
-            F[r] = cmul(x[k0 + 4 * hi], uf[1][0]);
-            cmac(F[r], x[k0 + 2 + 4 * hi], uf[1][1]);
+            F[r] = cmul_ca(m[k0 + 4 * hi], uf[1][0]);
+            cmac_ca(F[r], m[k0 + 2 + 4 * hi], uf[1][1]);
         }
         double2 AB[4], E[4], G[4];
 #pragma unroll
@@ -1601,17 +1614,21 @@ __global__ __launch_bounds__(256) void layer0_contract_kernel(
         }
     }
     // reduce the environments: 8 low wires x 2 entries + 4 register wires x 2
-    // entries, complex -> 48 doubles
+    // entries, complex -> 48 doubles.  The thread index is made opaque here so
+    // the per-thread factors below are not formed before the tile loop and
+    // held in registers across it.
+    uint32_t te = t;
+    asm volatile("" : "+v"(te));
 #pragma unroll
     for (int w = 0; w < 8; ++w) {
         double2 skip = make_double2(1.0, 0.0);  // prod_{v < 8, v != w} u_v(t_v)
         for (int v = 0; v < 8; ++v) {
             if (v != w) {
-                skip = cmul(skip, u_of(lc0[v], (t >> v) & 1));
+                skip = cmul(skip, u_of(lc0[v], (te >> v) & 1));
             }
         }
         const double2 val = cmul(bpk_sum, skip);
-        const int b = (t >> w) & 1;
+        const int b = (te >> w) & 1;
         const double vals[4] = {b ? 0.0 : val.x, b ? 0.0 : val.y, b ? val.x : 0.0,
                                 b ? val.y : 0.0};
 #pragma unroll
@@ -1632,6 +1649,166 @@ __global__ __launch_bounds__(256) void layer0_contract_kernel(
             const double r = wave_sum(vals[c]);
             if (lane == 0) {
                 red[wave][32 + 4 * e + c] = r;
+            }
+        }
+    }
+    __syncthreads();
+    if (t < 48) {
+        double acc = 0.0;
+        for (int w = 0; w < TL::WAVES; ++w) {
+            acc += red[w][t];
+        }
+        partials[(uint64_t)blockIdx.x * 48 + t] = acc;
+    }
+}
+
+// The same contraction with 512 threads x 8 registers: threads hold tile
+// offset bits 0..8 (wires 0..8), registers bits 9..11 (wires 9..11); twice
+// the waves of the 256-thread kernel keep more of the read stream in flight.
+// Partials use the same layout (4 doubles per wire w at column 4 w).
+__global__ __launch_bounds__(512, 4) void layer0_contract_r3_kernel(
+    const double2 *__restrict__ lam, const WireCoef *__restrict__ lc0, PassGeom G,
+    double2 *__restrict__ tile_sums, double *__restrict__ partials) {
+    constexpr int R = 3;
+    using TL = Tile<R>;
+    constexpr int T = TL::T;
+    constexpr int TB = T - R;  // thread bits: wires 0..8
+    __shared__ double2 lds[TL::SLOTS];
+    __shared__ double2 red2[TL::WAVES];
+    __shared__ double red[TL::WAVES][48];
+    const uint32_t t = threadIdx.x;
+    const int lane = t & 63;
+    const int wave = t >> 6;
+    double2 uf[3][2];  // u_{9+j}(b) for the register wires
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        const WireCoef kj = lc0[TB + j];
+        uf[j][0] = u0_of(kj);
+        uf[j][1] = u1_of(kj);
+    }
+    double2 a_lo = make_double2(1.0, 0.0);  // thread constant over wires 0..8
+    for (int v = 0; v < TB; ++v) {
+        a_lo = cmul(a_lo, u_of(lc0[v], (t >> v) & 1));
+    }
+    double2 bpk_sum = make_double2(0.0, 0.0);
+    double2 benv[3][2];
+#pragma unroll
+    for (int e = 0; e < 3; ++e) {
+        benv[e][0] = make_double2(0.0, 0.0);
+        benv[e][1] = make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+
+    for (uint64_t g = blockIdx.x; g < G.num_tiles; g += gridDim.x) {
+        // per-tile opaque thread index: the LDS slots are formed where used,
+        // not held (or spilled) across the loop
+        uint32_t tl = t;
+        asm volatile("" : "+v"(tl));
+        const uint64_t base = g << T;
+        const uint64_t pbase = ring_fwd(base, G.n);
+        const uint64_t lo_flip = pbase & 1u;
+        const uint64_t run = (pbase & ~uint64_t{TL::SIZE - 1}) + (tl ^ lo_flip);
+        const uint64_t run1 = run ^ G.hi_mask;
+        double2 m[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            m[k] = lam[(k < 4 ? run : run1) + ((uint64_t)k << TB)];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            lds[lds_slot<R>(ring_inv32(omap<R, 0>(tl, k), T))] = m[k];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            m[k] = lds[lds_slot<R>(omap<R, 0>(tl, k))];
+        }
+        double2 bg = make_double2(1.0, 0.0);  // high-wire product of this tile
+        for (int v = T; v < G.n; ++v) {
+            bg = cmul(bg, u_of(lc0[v], (int)((base >> v) & 1U)));
+        }
+        // against conj(mu), k = k0 + 2 k1 + 4 k2 (wires 9, 10, 11)
+        double2 A[4];  // k0 contracted: A[k1 + 2 k2]
+        double2 F[4];  // k1 contracted: F[k0 + 2 k2]
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            A[r] = cmul_ca(m[2 * r], uf[0][0]);
+            cmac_ca(A[r], m[2 * r + 1], uf[0][1]);
+            const int k0 = r & 1;
+            const int k2 = r >> 1;
+            F[r] = cmul_ca(m[k0 + 4 * k2], uf[1][0]);
+            cmac_ca(F[r], m[k0 + 2 + 4 * k2], uf[1][1]);
+        }
+        double2 env[3][2];
+        double2 AB[2];  // k0, k1 contracted: AB[k2]
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            AB[b] = cmul(A[2 * b], uf[1][0]);
+            cmac(AB[b], A[2 * b + 1], uf[1][1]);
+            env[2][b] = AB[b];
+            env[1][b] = cmul(A[b], uf[2][0]);      // contract k2 of A[k1 = b, k2]
+            cmac(env[1][b], A[b + 2], uf[2][1]);
+            env[0][b] = cmul(F[b], uf[2][0]);      // contract k2 of F[k0 = b, k2]
+            cmac(env[0][b], F[b + 2], uf[2][1]);
+        }
+        double2 pk = cmul(AB[0], uf[2][0]);
+        cmac(pk, AB[1], uf[2][1]);
+#pragma unroll
+        for (int e = 0; e < 3; ++e) {
+            cmac(benv[e][0], bg, env[e][0]);
+            cmac(benv[e][1], bg, env[e][1]);
+        }
+        cmac(bpk_sum, bg, pk);
+        const double2 st = cmul(pk, a_lo);
+        const double sx = wave_sum(st.x);
+        const double sy = wave_sum(st.y);
+        if (lane == 0) {
+            red2[wave] = make_double2(sx, sy);
+        }
+        __syncthreads();
+        if (t == 0) {
+            double2 tot = red2[0];
+            for (int w = 1; w < TL::WAVES; ++w) {
+                tot.x += red2[w].x;
+                tot.y += red2[w].y;
+            }
+            tile_sums[g] = tot;
+        }
+    }
+    // environments: 9 thread wires and 3 register wires, 4 doubles each (the
+    // thread index is opaque so the per-thread factors are formed here)
+    uint32_t te = t;
+    asm volatile("" : "+v"(te));
+    for (int w = 0; w < TB; ++w) {
+        double2 skip = make_double2(1.0, 0.0);  // prod_{v < 9, v != w} u_v(t_v)
+        for (int v = 0; v < TB; ++v) {
+            if (v != w) {
+                skip = cmul(skip, u_of(lc0[v], (te >> v) & 1));
+            }
+        }
+        const double2 val = cmul(bpk_sum, skip);
+        const int b = (te >> w) & 1;
+        const double vals[4] = {b ? 0.0 : val.x, b ? 0.0 : val.y, b ? val.x : 0.0,
+                                b ? val.y : 0.0};
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const double r = wave_sum(vals[c]);
+            if (lane == 0) {
+                red[wave][4 * w + c] = r;
+            }
+        }
+    }
+#pragma unroll
+    for (int e = 0; e < 3; ++e) {
+        const double2 v0 = cmul(a_lo, benv[e][0]);
+        const double2 v1 = cmul(a_lo, benv[e][1]);
+        const double vals[4] = {v0.x, v0.y, v1.x, v1.y};
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const double r = wave_sum(vals[c]);
+            if (lane == 0) {
+                red[wave][4 * (TB + e) + c] = r;
             }
         }
     }
@@ -1736,9 +1913,13 @@ __global__ __launch_bounds__(64) void layer0_finish_kernel(
     }
 }
 
-void launch_layer0_contract(const double2 *lam, const WireCoef *lc0, const PassGeom &g,
+void launch_layer0_contract(int R, const double2 *lam, const WireCoef *lc0, const PassGeom &g,
                             double2 *tile_sums, double *partials, int grid, hipStream_t s) {
-    layer0_contract_kernel<<<grid, 256, 0, s>>>(lam, lc0, g, tile_sums, partials);
+    if (R == 3) {
+        layer0_contract_r3_kernel<<<grid, 512, 0, s>>>(lam, lc0, g, tile_sums, partials);
+    } else {
+        layer0_contract_kernel<<<grid, 256, 0, s>>>(lam, lc0, g, tile_sums, partials);
+    }
 }
 
 void launch_layer0_high_env(const double2 *tile_sums, uint64_t num_tiles, const WireCoef *lc0,

@@ -138,7 +138,8 @@ void launch_ham_pass_scattered(int R, const double2 *psi, double2 *lam_out, cons
 // state: one read pass over lambda (gathered through the inverse ring CNOT
 // per low tile, R = 4), a reduction of the per-tile sums over the high wires,
 // and the final per-wire assembly into grad[0 .. 2n).
-void launch_layer0_contract(const double2 *lam, const WireCoef *lc0, const PassGeom &g,
+// R: register shape of the contraction (4: 256 threads x 16, 3: 512 x 8).
+void launch_layer0_contract(int R, const double2 *lam, const WireCoef *lc0, const PassGeom &g,
                             double2 *tile_sums, double *partials, int grid, hipStream_t s);
 void launch_layer0_high_env(const double2 *tile_sums, uint64_t num_tiles, const WireCoef *lc0,
                             int n, double *partials, int chunks, hipStream_t s);

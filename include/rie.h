@@ -179,7 +179,8 @@ rie_status rie_engine_tile_shapes(const rie_engine *engine, int *shapes_out3);
 /* The same per pass kind (no counterpart in the reference): shapes[i] is the
  * register shape (3 or 4) of kind i of rie_pass_kind, count = RIE_PASS_KINDS.
  * rie_engine_set_tile_shapes sets the three forward kinds to r_fwd, the
- * Hamiltonian kind to r_ham and the two backward kinds to r_bwd;
+ * Hamiltonian kind to r_ham and the two backward kinds to r_bwd (the layer-0
+ * contraction keeps its shape);
  * rie_engine_tile_shapes reports the high-pass forward, Hamiltonian and
  * high-pass backward kinds. */
 typedef enum rie_pass_kind {
@@ -189,7 +190,8 @@ typedef enum rie_pass_kind {
     RIE_PASS_HAM = 3,       /* Hamiltonian passes */
     RIE_PASS_BWD_LOW = 4,   /* adjoint, inverse ring CNOT + low wires */
     RIE_PASS_BWD_HIGH = 5,  /* adjoint, high wires */
-    RIE_PASS_KINDS = 6
+    RIE_PASS_LAYER0 = 6,    /* layer-0 gradient contraction (a read pass) */
+    RIE_PASS_KINDS = 7
 } rie_pass_kind;
 rie_status rie_engine_set_pass_shapes(rie_engine *engine, const int *shapes, size_t count);
 rie_status rie_engine_pass_shapes(const rie_engine *engine, int *shapes_out, size_t count);

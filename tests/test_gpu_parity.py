@@ -86,8 +86,8 @@ def test_tile_register_shapes(oracle, backend_module, n, layers, schedule, shape
     assert np.max(np.abs(np.asarray(out["gradient"]) - g_ref)) <= TOL
     with pytest.raises(ValueError):
         eng.set_tile_shapes(5, 4, 4)
-    # per pass kind: gen, fwd high, fwd low, H, bwd low, bwd high
-    mixed = [3, 4, 3, 4, 4, 3]
+    # per pass kind: gen, fwd high, fwd low, H, bwd low, bwd high, layer-0 contraction
+    mixed = [3, 4, 3, 4, 4, 3, 4]
     eng.set_pass_shapes(mixed)
     assert list(eng.pass_shapes) == mixed
     out = eng.energy_and_grad(params.reshape(-1))
