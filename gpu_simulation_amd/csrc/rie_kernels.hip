@@ -709,40 +709,43 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_pa
 
     const TileAddr ta = tile_addr<R>(t, G);
     for (uint64_t g = blockIdx.x; g < G.num_tiles; g += gridDim.x) {
+        // per-tile opaque thread index: LDS slots are formed where used
+        uint32_t tl = t;
+        asm volatile("" : "+v"(tl));
         const uint64_t base = tile_base(g, G, T);
         double2 c[TL::REGS];
         double2 l[TL::REGS];
-        bwd_load_tile<R, kGather, kHam>(c, l, cur_in, lam_in, base, ta, G, lds, t, field, e);
+        bwd_load_tile<R, kGather, kHam>(c, l, cur_in, lam_in, base, ta, G, lds, tl, field, e);
 
         // lam is exchanged before cur, so the LDS holds cur for the pair terms
         if constexpr (kGather) {
             constexpr uint32_t w1 = window_mask<R, 1>();
             constexpr uint32_t w2 = window_mask<R, 2>();
             phase_bwd<R, 1, kGather>(c, l, lc, G, wave_acc, lane, rz_tab);
-            exchange<R, 1, 2>(l, lds, t);
-            exchange<R, 1, 2>(c, lds, t);
+            exchange<R, 1, 2>(l, lds, tl);
+            exchange<R, 1, 2>(c, lds, tl);
             if constexpr (kHam && gather_pairs_reach<R, 2>()) {
-                add_pair_terms<R, 2>(c, l, lds, t, G, w1, field, e);
+                add_pair_terms<R, 2>(c, l, lds, tl, G, w1, field, e);
             }
             phase_bwd<R, 2, kGather>(c, l, lc, G, wave_acc, lane, rz_tab);
             if constexpr (TL::P == 4) {
-                exchange<R, 2, 3>(l, lds, t);
-                exchange<R, 2, 3>(c, lds, t);
+                exchange<R, 2, 3>(l, lds, tl);
+                exchange<R, 2, 3>(c, lds, tl);
                 if constexpr (kHam && gather_pairs_reach<R, 3>()) {
-                    add_pair_terms<R, 3>(c, l, lds, t, G, w1 | w2, field, e);
+                    add_pair_terms<R, 3>(c, l, lds, tl, G, w1 | w2, field, e);
                 }
                 phase_bwd<R, 3, kGather>(c, l, lc, G, wave_acc, lane, rz_tab);
-                exchange<R, 3, 0>(l, lds, t);
-                exchange<R, 3, 0>(c, lds, t);
+                exchange<R, 3, 0>(l, lds, tl);
+                exchange<R, 3, 0>(c, lds, tl);
                 if constexpr (kHam) {
-                    add_pair_terms<R, 0>(c, l, lds, t, G, w1 | w2 | window_mask<R, 3>(), field,
+                    add_pair_terms<R, 0>(c, l, lds, tl, G, w1 | w2 | window_mask<R, 3>(), field,
                                          e);
                 }
             } else {
-                exchange<R, 2, 0>(l, lds, t);
-                exchange<R, 2, 0>(c, lds, t);
+                exchange<R, 2, 0>(l, lds, tl);
+                exchange<R, 2, 0>(c, lds, tl);
                 if constexpr (kHam) {
-                    add_pair_terms<R, 0>(c, l, lds, t, G, w1 | w2, field, e);
+                    add_pair_terms<R, 0>(c, l, lds, tl, G, w1 | w2, field, e);
                 }
             }
             phase_bwd<R, 0, kGather>(c, l, lc, G, wave_acc, lane, rz_tab);
@@ -761,43 +764,43 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_pa
             constexpr uint32_t w0 = window_mask<R, 0>();
             phase_bwd<R, 0, kGather>(c, l, lc, G, wave_acc, lane, rz_tab);
             if constexpr (TL::P == 3) {
-                exchange<R, 0, 1>(l, lds, t);
-                exchange<R, 0, 1>(c, lds, t);
+                exchange<R, 0, 1>(l, lds, tl);
+                exchange<R, 0, 1>(c, lds, tl);
                 if constexpr (kHam) {
-                    add_pair_terms<R, 1>(c, l, lds, t, G, w0, field, e);
+                    add_pair_terms<R, 1>(c, l, lds, tl, G, w0, field, e);
                 }
                 phase_bwd<R, 1, kGather>(c, l, lc, G, wave_acc, lane, rz_tab);
-                exchange<R, 1, 2>(l, lds, t);
-                exchange<R, 1, 2>(c, lds, t);
+                exchange<R, 1, 2>(l, lds, tl);
+                exchange<R, 1, 2>(c, lds, tl);
                 if constexpr (kHam) {
-                    add_pair_terms<R, 2>(c, l, lds, t, G, w0 | window_mask<R, 1>(), field, e);
+                    add_pair_terms<R, 2>(c, l, lds, tl, G, w0 | window_mask<R, 1>(), field, e);
                 }
                 phase_bwd<R, 2, kGather>(c, l, lc, G, wave_acc, lane, rz_tab);
             } else {
                 uint32_t done = w0;
                 if (G.rot_lo < R) {
-                    exchange<R, 0, 1>(l, lds, t);
-                    exchange<R, 0, 1>(c, lds, t);
+                    exchange<R, 0, 1>(l, lds, tl);
+                    exchange<R, 0, 1>(c, lds, tl);
                     if constexpr (kHam) {
-                        add_pair_terms<R, 1>(c, l, lds, t, G, done, field, e);
+                        add_pair_terms<R, 1>(c, l, lds, tl, G, done, field, e);
                     }
                     phase_bwd<R, 1, kGather>(c, l, lc, G, wave_acc, lane, rz_tab);
                     done |= window_mask<R, 1>();
-                    exchange<R, 1, 2>(l, lds, t);
-                    exchange<R, 1, 2>(c, lds, t);
+                    exchange<R, 1, 2>(l, lds, tl);
+                    exchange<R, 1, 2>(c, lds, tl);
                 } else {
-                    exchange<R, 0, 2>(l, lds, t);
-                    exchange<R, 0, 2>(c, lds, t);
+                    exchange<R, 0, 2>(l, lds, tl);
+                    exchange<R, 0, 2>(c, lds, tl);
                 }
                 if constexpr (kHam) {
-                    add_pair_terms<R, 2>(c, l, lds, t, G, done, field, e);
+                    add_pair_terms<R, 2>(c, l, lds, tl, G, done, field, e);
                 }
                 phase_bwd<R, 2, kGather>(c, l, lc, G, wave_acc, lane, rz_tab);
                 done |= window_mask<R, 2>();
-                exchange<R, 2, 3>(l, lds, t);
-                exchange<R, 2, 3>(c, lds, t);
+                exchange<R, 2, 3>(l, lds, tl);
+                exchange<R, 2, 3>(c, lds, tl);
                 if constexpr (kHam) {
-                    add_pair_terms<R, 3>(c, l, lds, t, G, done, field, e);
+                    add_pair_terms<R, 3>(c, l, lds, tl, G, done, field, e);
                 }
                 phase_bwd<R, 3, kGather>(c, l, lc, G, wave_acc, lane, rz_tab);
             }
