@@ -219,6 +219,12 @@ class RingIsingCudaBackend {
 
     bool fused_hamiltonian() const { return rie_engine_fused_hamiltonian(engine_) == 1; }
 
+    void set_layer1_generation(int enabled) {
+        check(rie_engine_set_layer1_generation(engine_, enabled));
+    }
+
+    bool layer1_generation() const { return rie_engine_layer1_generation(engine_) == 1; }
+
     void set_kernel_timing(bool enable) { check(rie_engine_set_kernel_timing(engine_, enable)); }
 
     void reset_kernel_stats() { check(rie_engine_reset_kernel_stats(engine_)); }
@@ -553,6 +559,9 @@ PYBIND11_MODULE(_cuda_backend, m) {
         .def("set_fused_hamiltonian", &RingIsingCudaBackend::set_fused_hamiltonian,
              py::arg("enabled"))
         .def_property_readonly("fused_hamiltonian", &RingIsingCudaBackend::fused_hamiltonian)
+        .def("set_layer1_generation", &RingIsingCudaBackend::set_layer1_generation,
+             py::arg("enabled"))
+        .def_property_readonly("layer1_generation", &RingIsingCudaBackend::layer1_generation)
         .def_property_readonly("schedule", &RingIsingCudaBackend::schedule)
         .def("set_kernel_timing", &RingIsingCudaBackend::set_kernel_timing, py::arg("enable"))
         .def("reset_kernel_stats", &RingIsingCudaBackend::reset_kernel_stats)

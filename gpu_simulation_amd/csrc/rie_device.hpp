@@ -151,6 +151,78 @@ __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
     return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
+// acc + a * b
+__device__ __forceinline__ double2 cfma(double2 acc, double2 a, double2 b) {
+    return make_double2(acc.x + a.x * b.x - a.y * b.y, acc.y + a.x * b.y + a.y * b.x);
+}
+
+// 2x2 complex matrix [[a00, a01], [a10, a11]].
+struct CMat2 {
+    double2 a00, a01, a10, a11;
+};
+
+__device__ __forceinline__ CMat2 cmat_mul(const CMat2 &p, const CMat2 &q) {
+    CMat2 r;
+    r.a00 = cfma(cmul(p.a00, q.a00), p.a01, q.a10);
+    r.a01 = cfma(cmul(p.a00, q.a01), p.a01, q.a11);
+    r.a10 = cfma(cmul(p.a10, q.a00), p.a11, q.a10);
+    r.a11 = cfma(cmul(p.a10, q.a01), p.a11, q.a11);
+    return r;
+}
+
+// Tr(p q)
+__device__ __forceinline__ double2 cmat_trace_mul(const CMat2 &p, const CMat2 &q) {
+    double2 r = cmul(p.a00, q.a00);
+    r = cfma(r, p.a01, q.a10);
+    r = cfma(r, p.a10, q.a01);
+    return cfma(r, p.a11, q.a11);
+}
+
+// Layer-1 states as a ring of 2x2 matrices.  The layer-0 output after its
+// ring CNOT is chi(P^-1 y) with chi = (x) u_w and (P^-1 y)_0 = y_0 ^ y_{n-1},
+// (P^-1 y)_1 = y_1 ^ y_0 ^ y_{n-1}, (P^-1 y)_k = y_k ^ y_{k-1}.  Rotating a set
+// S of wires by U_w = RZ(phi_w) RY(theta_w) gives
+//   c(q) = sum_y prod_{w in S} U_w[q_w, y_w] prod_{w not in S} [q_w == y_w] chi(P^-1 y)
+//        = Tr(M_0[q_0] M_1[q_1] ... M_{n-1}[q_{n-1}])
+// with bond variables x_0 = y_0 ^ y_{n-1} between wires 0 and 1, y_{w} between
+// w and w + 1, and z = y_{n-1} closing the ring:
+//   M_0[q](z, x_0) = U_0[q, x_0 ^ z] u_0(x_0),
+//   M_w[q](x, y)   = U_w[q, y] u_w(y ^ x)   (w >= 1).
+// k0: the wire's layer-0 coefficients, k1: layer 1's; rot: w in S.
+__device__ __forceinline__ CMat2 chain_site(int w, int q, const WireCoef &k0, const WireCoef &k1,
+                                            bool rot) {
+    const double2 u[2] = {u0_of(k0), u1_of(k0)};
+    double2 U[2];  // U_w[q, 0], U_w[q, 1]
+    if (rot) {
+        // RZ(phi) RY(theta) = [[c e^-, -s e^-], [s e^+, c e^+]], e^-+ = e^{-+ i phi/2}
+        const double2 em = make_double2(k1.ch, -k1.sh);
+        const double2 ep = make_double2(k1.ch, k1.sh);
+        if (q == 0) {
+            U[0] = make_double2(k1.c * em.x, k1.c * em.y);
+            U[1] = make_double2(-k1.s * em.x, -k1.s * em.y);
+        } else {
+            U[0] = make_double2(k1.s * ep.x, k1.s * ep.y);
+            U[1] = make_double2(k1.c * ep.x, k1.c * ep.y);
+        }
+    } else {
+        U[0] = make_double2(q == 0 ? 1.0 : 0.0, 0.0);
+        U[1] = make_double2(q == 1 ? 1.0 : 0.0, 0.0);
+    }
+    CMat2 m;
+    if (w == 0) {
+        m.a00 = cmul(U[0], u[0]);
+        m.a01 = cmul(U[1], u[1]);
+        m.a10 = cmul(U[1], u[0]);
+        m.a11 = cmul(U[0], u[1]);
+    } else {
+        m.a00 = cmul(U[0], u[0]);
+        m.a01 = cmul(U[1], u[1]);
+        m.a10 = cmul(U[0], u[1]);
+        m.a11 = cmul(U[1], u[0]);
+    }
+    return m;
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
     for (int off = kWave / 2; off > 0; off >>= 1) {

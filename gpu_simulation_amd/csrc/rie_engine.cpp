@@ -18,6 +18,8 @@
 
 #include "rie.h"
 #include "rie_engine_internal.hpp"
+
+#include <cstdlib>
 #include "rie_kernels.hpp"
 
 namespace rie {
@@ -233,6 +235,8 @@ Engine::Engine(size_t num_qubits, size_t num_layers, double field, const std::st
         epartial_count = eparts;
         const uint64_t tiles = passes[0].g.num_tiles;
         d_tile_sums = (double2 *)device_alloc(sizeof(double2) * tiles, "alloc tile sums");
+        d_chain = (CMat2 *)device_alloc(sizeof(CMat2) * kChainMaxSets * kChainTabEntries,
+                                        "alloc chain tables");
         high_env_chunks = (int)std::min<uint64_t>(64, (tiles + 255) / 256);
         partial2_count = 4 * (size_t)high_env_chunks * (size_t)std::max(1, n - T);
         d_partials2 = (double *)device_alloc(sizeof(double) * partial2_count, "alloc partials");
@@ -267,6 +271,7 @@ Engine::~Engine() {
     device_free(d_epartials);
     device_free(d_drift);
     device_free(d_tile_sums);
+    device_free(d_chain);
     device_free(d_partials2);
     dense_free();
     for (auto &p : kpending) {
@@ -406,12 +411,26 @@ void Engine::structured_forward(hipStream_t s, bool save_ckpts) {
         final_ptr = pool[0];
         return;
     }
+    gen_fwd = gen_ready();
+    if (gen_fwd) {
+        build_chain_tables(s);
+    }
     if (schedule == RIE_SCHEDULE_STORED) {
         // Every pass writes its own buffer; layer 1's first pass generates the
-        // layer-0 state instead of reading it.
+        // layer-0 state instead of reading it (or, with layer-1 generation,
+        // layer 1 is one pass that writes only its output).
         const double2 *in = nullptr;
         for (int l = 1; l < L; ++l) {
             const WireCoef *lc = d_coef + (size_t)l * n;
+            if (l == 1 && gen_fwd) {
+                kt_begin(s);
+                launch_gen_layer1(pass_r[RIE_PASS_FWD_GEN], fbuf(1, K), d_chain, passes[0].g,
+                                  passes[0].grid, s);
+                check_launch("gen_layer1");
+                kt_end(s, "fwd_layer1_gen", S);
+                in = fbuf(1, K);
+                continue;
+            }
             for (int j = 0; j < K; ++j) {
                 const bool gen = (l == 1 && j == 0);
                 kt_begin(s);
@@ -442,7 +461,8 @@ void Engine::structured_forward(hipStream_t s, bool save_ckpts) {
     int cur_buf = -1;  // -1: generated (layer 1, first pass) or a checkpoint buffer
     for (int l = 1; l < L; ++l) {
         const WireCoef *lc = d_coef + (size_t)l * n;
-        for (int j = 0; j < K; ++j) {
+        const bool gen_layer = (l == 1 && gen_fwd);
+        for (int j = 0; j < (gen_layer ? 0 : K); ++j) {
             const bool gen = (l == 1 && j == 0);
             double2 *out = (cur_buf >= 0) ? pool[cur_buf] : pool[0];
             kt_begin(s);
@@ -469,10 +489,17 @@ void Engine::structured_forward(hipStream_t s, bool save_ckpts) {
             out = pool[out_buf];
         }
         kt_begin(s);
-        launch_fwd_pass(pass_r[RIE_PASS_FWD_LOW], true, false, cur, out, lc, d_coef, passes[0].g,
-                        passes[0].grid, s);
-        check_launch("fwd_pass scatter");
-        kt_end(s, "fwd_pass_low_cnot", 2 * S);
+        if (gen_layer) {
+            launch_gen_layer1(pass_r[RIE_PASS_FWD_GEN], out, d_chain, passes[0].g, passes[0].grid,
+                              s);
+            check_launch("gen_layer1");
+            kt_end(s, "fwd_layer1_gen", S);
+        } else {
+            launch_fwd_pass(pass_r[RIE_PASS_FWD_LOW], true, false, cur, out, lc, d_coef,
+                            passes[0].g, passes[0].grid, s);
+            check_launch("fwd_pass scatter");
+            kt_end(s, "fwd_pass_low_cnot", 2 * S);
+        }
         cur = out;
         cur_buf = out_buf;
     }
@@ -571,13 +598,20 @@ void Engine::structured_hamiltonian(hipStream_t s, bool write_lam) {
 
 void Engine::launch_bwd(int l, bool gather, bool write_cur, const double2 *cur_in,
                         const double2 *lam_in, double2 *cur_out, double2 *lam_out, size_t p,
-                        hipStream_t s) {
+                        hipStream_t s, int gen_set) {
     const WireCoef *lc = d_coef + (size_t)l * n;
     const PassPlan &pp = passes[p];
+    const CMat2 *gen_tab = nullptr;
+    if (gen_set >= 0) {
+        if (l != 1 || write_cur || gather || gen_set >= kChainMaxSets) {
+            fail(RIE_DEVICE_ERROR, "internal: generated state outside layer 1's lambda sweep");
+        }
+        gen_tab = d_chain + (size_t)gen_set * kChainTabEntries;
+    }
     if (l != fuse_layer) {
         launch_bwd_pass(pass_r[gather ? RIE_PASS_BWD_LOW : RIE_PASS_BWD_HIGH], gather, write_cur,
                         true, cur_in, lam_in, cur_out, lam_out, lc,
-                        pp.g, d_partials, pp.grid, s);
+                        pp.g, d_partials, pp.grid, s, false, 0.0, nullptr, gen_tab);
         return;
     }
     if (fuse_eoff + (size_t)pp.grid > epartial_count) {
@@ -585,8 +619,41 @@ void Engine::launch_bwd(int l, bool gather, bool write_cur, const double2 *cur_i
     }
     launch_bwd_pass(pass_r[gather ? RIE_PASS_BWD_LOW : RIE_PASS_BWD_HIGH], gather, write_cur, true,
                     cur_in, lam_in, cur_out, lam_out, lc,
-                    pp.g, d_partials, pp.grid, s, true, h, d_epartials + fuse_eoff);
+                    pp.g, d_partials, pp.grid, s, true, h, d_epartials + fuse_eoff, gen_tab);
     fuse_eoff += (size_t)pp.grid;
+}
+
+bool Engine::gen_ready() const {
+    return gen_enabled && strategy == RIE_STRATEGY_STRUCTURED_ADJOINT && !small && L >= 2 &&
+           n <= kChainMaxQubits && (int)passes.size() <= kChainMaxSets;
+}
+
+// Table sets of the current register shapes: 0 the forward (low tile, all of
+// layer 1 rotated, run at mapping 0), 1 + j the adjoint high pass j (the
+// wires of high groups 0 .. j rotated, run at mapping 0's registers).
+void Engine::build_chain_tables(hipStream_t s) {
+    ChainSetup cs{};
+    cs.n = n;
+    const uint64_t all = (n >= 64) ? ~uint64_t{0} : ((uint64_t{1} << n) - 1);
+    const int rf = pass_r[RIE_PASS_FWD_GEN];
+    const int rh = pass_r[RIE_PASS_BWD_HIGH];
+    cs.set[0] = ChainSet{T - rf, rf, all};
+    cs.count = 1;
+    for (size_t p = 1; p < passes.size(); ++p) {
+        const PassGeom &g = passes[p].g;
+        const uint64_t rot = all & ~((uint64_t{1} << T) - 1) &
+                             ((uint64_t{1} << passes[p].end_wire) - 1);
+        cs.set[cs.count++] = ChainSet{g.h0 + (T - rh - g.a), rh, rot};
+    }
+    for (int i = 0; i < cs.count; ++i) {
+        if (chain_chunks(n, cs.set[i].R) > kChainMaxChunks) {
+            fail(RIE_DEVICE_ERROR, "internal: chain tables too small for this size");
+        }
+    }
+    kt_begin(s);
+    launch_chain_tables(d_coef, d_coef + n, cs, d_chain, s);
+    check_launch("chain_tables");
+    kt_end(s, "chain_tables", 0.0);
 }
 
 bool Engine::fused_eval() const {
@@ -748,11 +815,18 @@ void Engine::structured_backward_stored(hipStream_t s, const double *scale) {
     // reads the state at each pass from there and only rewrites lambda.  The
     // low pass (inverse ring CNOT on load) comes first; the high passes
     // follow in reverse forward order.
+    // With layer-1 generation the forward wrote only layer 1's output: the low
+    // pass of layer 1 gathers it as usual, its high passes generate the state
+    // they need.
     const double S = (double)sizeof(double2) * (double)N;
     const int K = (int)passes.size() - 1;
+    if (gen_fwd) {
+        build_chain_tables(s);  // the register shapes may have changed since the forward
+    }
     int a = 0;  // lambda lives in lbuf(a)
     for (int l = L - 1; l >= 1; --l) {
         const int b = 1 - a;
+        const bool gen = (l == 1 && gen_fwd);
         require_partials((size_t)passes[0].grid * 2 * T);
         kt_begin(s);
         launch_bwd(l, true, false, fbuf(l, K), lbuf(a), nullptr, lbuf(b), 0, s);
@@ -766,9 +840,10 @@ void Engine::structured_backward_stored(hipStream_t s, const double *scale) {
             const PassPlan &pp = passes[1 + j];
             require_partials((size_t)pp.grid * 2 * T);
             kt_begin(s);
-            launch_bwd(l, false, false, fbuf(l, j), lbuf(b), nullptr, lbuf(b), (size_t)(1 + j), s);
+            launch_bwd(l, false, false, gen ? nullptr : fbuf(l, j), lbuf(b), nullptr, lbuf(b),
+                       (size_t)(1 + j), s, gen ? 1 + j : -1);
             check_launch("bwd_pass");
-            kt_end(s, "bwd_pass_high", 3 * S);
+            kt_end(s, gen ? "bwd_pass_high_gen" : "bwd_pass_high", (gen ? 2 : 3) * S);
             kt_begin(s);
             launch_reduce_grad(d_partials, pp.grid, T, pp.g, l, d_grad, scale, s);
             check_launch("reduce_grad");
@@ -1094,7 +1169,7 @@ void Engine::set_checkpoint_interval(size_t every) {
 // cpp/ising_cuda_rotation_kernels.cu:562-575).
 namespace {
 int pass_kind_of(const std::string &name) {
-    if (name == "fwd_pass_high_gen_init") {
+    if (name == "fwd_pass_high_gen_init" || name == "fwd_layer1_gen") {
         return RIE_PASS_FWD_GEN;
     }
     if (name == "fwd_pass_high") {
@@ -1109,7 +1184,7 @@ int pass_kind_of(const std::string &name) {
     if (name == "bwd_pass_low_cnot") {
         return RIE_PASS_BWD_LOW;
     }
-    if (name == "bwd_pass_high") {
+    if (name == "bwd_pass_high" || name == "bwd_pass_high_gen") {
         return RIE_PASS_BWD_HIGH;
     }
     if (name == "layer0_contract") {
@@ -1608,6 +1683,21 @@ rie_status rie_engine_set_fused_hamiltonian(rie_engine *engine, int enabled) {
 
 int rie_engine_fused_hamiltonian(const rie_engine *engine) {
     return engine == nullptr ? -1 : (reinterpret_cast<const Engine *>(engine)->fused_eval() ? 1 : 0);
+}
+
+rie_status rie_engine_set_layer1_generation(rie_engine *engine, int enabled) {
+    return guard([&]() {
+        Engine *e = as_engine(engine);
+        if (enabled != 0 && enabled != 1) {
+            rie::fail(RIE_INVALID_ARGUMENT, "enabled must be 0 or 1.");
+        }
+        e->gen_enabled = enabled == 1;
+        e->resident = false;
+    });
+}
+
+int rie_engine_layer1_generation(const rie_engine *engine) {
+    return engine == nullptr ? -1 : (reinterpret_cast<const Engine *>(engine)->gen_ready() ? 1 : 0);
 }
 
 rie_status rie_engine_set_kernel_timing(rie_engine *engine, int enable) {

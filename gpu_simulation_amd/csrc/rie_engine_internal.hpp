@@ -96,6 +96,15 @@ struct Engine {
     bool fuse_enabled = true;  // caller's switch
     int fuse_layer = -1;       // layer whose adjoint passes add pair terms now
     size_t fuse_eoff = 0;      // next energy-partial slot of the fused passes
+    // Layer-1 generation (L >= 2, n <= kChainMaxQubits): every state inside
+    // layer 1 before its ring CNOT is a trace of a ring of 2x2 matrices over
+    // the layer-0 product state, so layer 1's forward is one write pass and,
+    // in the stored schedule, its adjoint high passes generate the state
+    // instead of reading it.  Chain table sets: 0 the forward, 1 + j the
+    // adjoint high pass j.
+    bool gen_enabled = true;    // caller's switch
+    bool gen_fwd = false;       // the last forward generated layer 1
+    CMat2 *d_chain = nullptr;   // kChainMaxSets * kChainTabEntries matrices
 
     // device buffers
     std::vector<double2 *> pool;  // statevector buffers (16 * 2^n bytes each)
@@ -177,7 +186,9 @@ struct Engine {
                                bool staged);
     void launch_bwd(int l, bool gather, bool write_cur, const double2 *cur_in,
                     const double2 *lam_in, double2 *cur_out, double2 *lam_out, size_t p,
-                    hipStream_t s);
+                    hipStream_t s, int gen_set = -1);
+    bool gen_ready() const;
+    void build_chain_tables(hipStream_t s);
     void structured_backward(hipStream_t s, const double *scale, bool full_walk);
     void structured_backward_stored(hipStream_t s, const double *scale);
     void layer0_gradients(hipStream_t s, const double2 *lam, const double *scale);

@@ -17,6 +17,7 @@
 #include "rie_kernels.hpp"
 
 #include <algorithm>
+#include <stdexcept>
 
 namespace rie {
 
@@ -351,6 +352,39 @@ constexpr int tile_min_blocks() {
     return R == 3 ? 4 : 2;
 }
 
+// Stores a low tile (a == T) held in mapping M through the ring-CNOT
+// permutation: P(base ^ o) = P(base) ^ P_T(o) ^ (parity(o) ? hi : 0), and bit
+// T-1 of P_T(o) is parity(o), so the image is two contiguous runs.  The tile
+// is placed at P_T(o) in LDS and written out run by run.
+template <int R, int M>
+__device__ __forceinline__ void store_ring_scatter(const double2 (&v)[Tile<R>::REGS],
+                                                   double2 *lds, uint32_t t, uint64_t base,
+                                                   const PassGeom &G, double2 *out) {
+    using TL = Tile<R>;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < TL::REGS; ++k) {
+        lds[lds_slot<R>(ring_fwd32(omap<R, M>(t, k), TL::T))] = v[k];
+    }
+    __syncthreads();
+    // q = t | k << (T - R) is contiguous in the image run; P(base) has zero
+    // bits [1, T) and its bit 0 only swaps neighbours inside the run.
+    const uint64_t pbase = ring_fwd(base, G.n);
+    const uint64_t lo_flip = pbase & 1u;
+    const uint64_t run = (pbase & ~uint64_t{TL::SIZE - 1}) + (t ^ lo_flip);
+    double2 *dst0 = out + run;
+    double2 *dst1 = out + (run ^ G.hi_mask);
+#pragma unroll
+    for (int k = 0; k < TL::REGS; ++k) {
+        const double2 val = lds[lds_slot<R>(omap<R, 0>(t, k))];
+        if (k < TL::REGS / 2) {
+            dst0[(uint64_t)k << (TL::T - R)] = val;
+        } else {
+            dst1[(uint64_t)k << (TL::T - R)] = val;
+        }
+    }
+}
+
 // With kGenInit the pass does not read its input: the tile of the layer-0
 // state after its ring CNOT, chi(i) = prod_w u_w(bit w of P^-1(i)), is
 // generated from nibble tables of the layer-0 factors (lc0).
@@ -482,30 +516,7 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void fwd_pa
                 dst[k * ta.strideC] = v[k];
             }
         } else {
-            // Low tile (a == T): P(base ^ o) = P(base) ^ P_T(o) ^ (parity(o) ? hi : 0),
-            // and bit T-1 of P_T(o) is parity(o).
-            __syncthreads();
-#pragma unroll
-            for (int k = 0; k < TL::REGS; ++k) {
-                lds[lds_slot<R>(ring_fwd32(omap<R, TL::P - 1>(t, k), TL::T))] = v[k];
-            }
-            __syncthreads();
-            // q = t | k << (T - R) is contiguous in the image run; P(base) has zero
-            // bits [1, T) and its bit 0 only swaps neighbours inside the run.
-            const uint64_t pbase = ring_fwd(base, G.n);
-            const uint64_t lo_flip = pbase & 1u;
-            const uint64_t run = (pbase & ~uint64_t{TL::SIZE - 1}) + (t ^ lo_flip);
-            double2 *dst0 = out + run;
-            double2 *dst1 = out + (run ^ G.hi_mask);
-#pragma unroll
-            for (int k = 0; k < TL::REGS; ++k) {
-                const double2 val = lds[lds_slot<R>(omap<R, 0>(t, k))];
-                if (k < TL::REGS / 2) {
-                    dst0[(uint64_t)k << (TL::T - R)] = val;
-                } else {
-                    dst1[(uint64_t)k << (TL::T - R)] = val;
-                }
-            }
+            store_ring_scatter<R, TL::P - 1>(v, lds, t, base, G, out);
         }
     }
 }
@@ -535,6 +546,125 @@ void launch_fwd_pass(int R, bool scatter, bool gen_init, const double2 *in, doub
         launch_fwd_pass_r<4>(scatter, gen_init, in, out, layer_coef, layer0_coef, g, grid, s);
     } else {
         launch_fwd_pass_r<3>(scatter, gen_init, in, out, layer_coef, layer0_coef, g, grid, s);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Layer-1 states generated from the parameters (chain_site): tables.
+// ---------------------------------------------------------------------------
+
+// One workgroup per table set: the 2n site matrices in LDS, then the
+// register-run products tab[0, 2^R) and the chunk products tab[16 + 16c + v].
+__global__ __launch_bounds__(256) void chain_tables_kernel(const WireCoef *__restrict__ lc0,
+                                                           const WireCoef *__restrict__ lc1,
+                                                           ChainSetup cs, CMat2 *__restrict__ tabs) {
+    __shared__ CMat2 site[2 * 64];
+    const ChainSet st = cs.set[blockIdx.x];
+    const int n = cs.n;
+    for (int i = (int)threadIdx.x; i < 2 * n; i += (int)blockDim.x) {
+        const int w = i >> 1;
+        site[i] = chain_site(w, i & 1, lc0[w], lc1[w], ((st.rot >> w) & 1u) != 0);
+    }
+    __syncthreads();
+    CMat2 *tab = tabs + (size_t)blockIdx.x * kChainTabEntries;
+    const int chunks = chain_chunks(n, st.R);
+    for (int e = (int)threadIdx.x; e < kChainTabEntries; e += (int)blockDim.x) {
+        int first;
+        int len;
+        int v;
+        if (e < 16) {
+            if (e >= (1 << st.R)) {
+                continue;
+            }
+            first = st.r0;
+            len = st.R;
+            v = e;
+        } else {
+            const int c = (e - 16) >> 4;
+            v = (e - 16) & 15;
+            if (c >= chunks) {
+                continue;
+            }
+            first = (st.r0 + st.R + 4 * c) % n;
+            len = min(4, n - st.R - 4 * c);
+            if (v >= (1 << len)) {
+                continue;
+            }
+        }
+        CMat2 p = site[2 * first + (v & 1)];
+        for (int b = 1; b < len; ++b) {
+            const int w = (first + b) % n;
+            p = cmat_mul(p, site[2 * w + ((v >> b) & 1)]);
+        }
+        tab[e] = p;
+    }
+}
+
+void launch_chain_tables(const WireCoef *lc0, const WireCoef *lc1, const ChainSetup &cs,
+                         CMat2 *tabs, hipStream_t s) {
+    chain_tables_kernel<<<cs.count, 256, 0, s>>>(lc0, lc1, cs, tabs);
+}
+
+// Product of the chunk matrices at global index i, in ring order after the
+// register run [r0, r0 + R): chunk c reads index bits (r0 + R + 4c + b) mod n
+// (the run's own bits of i are not read).  chunks: the chunk products,
+// chunk c at chunks[16c + v] (a table set from its entry 16 on, or an LDS copy).
+template <int R>
+__device__ __forceinline__ CMat2 chain_y(const CMat2 *__restrict__ chunks_tab, uint64_t i,
+                                         int r0, int n) {
+    const int chunks = chain_chunks(n, R);
+    CMat2 y = {};
+#pragma unroll
+    for (int c = 0; c < kChainMaxChunks; ++c) {
+        if (c < chunks) {
+            const int w0 = (r0 + R + 4 * c) % n;
+            const int len = min(4, n - R - 4 * c);
+            // bits w0, w0 + 1, ... of i, wrapping from n - 1 to 0
+            const uint32_t v =
+                (uint32_t)(((i >> w0) | (i << (n - w0))) & ((uint64_t{1} << len) - 1));
+            const CMat2 m = chunks_tab[16 * c + (int)v];
+            y = (c == 0) ? m : cmat_mul(y, m);
+        }
+    }
+    return y;
+}
+
+// v[k] = Tr(run[k] Y): register bit b is the run's wire r0 + b.
+template <int R>
+__device__ __forceinline__ void chain_fill(double2 (&v)[Tile<R>::REGS],
+                                           const CMat2 *__restrict__ tab, const CMat2 &y) {
+#pragma unroll
+    for (int k = 0; k < Tile<R>::REGS; ++k) {
+        v[k] = cmat_trace_mul(tab[k], y);
+    }
+}
+
+// Layer 1 in one write pass: the low tile of V_1 P chi (every wire of layer 1
+// rotated; tables with r0 = T - R) generated in mapping 0, then the layer's
+// ring CNOT through LDS.
+template <int R>
+__global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void gen_layer1_kernel(
+    double2 *out, const CMat2 *__restrict__ tab, PassGeom G) {
+    using TL = Tile<R>;
+    __shared__ double2 lds[TL::SLOTS];
+    const uint32_t t = threadIdx.x;
+    for (uint64_t g = blockIdx.x; g < G.num_tiles; g += gridDim.x) {
+        uint32_t tl = t;
+        asm volatile("" : "+v"(tl));
+        const uint64_t base = tile_base(g, G, TL::T);
+        const CMat2 y = chain_y<R>(tab + 16, base + omap<R, 0>(tl, 0), TL::T - R, G.n);
+        double2 v[TL::REGS];
+        chain_fill<R>(v, tab, y);
+        store_ring_scatter<R, 0>(v, lds, tl, base, G, out);
+    }
+}
+
+void launch_gen_layer1(int R, double2 *out, const CMat2 *tab, const PassGeom &g, int grid,
+                       hipStream_t s) {
+    if (R == 4) {
+        gen_layer1_kernel<4><<<grid, Tile<4>::THREADS, 0, s>>>(out, tab, g);
+    } else {
+        gen_layer1_kernel<3><<<grid, Tile<3>::THREADS, 0, s>>>(out, tab, g);
     }
 }
 
@@ -616,13 +746,17 @@ __device__ __forceinline__ void add_pair_terms(const double2 (&c)[Tile<R>::REGS]
 // the low pass gathers through the ring CNOT (pre-CNOT frame, mapping 1),
 // a high pass reads mapping 0.  With kHam the pair terms of that first
 // window are added.
-template <int R, bool kGather, bool kHam>
+// kGen (high passes): the state is generated in mapping 0 from the chain table
+// set gen_tab (its chunk products copied to gen_chunks in LDS) instead of read.
+template <int R, bool kGather, bool kHam, bool kGen = false>
 __device__ __forceinline__ void bwd_load_tile(double2 (&c)[Tile<R>::REGS],
                                               double2 (&l)[Tile<R>::REGS],
                                               const double2 *cur_in, const double2 *lam_in,
                                               uint64_t base, const TileAddr &ta,
                                               const PassGeom &G, double2 *lds, uint32_t t,
-                                              double field, double &e) {
+                                              double field, double &e,
+                                              const CMat2 *__restrict__ gen_tab = nullptr,
+                                              const CMat2 *gen_chunks = nullptr) {
     using TL = Tile<R>;
     constexpr int T = TL::T;
     if constexpr (kGather) {
@@ -632,6 +766,7 @@ __device__ __forceinline__ void bwd_load_tile(double2 (&c)[Tile<R>::REGS],
         const uint64_t lo_flip = pbase & 1u;
         const uint64_t run = (pbase & ~uint64_t{TL::SIZE - 1}) + (t ^ lo_flip);
         const uint64_t run1 = run ^ G.hi_mask;
+        static_assert(!kGen, "the gather pass reads the state");
 #pragma unroll
         for (int k = 0; k < TL::REGS; ++k) {
             const uint64_t src = (k < TL::REGS / 2 ? run : run1) + ((uint64_t)k << (T - R));
@@ -668,7 +803,13 @@ __device__ __forceinline__ void bwd_load_tile(double2 (&c)[Tile<R>::REGS],
         for (int k = 0; k < TL::REGS; ++k) {
             const uint64_t idx = row + k * ta.strideA;
             l[k] = lam_in[idx];
-            c[k] = cur_in[idx];
+            if constexpr (!kGen) {
+                c[k] = cur_in[idx];
+            }
+        }
+        if constexpr (kGen) {
+            const CMat2 y = chain_y<R>(gen_chunks, row, tile_wire(T - R, G), G.n);
+            chain_fill<R>(c, gen_tab, y);
         }
         if constexpr (kHam) {
             // only the pair straddling the load window's lower edge needs
@@ -945,11 +1086,11 @@ __device__ __forceinline__ void store_lam(const double2 (&l)[Tile<R>::REGS], dou
     }
 }
 
-template <int R, bool kGather, bool kHam>
+template <int R, bool kGather, bool kHam, bool kGen>
 __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_lam_kernel(
     const double2 *cur_in, const double2 *lam_in, double2 *lam_out,
     const WireCoef *__restrict__ lc, PassGeom G, double *__restrict__ partials, double field,
-    double *__restrict__ e_partials) {
+    double *__restrict__ e_partials, const CMat2 *__restrict__ gen_tab) {
     using TL = Tile<R>;
     constexpr int T = TL::T;
     __shared__ double2 lds[TL::SLOTS];
@@ -957,6 +1098,9 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_la
     __shared__ double ered[kHam ? TL::WAVES : 1];
     __shared__ double2 rz_tab[TL::P * TL::REGS];
     __shared__ WireCoef tile_coef[T];
+    // chunk products of the generating table set: read at every tile start,
+    // where LDS latency is short next to L2 behind the streaming traffic
+    __shared__ CMat2 gen_chunks[kGen ? 16 * kChainMaxChunks : 1];
     const uint32_t t = threadIdx.x;
     const int lane = t & (kWave - 1);
     double *wave_acc = acc[t / kWave];
@@ -967,6 +1111,12 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_la
     build_rz_tables<R>(rz_tab, lc, G, t);
     if (t < (uint32_t)T) {
         tile_coef[t] = lc[tile_wire((int)t, G)];
+    }
+    if constexpr (kGen) {
+        const int entries = 16 * chain_chunks(G.n, R);
+        for (int i = (int)t; i < entries; i += TL::THREADS) {
+            gen_chunks[i] = gen_tab[16 + i];
+        }
     }
     __syncthreads();
 
@@ -980,7 +1130,8 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_la
         const uint64_t base = tile_base(g, G, T);
         double2 c[TL::REGS];
         double2 l[TL::REGS];
-        bwd_load_tile<R, kGather, kHam>(c, l, cur_in, lam_in, base, ta, G, lds, tl, field, e);
+        bwd_load_tile<R, kGather, kHam, kGen>(c, l, cur_in, lam_in, base, ta, G, lds, tl, field,
+                                              e, gen_tab, gen_chunks);
         // sweep 1: phases and gradients (lam exchanged before cur, so the LDS
         // holds cur for the pair terms); sweep 2: RY^dagger on lam
         if constexpr (kGather) {
@@ -1142,20 +1293,28 @@ template <int R>
 static void launch_bwd_pass_r(bool gather, bool write_cur, bool write_lam, const double2 *cur_in,
                               const double2 *lam_in, double2 *cur_out, double2 *lam_out,
                               const WireCoef *lc, const PassGeom &g, double *partials, int grid,
-                              hipStream_t s, bool ham, double field, double *e_partials) {
+                              hipStream_t s, bool ham, double field, double *e_partials,
+                              const CMat2 *gen_tab) {
     constexpr int TH = Tile<R>::THREADS;
+    const bool gen = gen_tab != nullptr;
     if (!write_cur && write_lam) {  // the state is not rewritten: lambda-only sweep
-#define RIE_BWD_LAM(GA, HM)                                                                  \
-    if (gather == GA && ham == HM) {                                                         \
-        bwd_lam_kernel<R, GA, HM><<<grid, TH, 0, s>>>(cur_in, lam_in, lam_out, lc, g,        \
-                                                      partials, field, e_partials);           \
+#define RIE_BWD_LAM(GA, HM, GN)                                                              \
+    if (gather == GA && ham == HM && gen == GN) {                                            \
+        bwd_lam_kernel<R, GA, HM, GN><<<grid, TH, 0, s>>>(cur_in, lam_in, lam_out, lc, g,    \
+                                                          partials, field, e_partials,       \
+                                                          gen_tab);                          \
         return;                                                                              \
     }
-        RIE_BWD_LAM(true, false)
-        RIE_BWD_LAM(false, false)
-        RIE_BWD_LAM(true, true)
-        RIE_BWD_LAM(false, true)
+        RIE_BWD_LAM(true, false, false)
+        RIE_BWD_LAM(false, false, false)
+        RIE_BWD_LAM(true, true, false)
+        RIE_BWD_LAM(false, true, false)
+        RIE_BWD_LAM(false, false, true)
+        RIE_BWD_LAM(false, true, true)
 #undef RIE_BWD_LAM
+    }
+    if (gen) {
+        throw std::invalid_argument("a generated state needs a high pass on the lambda-only sweep");
     }
 #define RIE_BWD(GA, WC, WL, HM)                                                              \
     if (gather == GA && write_cur == WC && write_lam == WL && ham == HM) {                   \
@@ -1177,13 +1336,16 @@ static void launch_bwd_pass_r(bool gather, bool write_cur, bool write_lam, const
 void launch_bwd_pass(int R, bool gather, bool write_cur, bool write_lam, const double2 *cur_in,
                      const double2 *lam_in, double2 *cur_out, double2 *lam_out,
                      const WireCoef *layer_coef, const PassGeom &g, double *partials, int grid,
-                     hipStream_t s, bool ham_pairs, double field, double *e_partials) {
+                     hipStream_t s, bool ham_pairs, double field, double *e_partials,
+                     const CMat2 *gen_tab) {
     if (R == 4) {
         launch_bwd_pass_r<4>(gather, write_cur, write_lam, cur_in, lam_in, cur_out, lam_out,
-                             layer_coef, g, partials, grid, s, ham_pairs, field, e_partials);
+                             layer_coef, g, partials, grid, s, ham_pairs, field, e_partials,
+                             gen_tab);
     } else {
         launch_bwd_pass_r<3>(gather, write_cur, write_lam, cur_in, lam_in, cur_out, lam_out,
-                             layer_coef, g, partials, grid, s, ham_pairs, field, e_partials);
+                             layer_coef, g, partials, grid, s, ham_pairs, field, e_partials,
+                             gen_tab);
     }
 }
 

@@ -109,6 +109,35 @@ void launch_fwd_pass(int R, bool scatter, bool gen_init, const double2 *in, doub
                      const WireCoef *layer_coef, const WireCoef *layer0_coef, const PassGeom &g,
                      int grid, hipStream_t s);
 
+// Layer-1 states from the parameters (chain_site in rie_device.hpp).  A table
+// set serves one pass: tab[0, 2^R) are the products of the register run's
+// site matrices (wires r0 .. r0 + R - 1; register bit b is wire r0 + b) and
+// tab[16 + 16c + v] the products of chunk c, the up to 4 wires
+// (r0 + R + 4c + b) mod n that follow the run around the ring, at bit values
+// v.  rot: the wires of layer 1 already rotated in the generated state.
+constexpr int kChainMaxChunks = 7;
+constexpr int kChainTabEntries = 16 + 16 * kChainMaxChunks;
+constexpr int kChainMaxSets = 6;
+constexpr int kChainMaxQubits = 31;  // chain_chunks(n, 3) <= kChainMaxChunks
+struct ChainSet {
+    int r0;
+    int R;
+    uint64_t rot;
+};
+struct ChainSetup {
+    int n;
+    int count;
+    ChainSet set[kChainMaxSets];
+};
+__host__ __device__ constexpr int chain_chunks(int n, int R) { return (n - R + 3) / 4; }
+// tabs: count * kChainTabEntries matrices, set i at tabs + i * kChainTabEntries.
+void launch_chain_tables(const WireCoef *lc0, const WireCoef *lc1, const ChainSetup &cs,
+                         CMat2 *tabs, hipStream_t s);
+// Layer 1 (every wire rotated) and its ring CNOT as one write pass over the
+// low tiles; tab: a set with r0 = 12 - R.
+void launch_gen_layer1(int R, double2 *out, const CMat2 *tab, const PassGeom &g, int grid,
+                       hipStream_t s);
+
 // Adjoint tile pass: un-rotates the pass's wires on (cur, lam) and writes
 // per-workgroup gradient partials [grid][2T].  gather = apply the inverse
 // ring-CNOT on load (low pass, out of place); write_cur / write_lam = store
@@ -117,11 +146,14 @@ void launch_fwd_pass(int R, bool scatter, bool gen_init, const double2 *in, doub
 // tile-bit pairs [g.pair_lo, g.pair_hi) to lam (-field * X_j X_{j+1} cur) and
 // write per-workgroup energy partials of those terms into e_partials.  In the
 // pre-CNOT frame of the last layer these pairs are the X terms X_j of H.
+// gen_tab (layer 1, high pass on the lambda-only sweep): the state is not read
+// from cur_in but generated from a chain table set whose run is mapping 0's
+// register wires.
 void launch_bwd_pass(int R, bool gather, bool write_cur, bool write_lam, const double2 *cur_in,
                      const double2 *lam_in, double2 *cur_out, double2 *lam_out,
                      const WireCoef *layer_coef, const PassGeom &g, double *partials, int grid,
                      hipStream_t s, bool ham_pairs = false, double field = 0.0,
-                     double *e_partials = nullptr);
+                     double *e_partials = nullptr, const CMat2 *gen_tab = nullptr);
 
 // Hamiltonian tile pass: adds the X terms of the pass's wires (and the ZZ
 // diagonal if diag) of H psi, optionally into lam, and writes per-workgroup

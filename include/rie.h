@@ -168,6 +168,20 @@ int rie_engine_schedule(const rie_engine *engine);
 rie_status rie_engine_set_fused_hamiltonian(rie_engine *engine, int enabled);
 int rie_engine_fused_hamiltonian(const rie_engine *engine);
 
+/* Layer-1 generation (no counterpart in the reference; on by default, tile
+ * path with at least two layers and at most 31 qubits): layer 0 acts on
+ * |0...0>, so every state inside layer 1 before its ring CNOT is a trace of
+ * a ring of 2x2 matrices (one per wire) over the layer-0 product state.
+ * Layer 1's forward becomes one write pass (generate, ring CNOT, store), and
+ * in the stored schedule layer 1's adjoint high passes generate the state
+ * they need instead of reading a stored one (the reference generates only layer 0
+ * as a product state and applies every later layer wire chunk by wire chunk:
+ * cpp/ising_cuda_structured_adjoint_modes.inc:108-145).  Results agree
+ * with the stored/walked evaluation to rounding.  enabled: 1 on, 0 off;
+ * rie_engine_layer1_generation reports whether evaluations use it. */
+rie_status rie_engine_set_layer1_generation(rie_engine *engine, int enabled);
+int rie_engine_layer1_generation(const rie_engine *engine);
+
 /* Register shape of the tile passes (no counterpart in the reference): each
  * of r_fwd, r_ham, r_bwd is 4 (256 threads x 16 registers per 4096-amplitude
  * tile, three register mappings) or 3 (512 threads x 8 registers, four

@@ -167,6 +167,73 @@ def test_fused_hamiltonian_device_path_and_scope(oracle, backend_module):
         eng.set_fused_hamiltonian(2)
 
 
+@pytest.mark.parametrize("schedule", ["stored", "inverse_walk"])
+@pytest.mark.parametrize("n,layers,shapes", [
+    (13, 2, None), (14, 3, None), (16, 2, [4, 3, 3, 3, 3, 3, 3]), (21, 2, None),
+    (22, 2, [3, 4, 4, 4, 3, 3, 4]), (22, 4, None), (25, 2, None)])
+def test_layer1_generation(oracle, backend_module, n, layers, shapes, schedule):
+    # layer 1 generated from the chain of 2x2 matrices (one forward write pass;
+    # in the stored schedule the adjoint high passes of layer 1 generate the state)
+    # against the oracle and against the pass-by-pass forward with stored
+    # states; fused and unfused Hamiltonian; both register shapes of the
+    # generator and of the generating adjoint passes
+    params = _params(2700 + n, n, layers)
+    print(f"case gen n={n} L={layers} shapes={shapes} schedule={schedule}", flush=True)
+    e_ref, g_ref = oracle.energy_and_grad(n, layers, 0.7, params)
+    eng = backend_module.RingIsingCudaBackend(n, layers, 0.7, "structured_adjoint", 8, False)
+    eng.set_schedule(schedule)
+    if shapes is not None:
+        eng.set_pass_shapes(shapes)
+    flat = np.ascontiguousarray(params.reshape(-1))
+    assert eng.layer1_generation is True
+    outs = {}
+    for gen in (1, 0):
+        eng.set_layer1_generation(gen)
+        assert eng.layer1_generation is bool(gen)
+        for fused in (1, 0):
+            eng.set_fused_hamiltonian(fused)
+            outs[gen, fused] = eng.energy_and_grad(flat)
+        outs[gen, "energy"] = eng.energy_and_grad(flat, False)["energy"]
+    for key, out in outs.items():
+        e = out if key[1] == "energy" else out["energy"]
+        assert abs(e - e_ref) <= TOL, key
+        if key[1] != "energy":
+            assert np.max(np.abs(np.asarray(out["gradient"]) - g_ref)) <= TOL, key
+    for fused in (1, 0):
+        a, b = outs[1, fused], outs[0, fused]
+        assert abs(a["energy"] - b["energy"]) <= 1e-12
+        assert np.max(np.abs(np.asarray(a["gradient"]) - np.asarray(b["gradient"]))) <= 1e-12
+    eng.set_layer1_generation(1)
+    eng.set_fused_hamiltonian(1)
+    again = eng.energy_and_grad(flat)
+    assert again["energy"] == outs[1, 1]["energy"]
+    assert np.array_equal(np.asarray(again["gradient"]), np.asarray(outs[1, 1]["gradient"]))
+
+
+def test_layer1_generation_three_high_groups_and_scope(backend_module):
+    # 31 qubits (three high groups, the largest size the chain tables cover):
+    # generated against stored layer-1 states, beyond the oracle's reach
+    n, layers = 31, 2
+    params = _params(2800, n, layers).reshape(-1)
+    print(f"case gen n={n} L={layers}", flush=True)
+    eng = backend_module.RingIsingCudaBackend(n, layers, 1.0, "structured_adjoint", 8, False)
+    assert eng.layer1_generation is True
+    gen = eng.energy_and_grad(params)
+    eng.set_layer1_generation(0)
+    plain = eng.energy_and_grad(params)
+    assert abs(gen["energy"] - plain["energy"]) <= 1e-10
+    assert np.max(np.abs(np.asarray(gen["gradient"]) - np.asarray(plain["gradient"]))) <= 1e-10
+    del eng
+    # one layer and the whole-circuit kernel keep their own paths
+    assert backend_module.RingIsingCudaBackend(16, 1, 1.0, "structured_adjoint", 8,
+                                               False).layer1_generation is False
+    assert backend_module.RingIsingCudaBackend(10, 3, 1.0, "structured_adjoint", 8,
+                                               False).layer1_generation is False
+    with pytest.raises(ValueError):
+        backend_module.RingIsingCudaBackend(16, 2, 1.0, "structured_adjoint", 8,
+                                            False).set_layer1_generation(2)
+
+
 def test_engine_autotune_keeps_results(oracle, backend_module):
     n, layers = 18, 3
     params = _params(2200, n, layers)
