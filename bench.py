@@ -61,6 +61,8 @@ def parse_args() -> argparse.Namespace:
                    help="skip timing the other backward schedule")
     p.add_argument("--no-fused-hamiltonian", action="store_true",
                    help="per-group Hamiltonian passes instead of the fused Hamiltonian")
+    p.add_argument("--no-layer1-generation", action="store_true",
+                   help="run layer 1 pass by pass with stored states instead of generating it")
     p.add_argument("--no-batched", action="store_true",
                    help="skip the batched uneven-mix measurement")
     p.add_argument("--batch-count", type=int, default=2048)
@@ -219,6 +221,8 @@ def main() -> int:
     engine = ext.RingIsingCudaBackend(n, layers, 1.0, "structured_adjoint", 8, False)
     if args.no_fused_hamiltonian:
         engine.set_fused_hamiltonian(0)
+    if args.no_layer1_generation:
+        engine.set_layer1_generation(0)
     tune = None
     if not args.no_autotune and args.schedule == "auto" and args.tile_shapes == "auto":
         t_tune = time.perf_counter()
@@ -334,6 +338,7 @@ def main() -> int:
             "tile_shapes": list(engine.tile_shapes),
             "pass_shapes": list(engine.pass_shapes),
             "fused_hamiltonian": bool(engine.fused_hamiltonian),
+            "layer1_generation": bool(engine.layer1_generation),
             "parallelism": "single GPU",
         },
         "autotune": tune,

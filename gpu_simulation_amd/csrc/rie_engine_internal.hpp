@@ -82,8 +82,9 @@ struct Engine {
     // register shape of the tile passes per pass kind: 4 = 256 threads x 16
     // registers, 3 = 512 threads x 8 registers (both 4096-amplitude tiles)
     // (defaults measured on MI355X at 30 qubits: the 512-thread shape is
-    // faster for the forward and H passes, the 256-thread one for backward)
-    int pass_r[RIE_PASS_KINDS] = {3, 3, 3, 3, 4, 4, 4};
+    // faster for the forward and H passes, the 256-thread one for the layer-1
+    // generator and the adjoint passes)
+    int pass_r[RIE_PASS_KINDS] = {4, 3, 3, 3, 4, 4, 4};
     int T = 12;
     std::vector<PassPlan> passes;  // [0]: low pass (a = T); [1..]: high passes
     // Fused Hamiltonian of gradient evaluations (L >= 2): one psi-frame pass

@@ -196,9 +196,10 @@ rie_status rie_engine_tile_shapes(const rie_engine *engine, int *shapes_out3);
  * Hamiltonian kind to r_ham and the two backward kinds to r_bwd (the layer-0
  * contraction keeps its shape);
  * rie_engine_tile_shapes reports the high-pass forward, Hamiltonian and
- * high-pass backward kinds. */
+ * high-pass backward kinds.  Defaults per kind: 4, 3, 3, 3, 4, 4, 4. */
 typedef enum rie_pass_kind {
-    RIE_PASS_FWD_GEN = 0,   /* first forward pass of layer 1 (layer 0 generated) */
+    RIE_PASS_FWD_GEN = 0,   /* layer 1 generated (one write pass), or without layer-1
+                               generation layer 1's first pass (layer 0 generated) */
     RIE_PASS_FWD_HIGH = 1,  /* forward, high wires */
     RIE_PASS_FWD_LOW = 2,   /* forward, low wires + ring CNOT */
     RIE_PASS_HAM = 3,       /* Hamiltonian passes */
