@@ -223,12 +223,30 @@ __device__ __forceinline__ CMat2 chain_site(int w, int q, const WireCoef &k0, co
     return m;
 }
 
+// v from lane (this lane permuted by the DPP control kCtrl) of the same row
+// of 16 lanes.
+template <int kCtrl>
+__device__ __forceinline__ double dpp_move(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), kCtrl, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), kCtrl, 0xf, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ double lane_value(double v, int lane) {
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), lane),
+                            __builtin_amdgcn_readlane(__double2loint(v), lane));
+}
+
+// Sum over the 64 lanes of a (fully active) wave, returned to every lane in a
+// fixed order.  Each row of 16 lanes reduces through DPP lane moves
+// (quad_perm [1,0,3,2], quad_perm [2,3,0,1], row_half_mirror, row_mirror),
+// which use no LDS; the four row sums are read from lanes 0, 16, 32, 48.
 __device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int off = kWave / 2; off > 0; off >>= 1) {
-        v += __shfl_xor(v, off, kWave);
-    }
-    return v;
+    v += dpp_move<0xB1>(v);
+    v += dpp_move<0x4E>(v);
+    v += dpp_move<0x141>(v);
+    v += dpp_move<0x140>(v);
+    return (lane_value(v, 0) + lane_value(v, 16)) + (lane_value(v, 32) + lane_value(v, 48));
 }
 
 }  // namespace rie
