@@ -1064,6 +1064,60 @@ __device__ __forceinline__ void ry_inv_window(double2 (&l)[Tile<R>::REGS],
     }
 }
 
+// High passes: window 1 of the 256-thread shape holds tile bits 0..3, of which
+// only bit 3 can be a rotated wire (bits 0..2 are the contiguous low index
+// bits that ride along).  In mappings 0 and 2 tile bit 3 is lane bit 3, so
+// its pair partner is the lane 8 apart in the same row of 16: DPP row_ror:8
+// brings it over without an LDS exchange.
+template <int R>
+__device__ __forceinline__ double2 lane_xor8(double2 v) {
+    return make_double2(dpp_move<0x128>(v.x), dpp_move<0x128>(v.y));
+}
+
+// RZ^dagger of tile bit 3 on both states (window 1's phase table at register
+// index 8 or 0), then its gradients as in grad_window.
+template <int R>
+__device__ __forceinline__ void grad_lane_bit3(double2 (&c)[Tile<R>::REGS],
+                                               double2 (&l)[Tile<R>::REGS],
+                                               double *__restrict__ wave_acc, int lane,
+                                               const double2 *rz_tab) {
+    const bool hi = ((lane >> 3) & 1) != 0;
+    const double2 z = rz_tab[Tile<R>::REGS + (hi ? 8 : 0)];
+    const double2 zc = make_double2(z.x, -z.y);
+    double dt = 0.0;
+    double dp = 0.0;
+#pragma unroll
+    for (int q = 0; q < Tile<R>::REGS; ++q) {
+        c[q] = cmul(c[q], zc);
+        l[q] = cmul(l[q], zc);
+        const double im = l[q].x * c[q].y - l[q].y * c[q].x;
+        dp += hi ? -im : im;
+        // bit set: Re(l1* c0); clear: -Re(l0* c1)
+        const double2 cp = lane_xor8<R>(c[q]);
+        const double re = l[q].x * cp.x + l[q].y * cp.y;
+        dt += hi ? re : -re;
+    }
+    dt = wave_sum(dt);
+    dp = wave_sum(dp);
+    if (lane == 0) {
+        wave_acc[2 * 3] += dt;
+        wave_acc[2 * 3 + 1] += dp;
+    }
+}
+
+// RY^dagger of tile bit 3 on lam (ry_inv_pair across the two lanes).
+template <int R>
+__device__ __forceinline__ void ry_inv_lane_bit3(double2 (&l)[Tile<R>::REGS],
+                                                 const WireCoef &k, int lane) {
+    const double s = ((lane >> 3) & 1) != 0 ? -k.s : k.s;
+#pragma unroll
+    for (int q = 0; q < Tile<R>::REGS; ++q) {
+        const double2 p = lane_xor8<R>(l[q]);
+        l[q].x = k.c * l[q].x + s * p.x;
+        l[q].y = k.c * l[q].y + s * p.y;
+    }
+}
+
 // Stores lam from mapping M: the load mapping of a high pass, any mapping of
 // the low tile (contiguous: offset = index - base).
 template <int R, int M, bool kGather>
@@ -1192,34 +1246,27 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_la
             constexpr uint32_t w0 = window_mask<R, 0>();
             grad_window<R, 0>(c, l, lc, G, wave_acc, lane, rz_tab);
             if constexpr (TL::P == 3) {
-                // windows 0, 2, 1, then lam through 0 to the store mapping 2
+                // windows 0 and 2, tile bit 3 across lanes (window 1 has no
+                // other rotated bit: its pairs with bit 4 were added at window
+                // 2), then lam back to mapping 0 for the store: three LDS
+                // exchanges
                 exchange<R, 0, 2>(l, lds, tl);
                 exchange<R, 0, 2>(c, lds, tl);
                 if constexpr (kHam) {
                     add_pair_terms<R, 2>(c, l, lds, tl, G, w0, field, e);
                 }
                 grad_window<R, 2>(c, l, lc, G, wave_acc, lane, rz_tab);
-                exchange<R, 2, 1>(l, lds, tl);
-                exchange<R, 2, 1>(c, lds, tl);
-                if constexpr (kHam) {
-                    add_pair_terms<R, 1>(c, l, lds, tl, G, w0 | window_mask<R, 2>(), field, e);
+                const bool bit3 = G.rot_lo <= 3 && G.rot_hi > 3;
+                if (bit3) {
+                    grad_lane_bit3<R>(c, l, wave_acc, lane, rz_tab);
                 }
-                grad_window<R, 1>(c, l, lc, G, wave_acc, lane, rz_tab);
-                ry_inv_window<R, 1>(l, tile_coef, G);
-                exchange<R, 1, 0>(l, lds, tl);
-                ry_inv_window<R, 0>(l, tile_coef, G);
-                exchange<R, 0, 2>(l, lds, tl);
                 ry_inv_window<R, 2>(l, tile_coef, G);
-                // 32-bit element offsets from the tile base (a tile spans
-                // fewer than 2^32 amplitudes): no 64-bit addresses kept live
-                // across the sweeps
-                double2 *dst = lam_out + base;
-                const uint32_t o0 = (uint32_t)ta.offC;
-                const uint32_t sc = (uint32_t)ta.strideC;
-#pragma unroll
-                for (int k = 0; k < TL::REGS; ++k) {
-                    dst[o0 + (uint32_t)k * sc] = l[k];
+                if (bit3) {
+                    ry_inv_lane_bit3<R>(l, tile_coef[3], lane);
                 }
+                exchange<R, 2, 0>(l, lds, tl);
+                ry_inv_window<R, 0>(l, tile_coef, G);
+                store_lam<R, 0, false>(l, lam_out, base, ta, tl);
             } else {
                 uint32_t done = w0;
                 const bool w1_rot = G.rot_lo < R;
@@ -1298,6 +1345,10 @@ static void launch_bwd_pass_r(bool gather, bool write_cur, bool write_lam, const
     constexpr int TH = Tile<R>::THREADS;
     const bool gen = gen_tab != nullptr;
     if (!write_cur && write_lam) {  // the state is not rewritten: lambda-only sweep
+        if (!gather && R == 4 && (g.a != 3 || g.rot_lo < 3)) {
+            // the sweep takes tile bit 3 across lanes and no other bit of window 1
+            throw std::invalid_argument("high pass geometry: tile bits 0..2 must ride along");
+        }
 #define RIE_BWD_LAM(GA, HM, GN)                                                              \
     if (gather == GA && ham == HM && gen == GN) {                                            \
         bwd_lam_kernel<R, GA, HM, GN><<<grid, TH, 0, s>>>(cur_in, lam_in, lam_out, lc, g,    \
