@@ -237,15 +237,21 @@ __device__ __forceinline__ double lane_value(double v, int lane) {
                             __builtin_amdgcn_readlane(__double2loint(v), lane));
 }
 
-// Sum over the 64 lanes of a (fully active) wave, returned to every lane in a
-// fixed order.  Each row of 16 lanes reduces through DPP lane moves
-// (quad_perm [1,0,3,2], quad_perm [2,3,0,1], row_half_mirror, row_mirror),
-// which use no LDS; the four row sums are read from lanes 0, 16, 32, 48.
-__device__ __forceinline__ double wave_sum(double v) {
+// Sum over this lane's row of 16 lanes, returned to every lane of the row,
+// through DPP lane moves (quad_perm [1,0,3,2], quad_perm [2,3,0,1],
+// row_half_mirror, row_mirror), which use no LDS.
+__device__ __forceinline__ double row_sum(double v) {
     v += dpp_move<0xB1>(v);
     v += dpp_move<0x4E>(v);
     v += dpp_move<0x141>(v);
     v += dpp_move<0x140>(v);
+    return v;
+}
+
+// Sum over the 64 lanes of a (fully active) wave, returned to every lane in a
+// fixed order: the four row sums read from lanes 0, 16, 32, 48.
+__device__ __forceinline__ double wave_sum(double v) {
+    v = row_sum(v);
     return (lane_value(v, 0) + lane_value(v, 16)) + (lane_value(v, 32) + lane_value(v, 48));
 }
 

@@ -996,7 +996,7 @@ template <int R, int M>
 __device__ __forceinline__ void grad_window(double2 (&c)[Tile<R>::REGS],
                                             double2 (&l)[Tile<R>::REGS],
                                             const WireCoef *__restrict__ lc, const PassGeom &G,
-                                            double *__restrict__ wave_acc, int lane,
+                                            double *__restrict__ row_acc, int lane,
                                             const double2 *rz_tab) {
     constexpr int P0 = phase_first_bit<R, M>();
     if (!window_rotates<R, M>(G)) {
@@ -1034,11 +1034,11 @@ __device__ __forceinline__ void grad_window(double2 (&c)[Tile<R>::REGS],
                     dt += (l1.x * c0.x + l1.y * c0.y) - (l0.x * c1.x + l0.y * c1.y);
                 }
             }
-            dt = wave_sum(dt);
-            const double dp = wave_sum(dps[rb]);
-            if (lane == 0) {
-                wave_acc[2 * j] += dt;
-                wave_acc[2 * j + 1] += dp;
+            dt = row_sum(dt);
+            const double dp = row_sum(dps[rb]);
+            if ((lane & 15) == 0) {
+                row_acc[2 * j] += dt;
+                row_acc[2 * j + 1] += dp;
             }
         }
     }
@@ -1079,7 +1079,7 @@ __device__ __forceinline__ double2 lane_xor8(double2 v) {
 template <int R>
 __device__ __forceinline__ void grad_lane_bit3(double2 (&c)[Tile<R>::REGS],
                                                double2 (&l)[Tile<R>::REGS],
-                                               double *__restrict__ wave_acc, int lane,
+                                               double *__restrict__ row_acc, int lane,
                                                const double2 *rz_tab) {
     const bool hi = ((lane >> 3) & 1) != 0;
     const double2 z = rz_tab[Tile<R>::REGS + (hi ? 8 : 0)];
@@ -1097,11 +1097,11 @@ __device__ __forceinline__ void grad_lane_bit3(double2 (&c)[Tile<R>::REGS],
         const double re = l[q].x * cp.x + l[q].y * cp.y;
         dt += hi ? re : -re;
     }
-    dt = wave_sum(dt);
-    dp = wave_sum(dp);
-    if (lane == 0) {
-        wave_acc[2 * 3] += dt;
-        wave_acc[2 * 3 + 1] += dp;
+    dt = row_sum(dt);
+    dp = row_sum(dp);
+    if ((lane & 15) == 0) {
+        row_acc[2 * 3] += dt;
+        row_acc[2 * 3 + 1] += dp;
     }
 }
 
@@ -1148,7 +1148,10 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_la
     using TL = Tile<R>;
     constexpr int T = TL::T;
     __shared__ double2 lds[TL::SLOTS];
-    __shared__ double acc[TL::WAVES][2 * T];
+    // gradient sums per row of 16 lanes: the per-tile reductions stop at the
+    // row (DPP only) and one lane of each row adds into its own slots
+    constexpr int ROWS = TL::THREADS / 16;
+    __shared__ double acc[ROWS][2 * T];
     __shared__ double ered[kHam ? TL::WAVES : 1];
     __shared__ double2 rz_tab[TL::P * TL::REGS];
     __shared__ WireCoef tile_coef[T];
@@ -1157,9 +1160,9 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_la
     __shared__ CMat2 gen_chunks[kGen ? 16 * kChainMaxChunks : 1];
     const uint32_t t = threadIdx.x;
     const int lane = t & (kWave - 1);
-    double *wave_acc = acc[t / kWave];
-    if (lane < 2 * T) {
-        wave_acc[lane] = 0.0;
+    double *wave_acc = acc[t / 16];
+    for (uint32_t i = t; i < (uint32_t)(ROWS * 2 * T); i += TL::THREADS) {
+        acc[i / (2 * T)][i % (2 * T)] = 0.0;
     }
     double e = 0.0;
     build_rz_tables<R>(rz_tab, lc, G, t);
@@ -1320,7 +1323,7 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_la
     if (t < 2 * T) {
         double total = 0.0;
 #pragma unroll
-        for (int w = 0; w < TL::WAVES; ++w) {
+        for (int w = 0; w < ROWS; ++w) {
             total += acc[w][t];
         }
         partials[(uint64_t)blockIdx.x * (2 * T) + t] = total;
