@@ -224,11 +224,12 @@ __device__ __forceinline__ CMat2 chain_site(int w, int q, const WireCoef &k0, co
 }
 
 // v from lane (this lane permuted by the DPP control kCtrl) of the same row
-// of 16 lanes.
+// of 16 lanes.  Every control used here permutes within a full row, so no
+// source lane is out of range and no "old" value needs to be set up.
 template <int kCtrl>
 __device__ __forceinline__ double dpp_move(double v) {
-    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), kCtrl, 0xf, 0xf, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), kCtrl, 0xf, 0xf, false);
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), kCtrl, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), kCtrl, 0xf, 0xf, true);
     return __hiloint2double(hi, lo);
 }
 

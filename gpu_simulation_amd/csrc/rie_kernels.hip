@@ -999,9 +999,9 @@ __device__ __forceinline__ void grad_window(double2 (&c)[Tile<R>::REGS],
                                             double *__restrict__ row_acc, int lane,
                                             const double2 *rz_tab) {
     constexpr int P0 = phase_first_bit<R, M>();
-    if (!window_rotates<R, M>(G)) {
-        return;
-    }
+    // no early exit for a window without rotated bits: its phases are 1 and
+    // its gradients are not read, and branch-free code keeps the register
+    // arrays free of copies at control-flow joins
     double dps[R];
 #pragma unroll
     for (int rb = 0; rb < R; ++rb) {
@@ -1046,19 +1046,18 @@ __device__ __forceinline__ void grad_window(double2 (&c)[Tile<R>::REGS],
 
 // tile_coef: the coefficients of the tile's wires by tile bit, in LDS
 template <int R, int M>
+// (tile bits outside [rot_lo, rot_hi) hold the identity: no branch, so the
+// register array needs no copies at control-flow joins)
 __device__ __forceinline__ void ry_inv_window(double2 (&l)[Tile<R>::REGS],
-                                              const WireCoef *tile_coef, const PassGeom &G) {
+                                              const WireCoef *tile_coef) {
     constexpr int P0 = phase_first_bit<R, M>();
 #pragma unroll
     for (int rb = 0; rb < R; ++rb) {
-        const int j = P0 + rb;
-        if (j >= G.rot_lo && j < G.rot_hi) {
-            const WireCoef k = tile_coef[j];
+        const WireCoef k = tile_coef[P0 + rb];
 #pragma unroll
-            for (int q = 0; q < Tile<R>::REGS; ++q) {
-                if (!(q & (1 << rb))) {
-                    ry_inv_pair(l[q], l[q | (1 << rb)], k);
-                }
+        for (int q = 0; q < Tile<R>::REGS; ++q) {
+            if (!(q & (1 << rb))) {
+                ry_inv_pair(l[q], l[q | (1 << rb)], k);
             }
         }
     }
@@ -1167,7 +1166,8 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_la
     double e = 0.0;
     build_rz_tables<R>(rz_tab, lc, G, t);
     if (t < (uint32_t)T) {
-        tile_coef[t] = lc[tile_wire((int)t, G)];
+        const bool rot = (int)t >= G.rot_lo && (int)t < G.rot_hi;
+        tile_coef[t] = rot ? lc[tile_wire((int)t, G)] : WireCoef{1.0, 0.0, 1.0, 0.0};
     }
     if constexpr (kGen) {
         const int entries = 16 * chain_chunks(G.n, R);
@@ -1215,13 +1215,13 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_la
                                          e);
                 }
                 grad_window<R, 0>(c, l, lc, G, wave_acc, lane, rz_tab);
-                ry_inv_window<R, 0>(l, tile_coef, G);
+                ry_inv_window<R, 0>(l, tile_coef);
                 exchange<R, 0, 1>(l, lds, tl);
-                ry_inv_window<R, 1>(l, tile_coef, G);
+                ry_inv_window<R, 1>(l, tile_coef);
                 exchange<R, 1, 2>(l, lds, tl);
-                ry_inv_window<R, 2>(l, tile_coef, G);
+                ry_inv_window<R, 2>(l, tile_coef);
                 exchange<R, 2, 3>(l, lds, tl);
-                ry_inv_window<R, 3>(l, tile_coef, G);
+                ry_inv_window<R, 3>(l, tile_coef);
                 store_lam<R, 3, true>(l, lam_out, base, ta, tl);
             } else {
                 // windows 1, 0, 2, then lam back through 1 to the contiguous
@@ -1238,11 +1238,11 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_la
                     add_pair_terms<R, 2>(c, l, lds, tl, G, w1 | window_mask<R, 0>(), field, e);
                 }
                 grad_window<R, 2>(c, l, lc, G, wave_acc, lane, rz_tab);
-                ry_inv_window<R, 2>(l, tile_coef, G);
+                ry_inv_window<R, 2>(l, tile_coef);
                 exchange<R, 2, 1>(l, lds, tl);
-                ry_inv_window<R, 1>(l, tile_coef, G);
+                ry_inv_window<R, 1>(l, tile_coef);
                 exchange<R, 1, 0>(l, lds, tl);
-                ry_inv_window<R, 0>(l, tile_coef, G);
+                ry_inv_window<R, 0>(l, tile_coef);
                 store_lam<R, 0, true>(l, lam_out, base, ta, tl);
             }
         } else {
@@ -1259,16 +1259,13 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_la
                     add_pair_terms<R, 2>(c, l, lds, tl, G, w0, field, e);
                 }
                 grad_window<R, 2>(c, l, lc, G, wave_acc, lane, rz_tab);
-                const bool bit3 = G.rot_lo <= 3 && G.rot_hi > 3;
-                if (bit3) {
-                    grad_lane_bit3<R>(c, l, wave_acc, lane, rz_tab);
-                }
-                ry_inv_window<R, 2>(l, tile_coef, G);
-                if (bit3) {
-                    ry_inv_lane_bit3<R>(l, tile_coef[3], lane);
-                }
+                // unconditional: an unrotated bit 3 has phase 1 and the
+                // identity in tile_coef, and its gradient slot is not read
+                grad_lane_bit3<R>(c, l, wave_acc, lane, rz_tab);
+                ry_inv_window<R, 2>(l, tile_coef);
+                ry_inv_lane_bit3<R>(l, tile_coef[3], lane);
                 exchange<R, 2, 0>(l, lds, tl);
-                ry_inv_window<R, 0>(l, tile_coef, G);
+                ry_inv_window<R, 0>(l, tile_coef);
                 store_lam<R, 0, false>(l, lam_out, base, ta, tl);
             } else {
                 uint32_t done = w0;
@@ -1298,17 +1295,17 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_la
                     add_pair_terms<R, 3>(c, l, lds, tl, G, done, field, e);
                 }
                 grad_window<R, 3>(c, l, lc, G, wave_acc, lane, rz_tab);
-                ry_inv_window<R, 3>(l, tile_coef, G);
+                ry_inv_window<R, 3>(l, tile_coef);
                 exchange<R, 3, 2>(l, lds, tl);
-                ry_inv_window<R, 2>(l, tile_coef, G);
+                ry_inv_window<R, 2>(l, tile_coef);
                 if (w1_rot) {
                     exchange<R, 2, 1>(l, lds, tl);
-                    ry_inv_window<R, 1>(l, tile_coef, G);
+                    ry_inv_window<R, 1>(l, tile_coef);
                     exchange<R, 1, 0>(l, lds, tl);
                 } else {
                     exchange<R, 2, 0>(l, lds, tl);
                 }
-                ry_inv_window<R, 0>(l, tile_coef, G);
+                ry_inv_window<R, 0>(l, tile_coef);
                 store_lam<R, 0, false>(l, lam_out, base, ta, tl);
             }
         }
