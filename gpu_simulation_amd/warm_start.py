@@ -17,8 +17,12 @@ Pipeline, all on the GPU:
 3. evaluation on held-out fields: every prediction is judged by the simulator
    (its energy against the random start's) and replaced by the random start
    when it is worse; then plain and warm-started runs are optimised side by
-   side and compared by the number of steps each needs to come within
-   ``tol`` of the plain run's converged energy.
+   side and compared by the number of steps each needs to converge: to come
+   within ``tol`` of the lowest energy of its own run.  The converged energy
+   is the run's minimum, not its last value: with a constant step size Adam
+   keeps oscillating near a minimum (about 1e-3 per step at lr 0.05), so the
+   last value depends on the phase the run stops in.  Beside it, the steps the
+   warm run needs to come within ``tol`` of the plain run's minimum.
 
 No counterpart in the reference, whose runs always start from the random
 initialisation.
@@ -121,7 +125,7 @@ def steps_to_reach(history: np.ndarray, targets: np.ndarray, tol: float) -> np.n
 
 
 def evaluate(model: WarmStartModel, fields, steps: int = 300, lr: float = 0.05,
-             tol: float = 1e-4, seed: int = 44) -> dict:
+             tol: float = 1e-3, seed: int = 44) -> dict:
     n, layers = model.num_qubits, model.layers
     fields = np.asarray(fields, dtype=np.float64)
     count = len(fields)
@@ -134,21 +138,28 @@ def evaluate(model: WarmStartModel, fields, steps: int = 300, lr: float = 0.05,
     e_plain0, e_pred0 = e_judge[:count], e_judge[count:]
     use_pred = e_pred0 <= e_plain0
     warm = [predicted[i] if use_pred[i] else plain[i] for i in range(count)]
-    hist, _, final = run_vqe(n, layers, np.concatenate([fields, fields]), plain + warm, steps, lr)
-    e_conv = final[:count]  # plain-init converged energy
-    s_plain = steps_to_reach(hist[:, :count], e_conv, tol)
-    s_warm = steps_to_reach(hist[:, count:], e_conv, tol)
+    hist, _, _ = run_vqe(n, layers, np.concatenate([fields, fields]), plain + warm, steps, lr)
+    h_plain, h_warm = hist[:, :count], hist[:, count:]
+    e_plain = h_plain.min(axis=0)  # converged energy of each run: its minimum
+    e_warm = h_warm.min(axis=0)
+    s_plain = steps_to_reach(h_plain, e_plain, tol)
+    s_warm = steps_to_reach(h_warm, e_warm, tol)
+    s_warm_to_plain = steps_to_reach(h_warm, e_plain, tol)
     med_plain = float(np.median(s_plain))
     med_warm = float(np.median(s_warm))
     return {
         "count": count,
+        "tol": tol,
         "median_steps_plain": med_plain,
         "median_steps_warm": med_warm,
         "median_ratio": med_warm / med_plain if med_plain > 0 else float("nan"),
+        "median_steps_warm_to_plain_energy": float(np.median(s_warm_to_plain)),
+        "warm_below_plain_fraction": float(np.mean(e_warm <= e_plain + tol)),
+        "mean_converged_energy_plain": float(e_plain.mean()),
+        "mean_converged_energy_warm": float(e_warm.mean()),
         "fallback_fraction": float(1.0 - use_pred.mean()),
         "mean_start_energy_plain": float(e_plain0.mean()),
         "mean_start_energy_warm": float(np.where(use_pred, e_pred0, e_plain0).mean()),
-        "warm_never_reached": int((s_warm >= steps).sum()),
         "steps_plain": s_plain,
         "steps_warm": s_warm,
     }
@@ -156,7 +167,7 @@ def evaluate(model: WarmStartModel, fields, steps: int = 300, lr: float = 0.05,
 
 def run_pipeline(num_qubits: int = 10, layers: int = 4, train_count: int = 4096,
                  eval_count: int = 512, steps: int = 300, lr: float = 0.05,
-                 mlp_steps: int = 2000, mlp_lr: float = 1e-3, tol: float = 1e-4) -> dict:
+                 mlp_steps: int = 2000, mlp_lr: float = 1e-3, tol: float = 1e-3) -> dict:
     data = generate_training_data(num_qubits, layers, train_count, steps, lr, seed=4)
     model = WarmStartModel(num_qubits, layers)
     losses = model.fit(data.fields, data.params, mlp_steps, mlp_lr)
