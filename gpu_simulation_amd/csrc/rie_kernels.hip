@@ -214,64 +214,67 @@ __device__ __forceinline__ void phase_fwd(double2 (&v)[Tile<R>::REGS],
 // kSweepDp: form dE/dphi of all the window's wires in the phase sweep (fewer
 // FP64 operations for windows of several rotated wires, four more live
 // registers); otherwise per pair next to RY^dagger.
-template <int R, int M, bool kSweepDp>
+template <int R, int M, bool kSweepDp, bool kHigh = false>
 __device__ __forceinline__ void phase_bwd(double2 (&c)[Tile<R>::REGS], double2 (&l)[Tile<R>::REGS],
-                                          const WireCoef *__restrict__ lc, const PassGeom &G,
-                                          double *__restrict__ wave_acc, int lane,
-                                          const double2 *rz_tab) {
+                                          const WireCoef *tile_coef, double *__restrict__ row_acc,
+                                          int lane, const double2 *rz_tab) {
     constexpr int P0 = phase_first_bit<R, M>();
     double dps[kSweepDp ? R : 1];
 #pragma unroll
     for (int rb = 0; rb < (kSweepDp ? R : 1); ++rb) {
         dps[rb] = 0.0;
     }
-    if (window_rotates<R, M>(G)) {
+    // branch-free: unrotated tile bits carry phase 1 and the identity in
+    // tile_coef, and their gradient slots are not read.  kHigh: tile bits 0..2
+    // of a high pass never rotate (the contiguous low index bits that ride
+    // along) and are skipped at compile time.
+    constexpr int kSkip = kHigh ? (3 - P0 > 0 ? (3 - P0 < R ? 3 - P0 : R) : 0) : 0;
+    if constexpr (kSkip == R) {
+        return;
+    }
 #pragma unroll
-        for (int q = 0; q < Tile<R>::REGS; ++q) {
-            const double2 z = rz_tab[M * Tile<R>::REGS + q];
-            const double2 zc = make_double2(z.x, -z.y);
-            c[q] = cmul(c[q], zc);
-            l[q] = cmul(l[q], zc);
-            if constexpr (kSweepDp) {
-                const double im = l[q].x * c[q].y - l[q].y * c[q].x;
+    for (int q = 0; q < Tile<R>::REGS; ++q) {
+        const double2 z = rz_tab[M * Tile<R>::REGS + q];
+        const double2 zc = make_double2(z.x, -z.y);
+        c[q] = cmul(c[q], zc);
+        l[q] = cmul(l[q], zc);
+        if constexpr (kSweepDp) {
+            const double im = l[q].x * c[q].y - l[q].y * c[q].x;
 #pragma unroll
-                for (int rb = 0; rb < R; ++rb) {
-                    dps[rb] += ((q >> rb) & 1) ? -im : im;
-                }
+            for (int rb = 0; rb < R; ++rb) {
+                dps[rb] += ((q >> rb) & 1) ? -im : im;
             }
         }
     }
 #pragma unroll
-    for (int rb = R - 1; rb >= 0; --rb) {
+    for (int rb = R - 1; rb >= kSkip; --rb) {
         const int j = P0 + rb;
-        if (j >= G.rot_lo && j < G.rot_hi) {
-            const WireCoef k = lc[tile_wire(j, G)];
-            double dt = 0.0;
-            double dp = 0.0;
+        const WireCoef k = tile_coef[j];
+        double dt = 0.0;
+        double dp = 0.0;
 #pragma unroll
-            for (int q = 0; q < Tile<R>::REGS; ++q) {
-                if (!(q & (1 << rb))) {
-                    double2 &c0 = c[q];
-                    double2 &c1 = c[q | (1 << rb)];
-                    double2 &l0 = l[q];
-                    double2 &l1 = l[q | (1 << rb)];
-                    if constexpr (!kSweepDp) {
-                        dp += (l0.x * c0.y - l0.y * c0.x) - (l1.x * c1.y - l1.y * c1.x);
-                    }
-                    ry_inv_pair(c0, c1, k);
-                    ry_inv_pair(l0, l1, k);
-                    dt += (l1.x * c0.x + l1.y * c0.y) - (l0.x * c1.x + l0.y * c1.y);
+        for (int q = 0; q < Tile<R>::REGS; ++q) {
+            if (!(q & (1 << rb))) {
+                double2 &c0 = c[q];
+                double2 &c1 = c[q | (1 << rb)];
+                double2 &l0 = l[q];
+                double2 &l1 = l[q | (1 << rb)];
+                if constexpr (!kSweepDp) {
+                    dp += (l0.x * c0.y - l0.y * c0.x) - (l1.x * c1.y - l1.y * c1.x);
                 }
+                ry_inv_pair(c0, c1, k);
+                ry_inv_pair(l0, l1, k);
+                dt += (l1.x * c0.x + l1.y * c0.y) - (l0.x * c1.x + l0.y * c1.y);
             }
-            if constexpr (kSweepDp) {
-                dp = dps[rb];
-            }
-            dt = wave_sum(dt);
-            dp = wave_sum(dp);
-            if (lane == 0) {
-                wave_acc[2 * j] += dt;
-                wave_acc[2 * j + 1] += dp;
-            }
+        }
+        if constexpr (kSweepDp) {
+            dp = dps[rb];
+        }
+        dt = row_sum(dt);
+        dp = row_sum(dp);
+        if ((lane & 15) == 0) {
+            row_acc[2 * j] += dt;
+            row_acc[2 * j + 1] += dp;
         }
     }
 }
@@ -835,17 +838,25 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_pa
     using TL = Tile<R>;
     constexpr int T = TL::T;
     __shared__ double2 lds[TL::SLOTS];
-    __shared__ double acc[TL::WAVES][2 * T];
+    // gradient sums per row of 16 lanes (DPP row reductions, one lane of each
+    // row adds into its own slots)
+    constexpr int ROWS = TL::THREADS / 16;
+    __shared__ double acc[ROWS][2 * T];
     __shared__ double ered[kHam ? TL::WAVES : 1];
     __shared__ double2 rz_tab[TL::P * TL::REGS];
+    __shared__ WireCoef tile_coef[T];
     const uint32_t t = threadIdx.x;
     const int lane = t & (kWave - 1);
-    double *wave_acc = acc[t / kWave];
-    if (lane < 2 * T) {
-        wave_acc[lane] = 0.0;
+    double *wave_acc = acc[t / 16];
+    for (uint32_t i = t; i < (uint32_t)(ROWS * 2 * T); i += TL::THREADS) {
+        acc[i / (2 * T)][i % (2 * T)] = 0.0;
     }
     double e = 0.0;
     build_rz_tables<R>(rz_tab, lc, G, t);
+    if (t < (uint32_t)T) {
+        const bool rot = (int)t >= G.rot_lo && (int)t < G.rot_hi;
+        tile_coef[t] = rot ? lc[tile_wire((int)t, G)] : WireCoef{1.0, 0.0, 1.0, 0.0};
+    }
     __syncthreads();
 
     const TileAddr ta = tile_addr<R>(t, G);
@@ -862,20 +873,20 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_pa
         if constexpr (kGather) {
             constexpr uint32_t w1 = window_mask<R, 1>();
             constexpr uint32_t w2 = window_mask<R, 2>();
-            phase_bwd<R, 1, kGather>(c, l, lc, G, wave_acc, lane, rz_tab);
+            phase_bwd<R, 1, kGather, !kGather>(c, l, tile_coef, wave_acc, lane, rz_tab);
             exchange<R, 1, 2>(l, lds, tl);
             exchange<R, 1, 2>(c, lds, tl);
             if constexpr (kHam && gather_pairs_reach<R, 2>()) {
                 add_pair_terms<R, 2>(c, l, lds, tl, G, w1, field, e);
             }
-            phase_bwd<R, 2, kGather>(c, l, lc, G, wave_acc, lane, rz_tab);
+            phase_bwd<R, 2, kGather, !kGather>(c, l, tile_coef, wave_acc, lane, rz_tab);
             if constexpr (TL::P == 4) {
                 exchange<R, 2, 3>(l, lds, tl);
                 exchange<R, 2, 3>(c, lds, tl);
                 if constexpr (kHam && gather_pairs_reach<R, 3>()) {
                     add_pair_terms<R, 3>(c, l, lds, tl, G, w1 | w2, field, e);
                 }
-                phase_bwd<R, 3, kGather>(c, l, lc, G, wave_acc, lane, rz_tab);
+                phase_bwd<R, 3, kGather, !kGather>(c, l, tile_coef, wave_acc, lane, rz_tab);
                 exchange<R, 3, 0>(l, lds, tl);
                 exchange<R, 3, 0>(c, lds, tl);
                 if constexpr (kHam) {
@@ -889,7 +900,7 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_pa
                     add_pair_terms<R, 0>(c, l, lds, tl, G, w1 | w2, field, e);
                 }
             }
-            phase_bwd<R, 0, kGather>(c, l, lc, G, wave_acc, lane, rz_tab);
+            phase_bwd<R, 0, kGather, !kGather>(c, l, tile_coef, wave_acc, lane, rz_tab);
             const uint64_t row = base + ta.offA;
 #pragma unroll
             for (int k = 0; k < TL::REGS; ++k) {
@@ -903,20 +914,20 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_pa
             }
         } else {
             constexpr uint32_t w0 = window_mask<R, 0>();
-            phase_bwd<R, 0, kGather>(c, l, lc, G, wave_acc, lane, rz_tab);
+            phase_bwd<R, 0, kGather, !kGather>(c, l, tile_coef, wave_acc, lane, rz_tab);
             if constexpr (TL::P == 3) {
                 exchange<R, 0, 1>(l, lds, tl);
                 exchange<R, 0, 1>(c, lds, tl);
                 if constexpr (kHam) {
                     add_pair_terms<R, 1>(c, l, lds, tl, G, w0, field, e);
                 }
-                phase_bwd<R, 1, kGather>(c, l, lc, G, wave_acc, lane, rz_tab);
+                phase_bwd<R, 1, kGather, !kGather>(c, l, tile_coef, wave_acc, lane, rz_tab);
                 exchange<R, 1, 2>(l, lds, tl);
                 exchange<R, 1, 2>(c, lds, tl);
                 if constexpr (kHam) {
                     add_pair_terms<R, 2>(c, l, lds, tl, G, w0 | window_mask<R, 1>(), field, e);
                 }
-                phase_bwd<R, 2, kGather>(c, l, lc, G, wave_acc, lane, rz_tab);
+                phase_bwd<R, 2, kGather, !kGather>(c, l, tile_coef, wave_acc, lane, rz_tab);
             } else {
                 uint32_t done = w0;
                 if (G.rot_lo < R) {
@@ -925,7 +936,7 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_pa
                     if constexpr (kHam) {
                         add_pair_terms<R, 1>(c, l, lds, tl, G, done, field, e);
                     }
-                    phase_bwd<R, 1, kGather>(c, l, lc, G, wave_acc, lane, rz_tab);
+                    phase_bwd<R, 1, kGather, !kGather>(c, l, tile_coef, wave_acc, lane, rz_tab);
                     done |= window_mask<R, 1>();
                     exchange<R, 1, 2>(l, lds, tl);
                     exchange<R, 1, 2>(c, lds, tl);
@@ -936,14 +947,14 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_pa
                 if constexpr (kHam) {
                     add_pair_terms<R, 2>(c, l, lds, tl, G, done, field, e);
                 }
-                phase_bwd<R, 2, kGather>(c, l, lc, G, wave_acc, lane, rz_tab);
+                phase_bwd<R, 2, kGather, !kGather>(c, l, tile_coef, wave_acc, lane, rz_tab);
                 done |= window_mask<R, 2>();
                 exchange<R, 2, 3>(l, lds, tl);
                 exchange<R, 2, 3>(c, lds, tl);
                 if constexpr (kHam) {
                     add_pair_terms<R, 3>(c, l, lds, tl, G, done, field, e);
                 }
-                phase_bwd<R, 3, kGather>(c, l, lc, G, wave_acc, lane, rz_tab);
+                phase_bwd<R, 3, kGather, !kGather>(c, l, tile_coef, wave_acc, lane, rz_tab);
             }
             const uint64_t row = base + ta.offC;
 #pragma unroll
@@ -968,7 +979,7 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_pa
     if (t < 2 * T) {
         double total = 0.0;
 #pragma unroll
-        for (int w = 0; w < TL::WAVES; ++w) {
+        for (int w = 0; w < ROWS; ++w) {
             total += acc[w][t];
         }
         partials[(uint64_t)blockIdx.x * (2 * T) + t] = total;
@@ -1366,6 +1377,10 @@ static void launch_bwd_pass_r(bool gather, bool write_cur, bool write_lam, const
     }
     if (gen) {
         throw std::invalid_argument("a generated state needs a high pass on the lambda-only sweep");
+    }
+    if (!gather && (g.a != 3 || g.rot_lo < 3)) {
+        // the one-sweep high pass skips tile bits 0..2 at compile time
+        throw std::invalid_argument("high pass geometry: tile bits 0..2 must ride along");
     }
 #define RIE_BWD(GA, WC, WL, HM)                                                              \
     if (gather == GA && write_cur == WC && write_lam == WL && ham == HM) {                   \
