@@ -201,9 +201,10 @@ __device__ __forceinline__ void phase_fwd(double2 (&v)[Tile<R>::REGS],
     }
 }
 
-// Per-wire gradient sums of one tile are reduced across the wave right away
-// and added by lane 0 to this wave's LDS slot (fixed order, no atomics), so
-// no per-thread accumulators stay live across phases and tiles.
+// Per-wire gradient sums of one tile are reduced across each row of 16 lanes
+// right away (DPP) and added by the row's first lane to that row's LDS slot
+// (fixed order, no atomics), so no per-thread accumulators stay live across
+// phases and tiles.
 // The window's RZ^dagger as one phase per register comes first.  dE/dphi of
 // a wire, sum of Im(l* c) signed by the wire's bit (Im(l0* c0) - Im(l1* c1)
 // per pair), does not change under the phases or under RY^dagger of the
@@ -847,7 +848,7 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_pa
     __shared__ WireCoef tile_coef[T];
     const uint32_t t = threadIdx.x;
     const int lane = t & (kWave - 1);
-    double *wave_acc = acc[t / 16];
+    double *row_acc = acc[t / 16];
     for (uint32_t i = t; i < (uint32_t)(ROWS * 2 * T); i += TL::THREADS) {
         acc[i / (2 * T)][i % (2 * T)] = 0.0;
     }
@@ -873,20 +874,20 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_pa
         if constexpr (kGather) {
             constexpr uint32_t w1 = window_mask<R, 1>();
             constexpr uint32_t w2 = window_mask<R, 2>();
-            phase_bwd<R, 1, kGather, !kGather>(c, l, tile_coef, wave_acc, lane, rz_tab);
+            phase_bwd<R, 1, kGather, !kGather>(c, l, tile_coef, row_acc, lane, rz_tab);
             exchange<R, 1, 2>(l, lds, tl);
             exchange<R, 1, 2>(c, lds, tl);
             if constexpr (kHam && gather_pairs_reach<R, 2>()) {
                 add_pair_terms<R, 2>(c, l, lds, tl, G, w1, field, e);
             }
-            phase_bwd<R, 2, kGather, !kGather>(c, l, tile_coef, wave_acc, lane, rz_tab);
+            phase_bwd<R, 2, kGather, !kGather>(c, l, tile_coef, row_acc, lane, rz_tab);
             if constexpr (TL::P == 4) {
                 exchange<R, 2, 3>(l, lds, tl);
                 exchange<R, 2, 3>(c, lds, tl);
                 if constexpr (kHam && gather_pairs_reach<R, 3>()) {
                     add_pair_terms<R, 3>(c, l, lds, tl, G, w1 | w2, field, e);
                 }
-                phase_bwd<R, 3, kGather, !kGather>(c, l, tile_coef, wave_acc, lane, rz_tab);
+                phase_bwd<R, 3, kGather, !kGather>(c, l, tile_coef, row_acc, lane, rz_tab);
                 exchange<R, 3, 0>(l, lds, tl);
                 exchange<R, 3, 0>(c, lds, tl);
                 if constexpr (kHam) {
@@ -900,7 +901,7 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_pa
                     add_pair_terms<R, 0>(c, l, lds, tl, G, w1 | w2, field, e);
                 }
             }
-            phase_bwd<R, 0, kGather, !kGather>(c, l, tile_coef, wave_acc, lane, rz_tab);
+            phase_bwd<R, 0, kGather, !kGather>(c, l, tile_coef, row_acc, lane, rz_tab);
             const uint64_t row = base + ta.offA;
 #pragma unroll
             for (int k = 0; k < TL::REGS; ++k) {
@@ -914,20 +915,20 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_pa
             }
         } else {
             constexpr uint32_t w0 = window_mask<R, 0>();
-            phase_bwd<R, 0, kGather, !kGather>(c, l, tile_coef, wave_acc, lane, rz_tab);
+            phase_bwd<R, 0, kGather, !kGather>(c, l, tile_coef, row_acc, lane, rz_tab);
             if constexpr (TL::P == 3) {
                 exchange<R, 0, 1>(l, lds, tl);
                 exchange<R, 0, 1>(c, lds, tl);
                 if constexpr (kHam) {
                     add_pair_terms<R, 1>(c, l, lds, tl, G, w0, field, e);
                 }
-                phase_bwd<R, 1, kGather, !kGather>(c, l, tile_coef, wave_acc, lane, rz_tab);
+                phase_bwd<R, 1, kGather, !kGather>(c, l, tile_coef, row_acc, lane, rz_tab);
                 exchange<R, 1, 2>(l, lds, tl);
                 exchange<R, 1, 2>(c, lds, tl);
                 if constexpr (kHam) {
                     add_pair_terms<R, 2>(c, l, lds, tl, G, w0 | window_mask<R, 1>(), field, e);
                 }
-                phase_bwd<R, 2, kGather, !kGather>(c, l, tile_coef, wave_acc, lane, rz_tab);
+                phase_bwd<R, 2, kGather, !kGather>(c, l, tile_coef, row_acc, lane, rz_tab);
             } else {
                 uint32_t done = w0;
                 if (G.rot_lo < R) {
@@ -936,7 +937,7 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_pa
                     if constexpr (kHam) {
                         add_pair_terms<R, 1>(c, l, lds, tl, G, done, field, e);
                     }
-                    phase_bwd<R, 1, kGather, !kGather>(c, l, tile_coef, wave_acc, lane, rz_tab);
+                    phase_bwd<R, 1, kGather, !kGather>(c, l, tile_coef, row_acc, lane, rz_tab);
                     done |= window_mask<R, 1>();
                     exchange<R, 1, 2>(l, lds, tl);
                     exchange<R, 1, 2>(c, lds, tl);
@@ -947,14 +948,14 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_pa
                 if constexpr (kHam) {
                     add_pair_terms<R, 2>(c, l, lds, tl, G, done, field, e);
                 }
-                phase_bwd<R, 2, kGather, !kGather>(c, l, tile_coef, wave_acc, lane, rz_tab);
+                phase_bwd<R, 2, kGather, !kGather>(c, l, tile_coef, row_acc, lane, rz_tab);
                 done |= window_mask<R, 2>();
                 exchange<R, 2, 3>(l, lds, tl);
                 exchange<R, 2, 3>(c, lds, tl);
                 if constexpr (kHam) {
                     add_pair_terms<R, 3>(c, l, lds, tl, G, done, field, e);
                 }
-                phase_bwd<R, 3, kGather, !kGather>(c, l, tile_coef, wave_acc, lane, rz_tab);
+                phase_bwd<R, 3, kGather, !kGather>(c, l, tile_coef, row_acc, lane, rz_tab);
             }
             const uint64_t row = base + ta.offC;
 #pragma unroll
@@ -1170,7 +1171,7 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_la
     __shared__ CMat2 gen_chunks[kGen ? 16 * kChainMaxChunks : 1];
     const uint32_t t = threadIdx.x;
     const int lane = t & (kWave - 1);
-    double *wave_acc = acc[t / 16];
+    double *row_acc = acc[t / 16];
     for (uint32_t i = t; i < (uint32_t)(ROWS * 2 * T); i += TL::THREADS) {
         acc[i / (2 * T)][i % (2 * T)] = 0.0;
     }
@@ -1191,7 +1192,7 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_la
     const TileAddr ta = tile_addr<R>(t, G);
     for (uint64_t g = blockIdx.x; g < G.num_tiles; g += gridDim.x) {
         // the thread index is made opaque per tile so the LDS addresses of the
-        // five exchanges are formed where they are used, not held in
+        // exchanges are formed where they are used, not held in
         // registers across the whole loop
         uint32_t tl = t;
         asm volatile("" : "+v"(tl));
@@ -1205,27 +1206,27 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_la
         if constexpr (kGather) {
             constexpr uint32_t w1 = window_mask<R, 1>();
             constexpr uint32_t w2 = window_mask<R, 2>();
-            grad_window<R, 1>(c, l, lc, G, wave_acc, lane, rz_tab);
+            grad_window<R, 1>(c, l, lc, G, row_acc, lane, rz_tab);
             if constexpr (TL::P == 4) {
                 exchange<R, 1, 2>(l, lds, tl);
                 exchange<R, 1, 2>(c, lds, tl);
                 if constexpr (kHam && gather_pairs_reach<R, 2>()) {
                     add_pair_terms<R, 2>(c, l, lds, tl, G, w1, field, e);
                 }
-                grad_window<R, 2>(c, l, lc, G, wave_acc, lane, rz_tab);
+                grad_window<R, 2>(c, l, lc, G, row_acc, lane, rz_tab);
                 exchange<R, 2, 3>(l, lds, tl);
                 exchange<R, 2, 3>(c, lds, tl);
                 if constexpr (kHam && gather_pairs_reach<R, 3>()) {
                     add_pair_terms<R, 3>(c, l, lds, tl, G, w1 | w2, field, e);
                 }
-                grad_window<R, 3>(c, l, lc, G, wave_acc, lane, rz_tab);
+                grad_window<R, 3>(c, l, lc, G, row_acc, lane, rz_tab);
                 exchange<R, 3, 0>(l, lds, tl);
                 exchange<R, 3, 0>(c, lds, tl);
                 if constexpr (kHam) {
                     add_pair_terms<R, 0>(c, l, lds, tl, G, w1 | w2 | window_mask<R, 3>(), field,
                                          e);
                 }
-                grad_window<R, 0>(c, l, lc, G, wave_acc, lane, rz_tab);
+                grad_window<R, 0>(c, l, lc, G, row_acc, lane, rz_tab);
                 ry_inv_window<R, 0>(l, tile_coef);
                 exchange<R, 0, 1>(l, lds, tl);
                 ry_inv_window<R, 1>(l, tile_coef);
@@ -1242,13 +1243,13 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_la
                 if constexpr (kHam) {
                     add_pair_terms<R, 0>(c, l, lds, tl, G, w1, field, e);
                 }
-                grad_window<R, 0>(c, l, lc, G, wave_acc, lane, rz_tab);
+                grad_window<R, 0>(c, l, lc, G, row_acc, lane, rz_tab);
                 exchange<R, 0, 2>(l, lds, tl);
                 exchange<R, 0, 2>(c, lds, tl);
                 if constexpr (kHam && gather_pairs_reach<R, 2>()) {
                     add_pair_terms<R, 2>(c, l, lds, tl, G, w1 | window_mask<R, 0>(), field, e);
                 }
-                grad_window<R, 2>(c, l, lc, G, wave_acc, lane, rz_tab);
+                grad_window<R, 2>(c, l, lc, G, row_acc, lane, rz_tab);
                 ry_inv_window<R, 2>(l, tile_coef);
                 exchange<R, 2, 1>(l, lds, tl);
                 ry_inv_window<R, 1>(l, tile_coef);
@@ -1258,7 +1259,7 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_la
             }
         } else {
             constexpr uint32_t w0 = window_mask<R, 0>();
-            grad_window<R, 0>(c, l, lc, G, wave_acc, lane, rz_tab);
+            grad_window<R, 0>(c, l, lc, G, row_acc, lane, rz_tab);
             if constexpr (TL::P == 3) {
                 // windows 0 and 2, tile bit 3 across lanes (window 1 has no
                 // other rotated bit: its pairs with bit 4 were added at window
@@ -1269,10 +1270,10 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_la
                 if constexpr (kHam) {
                     add_pair_terms<R, 2>(c, l, lds, tl, G, w0, field, e);
                 }
-                grad_window<R, 2>(c, l, lc, G, wave_acc, lane, rz_tab);
+                grad_window<R, 2>(c, l, lc, G, row_acc, lane, rz_tab);
                 // unconditional: an unrotated bit 3 has phase 1 and the
                 // identity in tile_coef, and its gradient slot is not read
-                grad_lane_bit3<R>(c, l, wave_acc, lane, rz_tab);
+                grad_lane_bit3<R>(c, l, row_acc, lane, rz_tab);
                 ry_inv_window<R, 2>(l, tile_coef);
                 ry_inv_lane_bit3<R>(l, tile_coef[3], lane);
                 exchange<R, 2, 0>(l, lds, tl);
@@ -1287,7 +1288,7 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_la
                     if constexpr (kHam) {
                         add_pair_terms<R, 1>(c, l, lds, tl, G, done, field, e);
                     }
-                    grad_window<R, 1>(c, l, lc, G, wave_acc, lane, rz_tab);
+                    grad_window<R, 1>(c, l, lc, G, row_acc, lane, rz_tab);
                     done |= window_mask<R, 1>();
                     exchange<R, 1, 2>(l, lds, tl);
                     exchange<R, 1, 2>(c, lds, tl);
@@ -1298,14 +1299,14 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_la
                 if constexpr (kHam) {
                     add_pair_terms<R, 2>(c, l, lds, tl, G, done, field, e);
                 }
-                grad_window<R, 2>(c, l, lc, G, wave_acc, lane, rz_tab);
+                grad_window<R, 2>(c, l, lc, G, row_acc, lane, rz_tab);
                 done |= window_mask<R, 2>();
                 exchange<R, 2, 3>(l, lds, tl);
                 exchange<R, 2, 3>(c, lds, tl);
                 if constexpr (kHam) {
                     add_pair_terms<R, 3>(c, l, lds, tl, G, done, field, e);
                 }
-                grad_window<R, 3>(c, l, lc, G, wave_acc, lane, rz_tab);
+                grad_window<R, 3>(c, l, lc, G, row_acc, lane, rz_tab);
                 ry_inv_window<R, 3>(l, tile_coef);
                 exchange<R, 3, 2>(l, lds, tl);
                 ry_inv_window<R, 2>(l, tile_coef);
