@@ -1080,7 +1080,6 @@ __device__ __forceinline__ void ry_inv_window(double2 (&l)[Tile<R>::REGS],
 // bits that ride along).  In mappings 0 and 2 tile bit 3 is lane bit 3, so
 // its pair partner is the lane 8 apart in the same row of 16: DPP row_ror:8
 // brings it over without an LDS exchange.
-template <int R>
 __device__ __forceinline__ double2 lane_xor8(double2 v) {
     return make_double2(dpp_move<0x128>(v.x), dpp_move<0x128>(v.y));
 }
@@ -1104,7 +1103,7 @@ __device__ __forceinline__ void grad_lane_bit3(double2 (&c)[Tile<R>::REGS],
         const double im = l[q].x * c[q].y - l[q].y * c[q].x;
         dp += hi ? -im : im;
         // bit set: Re(l1* c0); clear: -Re(l0* c1)
-        const double2 cp = lane_xor8<R>(c[q]);
+        const double2 cp = lane_xor8(c[q]);
         const double re = l[q].x * cp.x + l[q].y * cp.y;
         dt += hi ? re : -re;
     }
@@ -1123,7 +1122,7 @@ __device__ __forceinline__ void ry_inv_lane_bit3(double2 (&l)[Tile<R>::REGS],
     const double s = ((lane >> 3) & 1) != 0 ? -k.s : k.s;
 #pragma unroll
     for (int q = 0; q < Tile<R>::REGS; ++q) {
-        const double2 p = lane_xor8<R>(l[q]);
+        const double2 p = lane_xor8(l[q]);
         l[q].x = k.c * l[q].x + s * p.x;
         l[q].y = k.c * l[q].y + s * p.y;
     }
