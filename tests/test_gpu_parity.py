@@ -210,6 +210,26 @@ def test_layer1_generation(oracle, backend_module, n, layers, shapes, schedule):
     assert np.array_equal(np.asarray(again["gradient"]), np.asarray(outs[1, 1]["gradient"]))
 
 
+@pytest.mark.parametrize("kind", ["zero", "theta_pi", "quarter_turns"])
+def test_layer1_generation_special_angles(oracle, backend_module, kind):
+    # angles that put exact zeros and ones into the chain's site matrices
+    n, layers = 16, 2
+    params = np.zeros((layers, n, 2))
+    if kind == "theta_pi":
+        params[:, :, 0] = np.pi
+    elif kind == "quarter_turns":
+        params = (np.pi / 2) * np.random.default_rng(2900).integers(0, 4, size=(layers, n, 2))
+    print(f"case gen special n={n} L={layers} kind={kind}", flush=True)
+    e_ref, g_ref = oracle.energy_and_grad(n, layers, 0.9, params)
+    eng = backend_module.RingIsingCudaBackend(n, layers, 0.9, "structured_adjoint", 8, False)
+    flat = np.ascontiguousarray(params.reshape(-1))
+    for gen in (1, 0):
+        eng.set_layer1_generation(gen)
+        out = eng.energy_and_grad(flat)
+        assert abs(out["energy"] - e_ref) <= TOL, gen
+        assert np.max(np.abs(np.asarray(out["gradient"]) - g_ref)) <= TOL, gen
+
+
 def test_layer1_generation_three_high_groups_and_scope(backend_module):
     # 31 qubits (three high groups, the largest size the chain tables cover):
     # generated against stored layer-1 states, beyond the oracle's reach
