@@ -183,13 +183,13 @@ def batched_mix(count: int, steps: int, torch) -> dict:
 def warm_start_section() -> dict:
     """Learned warm start at 10 qubits x 4 layers: 4096 training VQE runs (300
     Adam steps, lr 0.05), MLP 3 -> 256 -> 256 -> 80 trained on the GPU, 512
-    held-out fields; median Adam steps for each run to come within 1e-3 of its
-    own converged (lowest) energy, plain vs warm."""
+    held-out fields; median Adam steps to come within 1e-4 of the plain-init
+    converged (lowest) energy, plain vs warm."""
     from gpu_simulation_amd import warm_start as ws
 
     t0 = time.perf_counter()
     res = ws.run_pipeline(num_qubits=10, layers=4, train_count=4096, eval_count=512, steps=300,
-                          lr=0.05, mlp_steps=2000, mlp_lr=1e-3, tol=1e-3)
+                          lr=0.05, mlp_steps=2000, mlp_lr=1e-3, tol=1e-4)
     out = {k: v for k, v in res.items() if k not in ("steps_plain", "steps_warm")}
     out["seconds"] = round(time.perf_counter() - t0, 2)
     out["workload"] = ("10 qubits x 4 layers, h ~ U[0.2, 2.0]; 4096 training runs (8 shared "

@@ -17,12 +17,13 @@ Pipeline, all on the GPU:
 3. evaluation on held-out fields: every prediction is judged by the simulator
    (its energy against the random start's) and replaced by the random start
    when it is worse; then plain and warm-started runs are optimised side by
-   side and compared by the number of steps each needs to converge: to come
-   within ``tol`` of the lowest energy of its own run.  The converged energy
-   is the run's minimum, not its last value: with a constant step size Adam
-   keeps oscillating near a minimum (about 1e-3 per step at lr 0.05), so the
-   last value depends on the phase the run stops in.  Beside it, the steps the
-   warm run needs to come within ``tol`` of the plain run's minimum.
+   side and compared by the number of steps each needs to come within
+   ``tol`` of the plain-init converged energy.  That energy is the plain
+   run's lowest value, not its last one: with a constant step size Adam keeps
+   oscillating near a minimum (about 1e-3 per step at lr 0.05), so the last
+   value depends on the phase in which the run stops.  Beside it: the steps
+   the warm run needs to come within ``tol`` of its own lowest energy, and
+   both mean converged energies.
 
 No counterpart in the reference, whose runs always start from the random
 initialisation.
@@ -125,7 +126,7 @@ def steps_to_reach(history: np.ndarray, targets: np.ndarray, tol: float) -> np.n
 
 
 def evaluate(model: WarmStartModel, fields, steps: int = 300, lr: float = 0.05,
-             tol: float = 1e-3, seed: int = 44) -> dict:
+             tol: float = 1e-4, seed: int = 44) -> dict:
     n, layers = model.num_qubits, model.layers
     fields = np.asarray(fields, dtype=np.float64)
     count = len(fields)
@@ -140,11 +141,11 @@ def evaluate(model: WarmStartModel, fields, steps: int = 300, lr: float = 0.05,
     warm = [predicted[i] if use_pred[i] else plain[i] for i in range(count)]
     hist, _, _ = run_vqe(n, layers, np.concatenate([fields, fields]), plain + warm, steps, lr)
     h_plain, h_warm = hist[:, :count], hist[:, count:]
-    e_plain = h_plain.min(axis=0)  # converged energy of each run: its minimum
+    e_plain = h_plain.min(axis=0)  # plain-init converged energy: the run's lowest value
     e_warm = h_warm.min(axis=0)
     s_plain = steps_to_reach(h_plain, e_plain, tol)
-    s_warm = steps_to_reach(h_warm, e_warm, tol)
-    s_warm_to_plain = steps_to_reach(h_warm, e_plain, tol)
+    s_warm = steps_to_reach(h_warm, e_plain, tol)
+    s_warm_own = steps_to_reach(h_warm, e_warm, tol)
     med_plain = float(np.median(s_plain))
     med_warm = float(np.median(s_warm))
     return {
@@ -153,7 +154,8 @@ def evaluate(model: WarmStartModel, fields, steps: int = 300, lr: float = 0.05,
         "median_steps_plain": med_plain,
         "median_steps_warm": med_warm,
         "median_ratio": med_warm / med_plain if med_plain > 0 else float("nan"),
-        "median_steps_warm_to_plain_energy": float(np.median(s_warm_to_plain)),
+        "warm_never_reached": int((s_warm >= steps).sum()),
+        "median_steps_warm_to_own_minimum": float(np.median(s_warm_own)),
         "warm_below_plain_fraction": float(np.mean(e_warm <= e_plain + tol)),
         "mean_converged_energy_plain": float(e_plain.mean()),
         "mean_converged_energy_warm": float(e_warm.mean()),
@@ -167,7 +169,7 @@ def evaluate(model: WarmStartModel, fields, steps: int = 300, lr: float = 0.05,
 
 def run_pipeline(num_qubits: int = 10, layers: int = 4, train_count: int = 4096,
                  eval_count: int = 512, steps: int = 300, lr: float = 0.05,
-                 mlp_steps: int = 2000, mlp_lr: float = 1e-3, tol: float = 1e-3) -> dict:
+                 mlp_steps: int = 2000, mlp_lr: float = 1e-3, tol: float = 1e-4) -> dict:
     data = generate_training_data(num_qubits, layers, train_count, steps, lr, seed=4)
     model = WarmStartModel(num_qubits, layers)
     losses = model.fit(data.fields, data.params, mlp_steps, mlp_lr)

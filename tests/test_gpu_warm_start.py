@@ -58,7 +58,7 @@ def test_learned_warm_start_beats_random_init():
     print("branch", data.branch, data.branch_mean_energies, "loss", losses[0], losses[-1],
           flush=True)
     assert losses[-1] < 0.1 * losses[0]
-    res = ws.evaluate(model, np.random.default_rng(44).uniform(0.2, 2.0, 64), steps, lr, 1e-3)
+    res = ws.evaluate(model, np.random.default_rng(44).uniform(0.2, 2.0, 64), steps, lr, 1e-4)
     print({k: v for k, v in res.items() if not isinstance(v, np.ndarray)}, flush=True)
     assert res["median_steps_warm"] < res["median_steps_plain"]
     # the warm runs converge at least as low as the plain ones on average
