@@ -1056,10 +1056,10 @@ __device__ __forceinline__ void grad_window(double2 (&c)[Tile<R>::REGS],
     }
 }
 
-// tile_coef: the coefficients of the tile's wires by tile bit, in LDS
+// tile_coef: the coefficients of the tile's wires by tile bit, in LDS (tile
+// bits outside [rot_lo, rot_hi) hold the identity: no branch, so the register
+// array needs no copies at control-flow joins)
 template <int R, int M>
-// (tile bits outside [rot_lo, rot_hi) hold the identity: no branch, so the
-// register array needs no copies at control-flow joins)
 __device__ __forceinline__ void ry_inv_window(double2 (&l)[Tile<R>::REGS],
                                               const WireCoef *tile_coef) {
     constexpr int P0 = phase_first_bit<R, M>();
