@@ -750,18 +750,43 @@ void Engine::structured_backward(hipStream_t s, const double *scale, bool full_w
         const int ncur = free_buffer({cur, lam});
         const int nlam = free_buffer({cur, lam, ncur});
         const bool only_low = passes.size() == 1;
+        // Layer 1 before the contracted layer 0, with layer-1 generation: its
+        // high passes generate the state, so nothing of the walked state is
+        // written (the low pass still reads the state after layer 1).
+        const bool gen = l == 1 && gen_fwd && !full_walk;
+        if (gen) {
+            build_chain_tables(s);  // the register shapes may have changed since the forward
+        }
+        const bool low_cur = !gen && (only_low ? need_cur_after : true);
         require_partials((size_t)passes[0].grid * 2 * T);
         kt_begin(s);
-        launch_bwd(l, true, only_low ? need_cur_after : true, cur_in, pool[lam], pool[ncur],
-                   pool[nlam], 0, s);
+        launch_bwd(l, true, low_cur, cur_in, pool[lam], pool[ncur], pool[nlam], 0, s);
         check_launch("bwd_pass gather");
-        kt_end(s, "bwd_pass_low_cnot", S * (3.0 + ((only_low && !need_cur_after) ? 0.0 : 1.0)));
+        kt_end(s, "bwd_pass_low_cnot", S * (3.0 + (low_cur ? 1.0 : 0.0)));
         kt_begin(s);
         launch_reduce_grad(d_partials, passes[0].grid, T, passes[0].g, l, d_grad, scale, s);
         check_launch("reduce_grad");
         kt_end(s, "reduce_grad", 0.0);
         cur = ncur;
         lam = nlam;
+        if (gen) {
+            // in the stored schedule's order, so that table set 1 + j (high
+            // groups 0 .. j rotated) holds the state each pass needs
+            for (int j = (int)passes.size() - 2; j >= 0; --j) {
+                const PassPlan &pp = passes[1 + (size_t)j];
+                require_partials((size_t)pp.grid * 2 * T);
+                kt_begin(s);
+                launch_bwd(l, false, false, nullptr, pool[lam], nullptr, pool[lam],
+                           (size_t)(1 + j), s, 1 + j);
+                check_launch("bwd_pass");
+                kt_end(s, "bwd_pass_high_gen", 2 * S);
+                kt_begin(s);
+                launch_reduce_grad(d_partials, pp.grid, T, pp.g, l, d_grad, scale, s);
+                check_launch("reduce_grad");
+                kt_end(s, "reduce_grad", 0.0);
+            }
+            continue;  // l == 1 is the last walked layer
+        }
         for (size_t p = 1; p < passes.size(); ++p) {
             const bool write_cur = (p + 1 < passes.size()) || need_cur_after;
             require_partials((size_t)passes[p].grid * 2 * T);
