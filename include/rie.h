@@ -149,6 +149,9 @@ rie_status rie_engine_energy_and_grad_device(rie_engine *engine,
  *   RIE_SCHEDULE_STORED: keep every forward pass output and read it back
  *     ((L-1)(K+1)+2 buffers, K = high passes per layer; each backward pass
  *     then reads state and lambda and writes lambda only).
+ * With layer-1 generation (rie_engine_set_layer1_generation) both schedules
+ * finish layer 1 the same way: its low pass reads the state after layer 1,
+ * its high passes generate the state they need.
  * A new engine uses STORED when it fits in 85% of free device memory. */
 typedef enum rie_schedule {
     RIE_SCHEDULE_INVERSE_WALK = 0,
@@ -173,8 +176,8 @@ int rie_engine_fused_hamiltonian(const rie_engine *engine);
  * |0...0>, so every state inside layer 1 before its ring CNOT is a trace of
  * a ring of 2x2 matrices (one per wire) over the layer-0 product state.
  * Layer 1's forward becomes one write pass (generate, ring CNOT, store), and
- * in the stored schedule layer 1's adjoint high passes generate the state
- * they need instead of reading a stored one (the reference generates only layer 0
+ * in both backward schedules layer 1's adjoint high passes generate the state
+ * they need instead of reading a stored or walked one (the reference generates only layer 0
  * as a product state and applies every later layer wire chunk by wire chunk:
  * cpp/ising_cuda_structured_adjoint_modes.inc:108-145).  Results agree
  * with the stored/walked evaluation to rounding.  enabled: 1 on, 0 off;
