@@ -100,9 +100,9 @@ struct Engine {
     // Layer-1 generation (L >= 2, n <= kChainMaxQubits): every state inside
     // layer 1 before its ring CNOT is a trace of a ring of 2x2 matrices over
     // the layer-0 product state, so layer 1's forward is one write pass and,
-    // in the stored schedule, its adjoint high passes generate the state
-    // instead of reading it.  Chain table sets: 0 the forward, 1 + j the
-    // adjoint high pass j.
+    // in both backward schedules, its adjoint high passes generate the state
+    // instead of reading or walking it.  Chain table sets: 0 the forward,
+    // 1 + j the adjoint high pass j.
     bool gen_enabled = true;    // caller's switch
     bool gen_fwd = false;       // the last forward generated layer 1
     CMat2 *d_chain = nullptr;   // kChainMaxSets * kChainTabEntries matrices
