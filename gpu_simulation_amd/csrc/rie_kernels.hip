@@ -175,29 +175,31 @@ __device__ __forceinline__ bool window_rotates(const PassGeom &G) {
     return G.rot_lo < P0 + R && G.rot_hi > P0;
 }
 
-template <int R, int M>
-__device__ __forceinline__ void phase_fwd(double2 (&v)[Tile<R>::REGS],
-                                          const WireCoef *__restrict__ lc, const PassGeom &G,
+// Branch-free: tile bits outside [rot_lo, rot_hi) carry the identity in
+// tile_coef and phase 1, so the register array needs no copies at control-flow
+// joins.  kHigh: tile bits 0..2 of a high pass never rotate (the contiguous low
+// index bits that ride along) and are skipped at compile time.
+template <int R, int M, bool kHigh>
+__device__ __forceinline__ void phase_fwd(double2 (&v)[Tile<R>::REGS], const WireCoef *tile_coef,
                                           const double2 *rz_tab) {
     constexpr int P0 = phase_first_bit<R, M>();
+    constexpr int kSkip = kHigh ? (3 - P0 > 0 ? (3 - P0 < R ? 3 - P0 : R) : 0) : 0;
+    if constexpr (kSkip == R) {
+        return;
+    }
 #pragma unroll
-    for (int rb = 0; rb < R; ++rb) {
-        const int j = P0 + rb;
-        if (j >= G.rot_lo && j < G.rot_hi) {
-            const WireCoef k = lc[tile_wire(j, G)];
+    for (int rb = kSkip; rb < R; ++rb) {
+        const WireCoef k = tile_coef[P0 + rb];
 #pragma unroll
-            for (int q = 0; q < Tile<R>::REGS; ++q) {
-                if (!(q & (1 << rb))) {
-                    ry_pair(v[q], v[q | (1 << rb)], k);
-                }
+        for (int q = 0; q < Tile<R>::REGS; ++q) {
+            if (!(q & (1 << rb))) {
+                ry_pair(v[q], v[q | (1 << rb)], k);
             }
         }
     }
-    if (window_rotates<R, M>(G)) {
 #pragma unroll
-        for (int q = 0; q < Tile<R>::REGS; ++q) {
-            v[q] = cmul(v[q], rz_tab[M * Tile<R>::REGS + q]);
-        }
+    for (int q = 0; q < Tile<R>::REGS; ++q) {
+        v[q] = cmul(v[q], rz_tab[M * Tile<R>::REGS + q]);
     }
 }
 
@@ -402,9 +404,14 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void fwd_pa
     __shared__ double2 lds[TL::SLOTS];
     __shared__ double2 nib[kGenInit ? kInitNibbles : 1][16];
     __shared__ double2 rz_tab[TL::P * TL::REGS];
+    __shared__ WireCoef tile_coef[TL::T];
     const uint32_t t = threadIdx.x;
     const TileAddr ta = tile_addr<R>(t, G);
     build_rz_tables<R>(rz_tab, lc, G, t);
+    if (t < (uint32_t)TL::T) {
+        const bool rot = (int)t >= G.rot_lo && (int)t < G.rot_hi;
+        tile_coef[t] = rot ? lc[tile_wire((int)t, G)] : WireCoef{1.0, 0.0, 1.0, 0.0};
+    }
     if constexpr (!kGenInit) {
         __syncthreads();  // gen_init syncs after its nibble tables
     }
@@ -495,23 +502,24 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void fwd_pa
                 }
             }
         }
-        phase_fwd<R, 0>(v, lc, G, rz_tab);
+        constexpr bool kHigh = !kScatter;  // high passes: tile bits 0..2 ride along
+        phase_fwd<R, 0, kHigh>(v, tile_coef, rz_tab);
         if constexpr (TL::P == 3) {
             exchange<R, 0, 1>(v, lds, t);
-            phase_fwd<R, 1>(v, lc, G, rz_tab);
+            phase_fwd<R, 1, kHigh>(v, tile_coef, rz_tab);
             exchange<R, 1, 2>(v, lds, t);
-            phase_fwd<R, 2>(v, lc, G, rz_tab);
+            phase_fwd<R, 2, kHigh>(v, tile_coef, rz_tab);
         } else {
-            if (G.rot_lo < R) {  // the low tile: offset bits [0, R) rotate too
+            if constexpr (!kHigh) {  // the low tile: offset bits [0, R) rotate too
                 exchange<R, 0, 1>(v, lds, t);
-                phase_fwd<R, 1>(v, lc, G, rz_tab);
+                phase_fwd<R, 1, kHigh>(v, tile_coef, rz_tab);
                 exchange<R, 1, 2>(v, lds, t);
             } else {
                 exchange<R, 0, 2>(v, lds, t);
             }
-            phase_fwd<R, 2>(v, lc, G, rz_tab);
+            phase_fwd<R, 2, kHigh>(v, tile_coef, rz_tab);
             exchange<R, 2, 3>(v, lds, t);
-            phase_fwd<R, 3>(v, lc, G, rz_tab);
+            phase_fwd<R, 3, kHigh>(v, tile_coef, rz_tab);
         }
         if constexpr (!kScatter) {
             double2 *dst = out + base + ta.offC;
@@ -530,6 +538,10 @@ static void launch_fwd_pass_r(bool scatter, bool gen_init, const double2 *in, do
                               const WireCoef *lc, const WireCoef *lc0, const PassGeom &g,
                               int grid, hipStream_t s) {
     constexpr int TH = Tile<R>::THREADS;
+    if (scatter ? g.a != Tile<R>::T || g.rot_lo != 0 : g.a != 3 || g.rot_lo < 3) {
+        // the kernel fixes which tile bits rotate at compile time
+        throw std::invalid_argument("forward pass geometry: low tile or 3 ride-along bits");
+    }
     if (gen_init) {
         if (scatter) {
             fwd_pass_kernel<R, true, true><<<grid, TH, 0, s>>>(in, out, lc, lc0, g);
