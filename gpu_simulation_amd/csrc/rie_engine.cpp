@@ -683,9 +683,10 @@ void Engine::structured_hamiltonian_fused(hipStream_t s) {
     fuse_eoff = (size_t)ham_fused_grid;
 }
 
-// Energy and gradient with the fused Hamiltonian: the energy partials of the
-// scattered pass and of the last layer's adjoint passes are reduced after the
-// backward sweep.
+// Energy and gradient with the fused Hamiltonian.  lambda = H psi is complete
+// once the last layer's adjoint passes have added their pair terms, and the
+// adjoint sweep is unitary, so at layer 0 <chi|mu> = <psi|H|psi>: the layer-0
+// contraction writes the energy.
 void Engine::structured_eval_fused(hipStream_t s, const double *scale, double *energy_out,
                                    bool staged) {
     if (staged) {
@@ -704,25 +705,22 @@ void Engine::structured_eval_fused(hipStream_t s, const double *scale, double *e
         stage_begin(s);
     }
     fuse_layer = L - 1;
+    l0_energy_out = energy_out;
     try {
         structured_backward(s, scale, false);
     } catch (...) {
         fuse_layer = -1;
+        l0_energy_out = nullptr;
         throw;
     }
     fuse_layer = -1;
+    l0_energy_out = nullptr;
     if (staged) {
         stage_end(s, "backward_sweep_ms");
         stage_begin(s);
+        stage_end(s, "energy_dot_ms");  // written by the layer-0 contraction
     }
-    energy_partials = fuse_eoff;
-    kt_begin(s);
-    launch_reduce_sum(d_epartials, (int)energy_partials, energy_out, s);
-    check_launch("energy reduce");
-    kt_end(s, "reduce_energy", 0.0);
-    if (staged) {
-        stage_end(s, "energy_dot_ms");
-    }
+    energy_partials = 0;
     resident = false;
 }
 
@@ -829,7 +827,7 @@ void Engine::layer0_gradients(hipStream_t s, const double2 *lam, const double *s
     launch_layer0_high_env(d_tile_sums, g0.num_tiles, d_coef, n, d_partials2, high_env_chunks, s);
     check_launch("layer0_high_env");
     launch_layer0_finish(d_partials, grid0, d_partials2, high_env_chunks, d_coef, n, d_grad, scale,
-                         s);
+                         s, l0_energy_out);
     check_launch("layer0_finish");
     kt_end(s, "layer0_reduce", 0.0);
 }

@@ -193,6 +193,7 @@ struct Engine {
     void structured_backward(hipStream_t s, const double *scale, bool full_walk);
     void structured_backward_stored(hipStream_t s, const double *scale);
     void layer0_gradients(hipStream_t s, const double2 *lam, const double *scale);
+    double *l0_energy_out = nullptr;  // set by the fused evaluation: energy from layer 0
     size_t stored_buffer_count() const;
     double2 *fbuf(int l, int j) const;  // stored schedule: output of pass j of layer l
     double2 *lbuf(int i) const;         // stored schedule: lambda buffers 0 and 1

@@ -712,7 +712,7 @@ constexpr bool gather_pairs_reach() {
     return phase_first_bit<R, M>() + R - 1 >= Tile<R>::T - 4;
 }
 
-template <int R, int M>
+template <int R, int M, bool kEnergy = false>
 __device__ __forceinline__ void add_pair_terms(const double2 (&c)[Tile<R>::REGS],
                                                double2 (&l)[Tile<R>::REGS], const double2 *lds,
                                                uint32_t t, const PassGeom &G, uint32_t done,
@@ -735,7 +735,9 @@ __device__ __forceinline__ void add_pair_terms(const double2 (&c)[Tile<R>::REGS]
                 const double2 p = lds[lds_slot<R>(omap<R, M>(t, k) ^ m)];
                 l[k].x -= field * p.x;
                 l[k].y -= field * p.y;
-                es += c[k].x * p.x + c[k].y * p.y;
+                if constexpr (kEnergy) {
+                    es += c[k].x * p.x + c[k].y * p.y;
+                }
             }
         }
     }
@@ -749,13 +751,15 @@ __device__ __forceinline__ void add_pair_terms(const double2 (&c)[Tile<R>::REGS]
                 const double2 p = c[k ^ (3 << rb)];
                 l[k].x -= field * p.x;
                 l[k].y -= field * p.y;
-                if ((k & (1 << rb)) == 0) {
+                if (kEnergy && (k & (1 << rb)) == 0) {
                     er += c[k].x * p.x + c[k].y * p.y;
                 }
             }
         }
     }
-    e -= field * (es + 2.0 * er);
+    if constexpr (kEnergy) {
+        e -= field * (es + 2.0 * er);
+    }
 }
 
 // Loads one tile of (cur, lam) into the first mapping of the adjoint pass:
@@ -2113,7 +2117,7 @@ __global__ __launch_bounds__(256) void layer0_high_env_kernel(
 __global__ __launch_bounds__(64) void layer0_finish_kernel(
     const double *__restrict__ low_partials, int low_blocks, const double *__restrict__ high_partials,
     int high_chunks, const WireCoef *__restrict__ lc0, int n, double *__restrict__ grad,
-    const double *__restrict__ scale) {
+    const double *__restrict__ scale, double *__restrict__ energy_out) {
     const int w = blockIdx.x;  // wire
     const int t = threadIdx.x;
     double e[4] = {0.0, 0.0, 0.0, 0.0};
@@ -2153,6 +2157,11 @@ __global__ __launch_bounds__(64) void layer0_finish_kernel(
         const double f = scale != nullptr ? *scale : 1.0;
         grad[2 * w] = dth * f;
         grad[2 * w + 1] = dph * f;
+        if (w == 0 && energy_out != nullptr) {
+            // with lambda = H psi carried back to layer 0, <chi|mu> = <psi|H|psi>:
+            // E = Re sum_b E_w[b] u_w(b) for any wire w
+            *energy_out = cmul(E0, u0_of(k)).x + cmul(E1, u1_of(k)).x;
+        }
     }
 }
 
@@ -2176,9 +2185,9 @@ void launch_layer0_high_env(const double2 *tile_sums, uint64_t num_tiles, const 
 
 void launch_layer0_finish(const double *low_partials, int low_blocks, const double *high_partials,
                           int high_chunks, const WireCoef *lc0, int n, double *grad,
-                          const double *scale, hipStream_t s) {
+                          const double *scale, hipStream_t s, double *energy_out) {
     layer0_finish_kernel<<<n, 64, 0, s>>>(low_partials, low_blocks, high_partials, high_chunks,
-                                          lc0, n, grad, scale);
+                                          lc0, n, grad, scale, energy_out);
 }
 
 // ---------------------------------------------------------------------------

@@ -143,9 +143,10 @@ void launch_gen_layer1(int R, double2 *out, const CMat2 *tab, const PassGeom &g,
 // ring-CNOT on load (low pass, out of place); write_cur / write_lam = store
 // the un-rotated state / adjoint state.
 // ham_pairs: before un-rotating, add the transverse-field terms of the
-// tile-bit pairs [g.pair_lo, g.pair_hi) to lam (-field * X_j X_{j+1} cur) and
-// write per-workgroup energy partials of those terms into e_partials.  In the
-// pre-CNOT frame of the last layer these pairs are the X terms X_j of H.
+// tile-bit pairs [g.pair_lo, g.pair_hi) to lam (-field * X_j X_{j+1} cur).  In
+// the pre-CNOT frame of the last layer these pairs are the X terms X_j of H;
+// their energy is not summed here (the layer-0 contraction yields the whole
+// energy once lambda = H psi is complete), e_partials receives zeros.
 // gen_tab (layer 1, high pass on the lambda-only sweep): the state is not read
 // from cur_in but generated from a chain table set whose run is mapping 0's
 // register wires.
@@ -175,9 +176,11 @@ void launch_layer0_contract(int R, const double2 *lam, const WireCoef *lc0, cons
                             double2 *tile_sums, double *partials, int grid, hipStream_t s);
 void launch_layer0_high_env(const double2 *tile_sums, uint64_t num_tiles, const WireCoef *lc0,
                             int n, double *partials, int chunks, hipStream_t s);
+// energy_out (optional): Re <chi|mu>, the energy when lambda = H psi was carried
+// back complete (fused Hamiltonian: the pair terms are all added by then).
 void launch_layer0_finish(const double *low_partials, int low_blocks, const double *high_partials,
                           int high_chunks, const WireCoef *lc0, int n, double *grad,
-                          const double *scale, hipStream_t s);
+                          const double *scale, hipStream_t s, double *energy_out = nullptr);
 
 // grad[(layer*n + wire)*2 + comp] = (sum over blocks) * scale for the pass's wires.
 void launch_reduce_grad(const double *partials, int num_blocks, int T, const PassGeom &g,
