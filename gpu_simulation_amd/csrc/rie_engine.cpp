@@ -225,9 +225,6 @@ Engine::Engine(size_t num_qubits, size_t num_layers, double field, const std::st
         for (const auto &p : passes) {
             eparts += (size_t)p.grid;
         }
-        // the fused evaluation: the scattered pass plus every adjoint pass of
-        // the last layer
-        eparts += (size_t)ham_fused_grid;
         // the layer-0 contraction writes 48 values per workgroup of the low pass
         partial_count = std::max({part, eparts, (size_t)passes[0].grid * 48});
         d_partials = (double *)device_alloc(sizeof(double) * partial_count, "alloc partials");
@@ -608,19 +605,10 @@ void Engine::launch_bwd(int l, bool gather, bool write_cur, const double2 *cur_i
         }
         gen_tab = d_chain + (size_t)gen_set * kChainTabEntries;
     }
-    if (l != fuse_layer) {
-        launch_bwd_pass(pass_r[gather ? RIE_PASS_BWD_LOW : RIE_PASS_BWD_HIGH], gather, write_cur,
-                        true, cur_in, lam_in, cur_out, lam_out, lc,
-                        pp.g, d_partials, pp.grid, s, false, 0.0, nullptr, gen_tab);
-        return;
-    }
-    if (fuse_eoff + (size_t)pp.grid > epartial_count) {
-        fail(RIE_DEVICE_ERROR, "internal: energy partial buffer too small");
-    }
+    const bool pairs = l == fuse_layer;
     launch_bwd_pass(pass_r[gather ? RIE_PASS_BWD_LOW : RIE_PASS_BWD_HIGH], gather, write_cur, true,
-                    cur_in, lam_in, cur_out, lam_out, lc,
-                    pp.g, d_partials, pp.grid, s, true, h, d_epartials + fuse_eoff, gen_tab);
-    fuse_eoff += (size_t)pp.grid;
+                    cur_in, lam_in, cur_out, lam_out, lc, pp.g, d_partials, pp.grid, s, pairs,
+                    pairs ? h : 0.0, gen_tab);
 }
 
 bool Engine::gen_ready() const {
@@ -671,16 +659,12 @@ void Engine::structured_hamiltonian_fused(hipStream_t s) {
         lam_idx = free_buffer({final_cur});
         lam = pool[lam_idx];
     }
-    if ((size_t)ham_fused_grid > epartial_count) {
-        fail(RIE_DEVICE_ERROR, "internal: energy partial buffer too small");
-    }
     const double S = (double)sizeof(double2) * (double)N;
     kt_begin(s);
-    launch_ham_pass_scattered(pass_r[RIE_PASS_HAM], final_state(), lam, ham_fused_g, h, d_epartials,
+    launch_ham_pass_scattered(pass_r[RIE_PASS_HAM], final_state(), lam, ham_fused_g, h,
                               ham_fused_grid, s);
     check_launch("ham_pass scattered");
     kt_end(s, "ham_pass_fused", 2.0 * S);
-    fuse_eoff = (size_t)ham_fused_grid;
 }
 
 // Energy and gradient with the fused Hamiltonian.  lambda = H psi is complete

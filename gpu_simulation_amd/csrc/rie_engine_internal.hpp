@@ -96,7 +96,6 @@ struct Engine {
     bool fuse_ready = false;   // the plan supports it
     bool fuse_enabled = true;  // caller's switch
     int fuse_layer = -1;       // layer whose adjoint passes add pair terms now
-    size_t fuse_eoff = 0;      // next energy-partial slot of the fused passes
     // Layer-1 generation (L >= 2, n <= kChainMaxQubits): every state inside
     // layer 1 before its ring CNOT is a trace of a ring of 2x2 matrices over
     // the layer-0 product state, so layer 1's forward is one write pass and,

@@ -696,8 +696,8 @@ constexpr uint32_t window_mask() {
 }
 
 // Transverse-field terms of the adjacent tile-bit pairs (j, j + 1),
-// pair_lo <= j < pair_hi - 1: lam += -field * X_j X_{j+1} cur, and e gets
-// their energy Re(conj(cur) * term).  Called before un-rotating the window of
+// pair_lo <= j < pair_hi - 1: lam += -field * X_j X_{j+1} cur (their energy is
+// part of the layer-0 contraction's).  Called before un-rotating the window of
 // mapping M, it adds the pairs that touch the window and no bit of `done`
 // (the windows un-rotated earlier in the pass, whose pairs were added then):
 // each pair is added once, while both of its bits are still rotated, so it
@@ -712,14 +712,13 @@ constexpr bool gather_pairs_reach() {
     return phase_first_bit<R, M>() + R - 1 >= Tile<R>::T - 4;
 }
 
-template <int R, int M, bool kEnergy = false>
+template <int R, int M>
 __device__ __forceinline__ void add_pair_terms(const double2 (&c)[Tile<R>::REGS],
                                                double2 (&l)[Tile<R>::REGS], const double2 *lds,
                                                uint32_t t, const PassGeom &G, uint32_t done,
-                                               double field, double &e) {
+                                               double field) {
     constexpr int T = Tile<R>::T;
     constexpr int lo = phase_first_bit<R, M>();
-    double es = 0.0;
     // the two pairs across the window's edges: one bit in registers, the
     // partner from LDS
 #pragma unroll
@@ -735,13 +734,10 @@ __device__ __forceinline__ void add_pair_terms(const double2 (&c)[Tile<R>::REGS]
                 const double2 p = lds[lds_slot<R>(omap<R, M>(t, k) ^ m)];
                 l[k].x -= field * p.x;
                 l[k].y -= field * p.y;
-                if constexpr (kEnergy) {
-                    es += c[k].x * p.x + c[k].y * p.y;
-                }
             }
         }
     }
-    double er = 0.0;  // pairs inside the window: each unordered pair once, doubled
+    // pairs inside the window
 #pragma unroll
     for (int rb = 0; rb + 1 < R; ++rb) {
         const int j = lo + rb;
@@ -751,14 +747,8 @@ __device__ __forceinline__ void add_pair_terms(const double2 (&c)[Tile<R>::REGS]
                 const double2 p = c[k ^ (3 << rb)];
                 l[k].x -= field * p.x;
                 l[k].y -= field * p.y;
-                if (kEnergy && (k & (1 << rb)) == 0) {
-                    er += c[k].x * p.x + c[k].y * p.y;
-                }
             }
         }
-    }
-    if constexpr (kEnergy) {
-        e -= field * (es + 2.0 * er);
     }
 }
 
@@ -774,7 +764,7 @@ __device__ __forceinline__ void bwd_load_tile(double2 (&c)[Tile<R>::REGS],
                                               const double2 *cur_in, const double2 *lam_in,
                                               uint64_t base, const TileAddr &ta,
                                               const PassGeom &G, double2 *lds, uint32_t t,
-                                              double field, double &e,
+                                              double field,
                                               const CMat2 *__restrict__ gen_tab = nullptr,
                                               const CMat2 *gen_chunks = nullptr) {
     using TL = Tile<R>;
@@ -815,7 +805,7 @@ __device__ __forceinline__ void bwd_load_tile(double2 (&c)[Tile<R>::REGS],
         }
         if constexpr (kHam && gather_pairs_reach<R, 1>()) {
             // the LDS still holds the gathered tile of cur in offset order
-            add_pair_terms<R, 1>(c, l, lds, t, G, 0u, field, e);
+            add_pair_terms<R, 1>(c, l, lds, t, G, 0u, field);
         }
     } else {
         const uint64_t row = base + ta.offA;
@@ -842,7 +832,7 @@ __device__ __forceinline__ void bwd_load_tile(double2 (&c)[Tile<R>::REGS],
                 }
                 __syncthreads();
             }
-            add_pair_terms<R, 0>(c, l, lds, t, G, 0u, field, e);
+            add_pair_terms<R, 0>(c, l, lds, t, G, 0u, field);
         }
     }
 }
@@ -850,8 +840,7 @@ __device__ __forceinline__ void bwd_load_tile(double2 (&c)[Tile<R>::REGS],
 template <int R, bool kGather, bool kWriteCur, bool kWriteLam, bool kHam>
 __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_pass_kernel(
     const double2 *cur_in, const double2 *lam_in, double2 *cur_out, double2 *lam_out,
-    const WireCoef *__restrict__ lc, PassGeom G, double *__restrict__ partials, double field,
-    double *__restrict__ e_partials) {
+    const WireCoef *__restrict__ lc, PassGeom G, double *__restrict__ partials, double field) {
     using TL = Tile<R>;
     constexpr int T = TL::T;
     __shared__ double2 lds[TL::SLOTS];
@@ -859,7 +848,6 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_pa
     // row adds into its own slots)
     constexpr int ROWS = TL::THREADS / 16;
     __shared__ double acc[ROWS][2 * T];
-    __shared__ double ered[kHam ? TL::WAVES : 1];
     __shared__ double2 rz_tab[TL::P * TL::REGS];
     __shared__ WireCoef tile_coef[T];
     const uint32_t t = threadIdx.x;
@@ -868,7 +856,6 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_pa
     for (uint32_t i = t; i < (uint32_t)(ROWS * 2 * T); i += TL::THREADS) {
         acc[i / (2 * T)][i % (2 * T)] = 0.0;
     }
-    double e = 0.0;
     build_rz_tables<R>(rz_tab, lc, G, t);
     if (t < (uint32_t)T) {
         const bool rot = (int)t >= G.rot_lo && (int)t < G.rot_hi;
@@ -884,7 +871,7 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_pa
         const uint64_t base = tile_base(g, G, T);
         double2 c[TL::REGS];
         double2 l[TL::REGS];
-        bwd_load_tile<R, kGather, kHam>(c, l, cur_in, lam_in, base, ta, G, lds, tl, field, e);
+        bwd_load_tile<R, kGather, kHam>(c, l, cur_in, lam_in, base, ta, G, lds, tl, field);
 
         // lam is exchanged before cur, so the LDS holds cur for the pair terms
         if constexpr (kGather) {
@@ -894,27 +881,26 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_pa
             exchange<R, 1, 2>(l, lds, tl);
             exchange<R, 1, 2>(c, lds, tl);
             if constexpr (kHam && gather_pairs_reach<R, 2>()) {
-                add_pair_terms<R, 2>(c, l, lds, tl, G, w1, field, e);
+                add_pair_terms<R, 2>(c, l, lds, tl, G, w1, field);
             }
             phase_bwd<R, 2, kGather, !kGather>(c, l, tile_coef, row_acc, lane, rz_tab);
             if constexpr (TL::P == 4) {
                 exchange<R, 2, 3>(l, lds, tl);
                 exchange<R, 2, 3>(c, lds, tl);
                 if constexpr (kHam && gather_pairs_reach<R, 3>()) {
-                    add_pair_terms<R, 3>(c, l, lds, tl, G, w1 | w2, field, e);
+                    add_pair_terms<R, 3>(c, l, lds, tl, G, w1 | w2, field);
                 }
                 phase_bwd<R, 3, kGather, !kGather>(c, l, tile_coef, row_acc, lane, rz_tab);
                 exchange<R, 3, 0>(l, lds, tl);
                 exchange<R, 3, 0>(c, lds, tl);
                 if constexpr (kHam) {
-                    add_pair_terms<R, 0>(c, l, lds, tl, G, w1 | w2 | window_mask<R, 3>(), field,
-                                         e);
+                    add_pair_terms<R, 0>(c, l, lds, tl, G, w1 | w2 | window_mask<R, 3>(), field);
                 }
             } else {
                 exchange<R, 2, 0>(l, lds, tl);
                 exchange<R, 2, 0>(c, lds, tl);
                 if constexpr (kHam) {
-                    add_pair_terms<R, 0>(c, l, lds, tl, G, w1 | w2, field, e);
+                    add_pair_terms<R, 0>(c, l, lds, tl, G, w1 | w2, field);
                 }
             }
             phase_bwd<R, 0, kGather, !kGather>(c, l, tile_coef, row_acc, lane, rz_tab);
@@ -936,13 +922,13 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_pa
                 exchange<R, 0, 1>(l, lds, tl);
                 exchange<R, 0, 1>(c, lds, tl);
                 if constexpr (kHam) {
-                    add_pair_terms<R, 1>(c, l, lds, tl, G, w0, field, e);
+                    add_pair_terms<R, 1>(c, l, lds, tl, G, w0, field);
                 }
                 phase_bwd<R, 1, kGather, !kGather>(c, l, tile_coef, row_acc, lane, rz_tab);
                 exchange<R, 1, 2>(l, lds, tl);
                 exchange<R, 1, 2>(c, lds, tl);
                 if constexpr (kHam) {
-                    add_pair_terms<R, 2>(c, l, lds, tl, G, w0 | window_mask<R, 1>(), field, e);
+                    add_pair_terms<R, 2>(c, l, lds, tl, G, w0 | window_mask<R, 1>(), field);
                 }
                 phase_bwd<R, 2, kGather, !kGather>(c, l, tile_coef, row_acc, lane, rz_tab);
             } else {
@@ -951,7 +937,7 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_pa
                     exchange<R, 0, 1>(l, lds, tl);
                     exchange<R, 0, 1>(c, lds, tl);
                     if constexpr (kHam) {
-                        add_pair_terms<R, 1>(c, l, lds, tl, G, done, field, e);
+                        add_pair_terms<R, 1>(c, l, lds, tl, G, done, field);
                     }
                     phase_bwd<R, 1, kGather, !kGather>(c, l, tile_coef, row_acc, lane, rz_tab);
                     done |= window_mask<R, 1>();
@@ -962,14 +948,14 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_pa
                     exchange<R, 0, 2>(c, lds, tl);
                 }
                 if constexpr (kHam) {
-                    add_pair_terms<R, 2>(c, l, lds, tl, G, done, field, e);
+                    add_pair_terms<R, 2>(c, l, lds, tl, G, done, field);
                 }
                 phase_bwd<R, 2, kGather, !kGather>(c, l, tile_coef, row_acc, lane, rz_tab);
                 done |= window_mask<R, 2>();
                 exchange<R, 2, 3>(l, lds, tl);
                 exchange<R, 2, 3>(c, lds, tl);
                 if constexpr (kHam) {
-                    add_pair_terms<R, 3>(c, l, lds, tl, G, done, field, e);
+                    add_pair_terms<R, 3>(c, l, lds, tl, G, done, field);
                 }
                 phase_bwd<R, 3, kGather, !kGather>(c, l, tile_coef, row_acc, lane, rz_tab);
             }
@@ -986,12 +972,6 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_pa
             }
         }
     }
-    if constexpr (kHam) {
-        const double ws = wave_sum(e);
-        if (lane == 0) {
-            ered[t / kWave] = ws;
-        }
-    }
     __syncthreads();
     if (t < 2 * T) {
         double total = 0.0;
@@ -1000,15 +980,6 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_pa
             total += acc[w][t];
         }
         partials[(uint64_t)blockIdx.x * (2 * T) + t] = total;
-    }
-    if constexpr (kHam) {
-        if (t == 0) {
-            double total = 0.0;
-            for (int w = 0; w < TL::WAVES; ++w) {
-                total += ered[w];
-            }
-            e_partials[blockIdx.x] = total;
-        }
     }
 }
 
@@ -1170,7 +1141,7 @@ template <int R, bool kGather, bool kHam, bool kGen>
 __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_lam_kernel(
     const double2 *cur_in, const double2 *lam_in, double2 *lam_out,
     const WireCoef *__restrict__ lc, PassGeom G, double *__restrict__ partials, double field,
-    double *__restrict__ e_partials, const CMat2 *__restrict__ gen_tab) {
+    const CMat2 *__restrict__ gen_tab) {
     using TL = Tile<R>;
     constexpr int T = TL::T;
     __shared__ double2 lds[TL::SLOTS];
@@ -1178,7 +1149,6 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_la
     // row (DPP only) and one lane of each row adds into its own slots
     constexpr int ROWS = TL::THREADS / 16;
     __shared__ double acc[ROWS][2 * T];
-    __shared__ double ered[kHam ? TL::WAVES : 1];
     __shared__ double2 rz_tab[TL::P * TL::REGS];
     __shared__ WireCoef tile_coef[T];
     // chunk products of the generating table set: read at every tile start,
@@ -1190,7 +1160,6 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_la
     for (uint32_t i = t; i < (uint32_t)(ROWS * 2 * T); i += TL::THREADS) {
         acc[i / (2 * T)][i % (2 * T)] = 0.0;
     }
-    double e = 0.0;
     build_rz_tables<R>(rz_tab, lc, G, t);
     if (t < (uint32_t)T) {
         const bool rot = (int)t >= G.rot_lo && (int)t < G.rot_hi;
@@ -1215,7 +1184,7 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_la
         double2 c[TL::REGS];
         double2 l[TL::REGS];
         bwd_load_tile<R, kGather, kHam, kGen>(c, l, cur_in, lam_in, base, ta, G, lds, tl, field,
-                                              e, gen_tab, gen_chunks);
+                                              gen_tab, gen_chunks);
         // sweep 1: phases and gradients (lam exchanged before cur, so the LDS
         // holds cur for the pair terms); sweep 2: RY^dagger on lam
         if constexpr (kGather) {
@@ -1226,20 +1195,19 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_la
                 exchange<R, 1, 2>(l, lds, tl);
                 exchange<R, 1, 2>(c, lds, tl);
                 if constexpr (kHam && gather_pairs_reach<R, 2>()) {
-                    add_pair_terms<R, 2>(c, l, lds, tl, G, w1, field, e);
+                    add_pair_terms<R, 2>(c, l, lds, tl, G, w1, field);
                 }
                 grad_window<R, 2>(c, l, lc, G, row_acc, lane, rz_tab);
                 exchange<R, 2, 3>(l, lds, tl);
                 exchange<R, 2, 3>(c, lds, tl);
                 if constexpr (kHam && gather_pairs_reach<R, 3>()) {
-                    add_pair_terms<R, 3>(c, l, lds, tl, G, w1 | w2, field, e);
+                    add_pair_terms<R, 3>(c, l, lds, tl, G, w1 | w2, field);
                 }
                 grad_window<R, 3>(c, l, lc, G, row_acc, lane, rz_tab);
                 exchange<R, 3, 0>(l, lds, tl);
                 exchange<R, 3, 0>(c, lds, tl);
                 if constexpr (kHam) {
-                    add_pair_terms<R, 0>(c, l, lds, tl, G, w1 | w2 | window_mask<R, 3>(), field,
-                                         e);
+                    add_pair_terms<R, 0>(c, l, lds, tl, G, w1 | w2 | window_mask<R, 3>(), field);
                 }
                 grad_window<R, 0>(c, l, lc, G, row_acc, lane, rz_tab);
                 ry_inv_window<R, 0>(l, tile_coef);
@@ -1256,13 +1224,13 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_la
                 exchange<R, 1, 0>(l, lds, tl);
                 exchange<R, 1, 0>(c, lds, tl);
                 if constexpr (kHam) {
-                    add_pair_terms<R, 0>(c, l, lds, tl, G, w1, field, e);
+                    add_pair_terms<R, 0>(c, l, lds, tl, G, w1, field);
                 }
                 grad_window<R, 0>(c, l, lc, G, row_acc, lane, rz_tab);
                 exchange<R, 0, 2>(l, lds, tl);
                 exchange<R, 0, 2>(c, lds, tl);
                 if constexpr (kHam && gather_pairs_reach<R, 2>()) {
-                    add_pair_terms<R, 2>(c, l, lds, tl, G, w1 | window_mask<R, 0>(), field, e);
+                    add_pair_terms<R, 2>(c, l, lds, tl, G, w1 | window_mask<R, 0>(), field);
                 }
                 grad_window<R, 2>(c, l, lc, G, row_acc, lane, rz_tab);
                 ry_inv_window<R, 2>(l, tile_coef);
@@ -1283,7 +1251,7 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_la
                 exchange<R, 0, 2>(l, lds, tl);
                 exchange<R, 0, 2>(c, lds, tl);
                 if constexpr (kHam) {
-                    add_pair_terms<R, 2>(c, l, lds, tl, G, w0, field, e);
+                    add_pair_terms<R, 2>(c, l, lds, tl, G, w0, field);
                 }
                 grad_window<R, 2>(c, l, lc, G, row_acc, lane, rz_tab);
                 // unconditional: an unrotated bit 3 has phase 1 and the
@@ -1301,7 +1269,7 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_la
                     exchange<R, 0, 1>(l, lds, tl);
                     exchange<R, 0, 1>(c, lds, tl);
                     if constexpr (kHam) {
-                        add_pair_terms<R, 1>(c, l, lds, tl, G, done, field, e);
+                        add_pair_terms<R, 1>(c, l, lds, tl, G, done, field);
                     }
                     grad_window<R, 1>(c, l, lc, G, row_acc, lane, rz_tab);
                     done |= window_mask<R, 1>();
@@ -1312,14 +1280,14 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_la
                     exchange<R, 0, 2>(c, lds, tl);
                 }
                 if constexpr (kHam) {
-                    add_pair_terms<R, 2>(c, l, lds, tl, G, done, field, e);
+                    add_pair_terms<R, 2>(c, l, lds, tl, G, done, field);
                 }
                 grad_window<R, 2>(c, l, lc, G, row_acc, lane, rz_tab);
                 done |= window_mask<R, 2>();
                 exchange<R, 2, 3>(l, lds, tl);
                 exchange<R, 2, 3>(c, lds, tl);
                 if constexpr (kHam) {
-                    add_pair_terms<R, 3>(c, l, lds, tl, G, done, field, e);
+                    add_pair_terms<R, 3>(c, l, lds, tl, G, done, field);
                 }
                 grad_window<R, 3>(c, l, lc, G, row_acc, lane, rz_tab);
                 ry_inv_window<R, 3>(l, tile_coef);
@@ -1337,12 +1305,6 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_la
             }
         }
     }
-    if constexpr (kHam) {
-        const double ws = wave_sum(e);
-        if (lane == 0) {
-            ered[t / kWave] = ws;
-        }
-    }
     __syncthreads();
     if (t < 2 * T) {
         double total = 0.0;
@@ -1352,23 +1314,13 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_la
         }
         partials[(uint64_t)blockIdx.x * (2 * T) + t] = total;
     }
-    if constexpr (kHam) {
-        if (t == 0) {
-            double total = 0.0;
-            for (int w = 0; w < TL::WAVES; ++w) {
-                total += ered[w];
-            }
-            e_partials[blockIdx.x] = total;
-        }
-    }
 }
 
 template <int R>
 static void launch_bwd_pass_r(bool gather, bool write_cur, bool write_lam, const double2 *cur_in,
                               const double2 *lam_in, double2 *cur_out, double2 *lam_out,
                               const WireCoef *lc, const PassGeom &g, double *partials, int grid,
-                              hipStream_t s, bool ham, double field, double *e_partials,
-                              const CMat2 *gen_tab) {
+                              hipStream_t s, bool ham, double field, const CMat2 *gen_tab) {
     constexpr int TH = Tile<R>::THREADS;
     const bool gen = gen_tab != nullptr;
     if (!write_cur && write_lam) {  // the state is not rewritten: lambda-only sweep
@@ -1379,8 +1331,7 @@ static void launch_bwd_pass_r(bool gather, bool write_cur, bool write_lam, const
 #define RIE_BWD_LAM(GA, HM, GN)                                                              \
     if (gather == GA && ham == HM && gen == GN) {                                            \
         bwd_lam_kernel<R, GA, HM, GN><<<grid, TH, 0, s>>>(cur_in, lam_in, lam_out, lc, g,    \
-                                                          partials, field, e_partials,       \
-                                                          gen_tab);                          \
+                                                          partials, field, gen_tab);         \
         return;                                                                              \
     }
         RIE_BWD_LAM(true, false, false)
@@ -1401,7 +1352,7 @@ static void launch_bwd_pass_r(bool gather, bool write_cur, bool write_lam, const
 #define RIE_BWD(GA, WC, WL, HM)                                                              \
     if (gather == GA && write_cur == WC && write_lam == WL && ham == HM) {                   \
         bwd_pass_kernel<R, GA, WC, WL, HM><<<grid, TH, 0, s>>>(                              \
-            cur_in, lam_in, cur_out, lam_out, lc, g, partials, field, e_partials);           \
+            cur_in, lam_in, cur_out, lam_out, lc, g, partials, field);                       \
         return;                                                                              \
     }
     RIE_BWD(true, true, true, false)
@@ -1418,16 +1369,13 @@ static void launch_bwd_pass_r(bool gather, bool write_cur, bool write_lam, const
 void launch_bwd_pass(int R, bool gather, bool write_cur, bool write_lam, const double2 *cur_in,
                      const double2 *lam_in, double2 *cur_out, double2 *lam_out,
                      const WireCoef *layer_coef, const PassGeom &g, double *partials, int grid,
-                     hipStream_t s, bool ham_pairs, double field, double *e_partials,
-                     const CMat2 *gen_tab) {
+                     hipStream_t s, bool ham_pairs, double field, const CMat2 *gen_tab) {
     if (R == 4) {
         launch_bwd_pass_r<4>(gather, write_cur, write_lam, cur_in, lam_in, cur_out, lam_out,
-                             layer_coef, g, partials, grid, s, ham_pairs, field, e_partials,
-                             gen_tab);
+                             layer_coef, g, partials, grid, s, ham_pairs, field, gen_tab);
     } else {
         launch_bwd_pass_r<3>(gather, write_cur, write_lam, cur_in, lam_in, cur_out, lam_out,
-                             layer_coef, g, partials, grid, s, ham_pairs, field, e_partials,
-                             gen_tab);
+                             layer_coef, g, partials, grid, s, ham_pairs, field, gen_tab);
     }
 }
 
@@ -1442,7 +1390,10 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void ham_pa
     using TL = Tile<R>;
     constexpr int T = TL::T;
     __shared__ double2 lds[TL::SLOTS];
-    __shared__ double red[TL::WAVES];
+    // the scattered pass serves gradient evaluations only, which take the
+    // energy from the layer-0 contraction: no energy partials there
+    constexpr bool kEnergy = !kScat;
+    __shared__ double red[kEnergy ? TL::WAVES : 1];
     const uint32_t t = threadIdx.x;
     // Index of register k of the load mapping relative to the thread's row:
     // a uniform stride for contiguous high bits, a deposit of k's bits into
@@ -1539,7 +1490,9 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void ham_pa
                 contrib.x += d * v[k].x;
                 contrib.y += d * v[k].y;
             }
-            e += v[k].x * contrib.x + v[k].y * contrib.y;
+            if constexpr (kEnergy) {
+                e += v[k].x * contrib.x + v[k].y * contrib.y;
+            }
             if constexpr (kWriteLam) {
                 double2 out = contrib;
                 if constexpr (kReadLam) {
@@ -1550,17 +1503,19 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void ham_pa
             }
         }
     }
-    const double ws = wave_sum(e);
-    if ((t & (kWave - 1)) == 0) {
-        red[t / kWave] = ws;
-    }
-    __syncthreads();
-    if (t == 0) {
-        double acc = 0.0;
-        for (int w = 0; w < TL::WAVES; ++w) {
-            acc += red[w];
+    if constexpr (kEnergy) {
+        const double ws = wave_sum(e);
+        if ((t & (kWave - 1)) == 0) {
+            red[t / kWave] = ws;
         }
-        e_partials[blockIdx.x] = acc;
+        __syncthreads();
+        if (t == 0) {
+            double acc = 0.0;
+            for (int w = 0; w < TL::WAVES; ++w) {
+                acc += red[w];
+            }
+            e_partials[blockIdx.x] = acc;
+        }
     }
 }
 
@@ -1583,13 +1538,13 @@ static void launch_ham_pass_r(bool diag, bool read_lam, bool write_lam, const do
 }
 
 void launch_ham_pass_scattered(int R, const double2 *psi, double2 *lam_out, const PassGeom &g,
-                               double field, double *e_partials, int grid, hipStream_t s) {
+                               double field, int grid, hipStream_t s) {
     if (R == 4) {
         ham_pass_kernel<4, true, false, true, true>
-            <<<grid, Tile<4>::THREADS, 0, s>>>(psi, nullptr, lam_out, g, field, e_partials);
+            <<<grid, Tile<4>::THREADS, 0, s>>>(psi, nullptr, lam_out, g, field, nullptr);
     } else {
         ham_pass_kernel<3, true, false, true, true>
-            <<<grid, Tile<3>::THREADS, 0, s>>>(psi, nullptr, lam_out, g, field, e_partials);
+            <<<grid, Tile<3>::THREADS, 0, s>>>(psi, nullptr, lam_out, g, field, nullptr);
     }
 }
 

@@ -146,7 +146,7 @@ void launch_gen_layer1(int R, double2 *out, const CMat2 *tab, const PassGeom &g,
 // tile-bit pairs [g.pair_lo, g.pair_hi) to lam (-field * X_j X_{j+1} cur).  In
 // the pre-CNOT frame of the last layer these pairs are the X terms X_j of H;
 // their energy is not summed here (the layer-0 contraction yields the whole
-// energy once lambda = H psi is complete), e_partials receives zeros.
+// energy once lambda = H psi is complete).
 // gen_tab (layer 1, high pass on the lambda-only sweep): the state is not read
 // from cur_in but generated from a chain table set whose run is mapping 0's
 // register wires.
@@ -154,7 +154,7 @@ void launch_bwd_pass(int R, bool gather, bool write_cur, bool write_lam, const d
                      const double2 *lam_in, double2 *cur_out, double2 *lam_out,
                      const WireCoef *layer_coef, const PassGeom &g, double *partials, int grid,
                      hipStream_t s, bool ham_pairs = false, double field = 0.0,
-                     double *e_partials = nullptr, const CMat2 *gen_tab = nullptr);
+                     const CMat2 *gen_tab = nullptr);
 
 // Hamiltonian tile pass: adds the X terms of the pass's wires (and the ZZ
 // diagonal if diag) of H psi, optionally into lam, and writes per-workgroup
@@ -163,9 +163,10 @@ void launch_ham_pass(int R, bool diag, bool read_lam, bool write_lam, const doub
                      const double2 *lam_in, double2 *lam_out, const PassGeom &g, double field,
                      double *e_partials, int grid, hipStream_t s);
 // The same with a scattered tile (g.sc_n > 0): the ZZ diagonal and the X terms
-// of all 12 tile wires, written into lam.
+// of all 12 tile wires, written into lam; no energy partials (its callers take
+// the energy from the layer-0 contraction).
 void launch_ham_pass_scattered(int R, const double2 *psi, double2 *lam_out, const PassGeom &g,
-                               double field, double *e_partials, int grid, hipStream_t s);
+                               double field, int grid, hipStream_t s);
 
 // Layer-0 gradients by contracting the adjoint state with the layer-0 product
 // state: one read pass over lambda (gathered through the inverse ring CNOT
