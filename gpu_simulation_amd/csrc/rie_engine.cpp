@@ -234,6 +234,11 @@ Engine::Engine(size_t num_qubits, size_t num_layers, double field, const std::st
         d_tile_sums = (double2 *)device_alloc(sizeof(double2) * tiles, "alloc tile sums");
         d_chain = (CMat2 *)device_alloc(sizeof(CMat2) * kChainMaxSets * kChainTabEntries,
                                         "alloc chain tables");
+        if (L >= 2 && n <= kChainMaxQubits && passes.size() <= (size_t)kChainMaxSets) {
+            // per-tile uniform segments of each table set (128 B per tile and set)
+            d_chain_u = (CMat2 *)device_alloc(sizeof(CMat2) * 2 * tiles * passes.size(),
+                                              "alloc chain segments");
+        }
         high_env_chunks = (int)std::min<uint64_t>(64, (tiles + 255) / 256);
         partial2_count = 4 * (size_t)high_env_chunks * (size_t)std::max(1, n - T);
         d_partials2 = (double *)device_alloc(sizeof(double) * partial2_count, "alloc partials");
@@ -269,6 +274,7 @@ Engine::~Engine() {
     device_free(d_drift);
     device_free(d_tile_sums);
     device_free(d_chain);
+    device_free(d_chain_u);
     device_free(d_partials2);
     dense_free();
     for (auto &p : kpending) {
@@ -421,8 +427,8 @@ void Engine::structured_forward(hipStream_t s, bool save_ckpts) {
             const WireCoef *lc = d_coef + (size_t)l * n;
             if (l == 1 && gen_fwd) {
                 kt_begin(s);
-                launch_gen_layer1(pass_r[RIE_PASS_FWD_GEN], fbuf(1, K), d_chain, passes[0].g,
-                                  passes[0].grid, s);
+                launch_gen_layer1(pass_r[RIE_PASS_FWD_GEN], fbuf(1, K), d_chain, d_chain_u,
+                                  passes[0].g, passes[0].grid, s);
                 check_launch("gen_layer1");
                 kt_end(s, "fwd_layer1_gen", S);
                 in = fbuf(1, K);
@@ -487,8 +493,8 @@ void Engine::structured_forward(hipStream_t s, bool save_ckpts) {
         }
         kt_begin(s);
         if (gen_layer) {
-            launch_gen_layer1(pass_r[RIE_PASS_FWD_GEN], out, d_chain, passes[0].g, passes[0].grid,
-                              s);
+            launch_gen_layer1(pass_r[RIE_PASS_FWD_GEN], out, d_chain, d_chain_u, passes[0].g,
+                              passes[0].grid, s);
             check_launch("gen_layer1");
             kt_end(s, "fwd_layer1_gen", S);
         } else {
@@ -599,21 +605,23 @@ void Engine::launch_bwd(int l, bool gather, bool write_cur, const double2 *cur_i
     const WireCoef *lc = d_coef + (size_t)l * n;
     const PassPlan &pp = passes[p];
     const CMat2 *gen_tab = nullptr;
+    const CMat2 *gen_utab = nullptr;
     if (gen_set >= 0) {
-        if (l != 1 || write_cur || gather || gen_set >= kChainMaxSets) {
+        if (l != 1 || write_cur || gather || gen_set >= (int)passes.size()) {
             fail(RIE_DEVICE_ERROR, "internal: generated state outside layer 1's lambda sweep");
         }
         gen_tab = d_chain + (size_t)gen_set * kChainTabEntries;
+        gen_utab = d_chain_u + (size_t)gen_set * 2 * passes[0].g.num_tiles;
     }
     const bool pairs = l == fuse_layer;
     launch_bwd_pass(pass_r[gather ? RIE_PASS_BWD_LOW : RIE_PASS_BWD_HIGH], gather, write_cur, true,
                     cur_in, lam_in, cur_out, lam_out, lc, pp.g, d_partials, pp.grid, s, pairs,
-                    pairs ? h : 0.0, gen_tab);
+                    pairs ? h : 0.0, gen_tab, gen_utab);
 }
 
 bool Engine::gen_ready() const {
     return gen_enabled && strategy == RIE_STRATEGY_STRUCTURED_ADJOINT && !small && L >= 2 &&
-           n <= kChainMaxQubits && (int)passes.size() <= kChainMaxSets;
+           n <= kChainMaxQubits && (int)passes.size() <= kChainMaxSets && d_chain_u != nullptr;
 }
 
 // Table sets of the current register shapes: 0 the forward (low tile, all of
@@ -622,24 +630,56 @@ bool Engine::gen_ready() const {
 void Engine::build_chain_tables(hipStream_t s) {
     ChainSetup cs{};
     cs.n = n;
+    cs.tiles = passes[0].g.num_tiles;
     const uint64_t all = (n >= 64) ? ~uint64_t{0} : ((uint64_t{1} << n) - 1);
     const int rf = pass_r[RIE_PASS_FWD_GEN];
     const int rh = pass_r[RIE_PASS_BWD_HIGH];
-    cs.set[0] = ChainSet{T - rf, rf, all};
+    // the forward's low tile: run at the tile's top, C0 = wires 0..3, C1 the
+    // rest below the run, no U_B
+    {
+        const PassGeom &g = passes[0].g;
+        ChainSet st{};
+        st.r0 = T - rf;
+        st.R = rf;
+        st.rot = all;
+        st.c0_first = 0;
+        st.c0_len = 4;
+        st.c1_first = 4;
+        st.c1_len = T - rf - 4;
+        st.ub_first = 0;
+        st.ub_len = 0;
+        st.ta = g.a;
+        st.th0 = g.h0;
+        cs.set[0] = st;
+    }
     cs.count = 1;
+    // adjoint high pass j: run at the top of the tile's high group, C0 = the
+    // tile's low wires [0, a), U_B = [a, h0), C1 = [h0, r0)
     for (size_t p = 1; p < passes.size(); ++p) {
         const PassGeom &g = passes[p].g;
-        const uint64_t rot = all & ~((uint64_t{1} << T) - 1) &
-                             ((uint64_t{1} << passes[p].end_wire) - 1);
-        cs.set[cs.count++] = ChainSet{g.h0 + (T - rh - g.a), rh, rot};
+        ChainSet st{};
+        st.r0 = g.h0 + (T - rh - g.a);
+        st.R = rh;
+        st.rot = all & ~((uint64_t{1} << T) - 1) & ((uint64_t{1} << passes[p].end_wire) - 1);
+        st.c0_first = 0;
+        st.c0_len = g.a;
+        st.c1_first = g.h0;
+        st.c1_len = T - rh - g.a;
+        st.ub_first = g.a;
+        st.ub_len = g.h0 - g.a;
+        st.ta = g.a;
+        st.th0 = g.h0;
+        cs.set[cs.count++] = st;
     }
     for (int i = 0; i < cs.count; ++i) {
-        if (chain_chunks(n, cs.set[i].R) > kChainMaxChunks) {
-            fail(RIE_DEVICE_ERROR, "internal: chain tables too small for this size");
+        const ChainSet &st = cs.set[i];
+        if (st.c0_len < 0 || st.c0_len > 4 || st.c1_len < 0 || st.c1_len > 6 || st.ub_len < 0 ||
+            st.r0 + st.R > n || passes[i].g.num_tiles != cs.tiles) {
+            fail(RIE_DEVICE_ERROR, "internal: chain tables do not fit this plan");
         }
     }
     kt_begin(s);
-    launch_chain_tables(d_coef, d_coef + n, cs, d_chain, s);
+    launch_chain_tables(d_coef, d_coef + n, cs, d_chain, d_chain_u, s);
     check_launch("chain_tables");
     kt_end(s, "chain_tables", 0.0);
 }

@@ -105,6 +105,7 @@ struct Engine {
     bool gen_enabled = true;    // caller's switch
     bool gen_fwd = false;       // the last forward generated layer 1
     CMat2 *d_chain = nullptr;   // kChainMaxSets * kChainTabEntries matrices
+    CMat2 *d_chain_u = nullptr; // per set and tile: the uniform segments U_A, U_B
 
     // device buffers
     std::vector<double2 *> pool;  // statevector buffers (16 * 2^n bytes each)

@@ -570,7 +570,8 @@ void launch_fwd_pass(int R, bool scatter, bool gen_init, const double2 *in, doub
 // ---------------------------------------------------------------------------
 
 // One workgroup per table set: the 2n site matrices in LDS, then the
-// register-run products tab[0, 2^R) and the chunk products tab[16 + 16c + v].
+// register-run products tab[0, 2^R) and the chunk products C0 at tab[16 + v]
+// and C1 at tab[32 + v].
 __global__ __launch_bounds__(256) void chain_tables_kernel(const WireCoef *__restrict__ lc0,
                                                            const WireCoef *__restrict__ lc1,
                                                            ChainSetup cs, CMat2 *__restrict__ tabs) {
@@ -583,66 +584,104 @@ __global__ __launch_bounds__(256) void chain_tables_kernel(const WireCoef *__res
     }
     __syncthreads();
     CMat2 *tab = tabs + (size_t)blockIdx.x * kChainTabEntries;
-    const int chunks = chain_chunks(n, st.R);
     for (int e = (int)threadIdx.x; e < kChainTabEntries; e += (int)blockDim.x) {
         int first;
         int len;
         int v;
-        if (e < 16) {
-            if (e >= (1 << st.R)) {
-                continue;
-            }
+        if (e < kChainC0) {
             first = st.r0;
             len = st.R;
             v = e;
+        } else if (e < kChainC1) {
+            first = st.c0_first;
+            len = st.c0_len;
+            v = e - kChainC0;
         } else {
-            const int c = (e - 16) >> 4;
-            v = (e - 16) & 15;
-            if (c >= chunks) {
-                continue;
-            }
-            first = (st.r0 + st.R + 4 * c) % n;
-            len = min(4, n - st.R - 4 * c);
-            if (v >= (1 << len)) {
-                continue;
-            }
+            first = st.c1_first;
+            len = st.c1_len;
+            v = e - kChainC1;
+        }
+        if (len == 0 || v >= (1 << len)) {
+            continue;
         }
         CMat2 p = site[2 * first + (v & 1)];
         for (int b = 1; b < len; ++b) {
-            const int w = (first + b) % n;
+            const int w = first + b;
             p = cmat_mul(p, site[2 * w + ((v >> b) & 1)]);
         }
         tab[e] = p;
     }
 }
 
-void launch_chain_tables(const WireCoef *lc0, const WireCoef *lc1, const ChainSetup &cs,
-                         CMat2 *tabs, hipStream_t s) {
-    chain_tables_kernel<<<cs.count, 256, 0, s>>>(lc0, lc1, cs, tabs);
+// Product of the site matrices of wires [first, first + len) at the bits of
+// index i (the identity when len is 0).
+__device__ __forceinline__ CMat2 chain_segment(const CMat2 *site, int first, int len,
+                                               uint64_t i) {
+    CMat2 p = {{1.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}, {1.0, 0.0}};
+    for (int b = 0; b < len; ++b) {
+        const int w = first + b;
+        const CMat2 &m = site[2 * w + (int)((i >> w) & 1u)];
+        p = (b == 0) ? m : cmat_mul(p, m);
+    }
+    return p;
 }
 
-// Product of the chunk matrices at global index i, in ring order after the
-// register run [r0, r0 + R): chunk c reads index bits (r0 + R + 4c + b) mod n
-// (the run's own bits of i are not read).  chunks: the chunk products,
-// chunk c at chunks[16c + v] (a table set from its entry 16 on, or an LDS copy).
-template <int R>
-__device__ __forceinline__ CMat2 chain_y(const CMat2 *__restrict__ chunks_tab, uint64_t i,
-                                         int r0, int n) {
-    const int chunks = chain_chunks(n, R);
-    CMat2 y = {};
-#pragma unroll
-    for (int c = 0; c < kChainMaxChunks; ++c) {
-        if (c < chunks) {
-            const int w0 = (r0 + R + 4 * c) % n;
-            const int len = min(4, n - R - 4 * c);
-            // bits w0, w0 + 1, ... of i, wrapping from n - 1 to 0
-            const uint32_t v =
-                (uint32_t)(((i >> w0) | (i << (n - w0))) & ((uint64_t{1} << len) - 1));
-            const CMat2 m = chunks_tab[16 * c + (int)v];
-            y = (c == 0) ? m : cmat_mul(y, m);
-        }
+// The tile-uniform segments of every tile: utab[2g] = U_A (wires [r0 + R, n)),
+// utab[2g + 1] = U_B, at the bits of tile g's base index.  grid.y: table set.
+__global__ __launch_bounds__(256) void chain_uniform_kernel(const WireCoef *__restrict__ lc0,
+                                                            const WireCoef *__restrict__ lc1,
+                                                            ChainSetup cs,
+                                                            CMat2 *__restrict__ utab) {
+    __shared__ CMat2 site[2 * 64];
+    const ChainSet st = cs.set[blockIdx.y];
+    const int n = cs.n;
+    for (int i = (int)threadIdx.x; i < 2 * n; i += (int)blockDim.x) {
+        const int w = i >> 1;
+        site[i] = chain_site(w, i & 1, lc0[w], lc1[w], ((st.rot >> w) & 1u) != 0);
     }
-    return y;
+    __syncthreads();
+    const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= cs.tiles) {
+        return;
+    }
+    PassGeom G{};
+    G.a = st.ta;
+    G.h0 = st.th0;
+    const uint64_t base = tile_base(g, G, Tile<4>::T);
+    CMat2 *u = utab + ((size_t)blockIdx.y * cs.tiles + g) * 2;
+    u[0] = chain_segment(site, st.r0 + st.R, n - st.r0 - st.R, base);
+    u[1] = chain_segment(site, st.ub_first, st.ub_len, base);
+}
+
+void launch_chain_tables(const WireCoef *lc0, const WireCoef *lc1, const ChainSetup &cs,
+                         CMat2 *tabs, CMat2 *utab, hipStream_t s) {
+    chain_tables_kernel<<<cs.count, 256, 0, s>>>(lc0, lc1, cs, tabs);
+    const dim3 grid((unsigned)((cs.tiles + 255) / 256), (unsigned)cs.count);
+    chain_uniform_kernel<<<grid, 256, 0, s>>>(lc0, lc1, cs, utab);
+}
+
+// Y of the forward's low tile at row i (run bits zero): U_A . C0 . C1 with C0
+// the index bits 0..3 and C1 the bits 4 .. T - R - 1.
+template <int R>
+__device__ __forceinline__ CMat2 chain_y_low(const CMat2 *__restrict__ tab,
+                                             const CMat2 *__restrict__ u, uint64_t i) {
+    constexpr int kC1Bits = Tile<R>::T - R - 4;
+    const CMat2 c0 = tab[kChainC0 + (int)(i & 15u)];
+    const CMat2 c1 = tab[kChainC1 + (int)((i >> 4) & ((1u << kC1Bits) - 1u))];
+    return cmat_mul(cmat_mul(u[0], c0), c1);
+}
+
+// Y of an adjoint high pass's tile at row i (run bits zero):
+// U_A . C0 . U_B . C1 with C0 the index bits [0, a) and C1 the bits
+// [h0, h0 + T - R - a); chunks: C0 at chunks[0, 16), C1 at chunks[16, 80).
+template <int R>
+__device__ __forceinline__ CMat2 chain_y_high(const CMat2 *chunks, const CMat2 *__restrict__ u,
+                                              uint64_t i, const PassGeom &G) {
+    const uint32_t v0 = (uint32_t)i & ((1u << G.a) - 1u);
+    const uint32_t v1 = (uint32_t)(i >> G.h0) & ((1u << (Tile<R>::T - R - G.a)) - 1u);
+    CMat2 y = cmat_mul(u[0], chunks[v0]);
+    y = cmat_mul(y, u[1]);
+    return cmat_mul(y, chunks[kChainC1 - kChainC0 + v1]);
 }
 
 // v[k] = Tr(run[k] Y): register bit b is the run's wire r0 + b.
@@ -660,7 +699,7 @@ __device__ __forceinline__ void chain_fill(double2 (&v)[Tile<R>::REGS],
 // ring CNOT through LDS.
 template <int R>
 __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void gen_layer1_kernel(
-    double2 *out, const CMat2 *__restrict__ tab, PassGeom G) {
+    double2 *out, const CMat2 *__restrict__ tab, const CMat2 *__restrict__ utab, PassGeom G) {
     using TL = Tile<R>;
     __shared__ double2 lds[TL::SLOTS];
     const uint32_t t = threadIdx.x;
@@ -668,19 +707,19 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void gen_la
         uint32_t tl = t;
         asm volatile("" : "+v"(tl));
         const uint64_t base = tile_base(g, G, TL::T);
-        const CMat2 y = chain_y<R>(tab + 16, base + omap<R, 0>(tl, 0), TL::T - R, G.n);
+        const CMat2 y = chain_y_low<R>(tab, utab + 2 * g, base + omap<R, 0>(tl, 0));
         double2 v[TL::REGS];
         chain_fill<R>(v, tab, y);
         store_ring_scatter<R, 0>(v, lds, tl, base, G, out);
     }
 }
 
-void launch_gen_layer1(int R, double2 *out, const CMat2 *tab, const PassGeom &g, int grid,
-                       hipStream_t s) {
+void launch_gen_layer1(int R, double2 *out, const CMat2 *tab, const CMat2 *utab,
+                       const PassGeom &g, int grid, hipStream_t s) {
     if (R == 4) {
-        gen_layer1_kernel<4><<<grid, Tile<4>::THREADS, 0, s>>>(out, tab, g);
+        gen_layer1_kernel<4><<<grid, Tile<4>::THREADS, 0, s>>>(out, tab, utab, g);
     } else {
-        gen_layer1_kernel<3><<<grid, Tile<3>::THREADS, 0, s>>>(out, tab, g);
+        gen_layer1_kernel<3><<<grid, Tile<3>::THREADS, 0, s>>>(out, tab, utab, g);
     }
 }
 
@@ -757,7 +796,8 @@ __device__ __forceinline__ void add_pair_terms(const double2 (&c)[Tile<R>::REGS]
 // a high pass reads mapping 0.  With kHam the pair terms of that first
 // window are added.
 // kGen (high passes): the state is generated in mapping 0 from the chain table
-// set gen_tab (its chunk products copied to gen_chunks in LDS) instead of read.
+// set gen_tab (its chunk products copied to gen_chunks in LDS) and the tile's
+// uniform segments gen_u instead of read.
 template <int R, bool kGather, bool kHam, bool kGen = false>
 __device__ __forceinline__ void bwd_load_tile(double2 (&c)[Tile<R>::REGS],
                                               double2 (&l)[Tile<R>::REGS],
@@ -766,7 +806,8 @@ __device__ __forceinline__ void bwd_load_tile(double2 (&c)[Tile<R>::REGS],
                                               const PassGeom &G, double2 *lds, uint32_t t,
                                               double field,
                                               const CMat2 *__restrict__ gen_tab = nullptr,
-                                              const CMat2 *gen_chunks = nullptr) {
+                                              const CMat2 *gen_chunks = nullptr,
+                                              const CMat2 *__restrict__ gen_u = nullptr) {
     using TL = Tile<R>;
     constexpr int T = TL::T;
     if constexpr (kGather) {
@@ -818,7 +859,7 @@ __device__ __forceinline__ void bwd_load_tile(double2 (&c)[Tile<R>::REGS],
             }
         }
         if constexpr (kGen) {
-            const CMat2 y = chain_y<R>(gen_chunks, row, tile_wire(T - R, G), G.n);
+            const CMat2 y = chain_y_high<R>(gen_chunks, gen_u, row, G);
             chain_fill<R>(c, gen_tab, y);
         }
         if constexpr (kHam) {
@@ -1141,7 +1182,7 @@ template <int R, bool kGather, bool kHam, bool kGen>
 __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_lam_kernel(
     const double2 *cur_in, const double2 *lam_in, double2 *lam_out,
     const WireCoef *__restrict__ lc, PassGeom G, double *__restrict__ partials, double field,
-    const CMat2 *__restrict__ gen_tab) {
+    const CMat2 *__restrict__ gen_tab, const CMat2 *__restrict__ gen_utab) {
     using TL = Tile<R>;
     constexpr int T = TL::T;
     __shared__ double2 lds[TL::SLOTS];
@@ -1153,7 +1194,7 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_la
     __shared__ WireCoef tile_coef[T];
     // chunk products of the generating table set: read at every tile start,
     // where LDS latency is short next to L2 behind the streaming traffic
-    __shared__ CMat2 gen_chunks[kGen ? 16 * kChainMaxChunks : 1];
+    __shared__ CMat2 gen_chunks[kGen ? kChainTabEntries - kChainC0 : 1];
     const uint32_t t = threadIdx.x;
     const int lane = t & (kWave - 1);
     double *row_acc = acc[t / 16];
@@ -1166,9 +1207,8 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_la
         tile_coef[t] = rot ? lc[tile_wire((int)t, G)] : WireCoef{1.0, 0.0, 1.0, 0.0};
     }
     if constexpr (kGen) {
-        const int entries = 16 * chain_chunks(G.n, R);
-        for (int i = (int)t; i < entries; i += TL::THREADS) {
-            gen_chunks[i] = gen_tab[16 + i];
+        for (int i = (int)t; i < kChainTabEntries - kChainC0; i += TL::THREADS) {
+            gen_chunks[i] = gen_tab[kChainC0 + i];
         }
     }
     __syncthreads();
@@ -1184,7 +1224,7 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_la
         double2 c[TL::REGS];
         double2 l[TL::REGS];
         bwd_load_tile<R, kGather, kHam, kGen>(c, l, cur_in, lam_in, base, ta, G, lds, tl, field,
-                                              gen_tab, gen_chunks);
+                                              gen_tab, gen_chunks, gen_utab + 2 * g);
         // sweep 1: phases and gradients (lam exchanged before cur, so the LDS
         // holds cur for the pair terms); sweep 2: RY^dagger on lam
         if constexpr (kGather) {
@@ -1320,9 +1360,15 @@ template <int R>
 static void launch_bwd_pass_r(bool gather, bool write_cur, bool write_lam, const double2 *cur_in,
                               const double2 *lam_in, double2 *cur_out, double2 *lam_out,
                               const WireCoef *lc, const PassGeom &g, double *partials, int grid,
-                              hipStream_t s, bool ham, double field, const CMat2 *gen_tab) {
+                              hipStream_t s, bool ham, double field, const CMat2 *gen_tab,
+                              const CMat2 *gen_utab) {
     constexpr int TH = Tile<R>::THREADS;
     const bool gen = gen_tab != nullptr;
+    if (gen && (gen_utab == nullptr || gather || g.a > 4 || Tile<R>::T - R - g.a > 6 ||
+                Tile<R>::T - R - g.a < 0)) {
+        // chunk C0 holds up to 4 tile bits, C1 up to 6
+        throw std::invalid_argument("generated state: high pass geometry outside the chain tables");
+    }
     if (!write_cur && write_lam) {  // the state is not rewritten: lambda-only sweep
         if (!gather && R == 4 && (g.a != 3 || g.rot_lo < 3)) {
             // the sweep takes tile bit 3 across lanes and no other bit of window 1
@@ -1331,7 +1377,7 @@ static void launch_bwd_pass_r(bool gather, bool write_cur, bool write_lam, const
 #define RIE_BWD_LAM(GA, HM, GN)                                                              \
     if (gather == GA && ham == HM && gen == GN) {                                            \
         bwd_lam_kernel<R, GA, HM, GN><<<grid, TH, 0, s>>>(cur_in, lam_in, lam_out, lc, g,    \
-                                                          partials, field, gen_tab);         \
+                                                          partials, field, gen_tab, gen_utab); \
         return;                                                                              \
     }
         RIE_BWD_LAM(true, false, false)
@@ -1369,13 +1415,16 @@ static void launch_bwd_pass_r(bool gather, bool write_cur, bool write_lam, const
 void launch_bwd_pass(int R, bool gather, bool write_cur, bool write_lam, const double2 *cur_in,
                      const double2 *lam_in, double2 *cur_out, double2 *lam_out,
                      const WireCoef *layer_coef, const PassGeom &g, double *partials, int grid,
-                     hipStream_t s, bool ham_pairs, double field, const CMat2 *gen_tab) {
+                     hipStream_t s, bool ham_pairs, double field, const CMat2 *gen_tab,
+                     const CMat2 *gen_utab) {
     if (R == 4) {
         launch_bwd_pass_r<4>(gather, write_cur, write_lam, cur_in, lam_in, cur_out, lam_out,
-                             layer_coef, g, partials, grid, s, ham_pairs, field, gen_tab);
+                             layer_coef, g, partials, grid, s, ham_pairs, field, gen_tab,
+                             gen_utab);
     } else {
         launch_bwd_pass_r<3>(gather, write_cur, write_lam, cur_in, lam_in, cur_out, lam_out,
-                             layer_coef, g, partials, grid, s, ham_pairs, field, gen_tab);
+                             layer_coef, g, partials, grid, s, ham_pairs, field, gen_tab,
+                             gen_utab);
     }
 }
 

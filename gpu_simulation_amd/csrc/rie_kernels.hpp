@@ -109,34 +109,52 @@ void launch_fwd_pass(int R, bool scatter, bool gen_init, const double2 *in, doub
                      const WireCoef *layer_coef, const WireCoef *layer0_coef, const PassGeom &g,
                      int grid, hipStream_t s);
 
-// Layer-1 states from the parameters (chain_site in rie_device.hpp).  A table
-// set serves one pass: tab[0, 2^R) are the products of the register run's
-// site matrices (wires r0 .. r0 + R - 1; register bit b is wire r0 + b) and
-// tab[16 + 16c + v] the products of chunk c, the up to 4 wires
-// (r0 + R + 4c + b) mod n that follow the run around the ring, at bit values
-// v.  rot: the wires of layer 1 already rotated in the generated state.
-constexpr int kChainMaxChunks = 7;
-constexpr int kChainTabEntries = 16 + 16 * kChainMaxChunks;
+// Layer-1 states from the parameters (chain_site in rie_device.hpp): the
+// state at index i is Tr(run[i's run bits] Y(i)), where the run is the pass's
+// register wires r0 .. r0 + R - 1 and Y(i) the product of the other site
+// matrices around the ring from wire r0 + R.  In a pass's tile that product
+// splits into tile-uniform segments and the thread's own bits:
+//   forward (low tile, run at the tile's top):
+//     Y = U_A . C0[bits 0..3] . C1[bits 4 .. r0 - 1]
+//   adjoint high pass (tile bits [0, a) and [h0, h0 + T - a), run at the top):
+//     Y = U_A . C0[bits 0 .. a - 1] . U_B . C1[bits h0 .. r0 - 1]
+// with U_A the wires [r0 + R, n) and U_B the wires [a, h0), both fixed by the
+// tile number.  A table set: tab[0, 2^R) the run products (register bit b is
+// wire r0 + b), tab[16 + v] chunk C0 and tab[32 + v] chunk C1 at bit values
+// v; utab[2g], utab[2g + 1] U_A and U_B of tile g (identity when empty).
+// rot: the wires of layer 1 already rotated in the generated state.
+constexpr int kChainC0 = 16;             // entry of chunk C0 (up to 4 bits)
+constexpr int kChainC1 = 32;             // entry of chunk C1 (up to 6 bits)
+constexpr int kChainTabEntries = 96;
 constexpr int kChainMaxSets = 6;
-constexpr int kChainMaxQubits = 31;  // chain_chunks(n, 3) <= kChainMaxChunks
+constexpr int kChainMaxQubits = 31;
 struct ChainSet {
     int r0;
     int R;
     uint64_t rot;
+    int c0_first;  // chunk C0: wires [c0_first, c0_first + c0_len)
+    int c0_len;
+    int c1_first;  // chunk C1
+    int c1_len;
+    int ub_first;  // segment U_B (U_A is [r0 + R, n))
+    int ub_len;
+    int ta;        // the pass's tile geometry (PassGeom a and h0)
+    int th0;
 };
 struct ChainSetup {
     int n;
     int count;
+    uint64_t tiles;  // tiles per pass (every pass has 2^(n - 12))
     ChainSet set[kChainMaxSets];
 };
-__host__ __device__ constexpr int chain_chunks(int n, int R) { return (n - R + 3) / 4; }
-// tabs: count * kChainTabEntries matrices, set i at tabs + i * kChainTabEntries.
+// tabs: count * kChainTabEntries matrices, set i at tabs + i * kChainTabEntries;
+// utab: count * tiles * 2 matrices, set i at utab + i * tiles * 2.
 void launch_chain_tables(const WireCoef *lc0, const WireCoef *lc1, const ChainSetup &cs,
-                         CMat2 *tabs, hipStream_t s);
+                         CMat2 *tabs, CMat2 *utab, hipStream_t s);
 // Layer 1 (every wire rotated) and its ring CNOT as one write pass over the
-// low tiles; tab: a set with r0 = 12 - R.
-void launch_gen_layer1(int R, double2 *out, const CMat2 *tab, const PassGeom &g, int grid,
-                       hipStream_t s);
+// low tiles; tab, utab: a set with r0 = 12 - R.
+void launch_gen_layer1(int R, double2 *out, const CMat2 *tab, const CMat2 *utab,
+                       const PassGeom &g, int grid, hipStream_t s);
 
 // Adjoint tile pass: un-rotates the pass's wires on (cur, lam) and writes
 // per-workgroup gradient partials [grid][2T].  gather = apply the inverse
@@ -147,14 +165,14 @@ void launch_gen_layer1(int R, double2 *out, const CMat2 *tab, const PassGeom &g,
 // the pre-CNOT frame of the last layer these pairs are the X terms X_j of H;
 // their energy is not summed here (the layer-0 contraction yields the whole
 // energy once lambda = H psi is complete).
-// gen_tab (layer 1, high pass on the lambda-only sweep): the state is not read
-// from cur_in but generated from a chain table set whose run is mapping 0's
-// register wires.
+// gen_tab, gen_utab (layer 1, high pass on the lambda-only sweep): the state
+// is not read from cur_in but generated from a chain table set whose run is
+// mapping 0's register wires, and that set's per-tile segments.
 void launch_bwd_pass(int R, bool gather, bool write_cur, bool write_lam, const double2 *cur_in,
                      const double2 *lam_in, double2 *cur_out, double2 *lam_out,
                      const WireCoef *layer_coef, const PassGeom &g, double *partials, int grid,
                      hipStream_t s, bool ham_pairs = false, double field = 0.0,
-                     const CMat2 *gen_tab = nullptr);
+                     const CMat2 *gen_tab = nullptr, const CMat2 *gen_utab = nullptr);
 
 // Hamiltonian tile pass: adds the X terms of the pass's wires (and the ZZ
 // diagonal if diag) of H psi, optionally into lam, and writes per-workgroup

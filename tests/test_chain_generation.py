@@ -4,8 +4,10 @@ The engine generates every state inside layer 1 (before its ring CNOT) as
 Tr(M_0[q_0] M_1[q_1] ... M_{n-1}[q_{n-1}]), 2x2 complex matrices per wire built
 from the layer-0 and layer-1 coefficients (``chain_site`` in
 ``gpu_simulation_amd/csrc/rie_device.hpp``), and evaluates the product per
-tile as a register-run table times the ordered product of 4-wire chunk
-tables (``chain_tables_kernel`` / ``chain_y`` / ``chain_fill``).  These tests
+tile as a register-run table times Y = U_A C0 U_B C1: two tile-uniform
+segments (``chain_uniform_kernel``, U_B the identity in the forward's tile)
+and two tables over the thread's own index bits (``chain_tables_kernel``,
+``chain_y_low`` / ``chain_y_high``, ``chain_fill``).  These tests
 restate both in numpy and compare with the oracle's layer-1 states: the state
 after layer 0 (oracle, one layer) rotated on a wire subset, and P^-1 applied
 to the oracle's two-layer state.  The GPU kernels themselves are checked
@@ -84,50 +86,67 @@ def _omap(R, M, t, k):
     return ((t >> lo) << (lo + R)) | (k << lo) | (t & ((1 << lo) - 1))
 
 
-def _tables(n, r0, R, rot, l0, l1):
+def _prod(sites, first, length, v):
+    """Product of the site matrices of wires first .. first + length - 1 at the
+    bits of v (bit b is wire first + b)."""
+    m = np.eye(2)
+    for b in range(length):
+        m = m @ sites[first + b][(v >> b) & 1]
+    return m
+
+
+def _set(n, R, ta, th0, rot, l0, l1):
+    """A table set as the engine lays it out for a pass whose tile is the index
+    bits [0, ta) and [th0, th0 + T - ta), run at the tile's top wires: the run
+    products, the thread chunks C0 and C1 (as (first, length, table)), and the
+    uniform segments U_A = [r0 + R, n) and U_B (as (first, length))."""
     sites = [[_site(w, q, l0, l1, (rot >> w) & 1) for q in (0, 1)] for w in range(n)]
-
-    def prod(first, length, v):
-        m = np.eye(2)
-        for b in range(length):
-            m = m @ sites[(first + b) % n][(v >> b) & 1]
-        return m
-
-    run = [prod(r0, R, v) for v in range(1 << R)]
-    chunks = []
-    for c in range((n - R + 3) // 4):
-        first, length = (r0 + R + 4 * c) % n, min(4, n - R - 4 * c)
-        chunks.append((first, length, [prod(first, length, v) for v in range(1 << length)]))
-    return run, chunks
+    if ta == T:  # the forward's low tile
+        r0 = T - R
+        c0, c1, ub = (0, 4), (4, T - R - 4), (0, 0)
+    else:  # an adjoint high pass
+        r0 = th0 + (T - R - ta)
+        c0, c1, ub = (0, ta), (th0, T - R - ta), (ta, th0 - ta)
+    run = [_prod(sites, r0, R, v) for v in range(1 << R)]
+    chunks = [(f, ln, [_prod(sites, f, ln, v) for v in range(1 << ln)]) for f, ln in (c0, c1)]
+    return sites, r0, run, chunks, (r0 + R, n - r0 - R), ub
 
 
-def _chain_y(chunks, i, n):
-    y = np.eye(2)
-    for first, length, tab in chunks:
-        v = ((i >> first) | (i << (n - first))) & ((1 << length) - 1)
-        y = y @ tab[v]
-    return y
+def _uniform(sites, seg, base):
+    first, length = seg
+    return _prod(sites, first, length, base >> first)
 
 
-@pytest.mark.parametrize("n", [13, 16])
+def _chain_y(sites, chunks, ua, ub, i, base):
+    """Y(i) = U_A . C0 . U_B . C1 (U_B the identity for the forward's tile)."""
+    (f0, n0, t0), (f1, n1, t1) = chunks
+    y = _uniform(sites, ua, base) @ t0[(i >> f0) & ((1 << n0) - 1)]
+    y = y @ _uniform(sites, ub, base)
+    return y @ t1[(i >> f1) & ((1 << n1) - 1)]
+
+
+@pytest.mark.parametrize("n", [13, 16, 22])
 @pytest.mark.parametrize("R", [3, 4])
 def test_tile_tables_reproduce_states(oracle, n, R):
-    # the forward's low tile (run = register bits of mapping 0, every wire
-    # rotated) and the adjoint high pass (run = mapping 0's register wires,
-    # the high group rotated), on sampled tiles and threads
+    # the forward's low tile (every wire rotated) and the adjoint high passes
+    # (the high group's wires rotated), on sampled tiles and threads
     rng = np.random.default_rng(60 + n + R)
     params = rng.uniform(0.0, 2.0 * np.pi, size=(2, n, 2))
     l0, l1 = _coefs(params[0]), _coefs(params[1])
     full = (1 << n) - 1
-    a, h0 = 3, min(T, n - 9)  # the first high group, placed as the engine plans it
-    cases = [
-        # (tile low bits a, first high index bit, run start r0, rotated wires)
-        (T, T, T - R, full),
-        (a, h0, h0 + (T - R - a), full & ~((1 << T) - 1)),
-    ]
-    for ta, th0, r0, rot in cases:
+    a = 3
+    cases = [(T, T, full)]  # (tile low bits, first high index bit, rotated wires)
+    # the high groups as the engine plans them: 9 wires each from wire 12,
+    # the last one ending at wire n - 1
+    g0 = T
+    while g0 < n:
+        h0 = min(g0, n - (T - a))
+        end = min(g0 + (T - a), n)
+        cases.append((a, h0, full & ~((1 << g0) - 1) & ((1 << end) - 1)))
+        g0 = end
+    for ta, th0, rot in cases:
         want = _rotated_reference(oracle, n, params, rot)
-        run, chunks = _tables(n, r0, R, rot, l0, l1)
+        sites, r0, run, chunks, ua, ub = _set(n, R, ta, th0, rot, l0, l1)
         gap = th0 - ta
         err = 0.0
         for g in rng.integers(0, 1 << (n - T), 4):
@@ -139,7 +158,7 @@ def test_tile_tables_reproduce_states(oracle, n, R):
                 def index(o):
                     return base | (o & ((1 << ta) - 1)) | ((o >> ta) << th0)
 
-                y = _chain_y(chunks, index(_omap(R, 0, t, 0)), n)
+                y = _chain_y(sites, chunks, ua, ub, index(_omap(R, 0, t, 0)), base)
                 for k in range(1 << R):
                     err = max(err, abs(np.trace(run[k] @ y) - want[index(_omap(R, 0, t, k))]))
-        assert err <= 1e-15, (ta, r0)
+        assert err <= 1e-14, (ta, th0, r0)
