@@ -136,14 +136,17 @@ def test_tile_tables_reproduce_states(oracle, n, R):
     full = (1 << n) - 1
     a = 3
     cases = [(T, T, full)]  # (tile low bits, first high index bit, rotated wires)
-    # the high groups as the engine plans them: 9 wires each from wire 12,
-    # the last one ending at wire n - 1
+    # the high groups as the engine plans them: the n - 12 high wires split
+    # into ceil((n - 12) / 9) groups of near-equal size, each tile's high
+    # part 9 wires from h0 = min(g0, n - 9)
+    rest, width = n - T, T - a
+    groups = (rest + width - 1) // width
     g0 = T
-    while g0 < n:
-        h0 = min(g0, n - (T - a))
-        end = min(g0 + (T - a), n)
-        cases.append((a, h0, full & ~((1 << g0) - 1) & ((1 << end) - 1)))
-        g0 = end
+    for gi in range(groups):
+        bg = rest // groups + (1 if gi < rest % groups else 0)
+        h0 = min(g0, n - width)
+        cases.append((a, h0, full & ~((1 << g0) - 1) & ((1 << (g0 + bg)) - 1)))
+        g0 += bg
     for ta, th0, rot in cases:
         want = _rotated_reference(oracle, n, params, rot)
         sites, r0, run, chunks, ua, ub = _set(n, R, ta, th0, rot, l0, l1)
