@@ -39,6 +39,28 @@ if str(ROOT) not in sys.path:
 HBM_PEAK_GBS = 8000.0
 METRIC = "energy_and_grad evals/s, % HBM roofline; batched vs single-launch @|\u0394E|\u22641e-10"
 TRAFFIC_FILE = ROOT / "profiles" / "hbm_traffic_structured_30q2l.json"
+PARITY_TOL = 1e-10
+
+
+def fixture_parity(n: int, layers: int, seed: int, host_params: np.ndarray, energy: float,
+                   grad: np.ndarray) -> dict | None:
+    """Error of the timed evaluation against the CPU oracle's gate-level adjoint
+    for the same draw (tests/golden/c5_<n>q<L>l.json, written by
+    tests/golden/make_c5_fixture.py), or None when no fixture covers the case."""
+    import hashlib
+
+    path = ROOT / "tests" / "golden" / f"c5_{n}q{layers}l.json"
+    if not path.exists():
+        return None
+    data = json.loads(path.read_text())
+    digest = hashlib.sha256(np.ascontiguousarray(host_params, dtype="<f8").tobytes()).hexdigest()
+    if data.get("seed") != seed or data.get("params_sha256") != digest or data.get("field") != 1.0:
+        return None
+    de = abs(energy - float(data["energy"]))
+    dg = float(np.max(np.abs(np.asarray(grad) - np.asarray(data["gradient"]))))
+    return {"reference": f"CPU oracle gate-level adjoint, {path.relative_to(ROOT)}",
+            "energy_ref": float(data["energy"]), "max_abs_dE": de, "max_abs_dgrad": dg,
+            "tol": PARITY_TOL, "within_tol": bool(de <= PARITY_TOL and dg <= PARITY_TOL)}
 
 
 def parse_args() -> argparse.Namespace:
@@ -256,6 +278,9 @@ def main() -> int:
     engine.set_kernel_timing(False)
     stats = engine.kernel_stats()
     stats_schedule = engine.schedule
+    # results of the last timed evaluation, checked below
+    e_timed = float(energy.item())
+    g_timed = grad.cpu().numpy()
 
     # the other schedule, timed the same way, for reference
     other = "inverse_walk" if engine.schedule == "stored" else "stored"
@@ -274,9 +299,10 @@ def main() -> int:
     except (RuntimeError, MemoryError, ValueError):
         other_ms = None
 
-    e_val = float(energy.item())
-    if not (abs(e_val) <= n * 2.0 + 1e-9 and torch.isfinite(grad).all()):
+    e_val = e_timed
+    if not (abs(e_val) <= n * 2.0 + 1e-9 and np.isfinite(g_timed).all()):
         raise RuntimeError(f"implausible result: energy {e_val}")
+    parity = fixture_parity(n, layers, args.seed, host_params, e_val, g_timed)
 
     ms_per_step = elapsed * 1e3 / args.steps
     kernels = []
@@ -347,6 +373,7 @@ def main() -> int:
         "roofline": roofline,
         "kernels": kernels,
         "energy": e_val,
+        "parity": parity,
     }
     if not args.no_batched:
         result["batched"] = batched_mix(args.batch_count, args.steps, torch)

@@ -44,6 +44,46 @@ def test_uneven_batch_matches_oracle(oracle, dense_max):
     assert np.max(np.abs(e_only - energies)) <= 1e-12
 
 
+def test_benchmark_mix_2048_every_size_and_depth(oracle):
+    # the benchmark's own workload: 2048 circuits, n ~ U{4..14},
+    # L ~ U{8, 32, 128}, h ~ U[0.2, 2], n <= 8 on dense_scan; all of them in
+    # one batch, and for every (n, L) pair its first circuit checked against
+    # the oracle: 128-leaf product trees at n <= 8, 13 and 14 qubits on the
+    # whole-circuit kernel's scratch state at 128 layers
+    specs, params = uneven_mix(2048, seed=3)
+    first = {}
+    for i, s in enumerate(specs):
+        first.setdefault((s.num_qubits, s.layers), i)
+    assert sorted(first) == [(n, L) for n in range(4, 15) for L in (8, 32, 128)]
+    picked = [first[key] for key in sorted(first)]
+    batch = RingIsingBatch(specs, dense_max_qubits=8)
+    assert sum(st == "dense_scan" for st in batch.strategies) == sum(
+        s.num_qubits <= 8 for s in specs)
+    refs = {i: oracle.energy_and_grad(specs[i].num_qubits, specs[i].layers, specs[i].field,
+                                      params[i]) for i in picked}
+
+    def check(label, energies, grads):
+        worst_e = worst_g = 0.0
+        for i in picked:
+            e_ref, g_ref = refs[i]
+            worst_e = max(worst_e, abs(energies[i] - e_ref))
+            worst_g = max(worst_g, float(np.max(np.abs(np.ravel(grads[i]) - g_ref))))
+        print(f"  {label}: max|dE| = {worst_e:.3e}  max|dgrad| = {worst_g:.3e}", flush=True)
+        assert worst_e <= TOL, (label, worst_e)
+        assert worst_g <= TOL, (label, worst_g)
+
+    print("case mix 2048 default configuration", flush=True)
+    energies, grads = batch.energy_and_grad(params)
+    check("default", energies, grads)
+    print("case mix 2048 autotuned, graph replay", flush=True)
+    batch.autotune(repeats=1)
+    batch.native.set_use_graph(True)
+    tuned_e, tuned_g = batch.energy_and_grad(params)
+    batch.native.set_use_graph(False)
+    check("autotuned", tuned_e, tuned_g)
+    assert np.max(np.abs(tuned_e - energies)) <= 1e-12
+
+
 def test_one_circuit_per_qubit_count_and_chunk_width(oracle, backend_module):
     # every qubit count 2..14 with every rotation-chunk width the kernel has
     specs = [CircuitSpec(n, 3, 0.7 + 0.05 * n) for n in range(2, 15)]
@@ -162,11 +202,11 @@ def test_plain_method_12q_3l_descent(oracle, backend_module):
     p = p0.reshape(-1).copy()
     for k in range(steps):
         out = eng.energy_and_grad(p)
-        assert abs(out["energy"] - h_ref[k]) <= 1e-9, k
+        assert abs(out["energy"] - h_ref[k]) <= TOL, k
         p = p - lr * np.asarray(out["gradient"])
-    assert np.max(np.abs(p - p_ref.reshape(-1))) <= 1e-9
+    assert np.max(np.abs(p - p_ref.reshape(-1))) <= TOL
     batch = RingIsingBatch([spec])
     batch.set_params([p0])
     hist = batch.optimize("gd", lr, steps)
-    assert np.max(np.abs(hist[:, 0] - h_ref)) <= 1e-9
-    assert np.max(np.abs(np.ravel(batch.get_params()[0]) - np.ravel(p_ref))) <= 1e-9
+    assert np.max(np.abs(hist[:, 0] - h_ref)) <= TOL
+    assert np.max(np.abs(np.ravel(batch.get_params()[0]) - np.ravel(p_ref))) <= TOL

@@ -297,9 +297,11 @@ def test_gate_level_strategy(oracle, backend_module, n, layers):
     _check(oracle, backend_module, n, layers, 0.9, "inverse_walk_cuQuantum", 2028 + n)
 
 
-@pytest.mark.parametrize("n,layers", [(3, 1), (5, 3), (6, 32), (7, 5), (8, 8), (8, 16)])
+@pytest.mark.parametrize("n,layers", [(3, 1), (5, 3), (6, 32), (7, 5), (8, 8), (8, 16),
+                                      (4, 128), (7, 64), (8, 128)])
 def test_dense_scan(oracle, backend_module, n, layers):
-    # reference: tests/test_backends_parity.py:197-235 (6q x 32L, 8q x 8L)
+    # reference: tests/test_backends_parity.py:197-235 (6q x 32L, 8q x 8L);
+    # 64- and 128-leaf product trees at the batch workload's depth
     _check(oracle, backend_module, n, layers, 1.0, "dense_scan", 2032 + n)
 
 
