@@ -15,6 +15,7 @@ from __future__ import annotations
 import argparse
 import csv
 import gc
+import re
 import statistics as st
 from dataclasses import dataclass
 from pathlib import Path
@@ -34,13 +35,15 @@ class BenchmarkCase:
     layers: int
 
 
+_CASE = re.compile(r"(\d+)[xX](\d+)")
+
+
 def parse_case(text: str) -> BenchmarkCase:
-    try:
-        q, layers = text.lower().split("x", maxsplit=1)
-        return BenchmarkCase(int(q), int(layers))
-    except Exception as exc:
-        raise argparse.ArgumentTypeError(
-            f"Invalid case {text!r}. Expected format like 12x16.") from exc
+    """``"QxL"`` (e.g. ``12x16``) -> ``BenchmarkCase(Q, L)``; argparse error otherwise."""
+    match = _CASE.fullmatch(text.strip())
+    if match is None:
+        raise argparse.ArgumentTypeError(f"Invalid case {text!r}. Expected format like 12x16.")
+    return BenchmarkCase(*map(int, match.groups()))
 
 
 def parse_args(argv=None) -> argparse.Namespace:
@@ -64,17 +67,27 @@ def parse_args(argv=None) -> argparse.Namespace:
 
 def stage_profile_summary(backend: RingIsingAdjointBackend, params: np.ndarray, *, warmup: int,
                           repeats: int) -> tuple[float, dict[str, float]]:
+    """Median total and per-stage medians of ``timings_ms`` over ``repeats`` profiled calls.
+
+    The stage timings come from HIP events inside the engine; Python's cyclic
+    collector is paused over the timed calls so that no collection lands
+    between an evaluation's stages.
+    """
     for _ in range(warmup):
         backend.profile_energy_and_grad(params)
-    series: dict[str, list[float]] = {}
-    totals = []
-    for _ in range(repeats):
-        gc.collect()
-        timings = backend.profile_energy_and_grad(params)["timings_ms"]
-        totals.append(sum(float(v) for v in timings.values()))
-        for name, value in timings.items():
-            series.setdefault(name, []).append(float(value))
-    return st.median(totals), {k: st.median(v) for k, v in sorted(series.items())}
+    runs: list[dict[str, float]] = []
+    collecting = gc.isenabled()
+    gc.disable()
+    try:
+        for _ in range(repeats):
+            timings = backend.profile_energy_and_grad(params)["timings_ms"]
+            runs.append({name: float(ms) for name, ms in timings.items()})
+    finally:
+        if collecting:
+            gc.enable()
+    stages = sorted({name for run in runs for name in run})
+    per_stage = {name: st.median([run[name] for run in runs if name in run]) for name in stages}
+    return st.median([sum(run.values()) for run in runs]), per_stage
 
 
 def kernel_profile(backend: RingIsingAdjointBackend, params: np.ndarray, *,

@@ -1,10 +1,15 @@
 """Plain gradient-descent loop that drives the engine.
 
-Behaviour of ``ring_ising/training/loop.py:16-139`` in the reference: one
-``step_fn`` evaluation per step, ``params <- apply_gradient_step(params, grad,
-stepsize)``, per-step metrics recorded at about ``DEFAULT_PROGRESS_PARTITIONS``
-evenly spaced steps when ``report_steps`` is set, and an optional one-line
-progress bar.
+Contract of ``ring_ising/training/loop.py:16-139`` in the reference: the
+``StepMetric`` / ``StepEvaluation`` / ``LoopResult`` records, one ``step_fn``
+evaluation and one ``apply_gradient_step`` update per step, and -- when
+``report_steps`` is set -- a metric for the first step of each of
+``DEFAULT_PROGRESS_PARTITIONS`` equal shares of the run (every step when the
+run is shorter than that).  The optional progress bar is redrawn on the same
+steps.
+
+Here the recorded steps are computed once before the loop (``marked_steps``),
+and the progress bar is a small object that owns its clock and its line.
 """
 
 from __future__ import annotations
@@ -22,6 +27,8 @@ ParamsT = TypeVar("ParamsT")
 
 @dataclass(frozen=True)
 class StepMetric:
+    """Energy, gradient norm and wall time of one recorded step."""
+
     step: int
     energy: float
     grad_norm: float
@@ -30,32 +37,55 @@ class StepMetric:
 
 @dataclass(frozen=True)
 class StepEvaluation:
+    """What ``step_fn`` returns: the gradient and the energy at the same parameters."""
+
     grad: np.ndarray
     energy: float
 
 
 @dataclass(frozen=True)
 class LoopResult(Generic[ParamsT]):
+    """Parameters after the last update, the recorded metrics and the loop's wall time."""
+
     final_params: ParamsT
     step_metrics: tuple[StepMetric, ...]
     wall_s: float
 
 
-def _progress_segment(step: int, total: int, *, partitions: int) -> int:
-    """Index (1..partitions) of the progress segment that ``step`` falls in."""
-    if total <= 0:
-        return 0
-    return min(partitions, max(1, (step * partitions - 1) // total + 1))
+def marked_steps(total: int, partitions: int) -> frozenset[int]:
+    """Steps (1-based) that open one of ``partitions`` equal shares of ``total`` steps.
+
+    Share k (k = 1..partitions) covers the steps s with (k-1)/partitions <
+    s/total <= k/partitions; its first step is floor((k-1) * total / partitions)
+    + 1.  Shares that hold no step (total < partitions) repeat a step, so the
+    set simply holds every step then.
+    """
+    if total <= 0 or partitions <= 0:
+        return frozenset()
+    return frozenset((k * total) // partitions + 1 for k in range(partitions))
 
 
-def _progress_line(step: int, total: int, *, elapsed_s: float, width: int = 24) -> str:
-    if total <= 0:
-        return "Progress [" + "-" * width + "] 0/0"
-    ratio = step / total
-    filled = min(width, int(round(width * ratio)))
-    bar = "#" * filled + "-" * (width - filled)
-    return (f"\rProgress [{bar}] {step}/{total} ({100.0 * ratio:5.1f}%) "
-            f"elapsed={elapsed_s:.3f}s")
+class _ProgressBar:
+    """One-line ``Progress [###---] i/n (pct%) elapsed=...s`` bar, redrawn in place."""
+
+    WIDTH = 24
+
+    def __init__(self, total: int) -> None:
+        self.total = total
+        self.started = time.perf_counter()
+
+    def line(self, done: int) -> str:
+        if self.total <= 0:
+            return "Progress [" + "-" * self.WIDTH + "] 0/0"
+        frac = done / self.total
+        hashes = min(self.WIDTH, int(round(self.WIDTH * frac)))
+        bar = "".join("#" if i < hashes else "-" for i in range(self.WIDTH))
+        pct = format(100.0 * frac, "5.1f")
+        elapsed = format(time.perf_counter() - self.started, ".3f")
+        return "\rProgress [%s] %d/%d (%s%%) elapsed=%ss" % (bar, done, self.total, pct, elapsed)
+
+    def draw(self, done: int) -> None:
+        print(self.line(done), end="", flush=True)
 
 
 def run_gradient_descent_loop(
@@ -70,28 +100,33 @@ def run_gradient_descent_loop(
     report_steps: bool,
     format_step: Callable[[StepMetric], str],
 ) -> LoopResult[ParamsT]:
-    metrics: list[StepMetric] = []
-    start = time.perf_counter()
-    last_segment = 0
+    """Run ``steps`` plain gradient-descent updates of ``params``.
+
+    The update is whatever ``apply_gradient_step(params, grad, stepsize)``
+    returns; the energy recorded for a step is the one ``step_fn`` reported at
+    the parameters *before* that step's update.
+    """
+    marks = marked_steps(steps, DEFAULT_PROGRESS_PARTITIONS)
+    bar = _ProgressBar(steps) if (verbose and show_progress) else None
+    recorded: list[StepMetric] = []
+    t_loop = bar.started if bar is not None else time.perf_counter()
+
     for step in range(1, steps + 1):
-        step_start = time.perf_counter()
-        segment = _progress_segment(step, steps, partitions=DEFAULT_PROGRESS_PARTITIONS)
-        record = report_steps and segment != last_segment
-        evaluation = step_fn(params)
-        grad = np.asarray(evaluation.grad, dtype=np.float64)
+        t_step = time.perf_counter()
+        ev = step_fn(params)
+        grad = np.asarray(ev.grad, dtype=np.float64)
         params = apply_gradient_step(params, grad, stepsize)
-        wall = time.perf_counter() - step_start
-        if record:
-            metric = StepMetric(step=step, energy=float(evaluation.energy),
-                                grad_norm=float(np.linalg.norm(grad)), step_wall_s=wall)
-            metrics.append(metric)
+        took = time.perf_counter() - t_step
+        if step not in marks:
+            continue
+        if report_steps:
+            metric = StepMetric(step, float(ev.energy), float(np.linalg.norm(grad)), took)
+            recorded.append(metric)
             if verbose:
                 print(format_step(metric))
-        if verbose and show_progress and segment != last_segment:
-            print(_progress_line(step, steps, elapsed_s=time.perf_counter() - start),
-                  end="", flush=True)
-        last_segment = segment
-    if verbose and show_progress and steps > 0:
+        if bar is not None:
+            bar.draw(step)
+
+    if bar is not None and steps > 0:
         print()
-    return LoopResult(final_params=params, step_metrics=tuple(metrics),
-                      wall_s=time.perf_counter() - start)
+    return LoopResult(params, tuple(recorded), time.perf_counter() - t_loop)

@@ -1,14 +1,21 @@
-"""Standalone (HIP) training workflow (``ring_ising/workflows/standalone.py:90-167``).
+"""Standalone (HIP) training workflow.
 
-Creates the backend, draws the initial parameters, runs the plain
-gradient-descent loop with one ``energy_and_grad`` per step and evaluates the
-energy of the final parameters.
+Contract of ``ring_ising/workflows/standalone.py:25-167``: ``run_standalone``
+accepts only ``backend="standalone"``, checks the shared run arguments, builds
+the engine from the ``RunConfig``, draws the initial parameters, runs the
+plain gradient-descent loop with one ``energy_and_grad`` per step, evaluates
+the energy of the final parameters and returns a ``TrainingRunResult`` whose
+metadata names the gradient strategy, the chunk width, the workspace estimate
+and the gate structure.  A verbose run prints the runtime banner, the problem
+block, the initial energy, one ``step=...`` line per recorded step and the
+summary block.
 """
 
 from __future__ import annotations
 
 import time
 from dataclasses import dataclass
+from functools import partial
 
 import numpy as np
 
@@ -16,110 +23,128 @@ from ..backends.standalone import RingIsingAdjointBackend, StandaloneBackendConf
 from ..config import RunConfig
 from ..training import (
     StepEvaluation,
+    StepMetric,
     TrainingRunResult,
     run_gradient_descent_loop,
     run_with_optional_telemetry,
     validate_common_run_args,
 )
-from .common import apply_gradient_step, make_initial_params, print_problem_summary, \
-    print_result_summary
+from .common import (
+    apply_gradient_step,
+    make_initial_params,
+    print_block,
+    print_problem_summary,
+    print_result_summary,
+)
+
+BACKEND_LABEL = "standalone_hip"
+# RunConfig fields handed to the engine's configuration unchanged
+_ENGINE_FIELDS = ("num_qubits", "layers", "field", "gradient_strategy", "double_buffer")
 
 
 @dataclass(frozen=True)
 class StandaloneWorkflow:
+    """A run's configuration, the engine configuration derived from it and the engine."""
+
     config: RunConfig
     backend_config: StandaloneBackendConfig
     backend: RingIsingAdjointBackend
+
+    @property
+    def chunk_width(self) -> int:
+        return self.backend_config.effective_structured_rotation_chunk_width
+
+    def evaluate(self, params: np.ndarray) -> StepEvaluation:
+        energy, grad = self.backend.energy_and_grad(params)
+        return StepEvaluation(grad=np.asarray(grad, dtype=np.float64), energy=float(energy))
+
+    def metadata(self) -> dict[str, object]:
+        eng = self.backend
+        return {
+            "gradient_strategy": eng.gradient_strategy,
+            "structured_rotation_chunk_width": self.chunk_width,
+            "estimated_workspace_gib": eng.estimated_workspace_gib,
+            "pennylane_gate_structure": eng.uses_pennylane_gate_structure,
+        }
+
+    def runtime_rows(self) -> list[tuple[str, object]]:
+        eng = self.backend
+        rows: list[tuple[str, object]] = [("Backend", "Standalone HIP (MI355X)"),
+                                          ("Gradient strategy", eng.gradient_strategy)]
+        if eng.gradient_strategy == "structured_adjoint":
+            rows.append(("Structured rotation chunk width", self.chunk_width))
+        structure = ("PennyLane-style gate structure" if eng.uses_pennylane_gate_structure
+                     else "Dense-scan fused gate structure")
+        rows.append((structure, True))
+        rows.append(("Estimated gradient workspace", f"{eng.estimated_workspace_gib:.2f} GiB"))
+        return rows
 
 
 StandaloneResult = TrainingRunResult
 
 
 def _backend_config(config: RunConfig) -> StandaloneBackendConfig:
+    shared = {name: getattr(config, name) for name in _ENGINE_FIELDS}
     return StandaloneBackendConfig(
-        num_qubits=config.num_qubits,
-        layers=config.layers,
-        field=config.field,
-        gradient_strategy=config.gradient_strategy,
         structured_rotation_chunk_width=config.effective_structured_rotation_chunk_width,
-        double_buffer=config.double_buffer,
-    )
+        **shared)
 
 
 def create_standalone_workflow(config: RunConfig) -> StandaloneWorkflow:
     backend_config = _backend_config(config)
-    return StandaloneWorkflow(config=config, backend_config=backend_config,
-                              backend=RingIsingAdjointBackend(backend_config))
+    return StandaloneWorkflow(config, backend_config, RingIsingAdjointBackend(backend_config))
 
 
 def print_standalone_runtime_summary(workflow: StandaloneWorkflow, params: np.ndarray) -> None:
-    config = workflow.config
-    print("Runtime:")
-    print("  Backend: Standalone HIP (MI355X)")
-    print(f"  Gradient strategy: {workflow.backend.gradient_strategy}")
-    if workflow.backend.gradient_strategy == "structured_adjoint":
-        print("  Structured rotation chunk width: "
-              f"{workflow.backend_config.effective_structured_rotation_chunk_width}")
-    if workflow.backend.uses_pennylane_gate_structure:
-        print("  PennyLane-style gate structure: True")
-    else:
-        print("  Dense-scan fused gate structure: True")
-    print(f"  Estimated gradient workspace: {workflow.backend.estimated_workspace_gib:.2f} GiB")
-    print()
-    print_problem_summary(config, params)
-    print("Backend details:")
-    print(f"  Initial energy: {workflow.backend.energy(params):.10f}")
-    print()
+    print_block("Runtime", workflow.runtime_rows())
+    print_problem_summary(workflow.config, params)
+    details: list[tuple[str, object]] = [
+        ("Initial energy", f"{workflow.backend.energy(params):.10f}")]
+    if workflow.config.gpu_telemetry:
+        details.append(("GPU telemetry", "not sampled on this platform (use rocprofv3)"))
+    print_block("Backend details", details)
+
+
+def _format_step(metric: StepMetric) -> str:
+    return "step=%03d energy=%.10f grad_norm=%.10f step_ms=%.3f" % (
+        metric.step, metric.energy, metric.grad_norm, 1000.0 * metric.step_wall_s)
+
+
+def _train(workflow: StandaloneWorkflow, start: np.ndarray) -> TrainingRunResult:
+    cfg = workflow.config
+    t0 = time.perf_counter()
+    loop = run_gradient_descent_loop(
+        start,
+        steps=cfg.steps,
+        stepsize=cfg.stepsize,
+        step_fn=workflow.evaluate,
+        apply_gradient_step=apply_gradient_step,
+        verbose=cfg.verbose,
+        show_progress=cfg.show_progress and not cfg.report_steps,
+        report_steps=cfg.report_steps,
+        format_step=_format_step,
+    )
+    final = np.asarray(loop.final_params, dtype=np.float64)
+    final_energy = float(workflow.backend.energy(final))
+    return TrainingRunResult(backend_label=BACKEND_LABEL, final_params=final,
+                             final_energy=final_energy, step_metrics=loop.step_metrics,
+                             wall_s=time.perf_counter() - t0, metadata=workflow.metadata())
 
 
 def run_standalone(config: RunConfig) -> TrainingRunResult:
+    """Run one standalone training workflow end to end (see the module docstring)."""
     if config.backend != "standalone":
         raise ValueError(f"run_standalone expected backend='standalone', got {config.backend!r}.")
     validate_common_run_args(num_qubits=config.num_qubits, layers=config.layers,
                              steps=config.steps, telemetry_interval_s=config.telemetry_interval_s)
     workflow = create_standalone_workflow(config)
-    params = make_initial_params(config)
+    start = make_initial_params(config)
     if config.verbose:
-        print_standalone_runtime_summary(workflow, params)
-
-    def step_fn(current: np.ndarray) -> StepEvaluation:
-        energy, grad = workflow.backend.energy_and_grad(current)
-        return StepEvaluation(energy=float(energy), grad=np.asarray(grad, dtype=np.float64))
-
-    def body() -> TrainingRunResult:
-        start = time.perf_counter()
-        loop = run_gradient_descent_loop(
-            params,
-            steps=config.steps,
-            stepsize=config.stepsize,
-            step_fn=step_fn,
-            apply_gradient_step=apply_gradient_step,
-            verbose=config.verbose,
-            show_progress=config.show_progress and not config.report_steps,
-            report_steps=config.report_steps,
-            format_step=lambda m: (f"step={m.step:03d} energy={m.energy:.10f} "
-                                   f"grad_norm={m.grad_norm:.10f} "
-                                   f"step_ms={1000.0 * m.step_wall_s:.3f}"),
-        )
-        final_energy = workflow.backend.energy(loop.final_params)
-        return TrainingRunResult(
-            backend_label="standalone_hip",
-            final_params=np.asarray(loop.final_params, dtype=np.float64),
-            final_energy=float(final_energy),
-            step_metrics=loop.step_metrics,
-            wall_s=time.perf_counter() - start,
-            metadata={
-                "gradient_strategy": workflow.backend.gradient_strategy,
-                "structured_rotation_chunk_width":
-                    workflow.backend_config.effective_structured_rotation_chunk_width,
-                "estimated_workspace_gib": workflow.backend.estimated_workspace_gib,
-                "pennylane_gate_structure": workflow.backend.uses_pennylane_gate_structure,
-            },
-        )
-
+        print_standalone_runtime_summary(workflow, start)
     result, summary = run_with_optional_telemetry(
         enabled=config.gpu_telemetry, interval_s=config.telemetry_interval_s,
-        live=config.telemetry_live, label="Standalone backend telemetry", body=body)
+        live=config.telemetry_live, label="Standalone backend telemetry",
+        body=partial(_train, workflow, start))
     result.gpu_telemetry_summary = summary
     if config.verbose:
         print_result_summary(result, config)

@@ -84,3 +84,14 @@ def test_uneven_mix_is_reproducible():
     assert all(np.array_equal(x, y) for x, y in zip(a_params, b_params))
     assert all(4 <= s.num_qubits <= 14 and s.layers in (8, 32, 128) and 0.2 <= s.field <= 2.0
                for s in a_specs)
+
+
+def test_reference_import_path_is_the_built_module():
+    """``standalone_backend._cuda_backend`` -- the name the reference's
+    ``RingIsingAdjointBackend._load_backend`` imports -- is this build's module."""
+    import importlib
+
+    ext = native.load_extension()
+    alias = importlib.import_module("standalone_backend._cuda_backend")
+    assert alias is ext
+    assert hasattr(alias, "RingIsingCudaBackend") and hasattr(alias, "energy_and_grad")
