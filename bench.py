@@ -74,6 +74,11 @@ def parse_args() -> argparse.Namespace:
     p.add_argument("--schedule", choices=["auto", "stored", "inverse_walk"], default="auto",
                    help="backward schedule (auto: stored when it fits in device memory)")
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--cpu-full", action="store_true",
+                   help="time one oracle evaluation of the workload itself instead of a "
+                        "bounded smaller-n sample")
+    p.add_argument("--no-extras", action="store_true",
+                   help="skip the 12q x 3L, drift, batched and warm-start sections")
     p.add_argument("--no-autotune", action="store_true",
                    help="keep the default tile shapes and schedule instead of autotuning")
     p.add_argument("--tile-shapes", default="auto",
@@ -93,13 +98,35 @@ def parse_args() -> argparse.Namespace:
     return p.parse_args()
 
 
-def cpu_baseline(n_target: int, layers: int, budget_s: float = 20.0) -> dict:
-    """Time the oracle on a bounded sample and scale to n_target."""
+def _cpu_threads() -> tuple[int, int]:
+    """(threads used, host cores visible): OMP_NUM_THREADS when the host sets
+    it (the process's CPU share), else every core in the affinity mask."""
+    visible = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    threads = int(env) if env.isdigit() and int(env) > 0 else visible
+    return min(threads, visible), visible
+
+
+def _cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(n_target: int, layers: int, budget_s: float = 25.0, full: bool = False) -> dict:
+    """The oracle's gate-level adjoint (PennyLane gate order, FP64, OpenMP)
+    on this host.  ``full``: one evaluation of the workload itself.  Otherwise
+    a bounded sample: the largest n whose evaluation fits the budget, scaled
+    to n_target by the 2^n * n cost of a gate-level sweep."""
     from oracle import oracle as cpu_oracle
 
     cpu_oracle.build()
-    cores = min(16, len(os.sched_getaffinity(0)))
-    cpu_oracle.set_num_threads(cores)
+    threads, visible = _cpu_threads()
+    cpu_oracle.set_num_threads(threads)
     rng = np.random.default_rng(5)
 
     def one_eval(n: int) -> float:
@@ -108,27 +135,110 @@ def cpu_baseline(n_target: int, layers: int, budget_s: float = 20.0) -> dict:
         cpu_oracle.energy_and_grad(n, layers, 1.0, params)
         return time.perf_counter() - t0
 
+    host = {"unit": "evals/s", "cores": threads, "kind": "port",
+            "host_cores_visible": visible, "cpu_model": _cpu_model()}
+    if full:
+        t = one_eval(n_target)
+        return {**host, "value": 1.0 / t,
+                "sample": (f"oracle gate-level adjoint (FP64, OpenMP, {threads} threads) at "
+                           f"{n_target} qubits x {layers} layers: one evaluation, {t:.1f} s, "
+                           "not scaled")}
     n = 18
     t = one_eval(n)
-    # grow n while 6 evaluations (1 warm-up + 5 timed) stay within the budget
-    while n < min(n_target, 26):
+    # grow n while 3 evaluations (1 warm-up + 2 timed) stay within the budget
+    while n < n_target:
         t_next = t * 2.0 * (n + 1) / n
-        if 6 * t_next > budget_s:
+        if 3 * t_next > budget_s:
             break
         n += 1
         t = one_eval(n)
-    samples = [one_eval(n) for _ in range(5)]
-    med = statistics.median(samples)
+    samples = [one_eval(n) for _ in range(2)]
+    med = min(samples)
     scale = (2.0 ** (n_target - n)) * (n_target / n)
+    return {**host, "value": 1.0 / (med * scale),
+            "sample": (f"oracle gate-level adjoint (FP64, OpenMP, {threads} threads) at {n} "
+                       f"qubits x {layers} layers: best of 2 runs after 1 warm-up "
+                       f"{med * 1e3:.1f} ms/eval, scaled to {n_target} qubits by 2^n * n "
+                       f"(x{scale:.3g})")}
+
+
+def plain_small(torch, steps: int) -> dict:
+    """12 qubits x 3 layers (one circuit, the whole-circuit kernel): evaluations
+    per second through the device entry point and through the host-array call,
+    beside the oracle's gate-level adjoint on this host's cores at the same size
+    (measured, not scaled), and the engine's error against that oracle."""
+    from gpu_simulation_amd import native
+    from oracle import oracle as cpu_oracle
+
+    n, layers = 12, 3
+    ext = native.load_extension()
+    host_params = np.random.default_rng(0).uniform(0.0, 2.0 * math.pi, size=(layers, n, 2))
+    flat = np.ascontiguousarray(host_params.reshape(-1))
+    eng = ext.RingIsingCudaBackend(n, layers, 1.0, "structured_adjoint", 8, False)
+    dev = torch.device("cuda:0")
+    p = torch.tensor(flat, device=dev)
+    e = torch.zeros(1, dtype=torch.float64, device=dev)
+    g = torch.zeros_like(p)
+    handle = torch.cuda.current_stream(dev).cuda_stream
+    reps = max(200, 50 * steps)
+
+    def device_step() -> None:
+        eng.energy_and_grad_device(p.data_ptr(), e.data_ptr(), g.data_ptr(), handle)
+
+    for _ in range(10):
+        device_step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        device_step()
+    torch.cuda.synchronize()
+    dev_s = (time.perf_counter() - t0) / reps
+    out = eng.energy_and_grad(flat)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        out = eng.energy_and_grad(flat)
+    host_s = (time.perf_counter() - t0) / reps
+
+    cpu_oracle.build()
+    threads, _ = _cpu_threads()
+    cpu_oracle.set_num_threads(threads)
+    e_ref, g_ref = cpu_oracle.energy_and_grad(n, layers, 1.0, host_params)
+    cpu_reps = 20
+    t0 = time.perf_counter()
+    for _ in range(cpu_reps):
+        cpu_oracle.energy_and_grad(n, layers, 1.0, host_params)
+    cpu_s = (time.perf_counter() - t0) / cpu_reps
     return {
-        "value": 1.0 / (med * scale),
-        "unit": "evals/s",
-        "cores": cores,
-        "kind": "port",
-        "sample": (f"oracle gate-level adjoint (FP64, OpenMP, {cores} threads) at {n} qubits x "
-                   f"{layers} layers: median {med * 1e3:.1f} ms/eval over 5 runs after 1 warm-up, "
-                   f"scaled to {n_target} qubits by 2^n * n (x{scale:.3g})"),
+        "workload": "structured_adjoint energy_and_grad, 12 qubits x 3 layers, params U[0,2pi) "
+                    "seed 0, J=h=1",
+        "device_entry_evals_per_s": round(1.0 / dev_s, 1),
+        "host_call_evals_per_s": round(1.0 / host_s, 1),
+        "cpu_oracle_evals_per_s": round(1.0 / cpu_s, 1),
+        "cpu_oracle": f"gate-level adjoint (FP64, OpenMP, up to {threads} threads; states of "
+                      "4096 amplitudes run on one)",
+        "speedup_device_vs_cpu": round(cpu_s / dev_s, 1),
+        "max_abs_dE": abs(float(out["energy"]) - e_ref),
+        "max_abs_dgrad": float(np.max(np.abs(np.asarray(out["gradient"]) - g_ref))),
     }
+
+
+def drift_report(torch) -> dict:
+    """Inverse-walk drift ||walked state - |0...0>||_2 at 20 qubits x 4 layers
+    (params U[0,2pi) seed 2024), with and without every-2-layer checkpoints."""
+    from gpu_simulation_amd import native
+
+    n, layers = 20, 4
+    ext = native.load_extension()
+    flat = np.ascontiguousarray(np.random.default_rng(2024).uniform(
+        0.0, 2.0 * math.pi, size=(layers, n, 2)).reshape(-1))
+    eng = ext.RingIsingCudaBackend(n, layers, 1.0, "structured_adjoint", 8, False)
+    eng.set_schedule("inverse_walk")
+    out = {"workload": "20 qubits x 4 layers, params U[0,2pi) seed 2024, J=h=1"}
+    for every in (0, 2):
+        eng.set_checkpoint_interval(every)
+        out[f"drift_checkpoint_every_{every}"] = float(eng.inverse_walk_drift(flat))
+    eng.set_checkpoint_interval(0)
+    return out
 
 
 def batched_mix(count: int, steps: int, torch) -> dict:
@@ -375,12 +485,16 @@ def main() -> int:
         "energy": e_val,
         "parity": parity,
     }
-    if not args.no_batched:
+    del engine  # frees its device buffers before the other sections allocate theirs
+    if not args.no_extras:
+        result["plain_12q3l"] = plain_small(torch, args.steps)
+        result["drift"] = drift_report(torch)
+    if not args.no_batched and not args.no_extras:
         result["batched"] = batched_mix(args.batch_count, args.steps, torch)
-    if not args.no_warm_start:
+    if not args.no_warm_start and not args.no_extras:
         result["warm_start"] = warm_start_section()
     if not args.no_cpu_baseline:
-        result["cpu_baseline"] = cpu_baseline(n, layers)
+        result["cpu_baseline"] = cpu_baseline(n, layers, full=args.cpu_full)
     print(json.dumps(result), flush=True)
     return 0
 

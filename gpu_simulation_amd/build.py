@@ -24,7 +24,6 @@ INCLUDE = ROOT / "include"
 OBJ_DIR = ROOT / "_build"
 LIB_DIR = PKG / "lib"
 LIB_PATH = LIB_DIR / "librie.so"
-PROBE_PATH = LIB_DIR / "rie_bwprobe"  # HBM calibration probe (executable)
 EXT_DIR = PKG / "standalone_backend"
 EXT_PATH = EXT_DIR / ("_cuda_backend" + (sysconfig.get_config_var("EXT_SUFFIX") or ".so"))
 
@@ -77,10 +76,6 @@ def build(verbose: bool = False) -> None:
     _run_parallel([[HIPCC, f"--offload-arch={OFFLOAD_ARCH}", "-shared", "-fPIC", "-o",
                     str(tmp_lib), *objects]], verbose)
     os.replace(tmp_lib, LIB_PATH)
-    tmp_probe = PROBE_PATH.with_name(PROBE_PATH.name + ".tmp")
-    _run_parallel([[HIPCC, *HIP_FLAGS, str(CSRC / "rie_bwprobe.hip"), "-o", str(tmp_probe)]],
-                  verbose)
-    os.replace(tmp_probe, PROBE_PATH)
 
     import pybind11
 

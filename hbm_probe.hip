@@ -1,0 +1,136 @@
+// HBM calibration probe for gfx950 (a measurement tool beside bench.py, not
+// part of the engine): streaming kernels of known byte counts.
+//
+//   hipcc --offload-arch=gfx950 -O3 -Xclang -target-feature -Xclang -packed-fp32-ops \
+//         hbm_probe.hip -o hbm_probe
+//   ./hbm_probe [GiB per buffer, default 8] [repeats, default 5]
+//
+// Prints one JSON object per kernel variant: algorithmic bytes per launch,
+// average launch time (HIP events) and GB/s.  Each variant keeps U
+// independent 16-byte loads per lane in flight (all U issued before the first
+// use), over a grid of G workgroups per CU; the sweep covers read-only
+// (a reduction that keeps every load), write-only and copy, each at
+// U in {1, 4, 8, 16} and G in {4, 8, 16, 32}.  The largest rate of each kind
+// is the achievable HBM rate for that access pattern; "copy" variants also
+// calibrate profiler FETCH_SIZE / WRITE_SIZE counters to bytes.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstdlib>
+
+#define CHECK(x)                                                                  \
+    do {                                                                          \
+        hipError_t e_ = (x);                                                      \
+        if (e_ != hipSuccess) {                                                   \
+            std::fprintf(stderr, "%s: %s\n", #x, hipGetErrorString(e_));          \
+            std::exit(1);                                                         \
+        }                                                                         \
+    } while (0)
+
+constexpr int kThreads = 256;
+
+// Workgroup w handles the blocks w, w + grid, ... of kThreads * U elements;
+// within a block lane t touches elements t, t + kThreads, ... (coalesced).
+template <int U>
+__global__ __launch_bounds__(kThreads) void read_kernel(const double2 *__restrict__ in, size_t n,
+                                                        double *__restrict__ sink) {
+    const size_t block = (size_t)kThreads * U;
+    double acc = 0.0;
+    for (size_t base = (size_t)blockIdx.x * block; base < n; base += (size_t)gridDim.x * block) {
+        double2 v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] = in[base + (size_t)u * kThreads + threadIdx.x];
+#pragma unroll
+        for (int u = 0; u < U; ++u) acc += v[u].x + v[u].y;
+    }
+    if (acc == 12345.678) sink[0] = acc;  // never true for the zero-filled input
+}
+
+template <int U>
+__global__ __launch_bounds__(kThreads) void write_kernel(double2 *__restrict__ out, size_t n) {
+    const size_t block = (size_t)kThreads * U;
+    for (size_t base = (size_t)blockIdx.x * block; base < n; base += (size_t)gridDim.x * block) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const size_t i = base + (size_t)u * kThreads + threadIdx.x;
+            out[i] = make_double2((double)i, 0.0);
+        }
+    }
+}
+
+template <int U>
+__global__ __launch_bounds__(kThreads) void copy_kernel(const double2 *__restrict__ in,
+                                                        double2 *__restrict__ out, size_t n) {
+    const size_t block = (size_t)kThreads * U;
+    for (size_t base = (size_t)blockIdx.x * block; base < n; base += (size_t)gridDim.x * block) {
+        double2 v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] = in[base + (size_t)u * kThreads + threadIdx.x];
+#pragma unroll
+        for (int u = 0; u < U; ++u) out[base + (size_t)u * kThreads + threadIdx.x] = v[u];
+    }
+}
+
+int main(int argc, char **argv) {
+    const double gib = argc > 1 ? std::atof(argv[1]) : 8.0;
+    const int repeats = argc > 2 ? std::atoi(argv[2]) : 5;
+    // whole multiples of the largest block (256 threads x 16 elements)
+    const size_t quantum = (size_t)kThreads * 16;
+    const size_t n = ((size_t)(gib * (double)(1ull << 30)) / sizeof(double2)) / quantum * quantum;
+    if (n == 0 || repeats < 1) {
+        std::fprintf(stderr, "usage: hbm_probe [GiB per buffer > 0] [repeats >= 1]\n");
+        return 2;
+    }
+    double2 *a = nullptr;
+    double2 *b = nullptr;
+    double *sink = nullptr;
+    CHECK(hipMalloc(&a, n * sizeof(double2)));
+    CHECK(hipMalloc(&b, n * sizeof(double2)));
+    CHECK(hipMalloc(&sink, sizeof(double)));
+    CHECK(hipMemset(a, 0, n * sizeof(double2)));
+    CHECK(hipMemset(b, 0, n * sizeof(double2)));
+    hipDeviceProp_t prop;
+    CHECK(hipGetDeviceProperties(&prop, 0));
+    const int cus = prop.multiProcessorCount;
+    hipEvent_t e0, e1;
+    CHECK(hipEventCreate(&e0));
+    CHECK(hipEventCreate(&e1));
+    const double S = (double)n * sizeof(double2);
+
+    auto run = [&](const char *kind, int u, int per_cu, double bytes, auto &&launch) {
+        launch();
+        CHECK(hipGetLastError());
+        CHECK(hipDeviceSynchronize());
+        CHECK(hipEventRecord(e0, 0));
+        for (int r = 0; r < repeats; ++r) launch();
+        CHECK(hipEventRecord(e1, 0));
+        CHECK(hipEventSynchronize(e1));
+        float ms = 0.0f;
+        CHECK(hipEventElapsedTime(&ms, e0, e1));
+        const double avg = ms / repeats;
+        std::printf("{\"kind\": \"%s\", \"loads_in_flight_per_lane\": %d, \"workgroups_per_cu\": %d, "
+                    "\"bytes\": %.0f, \"avg_ms\": %.4f, \"GBps\": %.1f}\n",
+                    kind, u, per_cu, bytes, avg, bytes / (avg * 1e-3) / 1e9);
+        std::fflush(stdout);
+    };
+
+    const int grids[] = {4, 8, 16, 32};
+    for (int g : grids) {
+        const int grid = cus * g;
+        run("read", 1, g, S, [&]() { read_kernel<1><<<grid, kThreads>>>(a, n, sink); });
+        run("read", 4, g, S, [&]() { read_kernel<4><<<grid, kThreads>>>(a, n, sink); });
+        run("read", 8, g, S, [&]() { read_kernel<8><<<grid, kThreads>>>(a, n, sink); });
+        run("read", 16, g, S, [&]() { read_kernel<16><<<grid, kThreads>>>(a, n, sink); });
+        run("write", 1, g, S, [&]() { write_kernel<1><<<grid, kThreads>>>(b, n); });
+        run("write", 4, g, S, [&]() { write_kernel<4><<<grid, kThreads>>>(b, n); });
+        run("write", 8, g, S, [&]() { write_kernel<8><<<grid, kThreads>>>(b, n); });
+        run("copy", 1, g, 2 * S, [&]() { copy_kernel<1><<<grid, kThreads>>>(a, b, n); });
+        run("copy", 4, g, 2 * S, [&]() { copy_kernel<4><<<grid, kThreads>>>(a, b, n); });
+        run("copy", 8, g, 2 * S, [&]() { copy_kernel<8><<<grid, kThreads>>>(a, b, n); });
+        run("copy", 16, g, 2 * S, [&]() { copy_kernel<16><<<grid, kThreads>>>(a, b, n); });
+    }
+    CHECK(hipFree(a));
+    CHECK(hipFree(b));
+    CHECK(hipFree(sink));
+    return 0;
+}

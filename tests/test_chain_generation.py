@@ -125,37 +125,60 @@ def _chain_y(sites, chunks, ua, ub, i, base):
     return y @ t1[(i >> f1) & ((1 << n1) - 1)]
 
 
-@pytest.mark.parametrize("n", [13, 16, 22])
-@pytest.mark.parametrize("R", [3, 4])
-def test_tile_tables_reproduce_states(oracle, n, R):
-    # the forward's low tile (every wire rotated) and the adjoint high passes
-    # (the high group's wires rotated), on sampled tiles and threads
-    rng = np.random.default_rng(60 + n + R)
-    params = rng.uniform(0.0, 2.0 * np.pi, size=(2, n, 2))
-    l0, l1 = _coefs(params[0]), _coefs(params[1])
+def _plan_cases(n):
+    """(tile low bits, first high index bit, rotated wires) of every table set
+    as the engine plans them: the forward's low tile (all wires rotated), then
+    one adjoint high pass per high group.  The n - 12 high wires split into
+    ceil((n - 12) / 9) groups of near-equal size; group j's tile takes 9 wires
+    from h0 = min(g0, n - 9), and its state still carries the layer-1
+    rotations of groups 0 .. j (wires [12, end of group j)), the low wires and
+    the groups above j being un-rotated already."""
     full = (1 << n) - 1
     a = 3
-    cases = [(T, T, full)]  # (tile low bits, first high index bit, rotated wires)
-    # the high groups as the engine plans them: the n - 12 high wires split
-    # into ceil((n - 12) / 9) groups of near-equal size, each tile's high
-    # part 9 wires from h0 = min(g0, n - 9)
+    cases = [(T, T, full)]
     rest, width = n - T, T - a
     groups = (rest + width - 1) // width
     g0 = T
     for gi in range(groups):
         bg = rest // groups + (1 if gi < rest % groups else 0)
         h0 = min(g0, n - width)
-        cases.append((a, h0, full & ~((1 << g0) - 1) & ((1 << (g0 + bg)) - 1)))
+        cases.append((a, h0, full & ~((1 << T) - 1) & ((1 << (g0 + bg)) - 1)))
         g0 += bg
-    for ta, th0, rot in cases:
-        want = _rotated_reference(oracle, n, params, rot)
+    return cases
+
+
+def _direct_trace(sites, n, i):
+    """Tr(M_0[i_0] ... M_{n-1}[i_{n-1}]): the layer-1 amplitude at index i,
+    wire by wire (tied to the oracle by test_chain_trace_equals_layer1_states)."""
+    m = np.eye(2)
+    for w in range(n):
+        m = m @ sites[w][(i >> w) & 1]
+    return np.trace(m)
+
+
+def test_plan_cases_follow_the_engine_groups():
+    # n = 30: two full 9-wire groups, h0 == g0 (wires 12..20 and 21..29)
+    c = _plan_cases(30)
+    assert [(ta, h0) for ta, h0, _ in c] == [(12, 12), (3, 12), (3, 21)]
+    assert c[1][2] == ((1 << 21) - 1) & ~((1 << 12) - 1)
+    assert c[2][2] == ((1 << 30) - 1) & ~((1 << 12) - 1)
+    # n = 31: three groups of 7, 6, 6 wires; the last two tiles overlap lower wires
+    c = _plan_cases(31)
+    assert [(ta, h0) for ta, h0, _ in c] == [(12, 12), (3, 12), (3, 19), (3, 22)]
+    assert c[2][2] == ((1 << 25) - 1) & ~((1 << 12) - 1)
+
+
+def _check_sampled(rng, n, R, params, want_at, tiles=4, threads=24):
+    l0, l1 = _coefs(params[0]), _coefs(params[1])
+    for ta, th0, rot in _plan_cases(n):
+        want = want_at(rot)
         sites, r0, run, chunks, ua, ub = _set(n, R, ta, th0, rot, l0, l1)
         gap = th0 - ta
         err = 0.0
-        for g in rng.integers(0, 1 << (n - T), 4):
+        for g in rng.integers(0, 1 << (n - T), tiles):
             g = int(g)
             base = ((g & ((1 << gap) - 1)) << ta) | ((g >> gap) << (th0 + T - ta))
-            for t in rng.integers(0, 1 << (T - R), 24):
+            for t in rng.integers(0, 1 << (T - R), threads):
                 t = int(t)
 
                 def index(o):
@@ -163,5 +186,35 @@ def test_tile_tables_reproduce_states(oracle, n, R):
 
                 y = _chain_y(sites, chunks, ua, ub, index(_omap(R, 0, t, 0)), base)
                 for k in range(1 << R):
-                    err = max(err, abs(np.trace(run[k] @ y) - want[index(_omap(R, 0, t, k))]))
+                    err = max(err, abs(np.trace(run[k] @ y) - want(index(_omap(R, 0, t, k)))))
         assert err <= 1e-14, (ta, th0, r0)
+
+
+@pytest.mark.parametrize("n", [13, 16, 22])
+@pytest.mark.parametrize("R", [3, 4])
+def test_tile_tables_reproduce_states(oracle, n, R):
+    # sampled tiles and threads of every table set against the oracle's states
+    rng = np.random.default_rng(60 + n + R)
+    params = rng.uniform(0.0, 2.0 * np.pi, size=(2, n, 2))
+
+    def want_at(rot):
+        v = _rotated_reference(oracle, n, params, rot)
+        return lambda i: v[i]
+
+    _check_sampled(rng, n, R, params, want_at)
+
+
+@pytest.mark.parametrize("n", [30, 31])
+@pytest.mark.parametrize("R", [3, 4])
+def test_tile_tables_at_the_largest_geometries(n, R):
+    # n = 30 (two full groups, h0 == g0) and n = 31 (three groups, overlapping
+    # tiles): sampled tiles and threads against the direct per-wire trace
+    rng = np.random.default_rng(90 + n + R)
+    params = rng.uniform(0.0, 2.0 * np.pi, size=(2, n, 2))
+    l0, l1 = _coefs(params[0]), _coefs(params[1])
+
+    def want_at(rot):
+        sites = [[_site(w, q, l0, l1, (rot >> w) & 1) for q in (0, 1)] for w in range(n)]
+        return lambda i: _direct_trace(sites, n, i)
+
+    _check_sampled(rng, n, R, params, want_at, tiles=3, threads=6)

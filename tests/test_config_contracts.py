@@ -93,3 +93,18 @@ def test_pennylane_backend_is_not_silently_substituted():
         run(RunConfig(backend="pennylane", verbose=False))
     with pytest.raises(ValueError):
         run(RunConfig(backend="cuda", verbose=False))
+
+
+@pytest.mark.parametrize("kwargs", [
+    {"drift_tol": 1e-12, "schedule": "stored"},
+    {"drift_tol": 1e-12, "gradient_strategy": "dense_scan"},
+    {"drift_tol": -1.0},
+    {"drift_tol": float("nan")},
+    {"drift_tol": 1e-12, "drift_check_every": -1},
+])
+def test_drift_triggered_checkpoints_reject_bad_settings(kwargs):
+    """Checked before any engine is created, so no GPU is needed."""
+    from gpu_simulation_amd.autograd import RingIsingEnergy
+
+    with pytest.raises(ValueError):
+        RingIsingEnergy(6, 2, 1.0, **kwargs)

@@ -104,3 +104,21 @@ def test_torch_optimizer_descends():
         opt.step()
         first = first if first is not None else float(e.item())
     assert float(module(p).item()) < first
+
+
+@pytest.mark.parametrize("tol, expect_active", [(0.0, True), (1.0, False)])
+def test_drift_triggered_checkpoints(oracle, tol, expect_active):
+    """drift_tol = 0 switches every-2-layer checkpoints on at the first call
+    (any floating-point walk drifts above zero), drift_tol = 1 never does; the
+    energy and gradient match the oracle either way, before and after."""
+    n, layers = 20, 4
+    params = _params(2024, n, layers)
+    module = RingIsingEnergy(n, layers, 1.0, drift_tol=tol, drift_check_every=3)
+    e_ref, g_ref = oracle.energy_and_grad(n, layers, 1.0, params, form="structured")
+    for call in range(4):
+        print(f"case drift-triggered checkpoints tol={tol} call={call}", flush=True)
+        e, g = _energy_and_grad(module, params)
+        assert abs(e - e_ref) <= 1e-10
+        assert np.max(np.abs(g - g_ref)) <= 1e-10
+    assert module.last_drift is not None and 0.0 < module.last_drift < 1e-11
+    assert module.checkpoints_active is expect_active
