@@ -14,7 +14,11 @@ Three independent checks of the engine at the pass geometry of this size
   every other angle zero gives |+>^n, so E = -n h;
 * the exact parameter-shift identity dE/dp = (E(p + pi/2) - E(p - pi/2)) / 2
   (RY and RZ have generators +-1/2) on all 120 components, from energy-only
-  evaluations, against the engine's adjoint gradient.
+  evaluations, against the engine's adjoint gradient;
+* a sparse brute force at full size: with only a few angles nonzero the state
+  has a handful of nonzero amplitudes, so a dictionary simulation of the gate
+  sequence (RY, RZ per wire, then CNOT(w, w+1) gate by gate) gives the exact
+  energy and, by parameter shift, all 120 gradients, against the engine.
 """
 
 from __future__ import annotations
@@ -26,6 +30,8 @@ from pathlib import Path
 
 import numpy as np
 import pytest
+from sparse_sim import ring_layer as _ring_layer
+from sparse_sim import sparse_energy, sparse_grad
 
 pytestmark = pytest.mark.gpu
 
@@ -41,14 +47,6 @@ def _fixture():
     digest = hashlib.sha256(np.ascontiguousarray(params, dtype="<f8").tobytes()).hexdigest()
     assert digest == data["params_sha256"], "regenerated parameters differ from the fixture's"
     return data, params
-
-
-def _ring_layer(index: int, n: int) -> int:
-    """Basis index after CNOT(0,1), CNOT(1,2), ..., CNOT(n-1,0), gate by gate."""
-    for w in range(n):
-        if (index >> w) & 1:
-            index ^= 1 << ((w + 1) % n)
-    return index
 
 
 def test_fixture_parameters_and_bounds():
@@ -116,9 +114,44 @@ def test_headline_zero_angles_closed_form(backend_module):
             for _ in range(LAYERS - layer):
                 m = _ring_layer(m, N)
             expected[layer, w] = -field if bin(m).count("1") == 1 else 0.0
-    # the rule is not vacuous here: some components are -h, most are 0
-    assert 0 < np.count_nonzero(expected) < expected.size
+    # at 30 qubits one or two ring layers spread every single bit over two or
+    # more wires, so the rule gives zero for every component (the nonzero
+    # gradients are covered by the sparse brute force below)
+    assert np.count_nonzero(expected) == 0
     assert np.max(np.abs(grad[:, :, 0] - expected)) <= TOL
+
+
+def _sparse_point(seed):
+    """Zero angles except six RY and four RZ angles spread over the low tile,
+    both high groups and the wires whose pairs straddle them (0, 11, 12, 20,
+    21, 29) in both layers."""
+    rng = np.random.default_rng(seed)
+    params = np.zeros((LAYERS, N, 2))
+    for layer, w in [(0, 0), (0, 11), (0, 21), (1, 12), (1, 20), (1, 29)]:
+        params[layer, w, 0] = rng.uniform(0.3, 2 * math.pi - 0.3)
+    for layer, w in [(0, 12), (0, 29), (1, 0), (1, 21)]:
+        params[layer, w, 1] = rng.uniform(0.3, 2 * math.pi - 0.3)
+    return params.reshape(-1)
+
+
+@pytest.mark.parametrize("schedule", ["stored", "inverse_walk"])
+def test_headline_sparse_brute_force(backend_module, schedule):
+    field = 0.9
+    params = _sparse_point(11)
+    e_ref = sparse_energy(N, LAYERS, field, params)
+    g_ref = sparse_grad(N, LAYERS, field, params)
+    # the point is not degenerate: the energy is not -n and many gradients are nonzero
+    assert abs(e_ref + N) > 1e-3 and np.count_nonzero(np.abs(g_ref) > 1e-6) >= 10
+    eng = backend_module.RingIsingCudaBackend(N, LAYERS, field, "structured_adjoint", 8, False)
+    eng.set_schedule(schedule)
+    for gen in (1, 0):
+        eng.set_layer1_generation(gen)
+        print(f"case 30q x 2L sparse point schedule={schedule} generation={gen}", flush=True)
+        out = eng.energy_and_grad(params)
+        de = abs(out["energy"] - e_ref)
+        dg = float(np.max(np.abs(np.asarray(out["gradient"]) - g_ref)))
+        print(f"  |dE| = {de:.3e}  max|dgrad| = {dg:.3e}", flush=True)
+        assert de <= TOL and dg <= TOL
 
 
 def test_headline_plus_state_energy(backend_module):

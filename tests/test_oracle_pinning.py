@@ -148,3 +148,24 @@ def test_gradient_checks_catch_a_planted_sign_error(oracle):
     shift = oracle.param_shift_grad(4, 2, 1.0, params)
     assert np.max(np.abs(grad - shift)) < 1e-12
     assert np.max(np.abs(planted - shift)) > 1e-3
+
+
+@pytest.mark.parametrize("n,layers", [(6, 3), (10, 2), (13, 2)])
+def test_sparse_simulation_matches_the_oracle(oracle, n, layers):
+    # the dictionary simulation used at 30 qubits (tests/sparse_sim.py), checked
+    # against both oracle forms where a dense state fits: energy and every
+    # gradient component at a point with a few nonzero angles
+    from sparse_sim import sparse_energy, sparse_grad
+
+    rng = np.random.default_rng(17 + n)
+    field = 0.8
+    params = np.zeros((layers, n, 2))
+    params[:, ::3, 0] = rng.uniform(0.0, 2 * np.pi, params[:, ::3, 0].shape)
+    params[:, 1::4, 1] = rng.uniform(0.0, 2 * np.pi, params[:, 1::4, 1].shape)
+    e = sparse_energy(n, layers, field, params)
+    g = sparse_grad(n, layers, field, params)
+    assert np.count_nonzero(np.abs(g) > 1e-6) >= 4
+    for form in ("gate", "structured"):
+        e_ref, g_ref = oracle.energy_and_grad(n, layers, field, params, form=form)
+        assert abs(e - e_ref) <= 1e-12
+        assert np.max(np.abs(g - g_ref)) <= 1e-12
