@@ -322,7 +322,7 @@ def warm_start_section() -> dict:
     t0 = time.perf_counter()
     res = ws.run_pipeline(num_qubits=10, layers=4, train_count=4096, eval_count=512, steps=300,
                           lr=0.05, mlp_steps=2000, mlp_lr=1e-3, tol=1e-4)
-    out = {k: v for k, v in res.items() if k not in ("steps_plain", "steps_warm")}
+    out = {k: v for k, v in res.items() if not k.startswith("steps_")}
     out["seconds"] = round(time.perf_counter() - t0, 2)
     out["workload"] = ("10 qubits x 4 layers, h ~ U[0.2, 2.0]; 4096 training runs (8 shared "
                        "random draws x 512 fields), 300 Adam steps lr 0.05; 512 held-out fields "

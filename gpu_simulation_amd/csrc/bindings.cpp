@@ -225,6 +225,10 @@ class RingIsingCudaBackend {
 
     bool layer1_generation() const { return rie_engine_layer1_generation(engine_) == 1; }
 
+    void set_double_buffer(int enabled) { check(rie_engine_set_double_buffer(engine_, enabled)); }
+
+    bool double_buffer() const { return rie_engine_double_buffer(engine_) == 1; }
+
     void set_kernel_timing(bool enable) { check(rie_engine_set_kernel_timing(engine_, enable)); }
 
     void reset_kernel_stats() { check(rie_engine_reset_kernel_stats(engine_)); }
@@ -562,6 +566,8 @@ PYBIND11_MODULE(_cuda_backend, m) {
         .def("set_layer1_generation", &RingIsingCudaBackend::set_layer1_generation,
              py::arg("enabled"))
         .def_property_readonly("layer1_generation", &RingIsingCudaBackend::layer1_generation)
+        .def("set_double_buffer", &RingIsingCudaBackend::set_double_buffer, py::arg("enabled"))
+        .def_property_readonly("double_buffer", &RingIsingCudaBackend::double_buffer)
         .def_property_readonly("schedule", &RingIsingCudaBackend::schedule)
         .def("set_kernel_timing", &RingIsingCudaBackend::set_kernel_timing, py::arg("enable"))
         .def("reset_kernel_stats", &RingIsingCudaBackend::reset_kernel_stats)

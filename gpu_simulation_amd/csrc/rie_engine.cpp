@@ -614,9 +614,13 @@ void Engine::launch_bwd(int l, bool gather, bool write_cur, const double2 *cur_i
         gen_utab = d_chain_u + (size_t)gen_set * 2 * passes[0].g.num_tiles;
     }
     const bool pairs = l == fuse_layer;
+    // double buffering: the lambda-only passes load each workgroup's next tile
+    // (lam of a high pass, the state of the gather pass) behind the current
+    // one; the walk's one-sweep passes do not
+    const bool prefetch = double_buffer && !write_cur;
     launch_bwd_pass(pass_r[gather ? RIE_PASS_BWD_LOW : RIE_PASS_BWD_HIGH], gather, write_cur, true,
                     cur_in, lam_in, cur_out, lam_out, lc, pp.g, d_partials, pp.grid, s, pairs,
-                    pairs ? h : 0.0, gen_tab, gen_utab);
+                    pairs ? h : 0.0, gen_tab, gen_utab, prefetch);
 }
 
 bool Engine::gen_ready() const {
@@ -1745,6 +1749,20 @@ rie_status rie_engine_set_layer1_generation(rie_engine *engine, int enabled) {
 
 int rie_engine_layer1_generation(const rie_engine *engine) {
     return engine == nullptr ? -1 : (reinterpret_cast<const Engine *>(engine)->gen_ready() ? 1 : 0);
+}
+
+rie_status rie_engine_set_double_buffer(rie_engine *engine, int enabled) {
+    return guard([&]() {
+        Engine *e = as_engine(engine);
+        if (enabled != 0 && enabled != 1) {
+            rie::fail(RIE_INVALID_ARGUMENT, "enabled must be 0 or 1.");
+        }
+        e->double_buffer = enabled == 1;
+    });
+}
+
+int rie_engine_double_buffer(const rie_engine *engine) {
+    return engine == nullptr ? -1 : (reinterpret_cast<const Engine *>(engine)->double_buffer ? 1 : 0);
 }
 
 rie_status rie_engine_set_kernel_timing(rie_engine *engine, int enable) {

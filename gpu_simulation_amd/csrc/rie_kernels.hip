@@ -798,7 +798,10 @@ __device__ __forceinline__ void add_pair_terms(const double2 (&c)[Tile<R>::REGS]
 // kGen (high passes): the state is generated in mapping 0 from the chain table
 // set gen_tab (its chunk products copied to gen_chunks in LDS) and the tile's
 // uniform segments gen_u instead of read.
-template <int R, bool kGather, bool kHam, bool kGen = false>
+// kReady, high passes: lam of the tile is already in l (prefetched); low
+// (gather) pass: c already holds the tile's state as loaded from the two
+// image runs (load_cur_gather), before its placement through LDS.
+template <int R, bool kGather, bool kHam, bool kGen = false, bool kReady = false>
 __device__ __forceinline__ void bwd_load_tile(double2 (&c)[Tile<R>::REGS],
                                               double2 (&l)[Tile<R>::REGS],
                                               const double2 *cur_in, const double2 *lam_in,
@@ -822,7 +825,9 @@ __device__ __forceinline__ void bwd_load_tile(double2 (&c)[Tile<R>::REGS],
         for (int k = 0; k < TL::REGS; ++k) {
             const uint64_t src = (k < TL::REGS / 2 ? run : run1) + ((uint64_t)k << (T - R));
             l[k] = lam_in[src];
-            c[k] = cur_in[src];
+            if constexpr (!kReady) {
+                c[k] = cur_in[src];
+            }
         }
         __syncthreads();
 #pragma unroll
@@ -853,7 +858,9 @@ __device__ __forceinline__ void bwd_load_tile(double2 (&c)[Tile<R>::REGS],
 #pragma unroll
         for (int k = 0; k < TL::REGS; ++k) {
             const uint64_t idx = row + k * ta.strideA;
-            l[k] = lam_in[idx];
+            if constexpr (!kReady) {
+                l[k] = lam_in[idx];
+            }
             if constexpr (!kGen) {
                 c[k] = cur_in[idx];
             }
@@ -1178,7 +1185,41 @@ __device__ __forceinline__ void store_lam(const double2 (&l)[Tile<R>::REGS], dou
     }
 }
 
-template <int R, bool kGather, bool kHam, bool kGen>
+// Loads the state of a low (gather) pass's tile from its two image runs under
+// the ring CNOT, in the order bwd_load_tile places them through LDS.
+template <int R>
+__device__ __forceinline__ void load_cur_gather(double2 (&c)[Tile<R>::REGS],
+                                                const double2 *cur_in, uint64_t base,
+                                                const PassGeom &G, uint32_t t) {
+    using TL = Tile<R>;
+    const uint64_t pbase = ring_fwd(base, G.n);
+    const uint64_t run = (pbase & ~uint64_t{TL::SIZE - 1}) + (t ^ (pbase & 1u));
+    const uint64_t run1 = run ^ G.hi_mask;
+#pragma unroll
+    for (int k = 0; k < TL::REGS; ++k) {
+        c[k] = cur_in[(k < TL::REGS / 2 ? run : run1) + ((uint64_t)k << (TL::T - R))];
+    }
+}
+
+// Loads lam of a high pass's tile in the load mapping (mapping 0).
+template <int R>
+__device__ __forceinline__ void load_lam_high(double2 (&l)[Tile<R>::REGS],
+                                              const double2 *lam_in, uint64_t base,
+                                              const TileAddr &ta) {
+    const uint64_t row = base + ta.offA;
+#pragma unroll
+    for (int k = 0; k < Tile<R>::REGS; ++k) {
+        l[k] = lam_in[row + k * ta.strideA];
+    }
+}
+
+// kPrefetch: once the tile's state is no longer needed (after its
+// gradients), the registers it held receive the workgroup's next tile --
+// lam for a high pass, the state for the low (gather) pass -- so that load is
+// in flight during the lam-only sweep, the store and the next tile's start
+// instead of stalling it (a register double buffer: the state and the
+// prefetched array share registers).
+template <int R, bool kGather, bool kHam, bool kGen, bool kPrefetch = false>
 __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_lam_kernel(
     const double2 *cur_in, const double2 *lam_in, double2 *lam_out,
     const WireCoef *__restrict__ lc, PassGeom G, double *__restrict__ partials, double field,
@@ -1214,17 +1255,64 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_la
     __syncthreads();
 
     const TileAddr ta = tile_addr<R>(t, G);
+    static_assert(!(kPrefetch && kGather && kGen), "the gather pass reads the state");
+    double2 x_next[kPrefetch ? TL::REGS : 1];
+    auto prefetch_tile = [&](uint64_t gn, const TileAddr &a, uint32_t tt) {
+        if constexpr (kPrefetch) {
+            if constexpr (kGather) {
+                load_cur_gather<R>(x_next, cur_in, tile_base(gn, G, T), G, tt);
+            } else {
+                load_lam_high<R>(x_next, lam_in, tile_base(gn, G, T), a);
+            }
+        }
+    };
+    if constexpr (kPrefetch) {
+        if ((uint64_t)blockIdx.x < G.num_tiles) {
+            prefetch_tile(blockIdx.x, ta, t);
+        }
+    }
     for (uint64_t g = blockIdx.x; g < G.num_tiles; g += gridDim.x) {
         // the thread index is made opaque per tile so the LDS addresses of the
         // exchanges are formed where they are used, not held in
         // registers across the whole loop
         uint32_t tl = t;
         asm volatile("" : "+v"(tl));
+        // with the prefetch the per-register addresses of the tile's loads and
+        // stores are formed per tile as well (hoisted out of the loop they
+        // would be spilled)
+        TileAddr tla = ta;
+        if constexpr (kPrefetch) {
+            asm volatile("" : "+v"(tla.offA));
+        }
         const uint64_t base = tile_base(g, G, T);
         double2 c[TL::REGS];
         double2 l[TL::REGS];
-        bwd_load_tile<R, kGather, kHam, kGen>(c, l, cur_in, lam_in, base, ta, G, lds, tl, field,
-                                              gen_tab, gen_chunks, gen_utab + 2 * g);
+        if constexpr (kPrefetch) {
+#pragma unroll
+            for (int k = 0; k < TL::REGS; ++k) {
+                if constexpr (kGather) {
+                    c[k] = x_next[k];
+                } else {
+                    l[k] = x_next[k];
+                }
+            }
+        }
+        bwd_load_tile<R, kGather, kHam, kGen, kPrefetch>(c, l, cur_in, lam_in, base, tla, G, lds,
+                                                         tl, field, gen_tab, gen_chunks,
+                                                         gen_utab + 2 * g);
+        // the next tile, into the registers the state frees (called right
+        // after the state's last use; scheduling barriers keep the loads from
+        // being hoisted above it, where both would need registers)
+        auto prefetch_next = [&]() {
+            if constexpr (kPrefetch) {
+                __builtin_amdgcn_sched_barrier(0);
+                const uint64_t gn = g + gridDim.x;
+                if (gn < G.num_tiles) {
+                    prefetch_tile(gn, tla, tl);
+                }
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        };
         // sweep 1: phases and gradients (lam exchanged before cur, so the LDS
         // holds cur for the pair terms); sweep 2: RY^dagger on lam
         if constexpr (kGather) {
@@ -1250,6 +1338,7 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_la
                     add_pair_terms<R, 0>(c, l, lds, tl, G, w1 | w2 | window_mask<R, 3>(), field);
                 }
                 grad_window<R, 0>(c, l, lc, G, row_acc, lane, rz_tab);
+                prefetch_next();
                 ry_inv_window<R, 0>(l, tile_coef);
                 exchange<R, 0, 1>(l, lds, tl);
                 ry_inv_window<R, 1>(l, tile_coef);
@@ -1257,7 +1346,7 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_la
                 ry_inv_window<R, 2>(l, tile_coef);
                 exchange<R, 2, 3>(l, lds, tl);
                 ry_inv_window<R, 3>(l, tile_coef);
-                store_lam<R, 3, true>(l, lam_out, base, ta, tl);
+                store_lam<R, 3, true>(l, lam_out, base, tla, tl);
             } else {
                 // windows 1, 0, 2, then lam back through 1 to the contiguous
                 // mapping 0 for the store
@@ -1273,12 +1362,13 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_la
                     add_pair_terms<R, 2>(c, l, lds, tl, G, w1 | window_mask<R, 0>(), field);
                 }
                 grad_window<R, 2>(c, l, lc, G, row_acc, lane, rz_tab);
+                prefetch_next();
                 ry_inv_window<R, 2>(l, tile_coef);
                 exchange<R, 2, 1>(l, lds, tl);
                 ry_inv_window<R, 1>(l, tile_coef);
                 exchange<R, 1, 0>(l, lds, tl);
                 ry_inv_window<R, 0>(l, tile_coef);
-                store_lam<R, 0, true>(l, lam_out, base, ta, tl);
+                store_lam<R, 0, true>(l, lam_out, base, tla, tl);
             }
         } else {
             constexpr uint32_t w0 = window_mask<R, 0>();
@@ -1297,11 +1387,12 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_la
                 // unconditional: an unrotated bit 3 has phase 1 and the
                 // identity in tile_coef, and its gradient slot is not read
                 grad_lane_bit3<R>(c, l, row_acc, lane, rz_tab);
+                prefetch_next();
                 ry_inv_window<R, 2>(l, tile_coef);
                 ry_inv_lane_bit3<R>(l, tile_coef[3], lane);
                 exchange<R, 2, 0>(l, lds, tl);
                 ry_inv_window<R, 0>(l, tile_coef);
-                store_lam<R, 0, false>(l, lam_out, base, ta, tl);
+                store_lam<R, 0, false>(l, lam_out, base, tla, tl);
             } else {
                 uint32_t done = w0;
                 const bool w1_rot = G.rot_lo < R;
@@ -1330,6 +1421,7 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_la
                     add_pair_terms<R, 3>(c, l, lds, tl, G, done, field);
                 }
                 grad_window<R, 3>(c, l, lc, G, row_acc, lane, rz_tab);
+                prefetch_next();
                 ry_inv_window<R, 3>(l, tile_coef);
                 exchange<R, 3, 2>(l, lds, tl);
                 ry_inv_window<R, 2>(l, tile_coef);
@@ -1341,7 +1433,7 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_la
                     exchange<R, 2, 0>(l, lds, tl);
                 }
                 ry_inv_window<R, 0>(l, tile_coef);
-                store_lam<R, 0, false>(l, lam_out, base, ta, tl);
+                store_lam<R, 0, false>(l, lam_out, base, tla, tl);
             }
         }
     }
@@ -1361,7 +1453,7 @@ static void launch_bwd_pass_r(bool gather, bool write_cur, bool write_lam, const
                               const double2 *lam_in, double2 *cur_out, double2 *lam_out,
                               const WireCoef *lc, const PassGeom &g, double *partials, int grid,
                               hipStream_t s, bool ham, double field, const CMat2 *gen_tab,
-                              const CMat2 *gen_utab) {
+                              const CMat2 *gen_utab, bool prefetch) {
     constexpr int TH = Tile<R>::THREADS;
     const bool gen = gen_tab != nullptr;
     if (gen && (gen_utab == nullptr || gather || g.a > 4 || Tile<R>::T - R - g.a > 6 ||
@@ -1374,18 +1466,24 @@ static void launch_bwd_pass_r(bool gather, bool write_cur, bool write_lam, const
             // the sweep takes tile bit 3 across lanes and no other bit of window 1
             throw std::invalid_argument("high pass geometry: tile bits 0..2 must ride along");
         }
-#define RIE_BWD_LAM(GA, HM, GN)                                                              \
-    if (gather == GA && ham == HM && gen == GN) {                                            \
-        bwd_lam_kernel<R, GA, HM, GN><<<grid, TH, 0, s>>>(cur_in, lam_in, lam_out, lc, g,    \
-                                                          partials, field, gen_tab, gen_utab); \
+#define RIE_BWD_LAM(GA, HM, GN, PF)                                                          \
+    if (gather == GA && ham == HM && gen == GN && prefetch == PF) {                          \
+        bwd_lam_kernel<R, GA, HM, GN, PF><<<grid, TH, 0, s>>>(                               \
+            cur_in, lam_in, lam_out, lc, g, partials, field, gen_tab, gen_utab);             \
         return;                                                                              \
     }
-        RIE_BWD_LAM(true, false, false)
-        RIE_BWD_LAM(false, false, false)
-        RIE_BWD_LAM(true, true, false)
-        RIE_BWD_LAM(false, true, false)
-        RIE_BWD_LAM(false, false, true)
-        RIE_BWD_LAM(false, true, true)
+        RIE_BWD_LAM(true, false, false, false)
+        RIE_BWD_LAM(false, false, false, false)
+        RIE_BWD_LAM(true, true, false, false)
+        RIE_BWD_LAM(false, true, false, false)
+        RIE_BWD_LAM(false, false, true, false)
+        RIE_BWD_LAM(false, true, true, false)
+        RIE_BWD_LAM(true, false, false, true)
+        RIE_BWD_LAM(true, true, false, true)
+        RIE_BWD_LAM(false, false, false, true)
+        RIE_BWD_LAM(false, true, false, true)
+        RIE_BWD_LAM(false, false, true, true)
+        RIE_BWD_LAM(false, true, true, true)
 #undef RIE_BWD_LAM
     }
     if (gen) {
@@ -1416,15 +1514,15 @@ void launch_bwd_pass(int R, bool gather, bool write_cur, bool write_lam, const d
                      const double2 *lam_in, double2 *cur_out, double2 *lam_out,
                      const WireCoef *layer_coef, const PassGeom &g, double *partials, int grid,
                      hipStream_t s, bool ham_pairs, double field, const CMat2 *gen_tab,
-                     const CMat2 *gen_utab) {
+                     const CMat2 *gen_utab, bool prefetch) {
     if (R == 4) {
         launch_bwd_pass_r<4>(gather, write_cur, write_lam, cur_in, lam_in, cur_out, lam_out,
                              layer_coef, g, partials, grid, s, ham_pairs, field, gen_tab,
-                             gen_utab);
+                             gen_utab, prefetch);
     } else {
         launch_bwd_pass_r<3>(gather, write_cur, write_lam, cur_in, lam_in, cur_out, lam_out,
                              layer_coef, g, partials, grid, s, ham_pairs, field, gen_tab,
-                             gen_utab);
+                             gen_utab, prefetch);
     }
 }
 

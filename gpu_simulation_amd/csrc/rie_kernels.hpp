@@ -168,11 +168,14 @@ void launch_gen_layer1(int R, double2 *out, const CMat2 *tab, const CMat2 *utab,
 // gen_tab, gen_utab (layer 1, high pass on the lambda-only sweep): the state
 // is not read from cur_in but generated from a chain table set whose run is
 // mapping 0's register wires, and that set's per-tile segments.
+// prefetch (high passes on the lambda-only sweep): each workgroup loads its
+// next tile's lam into the registers the current tile's state frees.
 void launch_bwd_pass(int R, bool gather, bool write_cur, bool write_lam, const double2 *cur_in,
                      const double2 *lam_in, double2 *cur_out, double2 *lam_out,
                      const WireCoef *layer_coef, const PassGeom &g, double *partials, int grid,
                      hipStream_t s, bool ham_pairs = false, double field = 0.0,
-                     const CMat2 *gen_tab = nullptr, const CMat2 *gen_utab = nullptr);
+                     const CMat2 *gen_tab = nullptr, const CMat2 *gen_utab = nullptr,
+                     bool prefetch = false);
 
 // Hamiltonian tile pass: adds the X terms of the pass's wires (and the ZZ
 // diagonal if diag) of H psi, optionally into lam, and writes per-workgroup

@@ -45,6 +45,8 @@ C_ABI_SYMBOLS = (
     "rie_engine_fused_hamiltonian",
     "rie_engine_set_layer1_generation",
     "rie_engine_layer1_generation",
+    "rie_engine_set_double_buffer",
+    "rie_engine_double_buffer",
     "rie_engine_set_kernel_timing",
     "rie_engine_reset_kernel_stats",
     "rie_engine_kernel_stat_count",

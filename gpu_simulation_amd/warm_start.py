@@ -66,6 +66,11 @@ def features(fields, num_qubits: int, layers: int) -> np.ndarray:
     return (x * INPUT_SCALE).astype(np.float32)
 
 
+def branch_seed(seed: int, branch: int) -> tuple[int, int]:
+    """Seed of the shared random draw of training branch ``branch``."""
+    return (seed, 1000 + branch)
+
+
 @dataclass
 class TrainingData:
     fields: np.ndarray      # (per_branch,) fields of the selected branch
@@ -81,7 +86,7 @@ def generate_training_data(num_qubits: int = 10, layers: int = 4, count: int = 4
                            branches: int = 8) -> TrainingData:
     per_branch = count // branches
     fields = np.random.default_rng(seed).uniform(0.2, 2.0, per_branch)
-    draws = [reference_init(num_qubits, layers, (seed, 1000 + b)) for b in range(branches)]
+    draws = [reference_init(num_qubits, layers, branch_seed(seed, b)) for b in range(branches)]
     all_fields = np.tile(fields, branches)
     inits = [draws[b] for b in range(branches) for _ in range(per_branch)]
     _, final, energies = run_vqe(num_qubits, layers, all_fields, inits, steps, lr)
@@ -126,7 +131,12 @@ def steps_to_reach(history: np.ndarray, targets: np.ndarray, tol: float) -> np.n
 
 
 def evaluate(model: WarmStartModel, fields, steps: int = 300, lr: float = 0.05,
-             tol: float = 1e-4, seed: int = 44) -> dict:
+             tol: float = 1e-4, seed: int = 44, branch_init: np.ndarray | None = None) -> dict:
+    """Plain (random init) and warm (MLP prediction, random fallback) runs
+    side by side.  With ``branch_init`` (the shared random draw whose branch
+    supplied the training targets) a third arm starts every field from that
+    draw without the network, which separates what the MLP adds from what
+    choosing the branch alone gives."""
     n, layers = model.num_qubits, model.layers
     fields = np.asarray(fields, dtype=np.float64)
     count = len(fields)
@@ -139,8 +149,13 @@ def evaluate(model: WarmStartModel, fields, steps: int = 300, lr: float = 0.05,
     e_plain0, e_pred0 = e_judge[:count], e_judge[count:]
     use_pred = e_pred0 <= e_plain0
     warm = [predicted[i] if use_pred[i] else plain[i] for i in range(count)]
-    hist, _, _ = run_vqe(n, layers, np.concatenate([fields, fields]), plain + warm, steps, lr)
-    h_plain, h_warm = hist[:, :count], hist[:, count:]
+    starts = plain + warm
+    arm_fields = [fields, fields]
+    if branch_init is not None:
+        starts += [np.asarray(branch_init, dtype=np.float64)] * count
+        arm_fields.append(fields)
+    hist, _, _ = run_vqe(n, layers, np.concatenate(arm_fields), starts, steps, lr)
+    h_plain, h_warm = hist[:, :count], hist[:, count:2 * count]
     e_plain = h_plain.min(axis=0)  # plain-init converged energy: the run's lowest value
     e_warm = h_warm.min(axis=0)
     s_plain = steps_to_reach(h_plain, e_plain, tol)
@@ -148,6 +163,24 @@ def evaluate(model: WarmStartModel, fields, steps: int = 300, lr: float = 0.05,
     s_warm_own = steps_to_reach(h_warm, e_warm, tol)
     med_plain = float(np.median(s_plain))
     med_warm = float(np.median(s_warm))
+    branch = {}
+    if branch_init is not None:
+        h_branch = hist[:, 2 * count:]
+        e_branch = h_branch.min(axis=0)
+        s_branch = steps_to_reach(h_branch, e_plain, tol)
+        # a target no arm saturates: within tol of the lowest energy any arm found
+        best = np.minimum(np.minimum(e_plain, e_warm), e_branch)
+        branch = {
+            "median_steps_branch_draw": float(np.median(s_branch)),
+            "branch_draw_never_reached": int((s_branch >= steps).sum()),
+            "mean_converged_energy_branch_draw": float(e_branch.mean()),
+            "mean_start_energy_branch_draw": float(h_branch[0].mean()),
+            "median_steps_to_best_plain": float(np.median(steps_to_reach(h_plain, best, tol))),
+            "median_steps_to_best_warm": float(np.median(steps_to_reach(h_warm, best, tol))),
+            "median_steps_to_best_branch_draw": float(
+                np.median(steps_to_reach(h_branch, best, tol))),
+            "steps_branch_draw": s_branch,
+        }
     return {
         "count": count,
         "tol": tol,
@@ -164,6 +197,7 @@ def evaluate(model: WarmStartModel, fields, steps: int = 300, lr: float = 0.05,
         "mean_start_energy_warm": float(np.where(use_pred, e_pred0, e_plain0).mean()),
         "steps_plain": s_plain,
         "steps_warm": s_warm,
+        **branch,
     }
 
 
@@ -174,7 +208,8 @@ def run_pipeline(num_qubits: int = 10, layers: int = 4, train_count: int = 4096,
     model = WarmStartModel(num_qubits, layers)
     losses = model.fit(data.fields, data.params, mlp_steps, mlp_lr)
     eval_fields = np.random.default_rng(44).uniform(0.2, 2.0, eval_count)
-    res = evaluate(model, eval_fields, steps, lr, tol, seed=44)
+    branch_init = reference_init(num_qubits, layers, branch_seed(4, data.branch))
+    res = evaluate(model, eval_fields, steps, lr, tol, seed=44, branch_init=branch_init)
     res["training_runs"] = data.runs
     res["training_branch_mean_energies"] = data.branch_mean_energies.tolist()
     res["mlp_loss_first"] = float(losses[0]) if len(losses) else float("nan")

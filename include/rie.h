@@ -67,7 +67,8 @@ int rie_device_count(void);
  * (cpp/ising_cuda_backend.hpp:29-31, cpp/ising_cuda_backend.cu:374-419):
  * num_qubits in [2, 34], num_layers >= 1, structured_rotation_chunk_width in
  * [1, 8] for structured_adjoint, dense_scan requires num_qubits <= 8.
- * double_buffer is accepted for interface compatibility. */
+ * double_buffer: 1 makes the adjoint high passes double-buffered (see
+ * rie_engine_set_double_buffer); results are identical either way. */
 rie_status rie_engine_create(size_t num_qubits, size_t num_layers, double field,
                              const char *gradient_strategy,
                              size_t structured_rotation_chunk_width,
@@ -184,6 +185,20 @@ int rie_engine_fused_hamiltonian(const rie_engine *engine);
  * rie_engine_layer1_generation reports whether evaluations use it. */
 rie_status rie_engine_set_layer1_generation(rie_engine *engine, int enabled);
 int rie_engine_layer1_generation(const rie_engine *engine);
+
+/* Double-buffered adjoint high passes.  Replaces the double_buffer flag of
+ * RingIsingCudaBackend (cpp/ising_cuda_backend.cu:374-419), which selects a
+ * backward rotation-chunk kernel that stages the next chunk while computing
+ * the current one (cpp/ising_cuda_structured_adjoint_modes.inc:174-185,
+ * cpp/ising_cuda_statevector_grad_kernels.cu:1742-1925).  Here the
+ * lambda-only high passes of the structured_adjoint backward sweep load each
+ * workgroup's next tile of lambda into the registers that the current tile's
+ * state frees after its gradients, so that load overlaps the rest of the
+ * tile.  The arithmetic is unchanged: results are bitwise identical with and
+ * without.  enabled: 1 on, 0 off; set at creation by double_buffer and
+ * reported by rie_engine_double_buffer. */
+rie_status rie_engine_set_double_buffer(rie_engine *engine, int enabled);
+int rie_engine_double_buffer(const rie_engine *engine);
 
 /* Register shape of the tile passes (no counterpart in the reference): each
  * of r_fwd, r_ham, r_bwd is 4 (256 threads x 16 registers per 4096-amplitude

@@ -365,3 +365,43 @@ def test_module_level_energy_and_grad(oracle, backend_module):
     assert np.max(np.abs(out["gradient"] - g_ref)) <= TOL
     with pytest.raises(ValueError):
         backend_module.energy_and_grad(n, layers, 1.0, "structured_adjoint", params)  # 3-D
+
+
+@pytest.mark.parametrize("schedule", ["stored", "inverse_walk"])
+@pytest.mark.parametrize("n,layers,shapes", [(13, 2, None), (16, 3, None), (22, 2, None),
+                                              (22, 3, (3, 3, 3)), (25, 2, (4, 3, 4)),
+                                              (31, 2, None)])
+def test_double_buffer_is_result_identical(oracle, backend_module, n, layers, shapes, schedule):
+    """double_buffer (next tile prefetched by the lambda-only adjoint passes)
+    changes no arithmetic: bitwise-equal energy and gradient with it on and
+    off, and the oracle's values where it fits."""
+    params = np.ascontiguousarray(_params(7000 + n, n, layers).reshape(-1))
+    outs = []
+    for db in (False, True):
+        print(f"case double_buffer={db} n={n} L={layers} schedule={schedule} shapes={shapes}",
+              flush=True)
+        eng = backend_module.RingIsingCudaBackend(n, layers, 1.0, "structured_adjoint", 8, db)
+        eng.set_schedule(schedule)
+        if shapes is not None:
+            eng.set_tile_shapes(*shapes)
+        assert eng.double_buffer is db
+        out = eng.energy_and_grad(params)
+        outs.append((out["energy"], np.asarray(out["gradient"])))
+        eng.set_double_buffer(int(not db))
+        assert eng.double_buffer is (not db)
+        again = eng.energy_and_grad(params)
+        assert again["energy"] == out["energy"]
+        assert np.array_equal(np.asarray(again["gradient"]), outs[-1][1])
+        del eng
+    assert outs[0][0] == outs[1][0]
+    assert np.array_equal(outs[0][1], outs[1][1])
+    if n <= 25:
+        e_ref, g_ref = oracle.energy_and_grad(n, layers, 1.0, params, form="structured")
+        assert abs(outs[1][0] - e_ref) <= TOL
+        assert np.max(np.abs(outs[1][1] - g_ref)) <= TOL
+
+
+def test_double_buffer_rejects_bad_values(backend_module):
+    eng = backend_module.RingIsingCudaBackend(13, 2, 1.0, "structured_adjoint", 8, False)
+    with pytest.raises(ValueError):
+        eng.set_double_buffer(2)

@@ -48,18 +48,28 @@ def test_training_steps_follow_oracle_adam(backend_module):
     assert np.max(np.abs(p - p_ref)) <= 1e-2 * lr
 
 
-def test_learned_warm_start_beats_random_init():
+@pytest.mark.parametrize("n,layers,count,mlp_steps", [(6, 3, 512, 1500), (10, 4, 1024, 2000)])
+def test_learned_warm_start_beats_random_init(n, layers, count, mlp_steps):
+    """Plain, warm and branch-draw arms (the last starts every field from the
+    selected training draw without the network); 10 qubits x 4 layers is the
+    configured warm-start shape, here with 1024 training runs and 64 held-out
+    fields instead of 4096 and 512."""
     from gpu_simulation_amd import warm_start as ws
 
-    n, layers, steps, lr = 6, 3, 200, 0.05
-    data = ws.generate_training_data(n, layers, count=512, steps=steps, lr=lr, seed=4)
+    steps, lr = 300 if n == 10 else 200, 0.05
+    print(f"case warm start n={n} L={layers} training runs={count}", flush=True)
+    data = ws.generate_training_data(n, layers, count=count, steps=steps, lr=lr, seed=4)
     model = ws.WarmStartModel(n, layers, hidden=256, seed=0)
-    losses = model.fit(data.fields, data.params, steps=1500, lr=1e-3)
+    losses = model.fit(data.fields, data.params, steps=mlp_steps, lr=1e-3)
     print("branch", data.branch, data.branch_mean_energies, "loss", losses[0], losses[-1],
           flush=True)
     assert losses[-1] < 0.1 * losses[0]
-    res = ws.evaluate(model, np.random.default_rng(44).uniform(0.2, 2.0, 64), steps, lr, 1e-4)
+    branch_init = ws.reference_init(n, layers, ws.branch_seed(4, data.branch))
+    res = ws.evaluate(model, np.random.default_rng(44).uniform(0.2, 2.0, 64), steps, lr, 1e-4,
+                      branch_init=branch_init)
     print({k: v for k, v in res.items() if not isinstance(v, np.ndarray)}, flush=True)
     assert res["median_steps_warm"] < res["median_steps_plain"]
     # the warm runs converge at least as low as the plain ones on average
     assert res["mean_converged_energy_warm"] <= res["mean_converged_energy_plain"] + 1e-3
+    # the network's predictions start lower than the shared draw of its branch
+    assert res["mean_start_energy_warm"] < res["mean_start_energy_branch_draw"]

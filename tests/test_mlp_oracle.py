@@ -36,3 +36,19 @@ def test_gelu_reference_values():
     h = 1e-6
     fd = (mlp_oracle.gelu(x + h) - mlp_oracle.gelu(x - h)) / (2 * h)
     assert np.max(np.abs(fd - mlp_oracle.gelu_grad(x))) <= 1e-9
+
+
+def test_adam_first_step_closed_form():
+    # after one step m_hat = g and v_hat = g^2 exactly, so
+    # p1 = p0 - lr * g / (|g| + eps): a sign step of size lr where |g| >> eps
+    inputs, hidden, outputs = 3, 8, 5
+    rng = np.random.default_rng(3)
+    count = inputs * hidden + hidden + hidden * hidden + hidden + hidden * outputs + outputs
+    p0 = 0.3 * rng.standard_normal(count)
+    x = rng.uniform(-1, 1, (16, inputs))
+    t = rng.standard_normal((16, outputs))
+    lr, eps = 1e-2, 1e-8
+    _, g = mlp_oracle.loss_and_grad(p0, x, t, inputs, hidden, outputs)
+    p1, losses = mlp_oracle.adam(p0, x, t, inputs, hidden, outputs, 1, lr, eps=eps)
+    np.testing.assert_allclose(p1, p0 - lr * g / (np.abs(g) + eps), rtol=0, atol=1e-15)
+    assert losses.shape == (1,)
