@@ -40,7 +40,7 @@ HIP_FLAGS = [
     f"-I{CSRC}",
 ]
 HIP_SOURCES = ["rie_kernels.hip", "rie_dense.hip", "rie_batch.hip", "rie_mlp.hip",
-               "rie_engine.cpp", "rie_batch.cpp", "rie_mlp.cpp"]
+               "rie_noalloc.hip", "rie_engine.cpp", "rie_batch.cpp", "rie_mlp.cpp"]
 
 
 def _run_parallel(commands: list[list[str]], verbose: bool) -> None:

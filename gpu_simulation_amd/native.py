@@ -28,6 +28,7 @@ C_ABI_SYMBOLS = (
     "rie_engine_timing",
     "rie_engine_benchmark_forward_graph",
     "rie_energy_and_grad",
+    "rie_energy_and_grad_noalloc",
     "rie_engine_num_qubits",
     "rie_engine_num_layers",
     "rie_engine_strategy",

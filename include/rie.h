@@ -112,6 +112,31 @@ rie_status rie_energy_and_grad(size_t num_qubits, size_t num_layers, double fiel
                                size_t structured_rotation_chunk_width,
                                double *energy_out, double *gradient_out);
 
+/* Allocation-free energy_and_grad of one small circuit (structured_adjoint,
+ * whole circuit in one workgroup).  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  It calls no
+ * hipMalloc, hipFree or allocating library routine: the coefficient table and
+ * the state scratch (n = 13, 14) live in a file-scope device array of
+ * RIE_NOALLOC_SCRATCH_BYTES per device, held until the process ends.  Every
+ * call on a device shares that array, so calls that overlap on one device
+ * race; serialize them (one stream, or wait between streams).
+ * num_qubits in [2, RIE_NOALLOC_MAX_QUBITS], num_layers in
+ * [1, RIE_NOALLOC_MAX_LAYERS]; larger problems get RIE_INVALID_ARGUMENT (use
+ * rie_energy_and_grad or an engine).  d_params (2*n*L doubles), d_energy (1)
+ * and d_gradient (2*n*L, or NULL for energy only) are device pointers; the
+ * call is asynchronous on stream (a hipStream_t, NULL: the default stream).
+ * Same results as rie_energy_and_grad with the structured_adjoint strategy
+ * and the same structured_rotation_chunk_width (bitwise for n <= 12). */
+#define RIE_NOALLOC_MAX_QUBITS 14
+#define RIE_NOALLOC_MAX_LAYERS 1024
+#define RIE_NOALLOC_SCRATCH_BYTES \
+    (32u * RIE_NOALLOC_MAX_QUBITS * RIE_NOALLOC_MAX_LAYERS + 32u * (1u << RIE_NOALLOC_MAX_QUBITS))
+rie_status rie_energy_and_grad_noalloc(size_t num_qubits, size_t num_layers, double field,
+                                       const double *d_params, size_t num_params,
+                                       size_t structured_rotation_chunk_width,
+                                       double *d_energy, double *d_gradient, void *stream);
+
 /* Engine properties. */
 size_t rie_engine_num_qubits(const rie_engine *engine);
 size_t rie_engine_num_layers(const rie_engine *engine);
