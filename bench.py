@@ -291,6 +291,17 @@ def batched_mix(count: int, steps: int, torch) -> dict:
              for i in range(count))
     same = bool(np.array_equal(e_b, e_p) and all(np.array_equal(a, b) for a, b in zip(g_b, g_p)))
     del engines
+    # the same circuits with n <= 8 on the whole-circuit statevector kernel
+    # instead of dense_scan (a measured alternative to the configured split)
+    sv = RingIsingBatch(specs, dense_max_qubits=0)
+    sv.set_params(params)
+    sv.autotune(repeats=2)
+    sv.native.set_use_graph(True)
+    sv_s = timed(lambda: sv.native.run_device(True, stream), max(3, steps))
+    e_sv, g_sv = sv.results()
+    de_sv = float(np.max(np.abs(np.asarray(e_sv) - np.asarray(e_b))))
+    dg_sv = max(float(np.max(np.abs(a - b))) for a, b in zip(g_sv, g_b))
+    del sv
     n_dense = sum(1 for s_ in specs if s_.num_qubits <= 8)
     return {
         "workload": (f"{count} circuits, n ~ U{{4..14}}, L ~ U{{8,32,128}}, h ~ U[0.2,2.0], "
@@ -307,6 +318,10 @@ def batched_mix(count: int, steps: int, torch) -> dict:
         "max_abs_dE_vs_engine": de,
         "max_abs_dgrad_vs_engine": dg,
         "bitwise_equal_to_per_circuit_launch": same,
+        "all_statevector_ms": round(sv_s * 1e3, 3),
+        "all_statevector_evals_per_s": round(count / sv_s, 1),
+        "all_statevector_max_abs_dE_vs_batched": de_sv,
+        "all_statevector_max_abs_dgrad_vs_batched": dg_sv,
         "autotune_s": round(tune_s, 2),
         "config": {k: cfg[k] for k in ("threads", "grid", "dense_tile", "chunk_by_n")},
     }

@@ -10,12 +10,15 @@
 // independent 16-byte loads per lane in flight (all U issued before the first
 // use), over a grid of G workgroups per CU; the sweep covers read-only
 // (a reduction that keeps every load), write-only and copy, each at
-// U in {1, 4, 8, 16} and G in {4, 8, 16, 32}.  The largest rate of each kind
-// is the achievable HBM rate for that access pattern; "copy" variants also
+// U in {1, 4, 8, 16} and G in {4, 8, 16, 32}, then the tile-pass patterns of
+// the engine (tile_kernel below: runs of 128 B .. 512 B at large strides, and
+// the contiguous 64 KB tile).  The largest rate of each kind is the
+// achievable HBM rate for that access pattern; "copy" variants also
 // calibrate profiler FETCH_SIZE / WRITE_SIZE counters to bytes.
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
+#include <cstdint>
 #include <cstdlib>
 
 #define CHECK(x)                                                                  \
@@ -69,6 +72,41 @@ __global__ __launch_bounds__(kThreads) void copy_kernel(const double2 *__restric
 #pragma unroll
         for (int u = 0; u < U; ++u) out[base + (size_t)u * kThreads + threadIdx.x] = v[u];
     }
+}
+
+// Tile-pass access pattern: a workgroup of 256 threads moves 4096-amplitude
+// tiles whose offset bits [0, a) are index bits [0, a) (runs of 16 * 2^a
+// contiguous bytes) and offset bits [a, 12) are index bits [h0, h0 + 12 - a);
+// thread t holds offsets t + 256 k, k = 0..15 (16 loads in flight per lane),
+// as the engine's high passes (a = 3, 128-byte runs) and low passes (a = 12)
+// load their tiles.
+template <bool kCopy>
+__global__ __launch_bounds__(kThreads) void tile_kernel(const double2 *__restrict__ in,
+                                                        double2 *__restrict__ out, int nbits,
+                                                        int a, int h0, double *__restrict__ sink) {
+    const uint64_t tiles = uint64_t{1} << (nbits - 12);
+    const int gap = h0 - a;
+    double acc = 0.0;
+    for (uint64_t g = blockIdx.x; g < tiles; g += gridDim.x) {
+        const uint64_t base = ((g & ((uint64_t{1} << gap) - 1)) << a) | ((g >> gap) << (h0 + 12 - a));
+        uint64_t idx[16];
+        double2 v[16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const uint32_t o = threadIdx.x + 256u * (uint32_t)k;
+            idx[k] = base | (o & ((1u << a) - 1u)) | ((uint64_t)(o >> a) << h0);
+            v[k] = in[idx[k]];
+        }
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            if constexpr (kCopy) {
+                out[idx[k]] = v[k];
+            } else {
+                acc += v[k].x + v[k].y;
+            }
+        }
+    }
+    if (acc == 12345.678) sink[0] = acc;  // never true for the zero-filled input
 }
 
 int main(int argc, char **argv) {
@@ -128,6 +166,28 @@ int main(int argc, char **argv) {
         run("copy", 4, g, 2 * S, [&]() { copy_kernel<4><<<grid, kThreads>>>(a, b, n); });
         run("copy", 8, g, 2 * S, [&]() { copy_kernel<8><<<grid, kThreads>>>(a, b, n); });
         run("copy", 16, g, 2 * S, [&]() { copy_kernel<16><<<grid, kThreads>>>(a, b, n); });
+    }
+    // tile-pass patterns over a 2^nbits-amplitude state (nbits = floor(log2 n))
+    int nbits = 0;
+    while ((size_t{2} << nbits) <= n) ++nbits;
+    for (int g : {2, 4, 8}) {
+        const int grid = cus * g;
+        for (int lo : {3, 4, 5, 12}) {
+            for (int h0 : {12, nbits - 12 + lo}) {
+                if (lo == 12 && h0 != 12) continue;  // the contiguous low tile
+                if (h0 < 12 || h0 + 12 - lo > nbits) continue;
+                const double Sb = (double)(size_t{1} << nbits) * sizeof(double2);
+                char name[64];
+                std::snprintf(name, sizeof(name), "tile_read_a%d_h%d", lo, h0);
+                run(name, 16, g, Sb, [&]() {
+                    tile_kernel<false><<<grid, kThreads>>>(a, b, nbits, lo, h0, sink);
+                });
+                std::snprintf(name, sizeof(name), "tile_copy_a%d_h%d", lo, h0);
+                run(name, 16, g, 2 * Sb, [&]() {
+                    tile_kernel<true><<<grid, kThreads>>>(a, b, nbits, lo, h0, sink);
+                });
+            }
+        }
     }
     CHECK(hipFree(a));
     CHECK(hipFree(b));
