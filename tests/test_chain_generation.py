@@ -218,3 +218,121 @@ def test_tile_tables_at_the_largest_geometries(n, R):
         return lambda i: _direct_trace(sites, n, i)
 
     _check_sampled(rng, n, R, params, want_at, tiles=3, threads=6)
+
+
+def _ring_fwd(i, n):
+    """P(i): bits k >= 1 the prefix XOR of bits 0..k, bit 0 the parity of bits 1..n-1."""
+    j = 0
+    acc = 0
+    for k in range(n):
+        acc ^= (i >> k) & 1
+        if k >= 1:
+            j |= acc << k
+    j |= (bin(i >> 1).count("1") & 1)
+    return j
+
+
+def _walls_after_ring(i):
+    """Domain walls of P(i) around the ring, in closed form from i."""
+    hi = bin(i >> 2).count("1")
+    b0 = i & 1
+    return hi + (b0 ^ (hi & 1)) + b0
+
+
+def _flip_mask(w, n):
+    """P^-1 e_w: bits w, w + 1 (w < n - 1), or n - 1, 0, 1 (w = n - 1)."""
+    return (1 << w) | (1 << (w + 1)) if w < n - 1 else (1 << (n - 1)) | 3
+
+
+def _crossing_wires(n):
+    """The X terms the generating low pass takes besides its in-tile pairs:
+    wire T - 1, the last wire of every high group but the last, and n - 1."""
+    rest, width = n - T, T - 3
+    groups = (rest + width - 1) // width
+    g0, firsts = T, []
+    for gi in range(groups):
+        firsts.append(g0)
+        g0 += rest // groups + (1 if gi < rest % groups else 0)
+    return sorted({T - 1, *(f - 1 for f in firsts[1:]), n - 1})
+
+
+def _high_group_pairs(n):
+    """X_w the high passes take: pairs w, w + 1 inside one high group (the
+    group's wires from its first to the one before its last)."""
+    starts = [T] + [c + 1 for c in _crossing_wires(n) if T <= c < n - 1]
+    ends = [c for c in _crossing_wires(n) if c >= T]
+    return [w for s, e in zip(starts, ends) for w in range(s, e)]
+
+
+@pytest.mark.parametrize("n", [13, 16, 22, 25, 30, 31])
+def test_hamiltonian_terms_are_split_exactly_once(n):
+    # in the pre-CNOT frame X_w of H is the flip P^-1 e_w; the generating low
+    # pass takes w = 0 .. T - 2 (pairs inside its tile) and the crossing wires,
+    # the high passes the pairs inside their groups: every w exactly once
+    taken = list(range(T - 1)) + _crossing_wires(n) + _high_group_pairs(n)
+    assert sorted(taken) == list(range(n))
+    # the groups' pairs lie inside the groups the engine plans
+    for w in _high_group_pairs(n):
+        assert any((rot >> w) & 1 and (rot >> (w + 1)) & 1 for _, _, rot in _plan_cases(n)[1:])
+    for w in range(n):
+        assert _ring_fwd(_flip_mask(w, n), n) == 1 << w  # P^-1 e_w is X_w's flip
+
+
+@pytest.mark.parametrize("n", [5, 9, 13])
+def test_diagonal_after_ring_closed_form(n):
+    for i in range(1 << n):
+        j = _ring_fwd(i, n)
+        rot = (j >> 1) | ((j & 1) << (n - 1))
+        assert bin(j ^ rot).count("1") == _walls_after_ring(i), i
+
+
+@pytest.mark.parametrize("n", [13, 16, 22])
+@pytest.mark.parametrize("R", [3, 4])
+def test_generating_low_pass_start_equals_hamiltonian(oracle, n, R):
+    # lambda at the start of layer 1's backward sweep, in the pre-CNOT frame:
+    # P^-1 H psi.  The generating low pass forms it for its tile as
+    # -ZZ(P i) phi(i) - h sum_{w < T-1} phi(i ^ m_w) - h sum_{w crossing} phi(i ^ m_w),
+    # the flipped amplitudes of crossing wires from the chain with U_A taken at
+    # base ^ (m_w above the tile) and the register / thread index flipped by m_w
+    # inside it; the high passes add -h phi(i ^ m_w) for the pairs inside
+    # their groups.  Sampled tiles and threads against the oracle.
+    rng = np.random.default_rng(130 + n + R)
+    field = 0.85
+    params = rng.uniform(0.0, 2.0 * np.pi, size=(2, n, 2))
+    l0, l1 = _coefs(params[0]), _coefs(params[1])
+    full = (1 << n) - 1
+    psi = oracle.state(n, 2, params)
+    hpsi = oracle.apply_hamiltonian(n, field, psi)
+    sites, r0, run, chunks, ua, ub = _set(n, R, T, T, full, l0, l1)
+    (f0, n0, t0), (f1, n1, t1) = chunks
+    hi_pairs = _high_group_pairs(n)
+
+    def phi_at(i):
+        return _direct_trace(sites, n, i)
+
+    err = 0.0
+    for g in rng.integers(0, 1 << (n - T), 3):
+        base = int(g) << T
+        for t in rng.integers(0, 1 << (T - R), 8):
+            t = int(t)
+            for k in range(1 << R):
+                o = (k << (T - R)) | t
+                i = base | o
+                y = _uniform(sites, ua, base) @ t0[t & 15] @ t1[(t >> 4) & ((1 << n1) - 1)]
+                ph = np.trace(run[k] @ y)
+                lam = -(n - 2 * _walls_after_ring(i)) * ph
+                for w in range(T - 1):
+                    lam -= field * phi_at(i ^ _flip_mask(w, n))
+                for w in _crossing_wires(n):
+                    m = _flip_mask(w, n)
+                    hi_m, lo_m = m & ~((1 << T) - 1), m & ((1 << T) - 1)
+                    k2 = k ^ ((lo_m >> (T - R)) & ((1 << R) - 1))
+                    t2 = t ^ (lo_m & ((1 << (T - R)) - 1))
+                    y2 = (_uniform(sites, ua, base ^ hi_m) @ t0[t2 & 15] @
+                          t1[(t2 >> 4) & ((1 << n1) - 1)])
+                    lam -= field * np.trace(run[k2] @ y2)
+                for w in hi_pairs:
+                    lam -= field * phi_at(i ^ _flip_mask(w, n))
+                want = hpsi[oracle.ring_cnot_index(i, n)]
+                err = max(err, abs(ph - psi[oracle.ring_cnot_index(i, n)]), abs(lam - want))
+    assert err <= 1e-12
