@@ -225,6 +225,12 @@ class RingIsingCudaBackend {
 
     bool layer1_generation() const { return rie_engine_layer1_generation(engine_) == 1; }
 
+    void set_generated_start(int enabled) {
+        check(rie_engine_set_generated_start(engine_, enabled));
+    }
+
+    bool generated_start() const { return rie_engine_generated_start(engine_) == 1; }
+
     void set_double_buffer(int enabled) { check(rie_engine_set_double_buffer(engine_, enabled)); }
 
     bool double_buffer() const { return rie_engine_double_buffer(engine_) == 1; }
@@ -566,6 +572,9 @@ PYBIND11_MODULE(_cuda_backend, m) {
         .def("set_layer1_generation", &RingIsingCudaBackend::set_layer1_generation,
              py::arg("enabled"))
         .def_property_readonly("layer1_generation", &RingIsingCudaBackend::layer1_generation)
+        .def("set_generated_start", &RingIsingCudaBackend::set_generated_start,
+             py::arg("enabled"))
+        .def_property_readonly("generated_start", &RingIsingCudaBackend::generated_start)
         .def("set_double_buffer", &RingIsingCudaBackend::set_double_buffer, py::arg("enabled"))
         .def_property_readonly("double_buffer", &RingIsingCudaBackend::double_buffer)
         .def_property_readonly("schedule", &RingIsingCudaBackend::schedule)

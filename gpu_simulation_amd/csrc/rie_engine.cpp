@@ -238,6 +238,11 @@ Engine::Engine(size_t num_qubits, size_t num_layers, double field, const std::st
             // per-tile uniform segments of each table set (128 B per tile and set)
             d_chain_u = (CMat2 *)device_alloc(sizeof(CMat2) * 2 * tiles * passes.size(),
                                               "alloc chain segments");
+            if (L == 2 && fuse_ready) {
+                // the crossing wires of the generated start (at most kMaxCross)
+                d_chain_x = (CMat2 *)device_alloc(sizeof(CMat2) * kMaxCross * tiles,
+                                                  "alloc chain flips");
+            }
         }
         high_env_chunks = (int)std::min<uint64_t>(64, (tiles + 255) / 256);
         partial2_count = 4 * (size_t)high_env_chunks * (size_t)std::max(1, n - T);
@@ -275,6 +280,7 @@ Engine::~Engine() {
     device_free(d_tile_sums);
     device_free(d_chain);
     device_free(d_chain_u);
+    device_free(d_chain_x);
     device_free(d_partials2);
     dense_free();
     for (auto &p : kpending) {
@@ -631,12 +637,14 @@ bool Engine::gen_ready() const {
 // Table sets of the current register shapes: 0 the forward (low tile, all of
 // layer 1 rotated, run at mapping 0), 1 + j the adjoint high pass j (the
 // wires of high groups 0 .. j rotated, run at mapping 0's registers).
-void Engine::build_chain_tables(hipStream_t s) {
+void Engine::build_chain_tables(hipStream_t s, int r_set0) {
     ChainSetup cs{};
     cs.n = n;
     cs.tiles = passes[0].g.num_tiles;
     const uint64_t all = (n >= 64) ? ~uint64_t{0} : ((uint64_t{1} << n) - 1);
-    const int rf = pass_r[RIE_PASS_FWD_GEN];
+    // set 0's register run: the forward generator's shape, or the caller's
+    // (the generated start's low pass)
+    const int rf = r_set0 > 0 ? r_set0 : pass_r[RIE_PASS_FWD_GEN];
     const int rh = pass_r[RIE_PASS_BWD_HIGH];
     // the forward's low tile: run at the tile's top, C0 = wires 0..3, C1 the
     // rest below the run, no U_B
@@ -685,7 +693,97 @@ void Engine::build_chain_tables(hipStream_t s) {
     kt_begin(s);
     launch_chain_tables(d_coef, d_coef + n, cs, d_chain, d_chain_u, s);
     check_launch("chain_tables");
+    if (r_set0 > 0) {
+        // U_A of set 0 at the crossing wires' flips above the tile
+        launch_chain_flips(d_coef, d_coef + n, cs, cross, d_chain_x, s);
+        check_launch("chain_flips");
+    }
     kt_end(s, "chain_tables", 0.0);
+}
+
+bool Engine::gen_start_ready() const {
+    return gen_start_enabled && gen_ready() && L == 2 && fuse_ready && fuse_enabled &&
+           d_chain_x != nullptr && ham_fused_g.sc_n <= kMaxCross;
+}
+
+// Gradient evaluation of a two-layer circuit without forward or Hamiltonian
+// passes (see gen_start_enabled): 1 S (generating low pass) + 2 S per high
+// pass + 1 S (layer-0 contraction).
+void Engine::structured_eval_gen_start(hipStream_t s, const double *scale, double *energy_out,
+                                       bool staged) {
+    const double S = (double)sizeof(double2) * (double)N;
+    const int r_low = pass_r[RIE_PASS_BWD_LOW];
+    // the crossing wires' flips P^-1 e_w split at the low tile's top bit
+    const uint64_t tile_mask = (uint64_t{1} << T) - 1;
+    cross = CrossFlips{};
+    cross.count = ham_fused_g.sc_n;
+    for (int c = 0; c < cross.count; ++c) {
+        const int w = ham_fused_g.sc[c];
+        const uint64_t m = w < n - 1 ? (uint64_t{3} << w) : ((uint64_t{1} << (n - 1)) | 3u);
+        const uint64_t lo = m & tile_mask;
+        cross.hi[c] = m & ~tile_mask;
+        cross.reg[c] = (int)((lo >> (T - r_low)) & ((1u << r_low) - 1u));
+        cross.thread[c] = (uint32_t)(lo & ((uint64_t{1} << (T - r_low)) - 1));
+    }
+    if (staged) {
+        stage_begin(s);
+    }
+    build_chain_tables(s, r_low);
+    gen_fwd = false;
+    final_ptr = nullptr;  // no state is written
+    if (staged) {
+        stage_end(s, "forward_apply_ms");
+        stage_begin(s);
+        stage_end(s, "hamiltonian_ms");  // formed inside the low adjoint pass
+        stage_begin(s);
+        stage_end(s, "gradient_zero_ms");
+        stage_begin(s);
+    }
+    PassGeom g0 = passes[0].g;
+    g0.pair_lo = 0;  // every adjacent pair inside the low tile
+    g0.pair_hi = T;
+    double2 *lam = pool[0];
+    const int K = (int)passes.size() - 1;
+    fuse_layer = 1;
+    l0_energy_out = energy_out;
+    try {
+        require_partials((size_t)passes[0].grid * 2 * T);
+        kt_begin(s);
+        launch_bwd_low_gen(r_low, lam, d_coef + n, g0, d_partials, passes[0].grid, s, h, d_chain,
+                           d_chain_u, d_chain_x, cross);
+        check_launch("bwd_pass low gen");
+        kt_end(s, "bwd_pass_low_gen", S);
+        kt_begin(s);
+        launch_reduce_grad(d_partials, passes[0].grid, T, passes[0].g, 1, d_grad, scale, s);
+        check_launch("reduce_grad");
+        kt_end(s, "reduce_grad", 0.0);
+        for (int j = K - 1; j >= 0; --j) {
+            const PassPlan &pp = passes[1 + (size_t)j];
+            require_partials((size_t)pp.grid * 2 * T);
+            kt_begin(s);
+            launch_bwd(1, false, false, nullptr, lam, nullptr, lam, (size_t)(1 + j), s, 1 + j);
+            check_launch("bwd_pass");
+            kt_end(s, "bwd_pass_high_gen", 2 * S);
+            kt_begin(s);
+            launch_reduce_grad(d_partials, pp.grid, T, pp.g, 1, d_grad, scale, s);
+            check_launch("reduce_grad");
+            kt_end(s, "reduce_grad", 0.0);
+        }
+        layer0_gradients(s, lam, scale);
+    } catch (...) {
+        fuse_layer = -1;
+        l0_energy_out = nullptr;
+        throw;
+    }
+    fuse_layer = -1;
+    l0_energy_out = nullptr;
+    if (staged) {
+        stage_end(s, "backward_sweep_ms");
+        stage_begin(s);
+        stage_end(s, "energy_dot_ms");  // written by the layer-0 contraction
+    }
+    energy_partials = 0;
+    resident = false;
 }
 
 bool Engine::fused_eval() const {
@@ -717,6 +815,10 @@ void Engine::structured_hamiltonian_fused(hipStream_t s) {
 // contraction writes the energy.
 void Engine::structured_eval_fused(hipStream_t s, const double *scale, double *energy_out,
                                    bool staged) {
+    if (gen_start_ready()) {
+        structured_eval_gen_start(s, scale, energy_out, staged);
+        return;
+    }
     if (staged) {
         stage_begin(s);
     }
@@ -1232,7 +1334,7 @@ int pass_kind_of(const std::string &name) {
     if (name.rfind("ham_pass", 0) == 0) {
         return RIE_PASS_HAM;
     }
-    if (name == "bwd_pass_low_cnot") {
+    if (name == "bwd_pass_low_cnot" || name == "bwd_pass_low_gen") {
         return RIE_PASS_BWD_LOW;
     }
     if (name == "bwd_pass_high" || name == "bwd_pass_high_gen") {
@@ -1434,6 +1536,10 @@ void Engine::copy_state(double *host_out, size_t count) {
     }
     if (strategy != RIE_STRATEGY_STRUCTURED_ADJOINT || small) {
         fail(RIE_UNSUPPORTED, "copy_state requires structured_adjoint with num_qubits > 12.");
+    }
+    if (final_state() == nullptr) {
+        fail(RIE_UNSUPPORTED, "no state is held: the last evaluation generated its states "
+                              "(an energy-only evaluation keeps the final state).");
     }
     if (ev_last_valid) {
         check(hipEventSynchronize(ev_last), "hipEventSynchronize");
@@ -1749,6 +1855,22 @@ rie_status rie_engine_set_layer1_generation(rie_engine *engine, int enabled) {
 
 int rie_engine_layer1_generation(const rie_engine *engine) {
     return engine == nullptr ? -1 : (reinterpret_cast<const Engine *>(engine)->gen_ready() ? 1 : 0);
+}
+
+rie_status rie_engine_set_generated_start(rie_engine *engine, int enabled) {
+    return guard([&]() {
+        Engine *e = as_engine(engine);
+        if (enabled != 0 && enabled != 1) {
+            rie::fail(RIE_INVALID_ARGUMENT, "enabled must be 0 or 1.");
+        }
+        e->gen_start_enabled = enabled == 1;
+        e->resident = false;
+    });
+}
+
+int rie_engine_generated_start(const rie_engine *engine) {
+    return engine == nullptr ? -1
+                             : (reinterpret_cast<const Engine *>(engine)->gen_start_ready() ? 1 : 0);
 }
 
 rie_status rie_engine_set_double_buffer(rie_engine *engine, int enabled) {

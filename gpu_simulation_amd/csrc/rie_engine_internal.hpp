@@ -104,6 +104,14 @@ struct Engine {
     // 1 + j the adjoint high pass j.
     bool gen_enabled = true;    // caller's switch
     bool gen_fwd = false;       // the last forward generated layer 1
+    // Generated start (L == 2 gradient evaluations with the fused
+    // Hamiltonian): no forward and no Hamiltonian pass; layer 1's low adjoint
+    // pass generates its state and forms lambda = P^-1 H psi for its tile
+    // itself (ZZ diagonal, in-tile pairs, crossing wires from flipped chain
+    // segments), then the high passes and the layer-0 contraction as usual.
+    bool gen_start_enabled = true;  // caller's switch
+    CMat2 *d_chain_x = nullptr;     // per tile: U_A at the crossing wires' flips
+    CrossFlips cross{};
     CMat2 *d_chain = nullptr;   // kChainMaxSets * kChainTabEntries matrices
     CMat2 *d_chain_u = nullptr; // per set and tile: the uniform segments U_A, U_B
 
@@ -189,7 +197,10 @@ struct Engine {
                     const double2 *lam_in, double2 *cur_out, double2 *lam_out, size_t p,
                     hipStream_t s, int gen_set = -1);
     bool gen_ready() const;
-    void build_chain_tables(hipStream_t s);
+    bool gen_start_ready() const;
+    void build_chain_tables(hipStream_t s, int r_set0 = -1);
+    void structured_eval_gen_start(hipStream_t s, const double *scale, double *energy_out,
+                                   bool staged);
     void structured_backward(hipStream_t s, const double *scale, bool full_walk);
     void structured_backward_stored(hipStream_t s, const double *scale);
     void layer0_gradients(hipStream_t s, const double2 *lam, const double *scale);

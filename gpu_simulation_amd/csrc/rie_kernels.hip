@@ -660,6 +660,39 @@ void launch_chain_tables(const WireCoef *lc0, const WireCoef *lc1, const ChainSe
     chain_uniform_kernel<<<grid, 256, 0, s>>>(lc0, lc1, cs, utab);
 }
 
+// U_A of table set 0 (wires [r0 + R, n)) at every tile's base with the
+// crossing wires' flips applied above the tile.
+__global__ __launch_bounds__(256) void chain_flip_kernel(const WireCoef *__restrict__ lc0,
+                                                         const WireCoef *__restrict__ lc1,
+                                                         ChainSetup cs, CrossFlips cf,
+                                                         CMat2 *__restrict__ xtab) {
+    __shared__ CMat2 site[2 * 64];
+    const ChainSet st = cs.set[0];
+    const int n = cs.n;
+    for (int i = (int)threadIdx.x; i < 2 * n; i += (int)blockDim.x) {
+        const int w = i >> 1;
+        site[i] = chain_site(w, i & 1, lc0[w], lc1[w], ((st.rot >> w) & 1u) != 0);
+    }
+    __syncthreads();
+    const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= cs.tiles) {
+        return;
+    }
+    PassGeom G{};
+    G.a = st.ta;
+    G.h0 = st.th0;
+    const uint64_t base = tile_base(g, G, Tile<4>::T);
+    for (int c = 0; c < cf.count; ++c) {
+        xtab[g * kMaxCross + (uint64_t)c] =
+            chain_segment(site, st.r0 + st.R, n - st.r0 - st.R, base ^ cf.hi[c]);
+    }
+}
+
+void launch_chain_flips(const WireCoef *lc0, const WireCoef *lc1, const ChainSetup &cs,
+                        const CrossFlips &cf, CMat2 *xtab, hipStream_t s) {
+    chain_flip_kernel<<<(unsigned)((cs.tiles + 255) / 256), 256, 0, s>>>(lc0, lc1, cs, cf, xtab);
+}
+
 // Y of the forward's low tile at row i (run bits zero): U_A . C0 . C1 with C0
 // the index bits 0..3 and C1 the bits 4 .. T - R - 1.
 template <int R>
@@ -1445,6 +1478,155 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_la
             total += acc[w][t];
         }
         partials[(uint64_t)blockIdx.x * (2 * T) + t] = total;
+    }
+}
+
+// H's diagonal -sum_w Z_w Z_{w+1} at P(i), from i (the pre-CNOT index):
+// bits k, k + 1 of P(i) differ where i_{k+1} = 1 (1 <= k <= n - 2), bits n - 1
+// and 0 where i_0 = 1, bits 0 and 1 where i_0 xor parity(i >> 2) = 1.
+__device__ __forceinline__ double ham_diag_pre_ring(uint64_t i, int n) {
+    const int hi = (int)__popcll(i >> 2);
+    const int b0 = (int)(i & 1u);
+    const int walls = hi + (b0 ^ (hi & 1)) + b0;
+    return (double)(2 * walls - n);
+}
+
+// Generating low pass of a two-layer circuit (see launch_bwd_low_gen).  The
+// tile is generated in mapping 0 and swept 0, 1, .., P - 1 (phases, pair
+// terms and gradients), then lambda alone goes back to mapping 0 under RY^dagger
+// and is stored there (contiguous rows).
+template <int R>
+__global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_low_gen_kernel(
+    double2 *lam_out, const WireCoef *__restrict__ lc, PassGeom G, double *__restrict__ partials,
+    double field, const CMat2 *__restrict__ tab, const CMat2 *__restrict__ utab,
+    const CMat2 *__restrict__ xtab, CrossFlips cf) {
+    using TL = Tile<R>;
+    constexpr int T = TL::T;
+    constexpr int kC1Bits = T - R - 4;
+    __shared__ double2 lds[TL::SLOTS];
+    constexpr int ROWS = TL::THREADS / 16;
+    __shared__ double acc[ROWS][2 * T];
+    __shared__ double2 rz_tab[TL::P * TL::REGS];
+    __shared__ WireCoef tile_coef[T];
+    // chunk products C0 (index bits 0..3) and C1 (bits 4 .. T - R - 1)
+    __shared__ CMat2 chunks[kChainTabEntries - kChainC0];
+    const uint32_t t = threadIdx.x;
+    const int lane = t & (kWave - 1);
+    double *row_acc = acc[t / 16];
+    for (uint32_t i = t; i < (uint32_t)(ROWS * 2 * T); i += TL::THREADS) {
+        acc[i / (2 * T)][i % (2 * T)] = 0.0;
+    }
+    build_rz_tables<R>(rz_tab, lc, G, t);
+    if (t < (uint32_t)T) {
+        tile_coef[t] = lc[t];  // every bit of the low tile is a rotated wire
+    }
+    for (int i = (int)t; i < kChainTabEntries - kChainC0; i += TL::THREADS) {
+        chunks[i] = tab[kChainC0 + i];
+    }
+    __syncthreads();
+    for (uint64_t g = blockIdx.x; g < G.num_tiles; g += gridDim.x) {
+        uint32_t tl = t;
+        asm volatile("" : "+v"(tl));
+        const uint64_t base = tile_base(g, G, T);
+        double2 c[TL::REGS];
+        double2 l[TL::REGS];
+        // phi of the tile in mapping 0: c[k] = Tr(run[k] U_A C0 C1); the
+        // thread's row holds tile bits 0 .. T - R - 1 (register bits zero)
+        const uint32_t row = omap<R, 0>(tl, 0);
+        {
+            const CMat2 y = cmat_mul(cmat_mul(utab[2 * g], chunks[row & 15u]),
+                                     chunks[kChainC1 - kChainC0 + ((row >> 4) & ((1u << kC1Bits) - 1u))]);
+            chain_fill<R>(c, tab, y);
+        }
+        // lambda = diagonal and crossing-wire terms of P^-1 H psi
+#pragma unroll
+        for (int k = 0; k < TL::REGS; ++k) {
+            const uint64_t i = base | omap<R, 0>(tl, k);
+            const double d = ham_diag_pre_ring(i, G.n);
+            l[k] = make_double2(d * c[k].x, d * c[k].y);
+        }
+        for (int x = 0; x < cf.count; ++x) {
+            const uint32_t r2 = row ^ cf.thread[x];
+            const CMat2 y = cmat_mul(cmat_mul(xtab[g * kMaxCross + (uint64_t)x], chunks[r2 & 15u]),
+                                     chunks[kChainC1 - kChainC0 + ((r2 >> 4) & ((1u << kC1Bits) - 1u))]);
+            const int rf = cf.reg[x];
+#pragma unroll
+            for (int k = 0; k < TL::REGS; ++k) {
+                const double2 v = cmat_trace_mul(tab[k ^ rf], y);
+                l[k].x -= field * v.x;
+                l[k].y -= field * v.y;
+            }
+        }
+        // pair terms of window 0 (its lower edge from an LDS copy), sweep 1
+        if (G.pair_lo <= T - R - 1 && G.pair_hi >= T - R + 1) {
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < TL::REGS; ++k) {
+                lds[lds_slot<R>(omap<R, 0>(tl, k))] = c[k];
+            }
+            __syncthreads();
+        }
+        add_pair_terms<R, 0>(c, l, lds, tl, G, 0u, field);
+        grad_window<R, 0>(c, l, lc, G, row_acc, lane, rz_tab);
+        constexpr uint32_t w0 = window_mask<R, 0>();
+        constexpr uint32_t w1 = window_mask<R, 1>();
+        exchange<R, 0, 1>(l, lds, tl);
+        exchange<R, 0, 1>(c, lds, tl);
+        add_pair_terms<R, 1>(c, l, lds, tl, G, w0, field);
+        grad_window<R, 1>(c, l, lc, G, row_acc, lane, rz_tab);
+        exchange<R, 1, 2>(l, lds, tl);
+        exchange<R, 1, 2>(c, lds, tl);
+        add_pair_terms<R, 2>(c, l, lds, tl, G, w0 | w1, field);
+        grad_window<R, 2>(c, l, lc, G, row_acc, lane, rz_tab);
+        if constexpr (TL::P == 3) {
+            // sweep 2: lambda back to mapping 0 under RY^dagger
+            ry_inv_window<R, 2>(l, tile_coef);
+            exchange<R, 2, 1>(l, lds, tl);
+            ry_inv_window<R, 1>(l, tile_coef);
+            exchange<R, 1, 0>(l, lds, tl);
+            ry_inv_window<R, 0>(l, tile_coef);
+        } else {
+            constexpr uint32_t w2 = window_mask<R, 2>();
+            exchange<R, 2, 3>(l, lds, tl);
+            exchange<R, 2, 3>(c, lds, tl);
+            add_pair_terms<R, 3>(c, l, lds, tl, G, w0 | w1 | w2, field);
+            grad_window<R, 3>(c, l, lc, G, row_acc, lane, rz_tab);
+            ry_inv_window<R, 3>(l, tile_coef);
+            exchange<R, 3, 2>(l, lds, tl);
+            ry_inv_window<R, 2>(l, tile_coef);
+            exchange<R, 2, 1>(l, lds, tl);
+            ry_inv_window<R, 1>(l, tile_coef);
+            exchange<R, 1, 0>(l, lds, tl);
+            ry_inv_window<R, 0>(l, tile_coef);
+        }
+        const TileAddr ta{};
+        store_lam<R, 0, true>(l, lam_out, base, ta, tl);
+    }
+    __syncthreads();
+    if (t < 2 * T) {
+        double total = 0.0;
+#pragma unroll
+        for (int w = 0; w < ROWS; ++w) {
+            total += acc[w][t];
+        }
+        partials[(uint64_t)blockIdx.x * (2 * T) + t] = total;
+    }
+}
+
+void launch_bwd_low_gen(int R, double2 *lam_out, const WireCoef *layer_coef, const PassGeom &g,
+                        double *partials, int grid, hipStream_t s, double field,
+                        const CMat2 *tab, const CMat2 *utab, const CMat2 *xtab,
+                        const CrossFlips &cf) {
+    if (g.a != Tile<4>::T || g.h0 != Tile<4>::T || g.rot_lo != 0 || g.rot_hi != Tile<4>::T ||
+        cf.count < 0 || cf.count > kMaxCross) {
+        throw std::invalid_argument("generating low pass: geometry outside the low tile");
+    }
+    if (R == 4) {
+        bwd_low_gen_kernel<4><<<grid, Tile<4>::THREADS, 0, s>>>(lam_out, layer_coef, g, partials,
+                                                              field, tab, utab, xtab, cf);
+    } else {
+        bwd_low_gen_kernel<3><<<grid, Tile<3>::THREADS, 0, s>>>(lam_out, layer_coef, g, partials,
+                                                              field, tab, utab, xtab, cf);
     }
 }
 

@@ -46,6 +46,8 @@ C_ABI_SYMBOLS = (
     "rie_engine_fused_hamiltonian",
     "rie_engine_set_layer1_generation",
     "rie_engine_layer1_generation",
+    "rie_engine_set_generated_start",
+    "rie_engine_generated_start",
     "rie_engine_set_double_buffer",
     "rie_engine_double_buffer",
     "rie_engine_set_kernel_timing",

@@ -211,6 +211,22 @@ int rie_engine_fused_hamiltonian(const rie_engine *engine);
 rie_status rie_engine_set_layer1_generation(rie_engine *engine, int enabled);
 int rie_engine_layer1_generation(const rie_engine *engine);
 
+/* Generated start of two-layer gradient evaluations (no counterpart in the
+ * reference, whose evaluations always run the forward sweep and H psi first:
+ * cpp/ising_cuda_structured_adjoint_modes.inc:309-373).  With L == 2, the
+ * fused Hamiltonian and layer-1 generation in use, energy_and_grad skips the
+ * forward and Hamiltonian passes: layer 1's low adjoint pass generates the
+ * pre-CNOT state of its tile and forms lambda = P^-1 H psi there (the ZZ
+ * diagonal at P(i), the X terms as flips P^-1 e_w inside the tile or, for
+ * flips leaving it, from the chain at the flipped tile), so the evaluation
+ * moves 6 state passes instead of 11.  Results agree with the other paths to
+ * rounding.  No final state is kept after such an evaluation
+ * (rie_engine_copy_state then reports RIE_UNSUPPORTED; energy-only calls keep
+ * it).  enabled: 1 on (default), 0 off; rie_engine_generated_start reports
+ * whether gradient evaluations use it. */
+rie_status rie_engine_set_generated_start(rie_engine *engine, int enabled);
+int rie_engine_generated_start(const rie_engine *engine);
+
 /* Double-buffered adjoint high passes.  Replaces the double_buffer flag of
  * RingIsingCudaBackend (cpp/ising_cuda_backend.cu:374-419), which selects a
  * backward rotation-chunk kernel that stages the next chunk while computing

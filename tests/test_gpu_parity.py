@@ -405,3 +405,50 @@ def test_double_buffer_rejects_bad_values(backend_module):
     eng = backend_module.RingIsingCudaBackend(13, 2, 1.0, "structured_adjoint", 8, False)
     with pytest.raises(ValueError):
         eng.set_double_buffer(2)
+
+
+@pytest.mark.parametrize("n,shape", [(13, 4), (14, 3), (16, 4), (21, 4), (22, 3), (22, 4),
+                                     (25, 4), (31, 4)])
+def test_generated_start(oracle, backend_module, n, shape):
+    """Two-layer gradient evaluations without forward or Hamiltonian passes:
+    the low adjoint pass forms P^-1 H psi of its tile from generated
+    amplitudes.  Against the oracle (n <= 25) and against the evaluation with
+    the forward and the fused Hamiltonian pass (all n), at two fields."""
+    for field, seed in [(1.0, 8100 + n), (0.37, 8200 + n)]:
+        params = np.ascontiguousarray(_params(seed, n, 2).reshape(-1))
+        eng = backend_module.RingIsingCudaBackend(n, 2, field, "structured_adjoint", 8, False)
+        shapes = list(eng.pass_shapes)
+        shapes[4] = shape  # the low adjoint pass's register shape
+        eng.set_pass_shapes(shapes)
+        print(f"case generated start n={n} shape={shape} h={field}", flush=True)
+        assert eng.generated_start
+        out = eng.energy_and_grad(params)
+        with pytest.raises((RuntimeError, ValueError)):
+            eng.final_state()  # no state is held after a generated-start evaluation
+        eng.set_generated_start(0)
+        assert not eng.generated_start
+        ref = eng.energy_and_grad(params)
+        assert abs(out["energy"] - ref["energy"]) <= TOL
+        assert np.max(np.abs(np.asarray(out["gradient"]) - np.asarray(ref["gradient"]))) <= TOL
+        if n <= 25:
+            e_ref, g_ref = oracle.energy_and_grad(n, 2, field, params, form="structured")
+            assert abs(out["energy"] - e_ref) <= TOL
+            assert np.max(np.abs(np.asarray(out["gradient"]) - g_ref)) <= TOL
+        e_only = eng.energy_and_grad(params, False)
+        assert abs(e_only["energy"] - out["energy"]) <= TOL
+        del eng
+
+
+def test_generated_start_scope(backend_module):
+    # three layers, unfused H or no layer-1 generation: the forward runs as before
+    eng = backend_module.RingIsingCudaBackend(16, 3, 1.0, "structured_adjoint", 8, False)
+    assert not eng.generated_start
+    eng = backend_module.RingIsingCudaBackend(16, 2, 1.0, "structured_adjoint", 8, False)
+    assert eng.generated_start
+    eng.set_fused_hamiltonian(0)
+    assert not eng.generated_start
+    eng.set_fused_hamiltonian(1)
+    eng.set_layer1_generation(0)
+    assert not eng.generated_start
+    with pytest.raises(ValueError):
+        eng.set_generated_start(3)
