@@ -161,6 +161,13 @@ struct CMat2 {
     double2 a00, a01, a10, a11;
 };
 
+__device__ __forceinline__ CMat2 cmat_add(const CMat2 &p, const CMat2 &q) {
+    return {make_double2(p.a00.x + q.a00.x, p.a00.y + q.a00.y),
+            make_double2(p.a01.x + q.a01.x, p.a01.y + q.a01.y),
+            make_double2(p.a10.x + q.a10.x, p.a10.y + q.a10.y),
+            make_double2(p.a11.x + q.a11.x, p.a11.y + q.a11.y)};
+}
+
 __device__ __forceinline__ CMat2 cmat_mul(const CMat2 &p, const CMat2 &q) {
     CMat2 r;
     r.a00 = cfma(cmul(p.a00, q.a00), p.a01, q.a10);

@@ -1495,8 +1495,17 @@ __device__ __forceinline__ double ham_diag_pre_ring(uint64_t i, int n) {
 // tile is generated in mapping 0 and swept 0, 1, .., P - 1 (phases, pair
 // terms and gradients), then lambda alone goes back to mapping 0 under RY^dagger
 // and is stored there (contiguous rows).
+// Waves per SIMD to budget registers for (the second launch-bounds value):
+// R = 3 aims at two 512-thread workgroups per CU, four waves per SIMD with
+// 128 VGPRs; the chunk tables stay in global memory so that two workgroups'
+// LDS fits.
 template <int R>
-__global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_low_gen_kernel(
+constexpr int low_gen_min_waves() {
+    return R == 3 ? 4 : 2;
+}
+
+template <int R>
+__global__ __launch_bounds__(Tile<R>::THREADS, low_gen_min_waves<R>()) void bwd_low_gen_kernel(
     double2 *lam_out, const WireCoef *__restrict__ lc, PassGeom G, double *__restrict__ partials,
     double field, const CMat2 *__restrict__ tab, const CMat2 *__restrict__ utab,
     const CMat2 *__restrict__ xtab, CrossFlips cf) {
@@ -1508,8 +1517,11 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_lo
     __shared__ double acc[ROWS][2 * T];
     __shared__ double2 rz_tab[TL::P * TL::REGS];
     __shared__ WireCoef tile_coef[T];
-    // chunk products C0 (index bits 0..3) and C1 (bits 4 .. T - R - 1)
-    __shared__ CMat2 chunks[kChainTabEntries - kChainC0];
+    // chunk products C0 (index bits 0..3) and C1 (bits 4 .. T - R - 1): in LDS
+    // for R = 4, read from the table in global memory (L2) for R = 3
+    constexpr bool kChunksLds = R == 4;
+    __shared__ CMat2 chunk_lds[kChunksLds ? kChainTabEntries - kChainC0 : 1];
+    const CMat2 *chunks = kChunksLds ? chunk_lds : tab + kChainC0;
     const uint32_t t = threadIdx.x;
     const int lane = t & (kWave - 1);
     double *row_acc = acc[t / 16];
@@ -1520,8 +1532,10 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_lo
     if (t < (uint32_t)T) {
         tile_coef[t] = lc[t];  // every bit of the low tile is a rotated wire
     }
-    for (int i = (int)t; i < kChainTabEntries - kChainC0; i += TL::THREADS) {
-        chunks[i] = tab[kChainC0 + i];
+    if constexpr (kChunksLds) {
+        for (int i = (int)t; i < kChainTabEntries - kChainC0; i += TL::THREADS) {
+            chunk_lds[i] = tab[kChainC0 + i];
+        }
     }
     __syncthreads();
     for (uint64_t g = blockIdx.x; g < G.num_tiles; g += gridDim.x) {
@@ -1530,31 +1544,41 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_lo
         const uint64_t base = tile_base(g, G, T);
         double2 c[TL::REGS];
         double2 l[TL::REGS];
-        // phi of the tile in mapping 0: c[k] = Tr(run[k] U_A C0 C1); the
-        // thread's row holds tile bits 0 .. T - R - 1 (register bits zero)
+        // phi of the tile in mapping 0, c[k] = Tr(run[k] U_A C0 C1), and lambda's
+        // diagonal and crossing-wire terms.  The thread's row holds tile bits
+        // 0 .. T - R - 1 (register bits zero).  Crossing flip 0 is wire
+        // T - 1's (bits T - 1, T: the register index's top bit and the tile's
+        // lowest bit above it), whose amplitude at register k ^ 2^(R-1) is
+        // Tr(run[k] Y_0); the other flips leave the register index alone and
+        // share one set of traces: Tr(run[k] sum_x Y_x) is the sum of their
+        // amplitudes.  Each run matrix serves all three traces.
         const uint32_t row = omap<R, 0>(tl, 0);
         {
-            const CMat2 y = cmat_mul(cmat_mul(utab[2 * g], chunks[row & 15u]),
-                                     chunks[kChainC1 - kChainC0 + ((row >> 4) & ((1u << kC1Bits) - 1u))]);
-            chain_fill<R>(c, tab, y);
-        }
-        // lambda = diagonal and crossing-wire terms of P^-1 H psi
-#pragma unroll
-        for (int k = 0; k < TL::REGS; ++k) {
-            const uint64_t i = base | omap<R, 0>(tl, k);
-            const double d = ham_diag_pre_ring(i, G.n);
-            l[k] = make_double2(d * c[k].x, d * c[k].y);
-        }
-        for (int x = 0; x < cf.count; ++x) {
-            const uint32_t r2 = row ^ cf.thread[x];
-            const CMat2 y = cmat_mul(cmat_mul(xtab[g * kMaxCross + (uint64_t)x], chunks[r2 & 15u]),
-                                     chunks[kChainC1 - kChainC0 + ((r2 >> 4) & ((1u << kC1Bits) - 1u))]);
-            const int rf = cf.reg[x];
+            auto y_of = [&](const CMat2 &u, uint32_t r2) {
+                return cmat_mul(cmat_mul(u, chunks[r2 & 15u]),
+                                chunks[kChainC1 - kChainC0 + ((r2 >> 4) & ((1u << kC1Bits) - 1u))]);
+            };
+            const CMat2 y = y_of(utab[2 * g], row);
+            const CMat2 y0 = y_of(xtab[g * kMaxCross], row ^ cf.thread[0]);
+            CMat2 ysum = y_of(xtab[g * kMaxCross + 1], row ^ cf.thread[1]);
+            for (int x = 2; x < cf.count; ++x) {
+                ysum = cmat_add(ysum, y_of(xtab[g * kMaxCross + (uint64_t)x], row ^ cf.thread[x]));
+            }
 #pragma unroll
             for (int k = 0; k < TL::REGS; ++k) {
-                const double2 v = cmat_trace_mul(tab[k ^ rf], y);
-                l[k].x -= field * v.x;
-                l[k].y -= field * v.y;
+                l[k] = make_double2(0.0, 0.0);
+            }
+#pragma unroll
+            for (int k = 0; k < TL::REGS; ++k) {
+                const CMat2 rk = tab[k];
+                c[k] = cmat_trace_mul(rk, y);
+                const double d = ham_diag_pre_ring(base | omap<R, 0>(tl, k), G.n);
+                const double2 v1 = cmat_trace_mul(rk, ysum);
+                const double2 v0 = cmat_trace_mul(rk, y0);
+                l[k].x += d * c[k].x - field * v1.x;
+                l[k].y += d * c[k].y - field * v1.y;
+                l[k ^ (1 << (R - 1))].x -= field * v0.x;
+                l[k ^ (1 << (R - 1))].y -= field * v0.y;
             }
         }
         // pair terms of window 0 (its lower edge from an LDS copy), sweep 1
@@ -1618,8 +1642,18 @@ void launch_bwd_low_gen(int R, double2 *lam_out, const WireCoef *layer_coef, con
                         const CMat2 *tab, const CMat2 *utab, const CMat2 *xtab,
                         const CrossFlips &cf) {
     if (g.a != Tile<4>::T || g.h0 != Tile<4>::T || g.rot_lo != 0 || g.rot_hi != Tile<4>::T ||
-        cf.count < 0 || cf.count > kMaxCross) {
+        cf.count < 2 || cf.count > kMaxCross) {
         throw std::invalid_argument("generating low pass: geometry outside the low tile");
+    }
+    // flip 0 must be wire T - 1's (register top bit, nothing below), the
+    // others must leave the register index alone
+    if (cf.reg[0] != (1 << (R - 1)) || cf.thread[0] != 0) {
+        throw std::invalid_argument("generating low pass: the first crossing flip is not wire T - 1's");
+    }
+    for (int x = 1; x < cf.count; ++x) {
+        if (cf.reg[x] != 0) {
+            throw std::invalid_argument("generating low pass: a crossing flip moves the register index");
+        }
     }
     if (R == 4) {
         bwd_low_gen_kernel<4><<<grid, Tile<4>::THREADS, 0, s>>>(lam_out, layer_coef, g, partials,
