@@ -239,8 +239,8 @@ Engine::Engine(size_t num_qubits, size_t num_layers, double field, const std::st
             d_chain_u = (CMat2 *)device_alloc(sizeof(CMat2) * 2 * tiles * passes.size(),
                                               "alloc chain segments");
             if (L == 2 && fuse_ready) {
-                // the crossing wires of the generated start (at most kMaxCross)
-                d_chain_x = (CMat2 *)device_alloc(sizeof(CMat2) * kMaxCross * tiles,
+                // the out-of-tile flips of the generated start
+                d_chain_x = (CMat2 *)device_alloc(sizeof(CMat2) * kFlipSegments * tiles,
                                                   "alloc chain flips");
             }
         }
@@ -694,8 +694,8 @@ void Engine::build_chain_tables(hipStream_t s, int r_set0) {
     launch_chain_tables(d_coef, d_coef + n, cs, d_chain, d_chain_u, s);
     check_launch("chain_tables");
     if (r_set0 > 0) {
-        // U_A of set 0 at the crossing wires' flips above the tile
-        launch_chain_flips(d_coef, d_coef + n, cs, cross, d_chain_x, s);
+        // U_A of set 0 at the out-of-tile flips
+        launch_chain_flips(d_coef, d_coef + n, cs, d_chain_x, s);
         check_launch("chain_flips");
     }
     kt_end(s, "chain_tables", 0.0);
@@ -703,7 +703,7 @@ void Engine::build_chain_tables(hipStream_t s, int r_set0) {
 
 bool Engine::gen_start_ready() const {
     return gen_start_enabled && gen_ready() && L == 2 && fuse_ready && fuse_enabled &&
-           d_chain_x != nullptr && ham_fused_g.sc_n <= kMaxCross;
+           d_chain_x != nullptr;
 }
 
 // Gradient evaluation of a two-layer circuit without forward or Hamiltonian
@@ -713,18 +713,6 @@ void Engine::structured_eval_gen_start(hipStream_t s, const double *scale, doubl
                                        bool staged) {
     const double S = (double)sizeof(double2) * (double)N;
     const int r_low = pass_r[RIE_PASS_BWD_LOW];
-    // the crossing wires' flips P^-1 e_w split at the low tile's top bit
-    const uint64_t tile_mask = (uint64_t{1} << T) - 1;
-    cross = CrossFlips{};
-    cross.count = ham_fused_g.sc_n;
-    for (int c = 0; c < cross.count; ++c) {
-        const int w = ham_fused_g.sc[c];
-        const uint64_t m = w < n - 1 ? (uint64_t{3} << w) : ((uint64_t{1} << (n - 1)) | 3u);
-        const uint64_t lo = m & tile_mask;
-        cross.hi[c] = m & ~tile_mask;
-        cross.reg[c] = (int)((lo >> (T - r_low)) & ((1u << r_low) - 1u));
-        cross.thread[c] = (uint32_t)(lo & ((uint64_t{1} << (T - r_low)) - 1));
-    }
     if (staged) {
         stage_begin(s);
     }
@@ -744,13 +732,15 @@ void Engine::structured_eval_gen_start(hipStream_t s, const double *scale, doubl
     g0.pair_hi = T;
     double2 *lam = pool[0];
     const int K = (int)passes.size() - 1;
-    fuse_layer = 1;
+    // every X term is in lambda once the low pass is done: the high passes
+    // add no pair terms (fuse_layer stays -1)
+    fuse_layer = -1;
     l0_energy_out = energy_out;
     try {
         require_partials((size_t)passes[0].grid * 2 * T);
         kt_begin(s);
         launch_bwd_low_gen(r_low, lam, d_coef + n, g0, d_partials, passes[0].grid, s, h, d_chain,
-                           d_chain_u, d_chain_x, cross);
+                           d_chain_u, d_chain_x);
         check_launch("bwd_pass low gen");
         kt_end(s, "bwd_pass_low_gen", S);
         kt_begin(s);

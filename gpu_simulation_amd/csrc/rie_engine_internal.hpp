@@ -107,11 +107,11 @@ struct Engine {
     // Generated start (L == 2 gradient evaluations with the fused
     // Hamiltonian): no forward and no Hamiltonian pass; layer 1's low adjoint
     // pass generates its state and forms lambda = P^-1 H psi for its tile
-    // itself (ZZ diagonal, in-tile pairs, crossing wires from flipped chain
-    // segments), then the high passes and the layer-0 contraction as usual.
+    // itself (ZZ diagonal, in-tile pairs, every other X term from flipped
+    // chain segments), then the high passes (without pair terms) and the
+    // layer-0 contraction.
     bool gen_start_enabled = true;  // caller's switch
-    CMat2 *d_chain_x = nullptr;     // per tile: U_A at the crossing wires' flips
-    CrossFlips cross{};
+    CMat2 *d_chain_x = nullptr;     // per tile: U_A at the out-of-tile flips
     CMat2 *d_chain = nullptr;   // kChainMaxSets * kChainTabEntries matrices
     CMat2 *d_chain_u = nullptr; // per set and tile: the uniform segments U_A, U_B
 

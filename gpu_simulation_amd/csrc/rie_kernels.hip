@@ -660,12 +660,14 @@ void launch_chain_tables(const WireCoef *lc0, const WireCoef *lc1, const ChainSe
     chain_uniform_kernel<<<grid, 256, 0, s>>>(lc0, lc1, cs, utab);
 }
 
-// U_A of table set 0 (wires [r0 + R, n)) at every tile's base with the
-// crossing wires' flips applied above the tile.
+// The flipped segments of table set 0 per tile (see launch_chain_flips in the
+// header).  The sum over the pairs above the tile is one left-to-right walk
+// over the sites T .. n-1 with three running products: A the plain prefix,
+// C the prefix whose last site is flipped (a pair waiting for its partner),
+// B the sum of prefixes with one completed flipped pair.
 __global__ __launch_bounds__(256) void chain_flip_kernel(const WireCoef *__restrict__ lc0,
                                                          const WireCoef *__restrict__ lc1,
-                                                         ChainSetup cs, CrossFlips cf,
-                                                         CMat2 *__restrict__ xtab) {
+                                                         ChainSetup cs, CMat2 *__restrict__ xtab) {
     __shared__ CMat2 site[2 * 64];
     const ChainSet st = cs.set[0];
     const int n = cs.n;
@@ -681,16 +683,29 @@ __global__ __launch_bounds__(256) void chain_flip_kernel(const WireCoef *__restr
     PassGeom G{};
     G.a = st.ta;
     G.h0 = st.th0;
+    const int lo = st.r0 + st.R;  // the low tile's top: U_A covers wires [lo, n)
     const uint64_t base = tile_base(g, G, Tile<4>::T);
-    for (int c = 0; c < cf.count; ++c) {
-        xtab[g * kMaxCross + (uint64_t)c] =
-            chain_segment(site, st.r0 + st.R, n - st.r0 - st.R, base ^ cf.hi[c]);
+    CMat2 *x = xtab + g * kFlipSegments;
+    x[0] = chain_segment(site, lo, n - lo, base ^ (uint64_t{1} << lo));
+    x[1] = chain_segment(site, lo, n - lo, base ^ (uint64_t{1} << (n - 1)));
+    const CMat2 zero = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+    CMat2 A = site[2 * lo + (int)((base >> lo) & 1u)];
+    CMat2 C = site[2 * lo + (int)(((base >> lo) & 1u) ^ 1u)];
+    CMat2 B = zero;
+    for (int v = lo + 1; v < n; ++v) {
+        const int b = (int)((base >> v) & 1u);
+        const CMat2 &sv = site[2 * v + b];
+        const CMat2 &sf = site[2 * v + (b ^ 1)];
+        B = cmat_add(cmat_mul(B, sv), cmat_mul(C, sf));
+        C = cmat_mul(A, sf);
+        A = cmat_mul(A, sv);
     }
+    x[2] = B;
 }
 
 void launch_chain_flips(const WireCoef *lc0, const WireCoef *lc1, const ChainSetup &cs,
-                        const CrossFlips &cf, CMat2 *xtab, hipStream_t s) {
-    chain_flip_kernel<<<(unsigned)((cs.tiles + 255) / 256), 256, 0, s>>>(lc0, lc1, cs, cf, xtab);
+                        CMat2 *xtab, hipStream_t s) {
+    chain_flip_kernel<<<(unsigned)((cs.tiles + 255) / 256), 256, 0, s>>>(lc0, lc1, cs, xtab);
 }
 
 // Y of the forward's low tile at row i (run bits zero): U_A . C0 . C1 with C0
@@ -1508,7 +1523,7 @@ template <int R>
 __global__ __launch_bounds__(Tile<R>::THREADS, low_gen_min_waves<R>()) void bwd_low_gen_kernel(
     double2 *lam_out, const WireCoef *__restrict__ lc, PassGeom G, double *__restrict__ partials,
     double field, const CMat2 *__restrict__ tab, const CMat2 *__restrict__ utab,
-    const CMat2 *__restrict__ xtab, CrossFlips cf) {
+    const CMat2 *__restrict__ xtab) {
     using TL = Tile<R>;
     constexpr int T = TL::T;
     constexpr int kC1Bits = T - R - 4;
@@ -1544,26 +1559,24 @@ __global__ __launch_bounds__(Tile<R>::THREADS, low_gen_min_waves<R>()) void bwd_
         const uint64_t base = tile_base(g, G, T);
         double2 c[TL::REGS];
         double2 l[TL::REGS];
-        // phi of the tile in mapping 0, c[k] = Tr(run[k] U_A C0 C1), and lambda's
-        // diagonal and crossing-wire terms.  The thread's row holds tile bits
-        // 0 .. T - R - 1 (register bits zero).  Crossing flip 0 is wire
-        // T - 1's (bits T - 1, T: the register index's top bit and the tile's
-        // lowest bit above it), whose amplitude at register k ^ 2^(R-1) is
-        // Tr(run[k] Y_0); the other flips leave the register index alone and
-        // share one set of traces: Tr(run[k] sum_x Y_x) is the sum of their
-        // amplitudes.  Each run matrix serves all three traces.
+        // phi of the tile in mapping 0, c[k] = Tr(run[k] U_A C0 C1), and
+        // lambda's diagonal and out-of-tile X terms.  The thread's row holds
+        // tile bits 0 .. T - R - 1 (register bits zero).  Wire T - 1's flip
+        // moves the register index's top bit: its amplitude at register
+        // k ^ 2^(R-1) is Tr(run[k] Y_0).  The other flips leaving the tile keep
+        // the register index and share one set of traces, Tr(run[k] Y_sum):
+        // wire n - 1's (thread index ^ 3) and the pairs above the tile, whose
+        // segments are summed per tile.  Each run matrix serves all three traces.
         const uint32_t row = omap<R, 0>(tl, 0);
         {
             auto y_of = [&](const CMat2 &u, uint32_t r2) {
                 return cmat_mul(cmat_mul(u, chunks[r2 & 15u]),
                                 chunks[kChainC1 - kChainC0 + ((r2 >> 4) & ((1u << kC1Bits) - 1u))]);
             };
+            const CMat2 *xg = xtab + g * kFlipSegments;
             const CMat2 y = y_of(utab[2 * g], row);
-            const CMat2 y0 = y_of(xtab[g * kMaxCross], row ^ cf.thread[0]);
-            CMat2 ysum = y_of(xtab[g * kMaxCross + 1], row ^ cf.thread[1]);
-            for (int x = 2; x < cf.count; ++x) {
-                ysum = cmat_add(ysum, y_of(xtab[g * kMaxCross + (uint64_t)x], row ^ cf.thread[x]));
-            }
+            const CMat2 y0 = y_of(xg[0], row);
+            const CMat2 ysum = cmat_add(y_of(xg[1], row ^ 3u), y_of(xg[2], row));
 #pragma unroll
             for (int k = 0; k < TL::REGS; ++k) {
                 l[k] = make_double2(0.0, 0.0);
@@ -1639,28 +1652,17 @@ __global__ __launch_bounds__(Tile<R>::THREADS, low_gen_min_waves<R>()) void bwd_
 
 void launch_bwd_low_gen(int R, double2 *lam_out, const WireCoef *layer_coef, const PassGeom &g,
                         double *partials, int grid, hipStream_t s, double field,
-                        const CMat2 *tab, const CMat2 *utab, const CMat2 *xtab,
-                        const CrossFlips &cf) {
+                        const CMat2 *tab, const CMat2 *utab, const CMat2 *xtab) {
     if (g.a != Tile<4>::T || g.h0 != Tile<4>::T || g.rot_lo != 0 || g.rot_hi != Tile<4>::T ||
-        cf.count < 2 || cf.count > kMaxCross) {
+        g.n <= Tile<4>::T) {
         throw std::invalid_argument("generating low pass: geometry outside the low tile");
-    }
-    // flip 0 must be wire T - 1's (register top bit, nothing below), the
-    // others must leave the register index alone
-    if (cf.reg[0] != (1 << (R - 1)) || cf.thread[0] != 0) {
-        throw std::invalid_argument("generating low pass: the first crossing flip is not wire T - 1's");
-    }
-    for (int x = 1; x < cf.count; ++x) {
-        if (cf.reg[x] != 0) {
-            throw std::invalid_argument("generating low pass: a crossing flip moves the register index");
-        }
     }
     if (R == 4) {
         bwd_low_gen_kernel<4><<<grid, Tile<4>::THREADS, 0, s>>>(lam_out, layer_coef, g, partials,
-                                                              field, tab, utab, xtab, cf);
+                                                              field, tab, utab, xtab);
     } else {
         bwd_low_gen_kernel<3><<<grid, Tile<3>::THREADS, 0, s>>>(lam_out, layer_coef, g, partials,
-                                                              field, tab, utab, xtab, cf);
+                                                              field, tab, utab, xtab);
     }
 }
 

@@ -147,30 +147,26 @@ struct ChainSetup {
     uint64_t tiles;  // tiles per pass (every pass has 2^(n - 12))
     ChainSet set[kChainMaxSets];
 };
-// The X terms of H on wires whose pre-CNOT flip P^-1 e_w leaves the low tile
-// (wire T - 1, the last wire of every high group but the last, wire n - 1):
-// the flip's bits above the tile (hi), and inside it the register-index and
-// thread-index parts of mapping 0 (reg, thread).
-constexpr int kMaxCross = 4;
-struct CrossFlips {
-    int count;
-    int reg[kMaxCross];
-    uint32_t thread[kMaxCross];
-    uint64_t hi[kMaxCross];
-};
-// xtab[g * kMaxCross + c] = U_A of table set 0 at tile g's base ^ cf.hi[c].
+// The X terms of H whose pre-CNOT flip P^-1 e_w leaves the low tile (bits
+// [0, T)), as tile-uniform segments of table set 0 (U_A, wires [T, n)):
+//   xtab[3g]     U_A at base ^ e_T            (wire T - 1: bits T - 1, T; the
+//                                               register index's top bit flips)
+//   xtab[3g + 1] U_A at base ^ e_{n-1}         (wire n - 1: bits n - 1, 0, 1;
+//                                               the thread index flips by 3)
+//   xtab[3g + 2] sum over w = T .. n - 2 of U_A at base ^ e_w ^ e_{w+1}
+//                                              (both bits above the tile)
+constexpr int kFlipSegments = 3;
 void launch_chain_flips(const WireCoef *lc0, const WireCoef *lc1, const ChainSetup &cs,
-                        const CrossFlips &cf, CMat2 *xtab, hipStream_t s);
+                        CMat2 *xtab, hipStream_t s);
 // Generating low pass (two-layer circuits, layer 1's backward start): per low
 // tile the pre-CNOT state phi of layer 1 from chain table set 0 (tab, utab),
-// lambda = P^-1 H psi restricted to the ZZ diagonal, the in-tile pairs
-// [g.pair_lo, g.pair_hi) and the crossing wires cf (flipped segments xtab),
-// then the adjoint sweep of wires 0..T-1 (gradients into partials) and lam_out
-// = lambda un-rotated on them.  Reads nothing but the tables: 1 S written.
+// lambda = P^-1 H psi (the ZZ diagonal, the in-tile pairs [g.pair_lo,
+// g.pair_hi) and every flip leaving the tile, from xtab), then the adjoint
+// sweep of wires 0..T-1 (gradients into partials) and lam_out = lambda
+// un-rotated on them.  Reads nothing but the tables: 1 S written.
 void launch_bwd_low_gen(int R, double2 *lam_out, const WireCoef *layer_coef, const PassGeom &g,
                         double *partials, int grid, hipStream_t s, double field,
-                        const CMat2 *tab, const CMat2 *utab, const CMat2 *xtab,
-                        const CrossFlips &cf);
+                        const CMat2 *tab, const CMat2 *utab, const CMat2 *xtab);
 
 // tabs: count * kChainTabEntries matrices, set i at tabs + i * kChainTabEntries;
 // utab: count * tiles * 2 matrices, set i at utab + i * tiles * 2.

@@ -218,8 +218,9 @@ int rie_engine_layer1_generation(const rie_engine *engine);
  * forward and Hamiltonian passes: layer 1's low adjoint pass generates the
  * pre-CNOT state of its tile and forms lambda = P^-1 H psi there (the ZZ
  * diagonal at P(i), the X terms as flips P^-1 e_w inside the tile or, for
- * flips leaving it, from the chain at the flipped tile), so the evaluation
- * moves 6 state passes instead of 11.  Results agree with the other paths to
+ * flips leaving it, from the chain at the flipped tile), and the adjoint high
+ * passes add no Hamiltonian terms, so the evaluation moves 6 state passes
+ * instead of 11.  Results agree with the other paths to
  * rounding.  No final state is kept after such an evaluation
  * (rie_engine_copy_state then reports RIE_UNSUPPORTED; energy-only calls keep
  * it).  enabled: 1 on (default), 0 off; rie_engine_generated_start reports

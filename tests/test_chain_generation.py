@@ -244,38 +244,61 @@ def _flip_mask(w, n):
     return (1 << w) | (1 << (w + 1)) if w < n - 1 else (1 << (n - 1)) | 3
 
 
-def _crossing_wires(n):
-    """The X terms the generating low pass takes besides its in-tile pairs:
-    wire T - 1, the last wire of every high group but the last, and n - 1."""
-    rest, width = n - T, T - 3
-    groups = (rest + width - 1) // width
-    g0, firsts = T, []
-    for gi in range(groups):
-        firsts.append(g0)
-        g0 += rest // groups + (1 if gi < rest % groups else 0)
-    return sorted({T - 1, *(f - 1 for f in firsts[1:]), n - 1})
-
-
-def _high_group_pairs(n):
-    """X_w the high passes take: pairs w, w + 1 inside one high group (the
-    group's wires from its first to the one before its last)."""
-    starts = [T] + [c + 1 for c in _crossing_wires(n) if T <= c < n - 1]
-    ends = [c for c in _crossing_wires(n) if c >= T]
-    return [w for s, e in zip(starts, ends) for w in range(s, e)]
-
-
 @pytest.mark.parametrize("n", [13, 16, 22, 25, 30, 31])
-def test_hamiltonian_terms_are_split_exactly_once(n):
-    # in the pre-CNOT frame X_w of H is the flip P^-1 e_w; the generating low
-    # pass takes w = 0 .. T - 2 (pairs inside its tile) and the crossing wires,
-    # the high passes the pairs inside their groups: every w exactly once
-    taken = list(range(T - 1)) + _crossing_wires(n) + _high_group_pairs(n)
-    assert sorted(taken) == list(range(n))
-    # the groups' pairs lie inside the groups the engine plans
-    for w in _high_group_pairs(n):
-        assert any((rot >> w) & 1 and (rot >> (w + 1)) & 1 for _, _, rot in _plan_cases(n)[1:])
+@pytest.mark.parametrize("R", [3, 4])
+def test_out_of_tile_flips_split(n, R):
+    # in the pre-CNOT frame X_w of H is the flip m_w = P^-1 e_w.  The
+    # generating low pass takes w = 0 .. T - 2 inside its tile, and the rest
+    # from flipped segments of U_A (the wires above the tile): wire T - 1
+    # flips the register index's top bit and U_A's first wire, wire n - 1
+    # flips thread bits 0, 1 and U_A's last wire, and w = T .. n - 2 flip two
+    # wires of U_A and nothing inside the tile
+    tile = (1 << T) - 1
     for w in range(n):
-        assert _ring_fwd(_flip_mask(w, n), n) == 1 << w  # P^-1 e_w is X_w's flip
+        m = _flip_mask(w, n)
+        assert _ring_fwd(m, n) == 1 << w
+        lo, hi = m & tile, m & ~tile
+        if w <= T - 2:
+            assert hi == 0
+        elif w == T - 1:
+            assert hi == 1 << T and lo == 1 << (T - 1)
+            assert (lo >> (T - R)) == 1 << (R - 1) and lo & ((1 << (T - R)) - 1) == 0
+        elif w == n - 1:
+            assert hi == 1 << (n - 1) and lo == 3 and (lo >> (T - R)) == 0
+        else:
+            assert lo == 0 and hi == (1 << w) | (1 << (w + 1))
+
+
+def _summed_pair_segment(sites, first, n, base):
+    """Sum over w = first .. n - 2 of the segment prod_{v >= first} site[v]
+    at base ^ e_w ^ e_{w+1}, by one walk with three running products: A the
+    plain prefix, C the prefix whose last site is flipped, B the prefixes with
+    one completed flipped pair."""
+    bit = lambda v: (base >> v) & 1
+    a = sites[first][bit(first)]
+    c = sites[first][bit(first) ^ 1]
+    b = np.zeros((2, 2), dtype=complex)
+    for v in range(first + 1, n):
+        s, f = sites[v][bit(v)], sites[v][bit(v) ^ 1]
+        b, c, a = b @ s + c @ f, a @ f, a @ s
+    return b
+
+
+@pytest.mark.parametrize("n", [13, 14, 17])
+def test_summed_pair_segment_matches_the_direct_sum(n):
+    rng = np.random.default_rng(40 + n)
+    params = rng.uniform(0.0, 2.0 * np.pi, size=(2, n, 2))
+    l0, l1 = _coefs(params[0]), _coefs(params[1])
+    full = (1 << n) - 1
+    sites = [[_site(w, q, l0, l1, True) for q in (0, 1)] for w in range(n)]
+    for base in [0, full & ~((1 << T) - 1)] + [int(x) << T for x in rng.integers(0, 1 << (n - T), 4)]:
+        want = np.zeros((2, 2), dtype=complex)
+        for w in range(T, n - 1):
+            want += _prod(sites, T, n - T, (base ^ _flip_mask(w, n)) >> T)
+        got = _summed_pair_segment(sites, T, n, base)
+        assert np.max(np.abs(got - want)) <= 1e-13 * max(1.0, np.max(np.abs(want)))
+    # one wire above the tile has no pair: the sum is empty
+    assert not np.any(_summed_pair_segment(sites[:T + 1], T, T + 1, 0))
 
 
 @pytest.mark.parametrize("n", [5, 9, 13])
@@ -290,12 +313,12 @@ def test_diagonal_after_ring_closed_form(n):
 @pytest.mark.parametrize("R", [3, 4])
 def test_generating_low_pass_start_equals_hamiltonian(oracle, n, R):
     # lambda at the start of layer 1's backward sweep, in the pre-CNOT frame:
-    # P^-1 H psi.  The generating low pass forms it for its tile as
-    # -ZZ(P i) phi(i) - h sum_{w < T-1} phi(i ^ m_w) - h sum_{w crossing} phi(i ^ m_w),
-    # the flipped amplitudes of crossing wires from the chain with U_A taken at
-    # base ^ (m_w above the tile) and the register / thread index flipped by m_w
-    # inside it; the high passes add -h phi(i ^ m_w) for the pairs inside
-    # their groups.  Sampled tiles and threads against the oracle.
+    # P^-1 H psi.  The generating low pass forms all of it for its tile as
+    # -ZZ(P i) phi(i) - h sum_{w < T-1} phi(i ^ m_w) - h Tr(run[k ^ 2^(R-1)] Y_0)
+    # - h Tr(run[k] Y_sum), with Y_0 from U_A at base ^ e_T (wire T - 1) and
+    # Y_sum from U_A at base ^ e_{n-1} on thread row t ^ 3 (wire n - 1) plus
+    # the summed pair segment on row t (w = T .. n - 2).  Sampled tiles and
+    # threads against the oracle.
     rng = np.random.default_rng(130 + n + R)
     field = 0.85
     params = rng.uniform(0.0, 2.0 * np.pi, size=(2, n, 2))
@@ -305,34 +328,33 @@ def test_generating_low_pass_start_equals_hamiltonian(oracle, n, R):
     hpsi = oracle.apply_hamiltonian(n, field, psi)
     sites, r0, run, chunks, ua, ub = _set(n, R, T, T, full, l0, l1)
     (f0, n0, t0), (f1, n1, t1) = chunks
-    hi_pairs = _high_group_pairs(n)
+    assert ua[0] == T and ua[0] + ua[1] == n
 
     def phi_at(i):
         return _direct_trace(sites, n, i)
 
+    def y_of(u, t):
+        return u @ t0[t & 15] @ t1[(t >> 4) & ((1 << n1) - 1)]
+
     err = 0.0
     for g in rng.integers(0, 1 << (n - T), 3):
         base = int(g) << T
+        u0 = _prod(sites, T, n - T, (base ^ (1 << T)) >> T)
+        u1 = _prod(sites, T, n - T, (base ^ (1 << (n - 1))) >> T)
+        usum = _summed_pair_segment(sites, T, n, base)
         for t in rng.integers(0, 1 << (T - R), 8):
             t = int(t)
+            y = y_of(_uniform(sites, ua, base), t)
+            y0 = y_of(u0, t)
+            ysum = y_of(u1, t ^ 3) + y_of(usum, t)
             for k in range(1 << R):
-                o = (k << (T - R)) | t
-                i = base | o
-                y = _uniform(sites, ua, base) @ t0[t & 15] @ t1[(t >> 4) & ((1 << n1) - 1)]
+                i = base | (k << (T - R)) | t
                 ph = np.trace(run[k] @ y)
                 lam = -(n - 2 * _walls_after_ring(i)) * ph
                 for w in range(T - 1):
                     lam -= field * phi_at(i ^ _flip_mask(w, n))
-                for w in _crossing_wires(n):
-                    m = _flip_mask(w, n)
-                    hi_m, lo_m = m & ~((1 << T) - 1), m & ((1 << T) - 1)
-                    k2 = k ^ ((lo_m >> (T - R)) & ((1 << R) - 1))
-                    t2 = t ^ (lo_m & ((1 << (T - R)) - 1))
-                    y2 = (_uniform(sites, ua, base ^ hi_m) @ t0[t2 & 15] @
-                          t1[(t2 >> 4) & ((1 << n1) - 1)])
-                    lam -= field * np.trace(run[k2] @ y2)
-                for w in hi_pairs:
-                    lam -= field * phi_at(i ^ _flip_mask(w, n))
+                lam -= field * np.trace(run[k ^ (1 << (R - 1))] @ y0)
+                lam -= field * np.trace(run[k] @ ysum)
                 want = hpsi[oracle.ring_cnot_index(i, n)]
                 err = max(err, abs(ph - psi[oracle.ring_cnot_index(i, n)]), abs(lam - want))
     assert err <= 1e-12
