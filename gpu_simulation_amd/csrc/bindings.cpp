@@ -235,6 +235,12 @@ class RingIsingCudaBackend {
 
     bool double_buffer() const { return rie_engine_double_buffer(engine_) == 1; }
 
+    void set_lambda_layouts(int enabled) {
+        check(rie_engine_set_lambda_layouts(engine_, enabled));
+    }
+
+    bool lambda_layouts() const { return rie_engine_lambda_layouts(engine_) == 1; }
+
     void set_kernel_timing(bool enable) { check(rie_engine_set_kernel_timing(engine_, enable)); }
 
     void reset_kernel_stats() { check(rie_engine_reset_kernel_stats(engine_)); }
@@ -577,6 +583,8 @@ PYBIND11_MODULE(_cuda_backend, m) {
         .def_property_readonly("generated_start", &RingIsingCudaBackend::generated_start)
         .def("set_double_buffer", &RingIsingCudaBackend::set_double_buffer, py::arg("enabled"))
         .def_property_readonly("double_buffer", &RingIsingCudaBackend::double_buffer)
+        .def("set_lambda_layouts", &RingIsingCudaBackend::set_lambda_layouts, py::arg("enabled"))
+        .def_property_readonly("lambda_layouts", &RingIsingCudaBackend::lambda_layouts)
         .def_property_readonly("schedule", &RingIsingCudaBackend::schedule)
         .def("set_kernel_timing", &RingIsingCudaBackend::set_kernel_timing, py::arg("enable"))
         .def("reset_kernel_stats", &RingIsingCudaBackend::reset_kernel_stats)

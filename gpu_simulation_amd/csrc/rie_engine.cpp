@@ -607,9 +607,10 @@ void Engine::structured_hamiltonian(hipStream_t s, bool write_lam) {
 
 void Engine::launch_bwd(int l, bool gather, bool write_cur, const double2 *cur_in,
                         const double2 *lam_in, double2 *cur_out, double2 *lam_out, size_t p,
-                        hipStream_t s, int gen_set) {
+                        hipStream_t s, int gen_set, const PassGeom *geom) {
     const WireCoef *lc = d_coef + (size_t)l * n;
     const PassPlan &pp = passes[p];
+    const PassGeom &pg = geom != nullptr ? *geom : pp.g;
     const CMat2 *gen_tab = nullptr;
     const CMat2 *gen_utab = nullptr;
     if (gen_set >= 0) {
@@ -625,8 +626,35 @@ void Engine::launch_bwd(int l, bool gather, bool write_cur, const double2 *cur_i
     // one; the walk's one-sweep passes do not
     const bool prefetch = double_buffer && !write_cur;
     launch_bwd_pass(pass_r[gather ? RIE_PASS_BWD_LOW : RIE_PASS_BWD_HIGH], gather, write_cur, true,
-                    cur_in, lam_in, cur_out, lam_out, lc, pp.g, d_partials, pp.grid, s, pairs,
+                    cur_in, lam_in, cur_out, lam_out, lc, pg, d_partials, pp.grid, s, pairs,
                     pairs ? h : 0.0, gen_tab, gen_utab, prefetch);
+}
+
+// The generated start's lambda layouts: the low pass writes high pass K-1's
+// layout, high pass j reads its own and writes high pass j-1's, high pass 0
+// writes the natural order.
+bool Engine::lam_layouts_fit() const {
+    const int K = (int)passes.size() - 1;
+    if (K < 1) {
+        return false;
+    }
+    const int r_low = pass_r[RIE_PASS_BWD_LOW];
+    const int r_high = pass_r[RIE_PASS_BWD_HIGH];
+    if (!lay_fits(passes[(size_t)K].g.h0, n, T - r_low, r_low)) {
+        return false;
+    }
+    for (int j = 0; j < K; ++j) {
+        const PassGeom &g = passes[1 + (size_t)j].g;
+        if (g.a != 3) {
+            return false;
+        }
+        const int first = g.h0 + (T - r_high - g.a);
+        const int out = j > 0 ? passes[(size_t)j].g.h0 : 0;
+        if (!lay_fits(g.h0, n, first, r_high) || !lay_fits(out, n, first, r_high)) {
+            return false;
+        }
+    }
+    return true;
 }
 
 bool Engine::gen_ready() const {
@@ -732,6 +760,14 @@ void Engine::structured_eval_gen_start(hipStream_t s, const double *scale, doubl
     g0.pair_hi = T;
     double2 *lam = pool[0];
     const int K = (int)passes.size() - 1;
+    // lambda layouts: high pass j reads its tiles as contiguous blocks.  A
+    // tile's write addresses then differ from its read addresses, so the
+    // high passes alternate between two buffers instead of working in place.
+    const bool layouts = lam_layouts_enabled && lam_layouts_fit() && pool.size() >= 2;
+    double2 *lam_other = layouts ? pool[1] : lam;
+    if (layouts) {
+        g0.lay_out = passes[(size_t)K].g.h0;
+    }
     // every X term is in lambda once the low pass is done: the high passes
     // add no pair terms (fuse_layer stays -1)
     fuse_layer = -1;
@@ -751,9 +787,16 @@ void Engine::structured_eval_gen_start(hipStream_t s, const double *scale, doubl
             const PassPlan &pp = passes[1 + (size_t)j];
             require_partials((size_t)pp.grid * 2 * T);
             kt_begin(s);
-            launch_bwd(1, false, false, nullptr, lam, nullptr, lam, (size_t)(1 + j), s, 1 + j);
+            PassGeom gj = pp.g;
+            if (layouts) {
+                gj.lay_in = gj.h0;
+                gj.lay_out = j > 0 ? passes[(size_t)j].g.h0 : 0;
+            }
+            launch_bwd(1, false, false, nullptr, lam, nullptr, lam_other, (size_t)(1 + j), s, 1 + j,
+                       &gj);
             check_launch("bwd_pass");
             kt_end(s, "bwd_pass_high_gen", 2 * S);
+            std::swap(lam, lam_other);
             kt_begin(s);
             launch_reduce_grad(d_partials, pp.grid, T, pp.g, 1, d_grad, scale, s);
             check_launch("reduce_grad");
@@ -1871,6 +1914,27 @@ rie_status rie_engine_set_double_buffer(rie_engine *engine, int enabled) {
         }
         e->double_buffer = enabled == 1;
     });
+}
+
+rie_status rie_engine_set_lambda_layouts(rie_engine *engine, int enabled) {
+    return guard([&]() {
+        Engine *e = as_engine(engine);
+        if (enabled != 0 && enabled != 1) {
+            rie::fail(RIE_INVALID_ARGUMENT, "enabled must be 0 or 1.");
+        }
+        e->lam_layouts_enabled = enabled == 1;
+    });
+}
+
+int rie_engine_lambda_layouts(const rie_engine *engine) {
+    if (engine == nullptr) {
+        return -1;
+    }
+    const Engine *e = reinterpret_cast<const Engine *>(engine);
+    return e->lam_layouts_enabled && e->gen_start_ready() && e->lam_layouts_fit() &&
+                   e->pool.size() >= 2
+               ? 1
+               : 0;
 }
 
 int rie_engine_double_buffer(const rie_engine *engine) {

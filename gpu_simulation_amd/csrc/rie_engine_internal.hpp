@@ -112,6 +112,10 @@ struct Engine {
     // layer-0 contraction.
     bool gen_start_enabled = true;  // caller's switch
     CMat2 *d_chain_x = nullptr;     // per tile: U_A at the out-of-tile flips
+    // Lambda layouts of the generated start: each high pass reads its tile
+    // as one contiguous block (the pass before it writes that layout).
+    bool lam_layouts_enabled = true;  // caller's switch
+    bool lam_layouts_fit() const;
     CMat2 *d_chain = nullptr;   // kChainMaxSets * kChainTabEntries matrices
     CMat2 *d_chain_u = nullptr; // per set and tile: the uniform segments U_A, U_B
 
@@ -195,7 +199,7 @@ struct Engine {
                                bool staged);
     void launch_bwd(int l, bool gather, bool write_cur, const double2 *cur_in,
                     const double2 *lam_in, double2 *cur_out, double2 *lam_out, size_t p,
-                    hipStream_t s, int gen_set = -1);
+                    hipStream_t s, int gen_set = -1, const PassGeom *geom = nullptr);
     bool gen_ready() const;
     bool gen_start_ready() const;
     void build_chain_tables(hipStream_t s, int r_set0 = -1);

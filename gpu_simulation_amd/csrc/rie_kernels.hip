@@ -858,7 +858,9 @@ __device__ __forceinline__ void bwd_load_tile(double2 (&c)[Tile<R>::REGS],
                                               double field,
                                               const CMat2 *__restrict__ gen_tab = nullptr,
                                               const CMat2 *gen_chunks = nullptr,
-                                              const CMat2 *__restrict__ gen_u = nullptr) {
+                                              const CMat2 *__restrict__ gen_u = nullptr,
+                                              uint64_t lam_base = 0,
+                                              const TileAddr *lam_ta = nullptr) {
     using TL = Tile<R>;
     constexpr int T = TL::T;
     if constexpr (kGather) {
@@ -902,15 +904,18 @@ __device__ __forceinline__ void bwd_load_tile(double2 (&c)[Tile<R>::REGS],
             add_pair_terms<R, 1>(c, l, lds, t, G, 0u, field);
         }
     } else {
+        // row: the index of register 0 (the state's address, the chain's
+        // argument); lambda may sit in another layout (lam_ta, lam_base)
         const uint64_t row = base + ta.offA;
+        const TileAddr &la = lam_ta != nullptr ? *lam_ta : ta;
+        const uint64_t lrow = (lam_ta != nullptr ? lam_base : base) + la.offA;
 #pragma unroll
         for (int k = 0; k < TL::REGS; ++k) {
-            const uint64_t idx = row + k * ta.strideA;
             if constexpr (!kReady) {
-                l[k] = lam_in[idx];
+                l[k] = lam_in[lrow + k * la.strideA];
             }
             if constexpr (!kGen) {
-                c[k] = cur_in[idx];
+                c[k] = cur_in[row + k * ta.strideA];
             }
         }
         if constexpr (kGen) {
@@ -1211,6 +1216,16 @@ __device__ __forceinline__ void ry_inv_lane_bit3(double2 (&l)[Tile<R>::REGS],
     }
 }
 
+// The load mapping's addresses in lambda layout h (a bit permutation: the
+// offset and the register step map bit by bit; the host checks that the
+// register bits stay contiguous).
+__device__ __forceinline__ TileAddr lay_tile_addr(const TileAddr &ta, int h) {
+    TileAddr r = ta;
+    r.offA = lay_addr(ta.offA, h);
+    r.strideA = lay_addr(ta.strideA, h);
+    return r;
+}
+
 // Stores lam from mapping M: the load mapping of a high pass, any mapping of
 // the low tile (contiguous: offset = index - base).
 template <int R, int M, bool kGather>
@@ -1223,12 +1238,12 @@ __device__ __forceinline__ void store_lam(const double2 (&l)[Tile<R>::REGS], dou
         }
     } else {
         static_assert(M == 0, "high passes store from the load mapping");
-        double2 *dst = lam_out + base;
-        const uint32_t o0 = (uint32_t)ta.offA;
-        const uint32_t sa = (uint32_t)ta.strideA;
+        // 64-bit offsets: at 33 and 34 qubits the register step of the top
+        // high group is 2^29 or more, and 15 steps leave 32 bits
+        double2 *dst = lam_out + base + ta.offA;
 #pragma unroll
         for (int k = 0; k < Tile<R>::REGS; ++k) {
-            dst[o0 + (uint32_t)k * sa] = l[k];
+            dst[(uint64_t)k * ta.strideA] = l[k];
         }
     }
 }
@@ -1303,6 +1318,10 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_la
     __syncthreads();
 
     const TileAddr ta = tile_addr<R>(t, G);
+    // lambda's layouts (high passes; the gather pass reads and writes the
+    // natural order)
+    const TileAddr ta_in = lay_tile_addr(ta, kGather ? 0 : G.lay_in);
+    const TileAddr ta_out = lay_tile_addr(ta, kGather ? 0 : G.lay_out);
     static_assert(!(kPrefetch && kGather && kGen), "the gather pass reads the state");
     double2 x_next[kPrefetch ? TL::REGS : 1];
     auto prefetch_tile = [&](uint64_t gn, const TileAddr &a, uint32_t tt) {
@@ -1310,13 +1329,13 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_la
             if constexpr (kGather) {
                 load_cur_gather<R>(x_next, cur_in, tile_base(gn, G, T), G, tt);
             } else {
-                load_lam_high<R>(x_next, lam_in, tile_base(gn, G, T), a);
+                load_lam_high<R>(x_next, lam_in, lay_addr(tile_base(gn, G, T), G.lay_in), a);
             }
         }
     };
     if constexpr (kPrefetch) {
         if ((uint64_t)blockIdx.x < G.num_tiles) {
-            prefetch_tile(blockIdx.x, ta, t);
+            prefetch_tile(blockIdx.x, ta_in, t);
         }
     }
     for (uint64_t g = blockIdx.x; g < G.num_tiles; g += gridDim.x) {
@@ -1329,8 +1348,10 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_la
         // stores are formed per tile as well (hoisted out of the loop they
         // would be spilled)
         TileAddr tla = ta;
+        TileAddr tla_in = ta_in;
         if constexpr (kPrefetch) {
             asm volatile("" : "+v"(tla.offA));
+            asm volatile("" : "+v"(tla_in.offA));
         }
         const uint64_t base = tile_base(g, G, T);
         double2 c[TL::REGS];
@@ -1347,7 +1368,8 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_la
         }
         bwd_load_tile<R, kGather, kHam, kGen, kPrefetch>(c, l, cur_in, lam_in, base, tla, G, lds,
                                                          tl, field, gen_tab, gen_chunks,
-                                                         gen_utab + 2 * g);
+                                                         gen_utab + 2 * g,
+                                                         lay_addr(base, G.lay_in), &tla_in);
         // the next tile, into the registers the state frees (called right
         // after the state's last use; scheduling barriers keep the loads from
         // being hoisted above it, where both would need registers)
@@ -1356,7 +1378,7 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_la
                 __builtin_amdgcn_sched_barrier(0);
                 const uint64_t gn = g + gridDim.x;
                 if (gn < G.num_tiles) {
-                    prefetch_tile(gn, tla, tl);
+                    prefetch_tile(gn, tla_in, tl);
                 }
                 __builtin_amdgcn_sched_barrier(0);
             }
@@ -1440,7 +1462,7 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_la
                 ry_inv_lane_bit3<R>(l, tile_coef[3], lane);
                 exchange<R, 2, 0>(l, lds, tl);
                 ry_inv_window<R, 0>(l, tile_coef);
-                store_lam<R, 0, false>(l, lam_out, base, tla, tl);
+                store_lam<R, 0, false>(l, lam_out, lay_addr(base, G.lay_out), ta_out, tl);
             } else {
                 uint32_t done = w0;
                 const bool w1_rot = G.rot_lo < R;
@@ -1481,7 +1503,7 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_la
                     exchange<R, 2, 0>(l, lds, tl);
                 }
                 ry_inv_window<R, 0>(l, tile_coef);
-                store_lam<R, 0, false>(l, lam_out, base, tla, tl);
+                store_lam<R, 0, false>(l, lam_out, lay_addr(base, G.lay_out), ta_out, tl);
             }
         }
     }
@@ -1636,8 +1658,12 @@ __global__ __launch_bounds__(Tile<R>::THREADS, low_gen_min_waves<R>()) void bwd_
             exchange<R, 1, 0>(l, lds, tl);
             ry_inv_window<R, 0>(l, tile_coef);
         }
-        const TileAddr ta{};
-        store_lam<R, 0, true>(l, lam_out, base, ta, tl);
+        // lambda in layout G.lay_out (a bit permutation: mapping 0's offset and
+        // its register step, tile bit T - R, map bit by bit)
+        TileAddr ta{};
+        ta.offA = lay_addr(omap<R, 0>(tl, 0), G.lay_out);
+        ta.strideA = lay_addr(uint64_t{1} << (T - R), G.lay_out);
+        store_lam<R, 0, false>(l, lam_out, lay_addr(base, G.lay_out), ta, tl);
     }
     __syncthreads();
     if (t < 2 * T) {
@@ -1654,7 +1680,7 @@ void launch_bwd_low_gen(int R, double2 *lam_out, const WireCoef *layer_coef, con
                         double *partials, int grid, hipStream_t s, double field,
                         const CMat2 *tab, const CMat2 *utab, const CMat2 *xtab) {
     if (g.a != Tile<4>::T || g.h0 != Tile<4>::T || g.rot_lo != 0 || g.rot_hi != Tile<4>::T ||
-        g.n <= Tile<4>::T) {
+        g.n <= Tile<4>::T || g.lay_in != 0 || !lay_fits(g.lay_out, g.n, Tile<4>::T - R, R)) {
         throw std::invalid_argument("generating low pass: geometry outside the low tile");
     }
     if (R == 4) {
@@ -1678,6 +1704,15 @@ static void launch_bwd_pass_r(bool gather, bool write_cur, bool write_lam, const
                 Tile<R>::T - R - g.a < 0)) {
         // chunk C0 holds up to 4 tile bits, C1 up to 6
         throw std::invalid_argument("generated state: high pass geometry outside the chain tables");
+    }
+    if (g.lay_in != 0 || g.lay_out != 0) {
+        // lambda layouts: lambda-only high passes whose register bits the
+        // layouts keep contiguous
+        const int first = g.h0 + (Tile<R>::T - R - g.a);
+        if (gather || write_cur || !write_lam || g.a != 3 || !lay_fits(g.lay_in, g.n, first, R) ||
+            !lay_fits(g.lay_out, g.n, first, R)) {
+            throw std::invalid_argument("lambda layouts: pass or geometry outside their scope");
+        }
     }
     if (!write_cur && write_lam) {  // the state is not rewritten: lambda-only sweep
         if (!gather && R == 4 && (g.a != 3 || g.rot_lo < 3)) {

@@ -32,7 +32,47 @@ struct PassGeom {
     // is unused.
     int sc_n;
     int sc[4];
+    // Layouts of lambda in HBM for the lambda-only high passes and the
+    // generating low pass (0 elsewhere): lay_in the layout read, lay_out the
+    // layout written; see lay_addr.
+    int lay_in;
+    int lay_out;
 };
+
+// Address of index i in lambda layout h.  h == 0: the natural order.  h > 0:
+// the order that makes the tile of a high pass (a = 3, first high index bit
+// h) one contiguous 2^12 block: index bits 0..2 stay, bits [h, h + 9) move
+// down to 3..11, bits [3, h) move up to [12, h + 9), the rest stay.  A bit
+// permutation, so lay_addr(x | y) = lay_addr(x) | lay_addr(y) for disjoint x, y.
+__host__ __device__ inline uint64_t lay_addr(uint64_t i, int h) {
+    if (h == 0) {
+        return i;
+    }
+    const uint64_t low = i & 7u;
+    const uint64_t blk = (i >> h) & 511u;
+    const uint64_t mid = (i >> 3) & ((uint64_t{1} << (h - 3)) - 1);
+    const uint64_t top = (i >> (h + 9)) << (h + 9);
+    return low | (blk << 3) | (mid << 12) | top;
+}
+
+// Whether layout h suits a pass whose load mapping holds the index bits
+// [first, first + count) in its register index: h permutes only bits below n
+// and keeps those bits contiguous and in order.
+inline bool lay_fits(int h, int n, int first, int count) {
+    if (h == 0) {
+        return true;
+    }
+    if (h < 3 || h + 9 > n) {
+        return false;
+    }
+    const uint64_t p0 = lay_addr(uint64_t{1} << first, h);
+    for (int b = 1; b < count; ++b) {
+        if (lay_addr(uint64_t{1} << (first + b), h) != (p0 << b)) {
+            return false;
+        }
+    }
+    return true;
+}
 
 // One independent circuit evaluated by a single workgroup (n <= 14).
 struct BatchJob {

@@ -228,6 +228,19 @@ int rie_engine_layer1_generation(const rie_engine *engine);
 rie_status rie_engine_set_generated_start(rie_engine *engine, int enabled);
 int rie_engine_generated_start(const rie_engine *engine);
 
+/* Lambda layouts of the generated start (no counterpart in the reference,
+ * whose passes keep every vector in index order).  Between the passes of a
+ * generated-start evaluation lambda is stored in the order that makes the
+ * next high pass's tiles contiguous 64 KiB blocks (index bits 0..2 and that
+ * pass's nine high bits at the bottom), so every high pass reads whole
+ * blocks and only its writes are 128-byte runs; the last high pass writes
+ * the natural order.  Results are bitwise identical either way.  enabled: 1
+ * on (default), 0 off; rie_engine_lambda_layouts reports whether gradient
+ * evaluations use them (the generated start in use and the pass geometry
+ * suited to them). */
+rie_status rie_engine_set_lambda_layouts(rie_engine *engine, int enabled);
+int rie_engine_lambda_layouts(const rie_engine *engine);
+
 /* Double-buffered adjoint high passes.  Replaces the double_buffer flag of
  * RingIsingCudaBackend (cpp/ising_cuda_backend.cu:374-419), which selects a
  * backward rotation-chunk kernel that stages the next chunk while computing

@@ -439,6 +439,49 @@ def test_generated_start(oracle, backend_module, n, shape):
         del eng
 
 
+@pytest.mark.parametrize("n,r_low,r_high", [(13, 4, 4), (16, 3, 4), (21, 4, 3), (25, 4, 4),
+                                             (28, 3, 3), (30, 4, 4)])
+def test_lambda_layouts(oracle, backend_module, n, r_low, r_high):
+    """Generated-start evaluations with lambda stored between the passes in
+    the layouts that make each high pass's tiles contiguous blocks: only the
+    addresses change, so the results equal the natural order's bitwise; and
+    against the oracle (n <= 25)."""
+    params = np.ascontiguousarray(_params(8300 + n, n, 2).reshape(-1))
+    eng = backend_module.RingIsingCudaBackend(n, 2, 0.8, "structured_adjoint", 8, False)
+    shapes = list(eng.pass_shapes)
+    shapes[4], shapes[5] = r_low, r_high
+    eng.set_pass_shapes(shapes)
+    print(f"case lambda layouts n={n} r_low={r_low} r_high={r_high}", flush=True)
+    assert eng.generated_start and eng.lambda_layouts
+    out = eng.energy_and_grad(params)
+    eng.set_lambda_layouts(0)
+    assert not eng.lambda_layouts
+    ref = eng.energy_and_grad(params)
+    assert out["energy"] == ref["energy"]
+    assert np.array_equal(np.asarray(out["gradient"]), np.asarray(ref["gradient"]))
+    if n <= 25:
+        e_ref, g_ref = oracle.energy_and_grad(n, 2, 0.8, params, form="structured")
+        assert abs(out["energy"] - e_ref) <= TOL
+        assert np.max(np.abs(np.asarray(out["gradient"]) - g_ref)) <= TOL
+    with pytest.raises(ValueError):
+        eng.set_lambda_layouts(2)
+    del eng
+
+
+def test_lambda_layouts_scope(backend_module):
+    # 22 and 31 qubits: a group's register bits straddle the layout below it,
+    # so lambda stays in the natural order; without the generated start the
+    # layouts are not used
+    for n in (22, 31):
+        eng = backend_module.RingIsingCudaBackend(n, 2, 1.0, "structured_adjoint", 8, False)
+        assert eng.generated_start and not eng.lambda_layouts
+        del eng
+    eng = backend_module.RingIsingCudaBackend(16, 2, 1.0, "structured_adjoint", 8, False)
+    assert eng.lambda_layouts
+    eng.set_generated_start(0)
+    assert not eng.lambda_layouts
+
+
 def test_generated_start_scope(backend_module):
     # three layers, unfused H or no layer-1 generation: the forward runs as before
     eng = backend_module.RingIsingCudaBackend(16, 3, 1.0, "structured_adjoint", 8, False)
