@@ -3,7 +3,9 @@
 //
 //   hipcc --offload-arch=gfx950 -O3 -Xclang -target-feature -Xclang -packed-fp32-ops \
 //         hbm_probe.hip -o hbm_probe
-//   ./hbm_probe [GiB per buffer, default 8] [repeats, default 5]
+//   ./hbm_probe [GiB per buffer, default 8] [repeats, default 5] [tiles]
+//
+// With "tiles" as the third argument only the tile-pass patterns run.
 //
 // Prints one JSON object per kernel variant: algorithmic bytes per launch,
 // average launch time (HIP events) and GB/s.  Each variant keeps U
@@ -20,6 +22,7 @@
 #include <cstdio>
 #include <cstdint>
 #include <cstdlib>
+#include <string>
 
 #define CHECK(x)                                                                  \
     do {                                                                          \
@@ -109,6 +112,35 @@ __global__ __launch_bounds__(kThreads) void tile_kernel(const double2 *__restric
     if (acc == 12345.678) sink[0] = acc;  // never true for the zero-filled input
 }
 
+// Read one tile pattern and write another: loads from tiles of (a_in, h_in),
+// stores the same amplitudes to tiles of (a_out, h_out) -- the passes of a
+// layout that keeps one side of every pass contiguous.
+__global__ __launch_bounds__(kThreads) void tile_mixed_kernel(const double2 *__restrict__ in,
+                                                              double2 *__restrict__ out, int nbits,
+                                                              int a_in, int h_in, int a_out,
+                                                              int h_out) {
+    const uint64_t tiles = uint64_t{1} << (nbits - 12);
+    const int gap_in = h_in - a_in;
+    const int gap_out = h_out - a_out;
+    for (uint64_t g = blockIdx.x; g < tiles; g += gridDim.x) {
+        const uint64_t b_in = ((g & ((uint64_t{1} << gap_in) - 1)) << a_in) |
+                              ((g >> gap_in) << (h_in + 12 - a_in));
+        const uint64_t b_out = ((g & ((uint64_t{1} << gap_out) - 1)) << a_out) |
+                               ((g >> gap_out) << (h_out + 12 - a_out));
+        double2 v[16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const uint32_t o = threadIdx.x + 256u * (uint32_t)k;
+            v[k] = in[b_in | (o & ((1u << a_in) - 1u)) | ((uint64_t)(o >> a_in) << h_in)];
+        }
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const uint32_t o = threadIdx.x + 256u * (uint32_t)k;
+            out[b_out | (o & ((1u << a_out) - 1u)) | ((uint64_t)(o >> a_out) << h_out)] = v[k];
+        }
+    }
+}
+
 int main(int argc, char **argv) {
     const double gib = argc > 1 ? std::atof(argv[1]) : 8.0;
     const int repeats = argc > 2 ? std::atoi(argv[2]) : 5;
@@ -152,8 +184,10 @@ int main(int argc, char **argv) {
         std::fflush(stdout);
     };
 
+    const bool tiles_only = argc > 3 && std::string(argv[3]) == "tiles";
     const int grids[] = {4, 8, 16, 32};
     for (int g : grids) {
+        if (tiles_only) break;
         const int grid = cus * g;
         run("read", 1, g, S, [&]() { read_kernel<1><<<grid, kThreads>>>(a, n, sink); });
         run("read", 4, g, S, [&]() { read_kernel<4><<<grid, kThreads>>>(a, n, sink); });
@@ -188,6 +222,30 @@ int main(int argc, char **argv) {
                 });
             }
         }
+    }
+    // one side contiguous, the other 128-byte runs (and both scattered, for reference)
+    for (int g : {2, 4, 8}) {
+        const int grid = cus * g;
+        const double Sb = (double)(size_t{1} << nbits) * sizeof(double2);
+        const int hs = nbits - 9;  // the top high group
+        run("mixed_rd_contig_wr_a3", 16, g, 2 * Sb, [&]() {
+            tile_mixed_kernel<<<grid, kThreads>>>(a, b, nbits, 12, 12, 3, hs);
+        });
+        run("mixed_rd_a3_wr_contig", 16, g, 2 * Sb, [&]() {
+            tile_mixed_kernel<<<grid, kThreads>>>(a, b, nbits, 3, hs, 12, 12);
+        });
+        run("mixed_rd_a3_wr_a3", 16, g, 2 * Sb, [&]() {
+            tile_mixed_kernel<<<grid, kThreads>>>(a, b, nbits, 3, hs, 3, hs);
+        });
+        // the scattered side's nine high bits right above the contiguous
+        // block (h = 12): the order that keeps one side whole and puts the
+        // other side's group at address bits 12..20
+        run("mixed_rd_a3h12_wr_contig", 16, g, 2 * Sb, [&]() {
+            tile_mixed_kernel<<<grid, kThreads>>>(a, b, nbits, 3, 12, 12, 12);
+        });
+        run("mixed_rd_contig_wr_a3h12", 16, g, 2 * Sb, [&]() {
+            tile_mixed_kernel<<<grid, kThreads>>>(a, b, nbits, 12, 12, 3, 12);
+        });
     }
     CHECK(hipFree(a));
     CHECK(hipFree(b));

@@ -439,9 +439,10 @@ def test_generated_start(oracle, backend_module, n, shape):
         del eng
 
 
-@pytest.mark.parametrize("n,r_low,r_high", [(13, 4, 4), (16, 3, 4), (21, 4, 3), (25, 4, 4),
-                                             (28, 3, 3), (30, 4, 4)])
-def test_lambda_layouts(oracle, backend_module, n, r_low, r_high):
+@pytest.mark.parametrize("n,r_low,r_high,r_l0", [(13, 4, 4, 4), (16, 3, 4, 3), (21, 4, 3, 4),
+                                                  (21, 3, 4, 3), (25, 4, 4, 4), (28, 3, 3, 3),
+                                                  (30, 4, 4, 4), (30, 3, 4, 3)])
+def test_lambda_layouts(oracle, backend_module, n, r_low, r_high, r_l0):
     """Generated-start evaluations with lambda stored between the passes in
     the layouts that make each high pass's tiles contiguous blocks: only the
     addresses change, so the results equal the natural order's bitwise; and
@@ -449,9 +450,9 @@ def test_lambda_layouts(oracle, backend_module, n, r_low, r_high):
     params = np.ascontiguousarray(_params(8300 + n, n, 2).reshape(-1))
     eng = backend_module.RingIsingCudaBackend(n, 2, 0.8, "structured_adjoint", 8, False)
     shapes = list(eng.pass_shapes)
-    shapes[4], shapes[5] = r_low, r_high
+    shapes[4], shapes[5], shapes[6] = r_low, r_high, r_l0
     eng.set_pass_shapes(shapes)
-    print(f"case lambda layouts n={n} r_low={r_low} r_high={r_high}", flush=True)
+    print(f"case lambda layouts n={n} r_low={r_low} r_high={r_high} r_l0={r_l0}", flush=True)
     assert eng.generated_start and eng.lambda_layouts
     out = eng.energy_and_grad(params)
     eng.set_lambda_layouts(0)
