@@ -263,4 +263,24 @@ __device__ __forceinline__ double wave_sum(double v) {
     return (lane_value(v, 0) + lane_value(v, 16)) + (lane_value(v, 32) + lane_value(v, 48));
 }
 
+// Streaming (non-temporal) 16-byte loads and stores for statevector arrays
+// that one pass reads or writes once and no later access of the same pass
+// touches again.  Measured at 30 qubits: the generating low pass, which only
+// writes, runs 12.9 -> 12.0 ms with them (its ordinary stores stall in the L2's
+// write path to HBM); the lambda loads of the high passes and the layer-0
+// contraction gain a little.
+typedef double stream_v2d __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ void store_stream(double2 *p, double2 v) {
+    stream_v2d x;
+    x.x = v.x;
+    x.y = v.y;
+    __builtin_nontemporal_store(x, reinterpret_cast<stream_v2d *>(p));
+}
+
+__device__ __forceinline__ double2 load_stream(const double2 *p) {
+    const stream_v2d x = __builtin_nontemporal_load(reinterpret_cast<const stream_v2d *>(p));
+    return make_double2(x.x, x.y);
+}
+
 }  // namespace rie

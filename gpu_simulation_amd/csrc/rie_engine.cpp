@@ -630,9 +630,21 @@ void Engine::launch_bwd(int l, bool gather, bool write_cur, const double2 *cur_i
                     pairs ? h : 0.0, gen_tab, gen_utab, prefetch);
 }
 
-// The generated start's lambda layouts: the low pass writes high pass K-1's
-// layout, high pass j reads its own and writes high pass j-1's, high pass 0
-// writes the natural order.
+// The generated start's lambda layouts: the low pass writes the layout that
+// high pass K-1 (the first to run) reads, high pass j reads its own and writes
+// the one high pass j-1 reads, high pass 0 writes the natural order.  Each
+// layout puts the reader's tile in one block and the writer's tile bits right
+// above it (make_lam_layout).
+LamLayout Engine::lam_layout_in(int j) const {
+    const int K = (int)passes.size() - 1;
+    const int w = j == K - 1 ? 3 : passes[2 + (size_t)j].g.h0;
+    return make_lam_layout(n, passes[1 + (size_t)j].g.h0, w);
+}
+
+LamLayout Engine::lam_layout_out(int j) const {
+    return j > 0 ? lam_layout_in(j - 1) : LamLayout{};
+}
+
 bool Engine::lam_layouts_fit() const {
     const int K = (int)passes.size() - 1;
     if (K < 1) {
@@ -640,7 +652,7 @@ bool Engine::lam_layouts_fit() const {
     }
     const int r_low = pass_r[RIE_PASS_BWD_LOW];
     const int r_high = pass_r[RIE_PASS_BWD_HIGH];
-    if (!lay_fits(passes[(size_t)K].g.h0, n, T - r_low, r_low)) {
+    if (!lay_fits(lam_layout_in(K - 1), n, T - r_low, r_low)) {
         return false;
     }
     for (int j = 0; j < K; ++j) {
@@ -649,8 +661,8 @@ bool Engine::lam_layouts_fit() const {
             return false;
         }
         const int first = g.h0 + (T - r_high - g.a);
-        const int out = j > 0 ? passes[(size_t)j].g.h0 : 0;
-        if (!lay_fits(g.h0, n, first, r_high) || !lay_fits(out, n, first, r_high)) {
+        if (!lay_fits(lam_layout_in(j), n, first, r_high) ||
+            !lay_fits(lam_layout_out(j), n, first, r_high)) {
             return false;
         }
     }
@@ -766,7 +778,7 @@ void Engine::structured_eval_gen_start(hipStream_t s, const double *scale, doubl
     const bool layouts = lam_layouts_enabled && lam_layouts_fit() && pool.size() >= 2;
     double2 *lam_other = layouts ? pool[1] : lam;
     if (layouts) {
-        g0.lay_out = passes[(size_t)K].g.h0;
+        g0.lay_out = lam_layout_in(K - 1);
     }
     // every X term is in lambda once the low pass is done: the high passes
     // add no pair terms (fuse_layer stays -1)
@@ -789,8 +801,8 @@ void Engine::structured_eval_gen_start(hipStream_t s, const double *scale, doubl
             kt_begin(s);
             PassGeom gj = pp.g;
             if (layouts) {
-                gj.lay_in = gj.h0;
-                gj.lay_out = j > 0 ? passes[(size_t)j].g.h0 : 0;
+                gj.lay_in = lam_layout_in(j);
+                gj.lay_out = lam_layout_out(j);
             }
             launch_bwd(1, false, false, nullptr, lam, nullptr, lam_other, (size_t)(1 + j), s, 1 + j,
                        &gj);

@@ -116,6 +116,8 @@ struct Engine {
     // as one contiguous block (the pass before it writes that layout).
     bool lam_layouts_enabled = true;  // caller's switch
     bool lam_layouts_fit() const;
+    LamLayout lam_layout_in(int j) const;   // read by high pass j
+    LamLayout lam_layout_out(int j) const;  // written by high pass j
     CMat2 *d_chain = nullptr;   // kChainMaxSets * kChainTabEntries matrices
     CMat2 *d_chain_u = nullptr; // per set and tile: the uniform segments U_A, U_B
 

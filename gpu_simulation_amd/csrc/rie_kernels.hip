@@ -912,7 +912,7 @@ __device__ __forceinline__ void bwd_load_tile(double2 (&c)[Tile<R>::REGS],
 #pragma unroll
         for (int k = 0; k < TL::REGS; ++k) {
             if constexpr (!kReady) {
-                l[k] = lam_in[lrow + k * la.strideA];
+                l[k] = load_stream(lam_in + lrow + k * la.strideA);
             }
             if constexpr (!kGen) {
                 c[k] = cur_in[row + k * ta.strideA];
@@ -1216,13 +1216,13 @@ __device__ __forceinline__ void ry_inv_lane_bit3(double2 (&l)[Tile<R>::REGS],
     }
 }
 
-// The load mapping's addresses in lambda layout h (a bit permutation: the
+// The load mapping's addresses in lambda layout L (a bit permutation: the
 // offset and the register step map bit by bit; the host checks that the
 // register bits stay contiguous).
-__device__ __forceinline__ TileAddr lay_tile_addr(const TileAddr &ta, int h) {
+__device__ __forceinline__ TileAddr lay_tile_addr(const TileAddr &ta, const LamLayout &L) {
     TileAddr r = ta;
-    r.offA = lay_addr(ta.offA, h);
-    r.strideA = lay_addr(ta.strideA, h);
+    r.offA = lay_addr(ta.offA, L);
+    r.strideA = lay_addr(ta.strideA, L);
     return r;
 }
 
@@ -1243,7 +1243,7 @@ __device__ __forceinline__ void store_lam(const double2 (&l)[Tile<R>::REGS], dou
         double2 *dst = lam_out + base + ta.offA;
 #pragma unroll
         for (int k = 0; k < Tile<R>::REGS; ++k) {
-            dst[(uint64_t)k * ta.strideA] = l[k];
+            store_stream(dst + (uint64_t)k * ta.strideA, l[k]);
         }
     }
 }
@@ -1272,7 +1272,7 @@ __device__ __forceinline__ void load_lam_high(double2 (&l)[Tile<R>::REGS],
     const uint64_t row = base + ta.offA;
 #pragma unroll
     for (int k = 0; k < Tile<R>::REGS; ++k) {
-        l[k] = lam_in[row + k * ta.strideA];
+        l[k] = load_stream(lam_in + row + k * ta.strideA);
     }
 }
 
@@ -1320,8 +1320,8 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_la
     const TileAddr ta = tile_addr<R>(t, G);
     // lambda's layouts (high passes; the gather pass reads and writes the
     // natural order)
-    const TileAddr ta_in = lay_tile_addr(ta, kGather ? 0 : G.lay_in);
-    const TileAddr ta_out = lay_tile_addr(ta, kGather ? 0 : G.lay_out);
+    const TileAddr ta_in = lay_tile_addr(ta, kGather ? LamLayout{} : G.lay_in);
+    const TileAddr ta_out = lay_tile_addr(ta, kGather ? LamLayout{} : G.lay_out);
     static_assert(!(kPrefetch && kGather && kGen), "the gather pass reads the state");
     double2 x_next[kPrefetch ? TL::REGS : 1];
     auto prefetch_tile = [&](uint64_t gn, const TileAddr &a, uint32_t tt) {
@@ -1680,7 +1680,7 @@ void launch_bwd_low_gen(int R, double2 *lam_out, const WireCoef *layer_coef, con
                         double *partials, int grid, hipStream_t s, double field,
                         const CMat2 *tab, const CMat2 *utab, const CMat2 *xtab) {
     if (g.a != Tile<4>::T || g.h0 != Tile<4>::T || g.rot_lo != 0 || g.rot_hi != Tile<4>::T ||
-        g.n <= Tile<4>::T || g.lay_in != 0 || !lay_fits(g.lay_out, g.n, Tile<4>::T - R, R)) {
+        g.n <= Tile<4>::T || g.lay_in.count != 0 || !lay_fits(g.lay_out, g.n, Tile<4>::T - R, R)) {
         throw std::invalid_argument("generating low pass: geometry outside the low tile");
     }
     if (R == 4) {
@@ -1705,7 +1705,7 @@ static void launch_bwd_pass_r(bool gather, bool write_cur, bool write_lam, const
         // chunk C0 holds up to 4 tile bits, C1 up to 6
         throw std::invalid_argument("generated state: high pass geometry outside the chain tables");
     }
-    if (g.lay_in != 0 || g.lay_out != 0) {
+    if (g.lay_in.count != 0 || g.lay_out.count != 0) {
         // lambda layouts: lambda-only high passes whose register bits the
         // layouts keep contiguous
         const int first = g.h0 + (Tile<R>::T - R - g.a);
@@ -2132,7 +2132,7 @@ __global__ __launch_bounds__(256) void layer0_contract_kernel(
         double2 m[16];
 #pragma unroll
         for (int k = 0; k < 16; ++k) {
-            m[k] = lam[(k < 8 ? run : run1) + ((uint64_t)k << 8)];
+            m[k] = load_stream(lam + (k < 8 ? run : run1) + ((uint64_t)k << 8));
         }
         __syncthreads();
 #pragma unroll
@@ -2314,7 +2314,7 @@ __global__ __launch_bounds__(512, 4) void layer0_contract_r3_kernel(
         double2 m[8];
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
-            m[k] = lam[(k < 4 ? run : run1) + ((uint64_t)k << TB)];
+            m[k] = load_stream(lam + (k < 4 ? run : run1) + ((uint64_t)k << TB));
         }
         __syncthreads();
 #pragma unroll

@@ -8,6 +8,17 @@
 
 namespace rie {
 
+// A bit permutation of statevector indices: a lambda layout.  Segment s moves
+// index bits [src[s], src[s] + len[s]) to address bits [dst[s], dst[s] + len[s]);
+// bits no segment names are zero (indices are below 2^n).  count == 0: the
+// natural order.
+struct LamLayout {
+    int count;
+    int8_t src[8];
+    int8_t len[8];
+    int8_t dst[8];
+};
+
 // Geometry of one tile pass.  A tile holds 2^T amplitudes (T = 3R): its
 // offset bits [0, a) are index bits [0, a) (contiguous in memory) and its
 // offset bits [a, T) are index bits [h0, h0 + T - a).  The remaining index
@@ -33,41 +44,99 @@ struct PassGeom {
     int sc_n;
     int sc[4];
     // Layouts of lambda in HBM for the lambda-only high passes and the
-    // generating low pass (0 elsewhere): lay_in the layout read, lay_out the
-    // layout written; see lay_addr.
-    int lay_in;
-    int lay_out;
+    // generating low pass (the natural order elsewhere): lay_in the layout
+    // read, lay_out the layout written; see make_lam_layout.
+    LamLayout lay_in;
+    LamLayout lay_out;
 };
 
-// Address of index i in lambda layout h.  h == 0: the natural order.  h > 0:
-// the order that makes the tile of a high pass (a = 3, first high index bit
-// h) one contiguous 2^12 block: index bits 0..2 stay, bits [h, h + 9) move
-// down to 3..11, bits [3, h) move up to [12, h + 9), the rest stay.  A bit
-// permutation, so lay_addr(x | y) = lay_addr(x) | lay_addr(y) for disjoint x, y.
-__host__ __device__ inline uint64_t lay_addr(uint64_t i, int h) {
-    if (h == 0) {
+// Address of index i in layout L (a bit permutation, so lay_addr(x | y) =
+// lay_addr(x) | lay_addr(y) for disjoint x, y, and strides map bit by bit).
+__host__ __device__ inline uint64_t lay_addr(uint64_t i, const LamLayout &L) {
+    if (L.count == 0) {
         return i;
     }
-    const uint64_t low = i & 7u;
-    const uint64_t blk = (i >> h) & 511u;
-    const uint64_t mid = (i >> 3) & ((uint64_t{1} << (h - 3)) - 1);
-    const uint64_t top = (i >> (h + 9)) << (h + 9);
-    return low | (blk << 3) | (mid << 12) | top;
+    uint64_t r = 0;
+    for (int s = 0; s < L.count; ++s) {
+        r |= ((i >> L.src[s]) & ((uint64_t{1} << L.len[s]) - 1)) << L.dst[s];
+    }
+    return r;
 }
 
-// Whether layout h suits a pass whose load mapping holds the index bits
-// [first, first + count) in its register index: h permutes only bits below n
-// and keeps those bits contiguous and in order.
-inline bool lay_fits(int h, int n, int first, int count) {
-    if (h == 0) {
+// The layout that a high pass with tile index bits {0, 1, 2} and [h, h + 9)
+// reads as one contiguous 2^12 block, written by a pass whose tile is index
+// bits {0, 1, 2} and [w, w + 9) (w = 3: the low tile [0, 12)): address bits
+// 0..2 and 3..11 hold the reader's tile bits in order, the writer's tile bits
+// outside the reader's tile come next (ascending), then every other bit below
+// n (ascending).  The writer's tile then spans 2^21 amplitudes (32 MB, 16 pages
+// of 2 MB) however far apart its bits lie in the index, so its stores stay in
+// few pages; with the writer's bits anywhere above, a top high group writing
+// for the group below it touched 512 pages per tile and missed in the L1 TLB
+// on a quarter of its translations.
+inline LamLayout make_lam_layout(int n, int h, int w) {
+    int order[64];
+    int count = 0;
+    bool placed[64] = {};
+    auto put = [&](int b) {
+        if (b >= 0 && b < n && !placed[b]) {
+            placed[b] = true;
+            order[count++] = b;
+        }
+    };
+    for (int b = 0; b < 3; ++b) {
+        put(b);
+    }
+    for (int b = h; b < h + 9; ++b) {
+        put(b);
+    }
+    for (int b = w; b < w + 9; ++b) {
+        put(b);
+    }
+    for (int b = 3; b < n; ++b) {
+        put(b);
+    }
+    LamLayout L{};
+    for (int d = 0; d < count; ++d) {
+        const int b = order[d];
+        if (L.count > 0 && L.src[L.count - 1] + L.len[L.count - 1] == b &&
+            L.dst[L.count - 1] + L.len[L.count - 1] == d) {
+            ++L.len[L.count - 1];
+            continue;
+        }
+        if (L.count == 8) {
+            return LamLayout{-1, {}, {}, {}};  // more pieces than a layout holds
+        }
+        L.src[L.count] = (int8_t)b;
+        L.len[L.count] = 1;
+        L.dst[L.count] = (int8_t)d;
+        ++L.count;
+    }
+    if (L.count == 1) {
+        L.count = 0;  // the identity: the natural order
+    }
+    return L;
+}
+
+// Whether layout L suits a pass whose load mapping holds the index bits
+// [first, first + count) in its register index: L is a permutation of the bits
+// below n and keeps those bits contiguous and in order.
+inline bool lay_fits(const LamLayout &L, int n, int first, int count) {
+    if (L.count == 0) {
         return true;
     }
-    if (h < 3 || h + 9 > n) {
+    if (L.count < 0 || L.count > 8) {
         return false;
     }
-    const uint64_t p0 = lay_addr(uint64_t{1} << first, h);
+    uint64_t image = 0;
+    for (int b = 0; b < n; ++b) {
+        image |= lay_addr(uint64_t{1} << b, L);
+    }
+    if (image != ((n >= 64) ? ~uint64_t{0} : ((uint64_t{1} << n) - 1))) {
+        return false;
+    }
+    const uint64_t p0 = lay_addr(uint64_t{1} << first, L);
     for (int b = 1; b < count; ++b) {
-        if (lay_addr(uint64_t{1} << (first + b), h) != (p0 << b)) {
+        if (lay_addr(uint64_t{1} << (first + b), L) != (p0 << b)) {
             return false;
         }
     }

@@ -440,8 +440,9 @@ def test_generated_start(oracle, backend_module, n, shape):
 
 
 @pytest.mark.parametrize("n,r_low,r_high,r_l0", [(13, 4, 4, 4), (16, 3, 4, 3), (21, 4, 3, 4),
-                                                  (21, 3, 4, 3), (25, 4, 4, 4), (28, 3, 3, 3),
-                                                  (30, 4, 4, 4), (30, 3, 4, 3)])
+                                                  (21, 3, 4, 3), (22, 4, 4, 4), (25, 4, 4, 4),
+                                                  (28, 3, 3, 3), (30, 4, 4, 4), (30, 3, 4, 3),
+                                                  (31, 4, 4, 4)])
 def test_lambda_layouts(oracle, backend_module, n, r_low, r_high, r_l0):
     """Generated-start evaluations with lambda stored between the passes in
     the layouts that make each high pass's tiles contiguous blocks: only the
@@ -470,10 +471,11 @@ def test_lambda_layouts(oracle, backend_module, n, r_low, r_high, r_l0):
 
 
 def test_lambda_layouts_scope(backend_module):
-    # 22 and 31 qubits: a group's register bits straddle the layout below it,
-    # so lambda stays in the natural order; without the generated start the
+    # 19 and 20 qubits: the single high group [n - 9, n) holds some of the low
+    # tile's register bits 8..11, which the low pass's layout then splits, so
+    # lambda stays in the natural order; without the generated start the
     # layouts are not used
-    for n in (22, 31):
+    for n in (19, 20):
         eng = backend_module.RingIsingCudaBackend(n, 2, 1.0, "structured_adjoint", 8, False)
         assert eng.generated_start and not eng.lambda_layouts
         del eng

@@ -2,15 +2,16 @@
 
 Between the passes of a generated-start evaluation lambda sits in the order
 that makes the next high pass's tile one contiguous 2^12 block.  These tests
-restate the layout (``lay_addr`` / ``lay_fits`` of
+restate the layout (``make_lam_layout`` / ``lay_addr`` / ``lay_fits`` of
 ``gpu_simulation_amd/csrc/rie_kernels.hpp``) and the engine's pass plan and
 check what the kernels rely on: each layout is a permutation of the index
 bits below n; under its own layout a high pass's tile is an aligned block
 whose offsets appear in order; the register bits of every pass's load mapping
 stay contiguous in the layouts it reads and writes, wherever the engine turns
-the layouts on (30 qubits among them); and the chain of layouts hands every
-amplitude from the pass that writes it to the pass that reads it.  The
-kernels themselves are compared with and without layouts on the GPU.
+the layouts on (all sizes but 18..20 qubits); each writer's tile stays within
+2^21 amplitudes; and the chain of layouts hands every amplitude from the pass
+that writes it to the pass that reads it.  The kernels themselves are compared
+with and without layouts on the GPU.
 """
 
 from __future__ import annotations
@@ -21,23 +22,48 @@ import pytest
 T = 12
 
 
-def lay_addr(i, h):
-    if h == 0:
+def make_lam_layout(n, h, w):
+    """make_lam_layout: address bits 0..11 hold the reader's tile bits
+    ({0, 1, 2} and [h, h + 9)), the writer's tile bits outside it ({0, 1, 2}
+    and [w, w + 9), w = 3 for the low tile) come next, then every other bit
+    below n, each group ascending.  Returned as segments (src, len, dst);
+    [] is the natural order."""
+    order = []
+    for b in list(range(3)) + list(range(h, h + 9)) + list(range(w, w + 9)) + list(range(3, n)):
+        if 0 <= b < n and b not in order:
+            order.append(b)
+    segs = []
+    for d, b in enumerate(order):
+        if segs and segs[-1][0] + segs[-1][1] == b and segs[-1][2] + segs[-1][1] == d:
+            segs[-1][1] += 1
+        else:
+            segs.append([b, 1, d])
+    if len(segs) > 8:
+        return None
+    return [] if len(segs) == 1 else [tuple(x) for x in segs]
+
+
+def lay_addr(i, L):
+    r = 0
+    if not L:
         return i
-    low = i & 7
-    blk = (i >> h) & 511
-    mid = (i >> 3) & ((1 << (h - 3)) - 1)
-    top = (i >> (h + 9)) << (h + 9)
-    return low | (blk << 3) | (mid << 12) | top
+    for src, ln, dst in L:
+        r |= ((i >> src) & ((1 << ln) - 1)) << dst
+    return r
 
 
-def lay_fits(h, n, first, count):
-    if h == 0:
-        return True
-    if h < 3 or h + 9 > n:
+def lay_fits(L, n, first, count):
+    if L is None:
         return False
-    p0 = lay_addr(1 << first, h)
-    return all(lay_addr(1 << (first + b), h) == p0 << b for b in range(1, count))
+    if not L:
+        return True
+    image = 0
+    for b in range(n):
+        image |= lay_addr(1 << b, L)
+    if image != (1 << n) - 1:
+        return False
+    p0 = lay_addr(1 << first, L)
+    return all(lay_addr(1 << (first + b), L) == p0 << b for b in range(1, count))
 
 
 def high_groups(n):
@@ -53,17 +79,29 @@ def high_groups(n):
     return out
 
 
-def layouts_fit(n, r_low=4, r_high=4):
-    """Engine::lam_layouts_fit: the low pass writes the layout of the last
-    high pass, high pass j reads its own and writes high pass j-1's."""
+def layout_in(n, j):
+    """Engine::lam_layout_in: high passes run j = K-1 .. 0; pass j's input was
+    written by pass j + 1 (the low pass for j = K - 1)."""
     h0s = high_groups(n)
     K = len(h0s)
-    if not lay_fits(h0s[K - 1], n, T - r_low, r_low):
+    w = 3 if j == K - 1 else h0s[j + 1]
+    return make_lam_layout(n, h0s[j], w)
+
+
+def layout_out(n, j):
+    return layout_in(n, j - 1) if j > 0 else []
+
+
+def layouts_fit(n, r_low=4, r_high=4):
+    """Engine::lam_layouts_fit."""
+    h0s = high_groups(n)
+    K = len(h0s)
+    if not lay_fits(layout_in(n, K - 1), n, T - r_low, r_low):
         return False
     for j in range(K):
         first = h0s[j] + (T - r_high - 3)
-        out = h0s[j - 1] if j > 0 else 0
-        if not (lay_fits(h0s[j], n, first, r_high) and lay_fits(out, n, first, r_high)):
+        if not (lay_fits(layout_in(n, j), n, first, r_high)
+                and lay_fits(layout_out(n, j), n, first, r_high)):
             return False
     return True
 
@@ -77,48 +115,81 @@ def tile_index(base, o, a, h0):
     return base | (o & ((1 << a) - 1)) | ((o >> a) << h0)
 
 
-@pytest.mark.parametrize("n,h", [(13, 4), (16, 7), (21, 12), (30, 12), (30, 21), (31, 22)])
-def test_layout_is_a_bit_permutation(n, h):
-    # every index bit below n lands on a distinct address bit below n
-    images = [lay_addr(1 << b, h) for b in range(n)]
-    assert all(v and v & (v - 1) == 0 for v in images)
-    assert sorted(images) == [1 << b for b in range(n)]
-    # and the map is linear over disjoint bits
-    rng = np.random.default_rng(n + h)
-    for x in rng.integers(0, 1 << n, 50):
-        x = int(x)
-        want = 0
-        for b in range(n):
-            if (x >> b) & 1:
-                want |= images[b]
-        assert lay_addr(x, h) == want
+def boundaries(n):
+    """(reader h0, writer tile first bit, layout) of every layout the chain uses."""
+    h0s = high_groups(n)
+    K = len(h0s)
+    out = []
+    for j in range(K):
+        w = 3 if j == K - 1 else h0s[j + 1]
+        out.append((h0s[j], w, layout_in(n, j)))
+    return out
 
 
-@pytest.mark.parametrize("n", [13, 16, 21, 25, 30])
+@pytest.mark.parametrize("n", list(range(13, 35)))
+def test_layout_is_a_bit_permutation(n):
+    # every index bit below n lands on a distinct address bit below n, and the
+    # map is linear over disjoint bits
+    rng = np.random.default_rng(n)
+    for h, w, L in boundaries(n):
+        images = [lay_addr(1 << b, L) for b in range(n)]
+        assert all(v and v & (v - 1) == 0 for v in images)
+        assert sorted(images) == [1 << b for b in range(n)]
+        for x in rng.integers(0, 1 << n, 50):
+            x = int(x)
+            want = 0
+            for b in range(n):
+                if (x >> b) & 1:
+                    want |= images[b]
+            assert lay_addr(x, L) == want
+
+
+@pytest.mark.parametrize("n", [13, 16, 21, 25, 30, 31])
 def test_own_layout_makes_each_high_tile_one_block(n):
     rng = np.random.default_rng(n)
-    for h0 in high_groups(n):
-        tiles = 1 << (n - T)
+    tiles = 1 << (n - T)
+    for h0, w, L in boundaries(n):
         for g in [0, tiles - 1] + [int(x) for x in rng.integers(0, tiles, 6)]:
             base = tile_base(g, 3, h0)
-            start = lay_addr(base, h0)
+            start = lay_addr(base, L)
             assert start % (1 << T) == 0
-            offs = [lay_addr(tile_index(base, o, 3, h0), h0) - start for o in range(1 << T)]
+            offs = [lay_addr(tile_index(base, o, 3, h0), L) - start for o in range(1 << T)]
             assert offs == list(range(1 << T))
 
 
-def test_layouts_are_on_at_the_headline_geometry():
+@pytest.mark.parametrize("n", list(range(13, 35)))
+def test_writer_tiles_stay_within_2_21_amplitudes(n):
+    # the writer of each boundary stores its tile within a 2^21-amplitude span
+    # (32 MB: 16 pages of 2 MB), the point of placing its bits right above the
+    # reader's block
+    rng = np.random.default_rng(n + 100)
+    tiles = 1 << (n - T)
+    for h0, w, L in boundaries(n):
+        for g in [0, tiles - 1] + [int(x) for x in rng.integers(0, tiles, 4)]:
+            if w == 3:
+                idx = [(g << T) | o for o in range(1 << T)]
+            else:
+                base = tile_base(g, 3, w)
+                idx = [tile_index(base, o, 3, w) for o in range(1 << T)]
+            addr = [lay_addr(i, L) for i in idx]
+            assert max(addr) - min(addr) < (1 << 21)
+
+
+def test_layouts_are_on_at_the_planned_sizes():
     assert high_groups(30) == [12, 21]
-    assert layouts_fit(30)
-    assert layouts_fit(30, r_low=3, r_high=3)
-    # where the last group's register bits straddle the layout below it
-    # (31 qubits: bits 27..30 against layout 19's edge at bit 28) the engine
-    # keeps the natural order
     assert high_groups(31) == [12, 19, 22]
-    assert not layouts_fit(31)
-    # the qubit counts with layouts (both register shapes of the high passes)
+    # the writer's bits sit right above the reader's block, so a high pass's
+    # register bits stay contiguous in the layouts it reads and writes; only
+    # where the single high group overlaps the low tile's register bits (18..20
+    # qubits: group [n - 9, n) against bits 8..11) does the engine keep the
+    # natural order
     assert [n for n in range(13, 35) if layouts_fit(n)] == [
-        13, 14, 15, 16, 17, 21, 25, 26, 27, 28, 29, 30, 32, 33, 34]
+        13, 14, 15, 16, 17, 21, 22, 23, 24, 25, 26, 27, 28, 29, 30, 31, 32, 33, 34]
+    assert [n for n in range(13, 35) if layouts_fit(n, r_low=3, r_high=3)] == [
+        13, 14, 15, 16, 17, 18, 21, 22, 23, 24, 25, 26, 27, 28, 29, 30, 31, 32, 33, 34]
+    # 30 qubits: the top group [21, 30) writes for the group [12, 21): its own
+    # bits go to address bits 12..20, the low tile's bits 3..11 above them
+    assert layout_out(30, 1) == [(0, 3, 0), (12, 18, 3), (3, 9, 21)]
 
 
 @pytest.mark.parametrize("n", list(range(13, 35)))
@@ -130,33 +201,34 @@ def test_register_bits_stay_contiguous_where_layouts_fit(n):
     h0s = high_groups(n)
     K = len(h0s)
 
-    def contiguous(h, bits):
-        pos = [lay_addr(1 << b, h).bit_length() - 1 for b in bits]
+    def contiguous(L, bits):
+        pos = [lay_addr(1 << b, L).bit_length() - 1 for b in bits]
         return pos == list(range(pos[0], pos[0] + len(bits)))
 
-    assert contiguous(h0s[K - 1], range(T - 4, T))
+    assert contiguous(layout_in(n, K - 1), range(T - 4, T))
     for j in range(K):
         first = h0s[j] + (T - 4 - 3)
-        assert contiguous(h0s[j], range(first, first + 4))
-        assert contiguous(h0s[j - 1] if j > 0 else 0, range(first, first + 4))
+        assert contiguous(layout_in(n, j), range(first, first + 4))
+        assert contiguous(layout_out(n, j), range(first, first + 4))
 
 
-def _lay_vec(i, h):
-    if h == 0:
+def _lay_vec(i, L):
+    if not L:
         return i
-    return ((i & 7) | (((i >> h) & 511) << 3) | (((i >> 3) & ((1 << (h - 3)) - 1)) << 12)
-            | ((i >> (h + 9)) << (h + 9)))
+    r = np.zeros_like(i)
+    for src, ln, dst in L:
+        r |= ((i >> src) & ((1 << ln) - 1)) << dst
+    return r
 
 
-@pytest.mark.parametrize("n", [16, 21, 25, 30])
+@pytest.mark.parametrize("n", [16, 21, 25, 30, 31])
 def test_kernel_address_arithmetic_lands_on_the_layout(n):
     # The kernels form an address as lay(base) + lay(thread offset) + k *
     # lay(register step) (lay_tile_addr, store_lam); reader and writer of a
     # boundary agree when that equals lay(index) for every (tile, thread,
     # register).  Low pass: offset t | k << 8 (mapping 0 of the low tile);
     # high pass with first high bit h0: offset (t & 7) | (t >> 3) << h0,
-    # register step 2^(h0 + 5).  All tiles for n <= 22, a sample at 30.
-    assert layouts_fit(n)
+    # register step 2^(h0 + 5).  All tiles for n <= 21, a sample above.
     h0s = high_groups(n)
     K = len(h0s)
     tiles = 1 << (n - T)
@@ -169,19 +241,19 @@ def test_kernel_address_arithmetic_lands_on_the_layout(n):
     k = np.arange(16, dtype=np.int64)
     G, Tt, Kk = np.meshgrid(gs, t, k, indexing="ij")
 
-    def check(base, off, step, h):
+    def check(base, off, step, L):
         index = base + off + Kk * step
-        addr = _lay_vec(base, h) + _lay_vec(off, h) + Kk * _lay_vec(np.int64(step), h)
-        assert np.array_equal(addr, _lay_vec(index, h))
+        addr = _lay_vec(base, L) + _lay_vec(off, L) + Kk * _lay_vec(np.int64(step), L)
+        assert np.array_equal(addr, _lay_vec(index, L))
         if len(gs) == tiles:  # the pass covers every index once
             assert np.array_equal(np.sort(index.ravel()), np.arange(1 << n))
             assert np.array_equal(np.sort(addr.ravel()), np.arange(1 << n))
 
-    # low pass store into the last high pass's layout
-    check(G << T, Tt, 1 << 8, h0s[K - 1])
+    # low pass store into the layout of the first high pass to run
+    check(G << T, Tt, 1 << 8, layout_in(n, K - 1))
     for j in range(K):
         h0 = h0s[j]
         base = ((G & ((1 << (h0 - 3)) - 1)) << 3) | ((G >> (h0 - 3)) << (h0 + 9))
         off = (Tt & 7) | ((Tt >> 3) << h0)
-        check(base, off, 1 << (h0 + 5), h0)                        # its read
-        check(base, off, 1 << (h0 + 5), h0s[j - 1] if j > 0 else 0)  # its write
+        check(base, off, 1 << (h0 + 5), layout_in(n, j))    # its read
+        check(base, off, 1 << (h0 + 5), layout_out(n, j))   # its write
