@@ -241,6 +241,12 @@ class RingIsingCudaBackend {
 
     bool lambda_layouts() const { return rie_engine_lambda_layouts(engine_) == 1; }
 
+    void set_layer0_in_pass(int enabled) {
+        check(rie_engine_set_layer0_in_pass(engine_, enabled));
+    }
+
+    bool layer0_in_pass() const { return rie_engine_layer0_in_pass(engine_) == 1; }
+
     void set_kernel_timing(bool enable) { check(rie_engine_set_kernel_timing(engine_, enable)); }
 
     void reset_kernel_stats() { check(rie_engine_reset_kernel_stats(engine_)); }
@@ -585,6 +591,8 @@ PYBIND11_MODULE(_cuda_backend, m) {
         .def_property_readonly("double_buffer", &RingIsingCudaBackend::double_buffer)
         .def("set_lambda_layouts", &RingIsingCudaBackend::set_lambda_layouts, py::arg("enabled"))
         .def_property_readonly("lambda_layouts", &RingIsingCudaBackend::lambda_layouts)
+        .def("set_layer0_in_pass", &RingIsingCudaBackend::set_layer0_in_pass, py::arg("enabled"))
+        .def_property_readonly("layer0_in_pass", &RingIsingCudaBackend::layer0_in_pass)
         .def_property_readonly("schedule", &RingIsingCudaBackend::schedule)
         .def("set_kernel_timing", &RingIsingCudaBackend::set_kernel_timing, py::arg("enable"))
         .def("reset_kernel_stats", &RingIsingCudaBackend::reset_kernel_stats)

@@ -225,13 +225,19 @@ Engine::Engine(size_t num_qubits, size_t num_layers, double field, const std::st
         for (const auto &p : passes) {
             eparts += (size_t)p.grid;
         }
-        // the layer-0 contraction writes 48 values per workgroup of the low pass
-        partial_count = std::max({part, eparts, (size_t)passes[0].grid * 48});
+        // the layer-0 contraction writes 48 values per workgroup of the low pass;
+        // inside the top group's pass, 2T + kTopL0Cols per workgroup of that pass
+        partial_count = std::max({part, eparts, (size_t)passes[0].grid * 48,
+                                  (size_t)passes.back().grid * (2 * T + kTopL0Cols)});
         d_partials = (double *)device_alloc(sizeof(double) * partial_count, "alloc partials");
         d_epartials = (double *)device_alloc(sizeof(double) * eparts, "alloc energy partials");
         epartial_count = eparts;
         const uint64_t tiles = passes[0].g.num_tiles;
-        d_tile_sums = (double2 *)device_alloc(sizeof(double2) * tiles, "alloc tile sums");
+        // per-tile sums: one per tile for the low-tile contraction; for the
+        // contraction inside the top group's pass one per wave of a tile plus
+        // the tile's fixed-wire factor
+        tile_sum_count = L == 2 ? tiles * (Tile4Waves + 1) : tiles;
+        d_tile_sums = (double2 *)device_alloc(sizeof(double2) * tile_sum_count, "alloc tile sums");
         d_chain = (CMat2 *)device_alloc(sizeof(CMat2) * kChainMaxSets * kChainTabEntries,
                                         "alloc chain tables");
         if (L >= 2 && n <= kChainMaxQubits && passes.size() <= (size_t)kChainMaxSets) {
@@ -630,29 +636,34 @@ void Engine::launch_bwd(int l, bool gather, bool write_cur, const double2 *cur_i
                     pairs ? h : 0.0, gen_tab, gen_utab, prefetch);
 }
 
-// The generated start's lambda layouts: the low pass writes the layout that
-// high pass K-1 (the first to run) reads, high pass j reads its own and writes
-// the one high pass j-1 reads, high pass 0 writes the natural order.  Each
-// layout puts the reader's tile in one block and the writer's tile bits right
-// above it (make_lam_layout).
-LamLayout Engine::lam_layout_in(int j) const {
+// The generated start's lambda layouts.  The high passes run either from the
+// lowest group up (bottom_up: then the last one holds the top group's tiles and
+// can contract layer 0 itself) or from the top group down.  The low pass
+// writes the layout of the first high pass, each high pass reads its own and
+// writes the next one's, the last writes the natural order for the layer-0
+// contraction (or nothing).  Each layout puts the reader's tile in one block
+// and the writer's tile bits right above it (make_lam_layout).
+LamLayout Engine::lam_layout_in(int j, bool bottom_up) const {
     const int K = (int)passes.size() - 1;
-    const int w = j == K - 1 ? 3 : passes[2 + (size_t)j].g.h0;
+    const int prev = bottom_up ? j - 1 : j + 1;  // the pass that runs before j
+    const int w = (prev < 0 || prev >= K) ? 3 : passes[1 + (size_t)prev].g.h0;
     return make_lam_layout(n, passes[1 + (size_t)j].g.h0, w);
 }
 
-LamLayout Engine::lam_layout_out(int j) const {
-    return j > 0 ? lam_layout_in(j - 1) : LamLayout{};
+LamLayout Engine::lam_layout_out(int j, bool bottom_up) const {
+    const int K = (int)passes.size() - 1;
+    const int next = bottom_up ? j + 1 : j - 1;
+    return (next >= 0 && next < K) ? lam_layout_in(next, bottom_up) : LamLayout{};
 }
 
-bool Engine::lam_layouts_fit() const {
+bool Engine::lam_layouts_fit(bool bottom_up) const {
     const int K = (int)passes.size() - 1;
     if (K < 1) {
         return false;
     }
     const int r_low = pass_r[RIE_PASS_BWD_LOW];
     const int r_high = pass_r[RIE_PASS_BWD_HIGH];
-    if (!lay_fits(lam_layout_in(K - 1), n, T - r_low, r_low)) {
+    if (!lay_fits(lam_layout_in(bottom_up ? 0 : K - 1, bottom_up), n, T - r_low, r_low)) {
         return false;
     }
     for (int j = 0; j < K; ++j) {
@@ -661,12 +672,32 @@ bool Engine::lam_layouts_fit() const {
             return false;
         }
         const int first = g.h0 + (T - r_high - g.a);
-        if (!lay_fits(lam_layout_in(j), n, first, r_high) ||
-            !lay_fits(lam_layout_out(j), n, first, r_high)) {
+        if (!lay_fits(lam_layout_in(j, bottom_up), n, first, r_high) ||
+            !lay_fits(lam_layout_out(j, bottom_up), n, first, r_high)) {
             return false;
         }
     }
     return true;
+}
+
+// The generated start runs its high passes from the lowest group up (and
+// contracts layer 0 in the last) where that is possible and does not cost the
+// layouts that the other order would have (at 27-29 and 31-34 qubits the top
+// group's tile overlaps the one below it, and bottom-up the lower pass's
+// register bits would straddle the layout it writes); otherwise from the top
+// group down with the separate contraction.
+bool Engine::gen_bottom_up() const {
+    if (!l0_in_pass_ready()) {
+        return false;
+    }
+    if (!lam_layouts_enabled || pool.size() < 2) {
+        return true;
+    }
+    return lam_layouts_fit(true) || !lam_layouts_fit(false);
+}
+
+bool Engine::gen_layouts() const {
+    return lam_layouts_enabled && pool.size() >= 2 && lam_layouts_fit(gen_bottom_up());
 }
 
 bool Engine::gen_ready() const {
@@ -675,9 +706,12 @@ bool Engine::gen_ready() const {
 }
 
 // Table sets of the current register shapes: 0 the forward (low tile, all of
-// layer 1 rotated, run at mapping 0), 1 + j the adjoint high pass j (the
-// wires of high groups 0 .. j rotated, run at mapping 0's registers).
-void Engine::build_chain_tables(hipStream_t s, int r_set0) {
+// layer 1 rotated, run at mapping 0), 1 + j the adjoint high pass j, run at
+// mapping 0's registers, with the wires still rotated when it runs: high
+// groups 0 .. j when the high passes run from the top group down (the stored
+// schedule and the walk), groups j .. K-1 when they run from the lowest group
+// up (the generated start, bottom_up).
+void Engine::build_chain_tables(hipStream_t s, int r_set0, bool bottom_up) {
     ChainSetup cs{};
     cs.n = n;
     cs.tiles = passes[0].g.num_tiles;
@@ -712,7 +746,9 @@ void Engine::build_chain_tables(hipStream_t s, int r_set0) {
         ChainSet st{};
         st.r0 = g.h0 + (T - rh - g.a);
         st.R = rh;
-        st.rot = all & ~((uint64_t{1} << T) - 1) & ((uint64_t{1} << passes[p].end_wire) - 1);
+        st.rot = bottom_up ? all & ~((uint64_t{1} << passes[p].first_wire) - 1)
+                           : all & ~((uint64_t{1} << T) - 1) &
+                                 ((uint64_t{1} << passes[p].end_wire) - 1);
         st.c0_first = 0;
         st.c0_len = g.a;
         st.c1_first = g.h0;
@@ -746,9 +782,24 @@ bool Engine::gen_start_ready() const {
            d_chain_x != nullptr;
 }
 
+// Whether the generated start contracts layer 0 inside its last high pass
+// (launch_bwd_top_l0): register shape 4 for the high passes, the top group's
+// tiles whole in the layout that pass reads.
+bool Engine::l0_in_pass_ready() const {
+    const int K = (int)passes.size() - 1;
+    if (!l0_in_pass_enabled || K < 1 || pass_r[RIE_PASS_BWD_HIGH] != 4) {
+        return false;
+    }
+    const PassGeom &g = passes[(size_t)K].g;
+    return g.a == 3 && g.h0 + 9 == n && g.h0 >= 4 && g.rot_lo >= 3;
+}
+
 // Gradient evaluation of a two-layer circuit without forward or Hamiltonian
 // passes (see gen_start_enabled): 1 S (generating low pass) + 2 S per high
-// pass + 1 S (layer-0 contraction).
+// pass but the last + 1 S for the last, which contracts layer 0 itself when
+// the high passes run from the lowest group up (gen_bottom_up: the last one
+// holds the top group's tiles); otherwise 2 S for it and the 1 S layer-0
+// contraction.
 void Engine::structured_eval_gen_start(hipStream_t s, const double *scale, double *energy_out,
                                        bool staged) {
     const double S = (double)sizeof(double2) * (double)N;
@@ -756,7 +807,8 @@ void Engine::structured_eval_gen_start(hipStream_t s, const double *scale, doubl
     if (staged) {
         stage_begin(s);
     }
-    build_chain_tables(s, r_low);
+    const bool bottom_up = gen_bottom_up();
+    build_chain_tables(s, r_low, bottom_up);
     gen_fwd = false;
     final_ptr = nullptr;  // no state is written
     if (staged) {
@@ -775,11 +827,12 @@ void Engine::structured_eval_gen_start(hipStream_t s, const double *scale, doubl
     // lambda layouts: high pass j reads its tiles as contiguous blocks.  A
     // tile's write addresses then differ from its read addresses, so the
     // high passes alternate between two buffers instead of working in place.
-    const bool layouts = lam_layouts_enabled && lam_layouts_fit() && pool.size() >= 2;
+    const bool layouts = gen_layouts();
     double2 *lam_other = layouts ? pool[1] : lam;
     if (layouts) {
-        g0.lay_out = lam_layout_in(K - 1);
+        g0.lay_out = lam_layout_in(bottom_up ? 0 : K - 1, bottom_up);
     }
+    const bool l0_in_pass = bottom_up;
     // every X term is in lambda once the low pass is done: the high passes
     // add no pair terms (fuse_layer stays -1)
     fuse_layer = -1;
@@ -795,15 +848,42 @@ void Engine::structured_eval_gen_start(hipStream_t s, const double *scale, doubl
         launch_reduce_grad(d_partials, passes[0].grid, T, passes[0].g, 1, d_grad, scale, s);
         check_launch("reduce_grad");
         kt_end(s, "reduce_grad", 0.0);
-        for (int j = K - 1; j >= 0; --j) {
+        for (int step = 0; step < K; ++step) {
+            const int j = bottom_up ? step : K - 1 - step;
             const PassPlan &pp = passes[1 + (size_t)j];
-            require_partials((size_t)pp.grid * 2 * T);
-            kt_begin(s);
             PassGeom gj = pp.g;
             if (layouts) {
-                gj.lay_in = lam_layout_in(j);
-                gj.lay_out = lam_layout_out(j);
+                gj.lay_in = lam_layout_in(j, bottom_up);
+                gj.lay_out = lam_layout_out(j, bottom_up);
             }
+            if (j == K - 1 && l0_in_pass) {
+                const uint64_t tiles = pp.g.num_tiles;
+                require_partials((size_t)pp.grid * (2 * T + kTopL0Cols));
+                if ((size_t)high_env_chunks * 4 * (size_t)std::max(1, n - T) > partial2_count ||
+                    tile_sum_count < tiles * (Tile4Waves + 1)) {
+                    fail(RIE_DEVICE_ERROR, "internal: layer-0 partial buffer too small");
+                }
+                double *l0_part = d_partials + (size_t)pp.grid * 2 * T;
+                double2 *phi_c = d_tile_sums + tiles * Tile4Waves;
+                kt_begin(s);
+                launch_bwd_top_l0(lam, d_coef + n, gj, d_partials, pp.grid, s,
+                                  d_chain + (size_t)(1 + j) * kChainTabEntries,
+                                  d_chain_u + (size_t)(1 + j) * 2 * passes[0].g.num_tiles, d_coef,
+                                  phi_c, d_tile_sums, l0_part);
+                check_launch("bwd_pass high gen layer0");
+                kt_end(s, "bwd_pass_high_gen_layer0", S);
+                kt_begin(s);
+                launch_reduce_grad(d_partials, pp.grid, T, pp.g, 1, d_grad, scale, s);
+                check_launch("reduce_grad");
+                launch_layer0_top_finish(d_tile_sums, tiles, l0_part, pp.grid, d_coef, n,
+                                         d_partials2, high_env_chunks, d_grad, scale, s,
+                                         l0_energy_out);
+                check_launch("layer0_top_finish");
+                kt_end(s, "layer0_reduce", 0.0);
+                break;
+            }
+            require_partials((size_t)pp.grid * 2 * T);
+            kt_begin(s);
             launch_bwd(1, false, false, nullptr, lam, nullptr, lam_other, (size_t)(1 + j), s, 1 + j,
                        &gj);
             check_launch("bwd_pass");
@@ -814,7 +894,9 @@ void Engine::structured_eval_gen_start(hipStream_t s, const double *scale, doubl
             check_launch("reduce_grad");
             kt_end(s, "reduce_grad", 0.0);
         }
-        layer0_gradients(s, lam, scale);
+        if (!l0_in_pass) {
+            layer0_gradients(s, lam, scale);
+        }
     } catch (...) {
         fuse_layer = -1;
         l0_energy_out = nullptr;
@@ -1938,15 +2020,30 @@ rie_status rie_engine_set_lambda_layouts(rie_engine *engine, int enabled) {
     });
 }
 
+rie_status rie_engine_set_layer0_in_pass(rie_engine *engine, int enabled) {
+    return guard([&]() {
+        Engine *e = as_engine(engine);
+        if (enabled != 0 && enabled != 1) {
+            rie::fail(RIE_INVALID_ARGUMENT, "enabled must be 0 or 1.");
+        }
+        e->l0_in_pass_enabled = enabled == 1;
+    });
+}
+
+int rie_engine_layer0_in_pass(const rie_engine *engine) {
+    if (engine == nullptr) {
+        return -1;
+    }
+    const Engine *e = reinterpret_cast<const Engine *>(engine);
+    return e->gen_start_ready() && e->gen_bottom_up() ? 1 : 0;
+}
+
 int rie_engine_lambda_layouts(const rie_engine *engine) {
     if (engine == nullptr) {
         return -1;
     }
     const Engine *e = reinterpret_cast<const Engine *>(engine);
-    return e->lam_layouts_enabled && e->gen_start_ready() && e->lam_layouts_fit() &&
-                   e->pool.size() >= 2
-               ? 1
-               : 0;
+    return e->gen_start_ready() && e->gen_layouts() ? 1 : 0;
 }
 
 int rie_engine_double_buffer(const rie_engine *engine) {

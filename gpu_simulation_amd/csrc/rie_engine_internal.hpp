@@ -115,9 +115,13 @@ struct Engine {
     // Lambda layouts of the generated start: each high pass reads its tile
     // as one contiguous block (the pass before it writes that layout).
     bool lam_layouts_enabled = true;  // caller's switch
-    bool lam_layouts_fit() const;
-    LamLayout lam_layout_in(int j) const;   // read by high pass j
-    LamLayout lam_layout_out(int j) const;  // written by high pass j
+    bool l0_in_pass_enabled = true;   // caller's switch: layer 0 inside the last high pass
+    bool l0_in_pass_ready() const;
+    bool lam_layouts_fit(bool bottom_up) const;
+    LamLayout lam_layout_in(int j, bool bottom_up) const;   // read by high pass j
+    LamLayout lam_layout_out(int j, bool bottom_up) const;  // written by high pass j
+    bool gen_bottom_up() const;  // generated start: high passes from the lowest group up
+    bool gen_layouts() const;    // generated start: lambda layouts in use
     CMat2 *d_chain = nullptr;   // kChainMaxSets * kChainTabEntries matrices
     CMat2 *d_chain_u = nullptr; // per set and tile: the uniform segments U_A, U_B
 
@@ -131,7 +135,8 @@ struct Engine {
     double *d_partials = nullptr;
     double *d_epartials = nullptr;
     double *d_drift = nullptr;
-    double2 *d_tile_sums = nullptr;  // layer-0 contraction: one value per low tile
+    double2 *d_tile_sums = nullptr;  // layer-0 contraction: per-tile sums (tile_sum_count)
+    uint64_t tile_sum_count = 0;
     double *d_partials2 = nullptr;   // layer-0 contraction: high-wire partials
     int high_env_chunks = 1;
     size_t partial_count = 0;    // capacity of d_partials (doubles)
@@ -204,7 +209,7 @@ struct Engine {
                     hipStream_t s, int gen_set = -1, const PassGeom *geom = nullptr);
     bool gen_ready() const;
     bool gen_start_ready() const;
-    void build_chain_tables(hipStream_t s, int r_set0 = -1);
+    void build_chain_tables(hipStream_t s, int r_set0 = -1, bool bottom_up = false);
     void structured_eval_gen_start(hipStream_t s, const double *scale, double *energy_out,
                                    bool staged);
     void structured_backward(hipStream_t s, const double *scale, bool full_walk);

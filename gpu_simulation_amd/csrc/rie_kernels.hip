@@ -2083,6 +2083,56 @@ __device__ __forceinline__ double2 u_of(const WireCoef &k, int bit) {
     return bit ? u1_of(k) : u0_of(k);
 }
 
+// Contraction of one thread's 16 registers m[k] (k = k0 + 2 k1 + 4 k2 + 8 k3)
+// against conj(m) and the product state uf[j][k_j] of the four register wires:
+// pk = sum_k conj(m_k) prod_j uf[j][k_j], and env[e][b] the same sum over
+// k_e = b without wire e's factor.  Hierarchical: A contracts k0, AB k0..k1,
+// ABC k0..k2; E, F, G keep one bit uncontracted for the environments.
+__device__ __forceinline__ void contract_reg4(const double2 (&m)[16], const double2 (&uf)[4][2],
+                                              double2 &pk, double2 (&env)[4][2]) {
+    double2 A[8], F[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {  // r = k1 + 2 k2 + 4 k3  /  k0 + 2 k2 + 4 k3
+        A[r] = cmul_ca(m[2 * r], uf[0][0]);
+        cmac_ca(A[r], m[2 * r + 1], uf[0][1]);
+        const int k0 = r & 1;
+        const int hi = r >> 1;  // k2 + 2 k3
+        F[r] = cmul_ca(m[k0 + 4 * hi], uf[1][0]);
+        cmac_ca(F[r], m[k0 + 2 + 4 * hi], uf[1][1]);
+    }
+    double2 AB[4], E[4], G[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        // AB[k2 + 2 k3]
+        AB[r] = cmul(A[2 * r], uf[1][0]);
+        cmac(AB[r], A[2 * r + 1], uf[1][1]);
+        // E[k1 + 2 k3] = contract k2 of A[k1, k2, k3]
+        const int k1 = r & 1;
+        const int k3 = r >> 1;
+        E[r] = cmul(A[k1 + 4 * k3], uf[2][0]);
+        cmac(E[r], A[k1 + 2 + 4 * k3], uf[2][1]);
+        // G[k0 + 2 k3] = contract k2 of F[k0, k2, k3]
+        G[r] = cmul(F[k1 + 4 * k3], uf[2][0]);
+        cmac(G[r], F[k1 + 2 + 4 * k3], uf[2][1]);
+    }
+    double2 ABC[2];
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+        ABC[b] = cmul(AB[2 * b], uf[2][0]);          // contract k2 of AB[k2, k3=b]
+        cmac(ABC[b], AB[2 * b + 1], uf[2][1]);
+        env[3][b] = ABC[b];
+        env[2][b] = cmul(AB[b], uf[3][0]);           // contract k3 of AB[k2=b, k3]
+        cmac(env[2][b], AB[b + 2], uf[3][1]);
+        env[1][b] = cmul(E[b], uf[3][0]);            // contract k3 of E[k1=b, k3]
+        cmac(env[1][b], E[b + 2], uf[3][1]);
+        env[0][b] = cmul(G[b], uf[3][0]);            // contract k3 of G[k0=b, k3]
+        cmac(env[0][b], G[b + 2], uf[3][1]);
+    }
+    pk = cmul(ABC[0], uf[3][0]);
+    cmac(pk, ABC[1], uf[3][1]);
+}
+
+
 // R = 4 (T = 12) low tiles; 256 threads, thread t holds offsets t | k << 8.
 __global__ __launch_bounds__(256) void layer0_contract_kernel(
     const double2 *__restrict__ lam, const WireCoef *__restrict__ lc0, PassGeom G,
@@ -2149,49 +2199,9 @@ __global__ __launch_bounds__(256) void layer0_contract_kernel(
         for (int v = T; v < G.n; ++v) {
             bg = cmul(bg, u_of(lc0[v], (int)((base >> v) & 1U)));
         }
-        // contract the register bits k0..k3 (wires 8..11) hierarchically
-        // against conj(mu): A contracts k0, AB k0..k1, ABC k0..k2; the
-        // environments keep one bit uncontracted.
-        double2 A[8], F[8];
-#pragma unroll
-        for (int r = 0; r < 8; ++r) {  // r = k1 + 2 k2 + 4 k3  /  k0 + 2 k2 + 4 k3
-            A[r] = cmul_ca(m[2 * r], uf[0][0]);
-            cmac_ca(A[r], m[2 * r + 1], uf[0][1]);
-            const int k0 = r & 1;
-            const int hi = r >> 1;  // k2 + 2 k3
-            F[r] = cmul_ca(m[k0 + 4 * hi], uf[1][0]);
-            cmac_ca(F[r], m[k0 + 2 + 4 * hi], uf[1][1]);
-        }
-        double2 AB[4], E[4], G[4];
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            // AB[k2 + 2 k3]
-            AB[r] = cmul(A[2 * r], uf[1][0]);
-            cmac(AB[r], A[2 * r + 1], uf[1][1]);
-            // E[k1 + 2 k3] = contract k2 of A[k1, k2, k3]
-            const int k1 = r & 1;
-            const int k3 = r >> 1;
-            E[r] = cmul(A[k1 + 4 * k3], uf[2][0]);
-            cmac(E[r], A[k1 + 2 + 4 * k3], uf[2][1]);
-            // G[k0 + 2 k3] = contract k2 of F[k0, k2, k3]
-            G[r] = cmul(F[k1 + 4 * k3], uf[2][0]);
-            cmac(G[r], F[k1 + 2 + 4 * k3], uf[2][1]);
-        }
-        double2 ABC[2], env[4][2];
-#pragma unroll
-        for (int b = 0; b < 2; ++b) {
-            ABC[b] = cmul(AB[2 * b], uf[2][0]);          // contract k2 of AB[k2, k3=b]
-            cmac(ABC[b], AB[2 * b + 1], uf[2][1]);
-            env[3][b] = ABC[b];
-            env[2][b] = cmul(AB[b], uf[3][0]);           // contract k3 of AB[k2=b, k3]
-            cmac(env[2][b], AB[b + 2], uf[3][1]);
-            env[1][b] = cmul(E[b], uf[3][0]);            // contract k3 of E[k1=b, k3]
-            cmac(env[1][b], E[b + 2], uf[3][1]);
-            env[0][b] = cmul(G[b], uf[3][0]);            // contract k3 of G[k0=b, k3]
-            cmac(env[0][b], G[b + 2], uf[3][1]);
-        }
-        double2 pk = cmul(ABC[0], uf[3][0]);
-        cmac(pk, ABC[1], uf[3][1]);
+        // the register bits k0..k3 (wires 8..11) against conj(mu)
+        double2 pk, env[4][2];
+        contract_reg4(m, uf, pk, env);
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
             cmac(benv[e][0], bg, env[e][0]);
@@ -2467,8 +2477,36 @@ __global__ __launch_bounds__(256) void layer0_high_env_kernel(
     }
 }
 
-// Gradients of layer 0: d u / d theta = (-s/2 e-, c/2 e+),
-// d u / d phi = (-i/2 c e-, +i/2 s e+), e-+ = e^{-+i phi/2}.
+// Gradients of layer-0 wire w from its environments e = (E_w[0], E_w[1]) as
+// {re, im, re, im}: d u / d theta = (-s/2 e-, c/2 e+),
+// d u / d phi = (-i/2 c e-, +i/2 s e+), e-+ = e^{-+i phi/2}.  Wire 0 also
+// writes the energy: with lambda = H psi carried back to layer 0,
+// <chi|mu> = <psi|H|psi>, so E = Re sum_b E_w[b] u_w(b) for any wire w.
+__device__ __forceinline__ void layer0_wire_result(const WireCoef &k, const double (&e)[4], int w,
+                                                   double *__restrict__ grad,
+                                                   const double *__restrict__ scale,
+                                                   double *__restrict__ energy_out) {
+    const double2 em = make_double2(k.ch, -k.sh);
+    const double2 ep = make_double2(k.ch, k.sh);
+    const double2 dth0 = make_double2(-0.5 * k.s * em.x, -0.5 * k.s * em.y);
+    const double2 dth1 = make_double2(0.5 * k.c * ep.x, 0.5 * k.c * ep.y);
+    // -i/2 c e- and +i/2 s e+
+    const double2 dph0 = make_double2(0.5 * k.c * em.y, -0.5 * k.c * em.x);
+    const double2 dph1 = make_double2(-0.5 * k.s * ep.y, 0.5 * k.s * ep.x);
+    const double2 E0 = make_double2(e[0], e[1]);
+    const double2 E1 = make_double2(e[2], e[3]);
+    const double dth = 2.0 * (cmul(E0, dth0).x + cmul(E1, dth1).x);
+    const double dph = 2.0 * (cmul(E0, dph0).x + cmul(E1, dph1).x);
+    const double f = scale != nullptr ? *scale : 1.0;
+    grad[2 * w] = dth * f;
+    grad[2 * w + 1] = dph * f;
+    if (w == 0 && energy_out != nullptr) {
+        *energy_out = cmul(E0, u0_of(k)).x + cmul(E1, u1_of(k)).x;
+    }
+}
+
+// Layer-0 gradients from the environments of the low-tile contraction
+// (wires 0..11, per workgroup) and of layer0_high_env (wires 12..n-1).
 __global__ __launch_bounds__(64) void layer0_finish_kernel(
     const double *__restrict__ low_partials, int low_blocks, const double *__restrict__ high_partials,
     int high_chunks, const WireCoef *__restrict__ lc0, int n, double *__restrict__ grad,
@@ -2497,26 +2535,7 @@ __global__ __launch_bounds__(64) void layer0_finish_kernel(
         e[c] = wave_sum(e[c]);
     }
     if (t == 0) {
-        const WireCoef k = lc0[w];
-        const double2 em = make_double2(k.ch, -k.sh);
-        const double2 ep = make_double2(k.ch, k.sh);
-        const double2 dth0 = make_double2(-0.5 * k.s * em.x, -0.5 * k.s * em.y);
-        const double2 dth1 = make_double2(0.5 * k.c * ep.x, 0.5 * k.c * ep.y);
-        // -i/2 c e- and +i/2 s e+
-        const double2 dph0 = make_double2(0.5 * k.c * em.y, -0.5 * k.c * em.x);
-        const double2 dph1 = make_double2(-0.5 * k.s * ep.y, 0.5 * k.s * ep.x);
-        const double2 E0 = make_double2(e[0], e[1]);
-        const double2 E1 = make_double2(e[2], e[3]);
-        const double dth = 2.0 * (cmul(E0, dth0).x + cmul(E1, dth1).x);
-        const double dph = 2.0 * (cmul(E0, dph0).x + cmul(E1, dph1).x);
-        const double f = scale != nullptr ? *scale : 1.0;
-        grad[2 * w] = dth * f;
-        grad[2 * w + 1] = dph * f;
-        if (w == 0 && energy_out != nullptr) {
-            // with lambda = H psi carried back to layer 0, <chi|mu> = <psi|H|psi>:
-            // E = Re sum_b E_w[b] u_w(b) for any wire w
-            *energy_out = cmul(E0, u0_of(k)).x + cmul(E1, u1_of(k)).x;
-        }
+        layer0_wire_result(lc0[w], e, w, grad, scale, energy_out);
     }
 }
 
@@ -2543,6 +2562,364 @@ void launch_layer0_finish(const double *low_partials, int low_blocks, const doub
                           const double *scale, hipStream_t s, double *energy_out) {
     layer0_finish_kernel<<<n, 64, 0, s>>>(low_partials, low_blocks, high_partials, high_chunks,
                                           lc0, n, grad, scale, energy_out);
+}
+
+// ---------------------------------------------------------------------------
+// The last high pass of a two-layer evaluation, with the layer-0 contraction.
+//
+// When the generated start runs its high passes from the lowest group up, the
+// last one holds the top group's tiles: index bits {0, 1, 2} and [h, n)
+// (h = n - 9) of the frame after layer 0's ring CNOT, the bits [3, h) fixed by
+// the tile.  Once the group is un-rotated, lambda of the tile is mu = P^-1 lambda
+// at the layer-0 coordinates i = P^-1 j:
+//   i_0 = j_0 ^ j_{n-1},  i_1 = j_0 ^ j_1 ^ j_{n-1},  i_2 = j_1 ^ j_2,
+//   i_w = j_w ^ j_{w-1} (w >= 3),
+// so q = (i_0, i_1, i_2, i_h, .., i_{n-1}) runs over all 2^12 values in the tile,
+// i_4 .. i_{h-1} are fixed by it, and i_3 = j_3 ^ i_0 ^ i_1 ^ i_2 (j_2 =
+// i_0 ^ i_1 ^ i_2).  Instead of storing lambda the pass permutes it through LDS
+// to q (threads: q bits 0..7, registers: q bits 8..11) and contracts it with
+// the layer-0 product state as layer0_contract_kernel does with a low tile:
+// the register wires n-4 .. n-1 per thread (contract_reg4), the thread wires
+// from per-thread sums split by x = j_3 ^ parity(q_0, q_1, q_2) (wire 3's
+// factor is u_3(x)), the fixed wires 4 .. h-1 from per-tile sums S(g)
+// (layer0_fixed_env_kernel) and the per-tile product phi_c(g) of their factors.
+// ---------------------------------------------------------------------------
+
+// q of top-tile offset o (bits 0..2: j_0..j_2, bits 3..11: j_h..j_{n-1})
+// without the tile's j_{h-1}, which flips q bit 3.
+__device__ __forceinline__ uint32_t top_q_of_offset(uint32_t o) {
+    const uint32_t d = (o ^ (o << 1)) & 0xFFFu;
+    const uint32_t q = (d & ~8u) | (o & 8u);
+    return q ^ (((o >> 11) & 1u) * 3u);
+}
+
+// Wire of q bit b in the top tile.
+__device__ __forceinline__ int top_wire(int b, int h) {
+    return b < 3 ? b : h + (b - 3);
+}
+
+// phi_c[g] = prod_{v = 4}^{h-1} u_v(j_v ^ j_{v-1}), j_{3+m} = bit m of g.
+__global__ __launch_bounds__(256) void layer0_top_factor_kernel(const WireCoef *__restrict__ lc0,
+                                                                int h, uint64_t num_tiles,
+                                                                double2 *__restrict__ phi_c) {
+    const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= num_tiles) {
+        return;
+    }
+    double2 f = make_double2(1.0, 0.0);
+    for (int v = 4; v < h; ++v) {
+        f = cmul(f, u_of(lc0[v], (int)(((g >> (v - 3)) ^ (g >> (v - 4))) & 1u)));
+    }
+    phi_c[g] = f;
+}
+
+// Per-block partials of the fused pass: layer-1 gradients as bwd_lam_kernel
+// (2T per block), then kTopL0Cols layer-0 values per block: 4 doubles
+// (E_w[0], E_w[1]) per thread wire at column 4b (q bit b = 0..7), per register
+// wire at 32 + 4e (wire n - 4 + e), wire 3 at 48.
+static_assert(kTopL0Cols == 52 && Tile4Waves == Tile<4>::WAVES, "fused top pass layout");
+
+__global__ __launch_bounds__(256, 2) void bwd_top_l0_kernel(
+    const double2 *lam_in, const WireCoef *__restrict__ lc, PassGeom G,
+    double *__restrict__ partials, const CMat2 *__restrict__ gen_tab,
+    const CMat2 *__restrict__ gen_utab, const WireCoef *__restrict__ lc0,
+    const double2 *__restrict__ phi_c, double2 *__restrict__ tile_sums,
+    double *__restrict__ l0_partials) {
+    constexpr int R = 4;
+    using TL = Tile<R>;
+    constexpr int T = TL::T;
+    __shared__ double2 lds[TL::SLOTS];
+    constexpr int ROWS = TL::THREADS / 16;
+    __shared__ double acc[ROWS][2 * T];
+    __shared__ double2 rz_tab[TL::P * TL::REGS];
+    __shared__ WireCoef tile_coef[T];
+    __shared__ CMat2 gen_chunks[kChainTabEntries - kChainC0];
+    // u_{n-4+j}(b) at 2j + b (the register wires), u_3(b) at 8 + b
+    __shared__ double2 l0_u[10];
+    const uint32_t t = threadIdx.x;
+    const int lane = t & (kWave - 1);
+    const int wave = (int)(t / kWave);
+    const int n = G.n;
+    const int h = G.h0;
+    double *row_acc = acc[t / 16];
+    for (uint32_t i = t; i < (uint32_t)(ROWS * 2 * T); i += TL::THREADS) {
+        acc[i / (2 * T)][i % (2 * T)] = 0.0;
+    }
+    build_rz_tables<R>(rz_tab, lc, G, t);
+    if (t < (uint32_t)T) {
+        const bool rot = (int)t >= G.rot_lo && (int)t < G.rot_hi;
+        tile_coef[t] = rot ? lc[tile_wire((int)t, G)] : WireCoef{1.0, 0.0, 1.0, 0.0};
+    }
+    for (int i = (int)t; i < kChainTabEntries - kChainC0; i += TL::THREADS) {
+        gen_chunks[i] = gen_tab[kChainC0 + i];
+    }
+    if (t < 10) {
+        const int w = t < 8 ? n - 4 + (int)(t >> 1) : 3;
+        l0_u[t] = u_of(lc0[w], (int)(t & 1u));
+    }
+    __syncthreads();
+
+    const TileAddr ta = tile_addr<R>(t, G);
+    const TileAddr ta_in = lay_tile_addr(ta, G.lay_in);
+    // the thread's q bits 0..7 are its index: parity of q_0..q_2 and the
+    // product a_t over its wires 0, 1, 2, h .. h + 4 (formed where used, so it
+    // holds no registers across the sweep)
+    auto thread_factor = [&](uint32_t tt) {
+        double2 a = make_double2(1.0, 0.0);
+        for (int b = 0; b < 8; ++b) {
+            a = cmul(a, u_of(lc0[top_wire(b, h)], (int)((tt >> b) & 1u)));
+        }
+        return a;
+    };
+    // per-thread sums over this workgroup's tiles: zc[x] of phi_c pk with
+    // x = j_3 ^ par, benv[e][b] of u_3(x) phi_c env[e][b]
+    double2 zc0 = make_double2(0.0, 0.0);
+    double2 zc1 = make_double2(0.0, 0.0);
+    double2 benv[4][2];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        benv[e][0] = make_double2(0.0, 0.0);
+        benv[e][1] = make_double2(0.0, 0.0);
+    }
+    for (uint64_t g = blockIdx.x; g < G.num_tiles; g += gridDim.x) {
+        uint32_t tl = t;
+        asm volatile("" : "+v"(tl));
+        const uint64_t base = tile_base(g, G, T);
+        double2 c[TL::REGS];
+        double2 l[TL::REGS];
+        bwd_load_tile<R, false, false, true>(c, l, nullptr, lam_in, base, ta, G, lds, tl, 0.0,
+                                             gen_tab, gen_chunks, gen_utab + 2 * g,
+                                             lay_addr(base, G.lay_in), &ta_in);
+        // the sweep of the lambda-only high pass (bwd_lam_kernel, R = 4)
+        grad_window<R, 0>(c, l, lc, G, row_acc, lane, rz_tab);
+        exchange<R, 0, 2>(l, lds, tl);
+        exchange<R, 0, 2>(c, lds, tl);
+        grad_window<R, 2>(c, l, lc, G, row_acc, lane, rz_tab);
+        grad_lane_bit3<R>(c, l, row_acc, lane, rz_tab);
+        ry_inv_window<R, 2>(l, tile_coef);
+        ry_inv_lane_bit3<R>(l, tile_coef[3], lane);
+        exchange<R, 2, 0>(l, lds, tl);
+        ry_inv_window<R, 0>(l, tile_coef);
+        // lambda to the layer-0 coordinates q (mapping 0 of q: thread bits 0..7)
+        const uint32_t cflip = (uint32_t)((base >> (h - 1)) & 1u) << 3;
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < TL::REGS; ++k) {
+            lds[lds_slot<R>(top_q_of_offset(omap<R, 0>(tl, k)) ^ cflip)] = l[k];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < TL::REGS; ++k) {
+            l[k] = lds[lds_slot<R>(omap<R, 0>(tl, k))];
+        }
+        double2 uf[4][2];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            uf[j][0] = l0_u[2 * j];
+            uf[j][1] = l0_u[2 * j + 1];
+        }
+        double2 pk, env[4][2];
+        contract_reg4(l, uf, pk, env);
+        const uint32_t x = (uint32_t)(((base >> 3) ^ tl ^ (tl >> 1) ^ (tl >> 2)) & 1u);
+        const double2 phi = phi_c[g];
+        const double2 u3x = l0_u[8 + x];
+        const double2 pp = cmul(phi, pk);
+        zc0.x += x ? 0.0 : pp.x;
+        zc0.y += x ? 0.0 : pp.y;
+        zc1.x += x ? pp.x : 0.0;
+        zc1.y += x ? pp.y : 0.0;
+        const double2 bgw = cmul(phi, u3x);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            cmac(benv[e][0], bgw, env[e][0]);
+            cmac(benv[e][1], bgw, env[e][1]);
+        }
+        // the tile's sum without phi_c, per wave
+        const double2 st = cmul(cmul(thread_factor(tl), u3x), pk);
+        const double sx = wave_sum(st.x);
+        const double sy = wave_sum(st.y);
+        if (lane == 0) {
+            tile_sums[g * TL::WAVES + (uint64_t)wave] = make_double2(sx, sy);
+        }
+    }
+    __syncthreads();
+    if (t < 2 * T) {
+        double total = 0.0;
+#pragma unroll
+        for (int w = 0; w < ROWS; ++w) {
+            total += acc[w][t];
+        }
+        partials[(uint64_t)blockIdx.x * (2 * T) + t] = total;
+    }
+    // layer-0 environments of this workgroup's tiles; the exchange buffer
+    // holds the per-wave sums (the thread index is opaque here so the factors
+    // below are formed after the loop)
+    double *red = reinterpret_cast<double *>(lds);
+    uint32_t te = t;
+    asm volatile("" : "+v"(te));
+    const double2 a_t = thread_factor(te);
+    const double2 u30 = l0_u[8];
+    const double2 u31 = l0_u[9];
+    double2 zsum = cmul(u30, zc0);
+    cmac(zsum, u31, zc1);
+#pragma unroll
+    for (int bw = 0; bw < 8; ++bw) {
+        double2 skip = make_double2(1.0, 0.0);
+        for (int b = 0; b < 8; ++b) {
+            if (b != bw) {
+                skip = cmul(skip, u_of(lc0[top_wire(b, h)], (int)((te >> b) & 1u)));
+            }
+        }
+        const double2 val = cmul(zsum, skip);
+        const int bit = (int)((te >> bw) & 1u);
+        const double vals[4] = {bit ? 0.0 : val.x, bit ? 0.0 : val.y, bit ? val.x : 0.0,
+                                bit ? val.y : 0.0};
+#pragma unroll
+        for (int cc = 0; cc < 4; ++cc) {
+            const double r = wave_sum(vals[cc]);
+            if (lane == 0) {
+                red[wave * kTopL0Cols + 4 * bw + cc] = r;
+            }
+        }
+    }
+#pragma unroll
+    for (int e = 0; e < 5; ++e) {
+        // e < 4: register wire n - 4 + e; e == 4: wire 3
+        const double2 v0 = cmul(a_t, e < 4 ? benv[e & 3][0] : zc0);
+        const double2 v1 = cmul(a_t, e < 4 ? benv[e & 3][1] : zc1);
+        const double vals[4] = {v0.x, v0.y, v1.x, v1.y};
+#pragma unroll
+        for (int cc = 0; cc < 4; ++cc) {
+            const double r = wave_sum(vals[cc]);
+            if (lane == 0) {
+                red[wave * kTopL0Cols + 32 + 4 * e + cc] = r;
+            }
+        }
+    }
+    __syncthreads();
+    if (t < (uint32_t)kTopL0Cols) {
+        double sum = 0.0;
+        for (int w = 0; w < TL::WAVES; ++w) {
+            sum += red[w * kTopL0Cols + t];
+        }
+        l0_partials[(uint64_t)blockIdx.x * kTopL0Cols + t] = sum;
+    }
+}
+
+// Environments of the fixed wires v in [4, h) of the top-group tiles from the
+// per-tile sums (waves partials per tile):
+// E_v[b] = sum_{g : i_v(g) = b} S(g) prod_{u in [4, h), u != v} u_u(i_u(g)),
+// i_u(g) = j_u ^ j_{u-1}, j_{3+m} = bit m of g.  Block (v - 4, chunk) writes
+// 4 doubles.
+__global__ __launch_bounds__(256) void layer0_fixed_env_kernel(
+    const double2 *__restrict__ tile_sums, int waves, uint64_t num_tiles,
+    const WireCoef *__restrict__ lc0, int h, double *__restrict__ partials) {
+    __shared__ double red[4][4];
+    const int v = 4 + (int)blockIdx.y;
+    const int t = threadIdx.x;
+    double2 e0 = make_double2(0.0, 0.0);
+    double2 e1 = make_double2(0.0, 0.0);
+    const uint64_t stride = (uint64_t)gridDim.x * 256;
+    for (uint64_t g = (uint64_t)blockIdx.x * 256 + t; g < num_tiles; g += stride) {
+        double2 f = tile_sums[g * waves];
+        for (int w = 1; w < waves; ++w) {
+            const double2 p = tile_sums[g * waves + w];
+            f.x += p.x;
+            f.y += p.y;
+        }
+        for (int u = 4; u < h; ++u) {
+            if (u != v) {
+                f = cmul(f, u_of(lc0[u], (int)(((g >> (u - 3)) ^ (g >> (u - 4))) & 1u)));
+            }
+        }
+        if (((g >> (v - 3)) ^ (g >> (v - 4))) & 1u) {
+            e1.x += f.x;
+            e1.y += f.y;
+        } else {
+            e0.x += f.x;
+            e0.y += f.y;
+        }
+    }
+    const double vals[4] = {e0.x, e0.y, e1.x, e1.y};
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const double r = wave_sum(vals[c]);
+        if ((t & 63) == 0) {
+            red[t >> 6][c] = r;
+        }
+    }
+    __syncthreads();
+    if (t < 4) {
+        partials[((uint64_t)blockIdx.y * gridDim.x + blockIdx.x) * 4 + t] =
+            red[0][t] + red[1][t] + red[2][t] + red[3][t];
+    }
+}
+
+// Layer-0 gradients (and the energy) from the fused pass's per-block partials
+// and the fixed wires' chunks.
+__global__ __launch_bounds__(64) void layer0_finish_top_kernel(
+    const double *__restrict__ blk_partials, int blocks, const double *__restrict__ fixed_partials,
+    int chunks, const WireCoef *__restrict__ lc0, int n, double *__restrict__ grad,
+    const double *__restrict__ scale, double *__restrict__ energy_out) {
+    const int w = blockIdx.x;
+    const int t = threadIdx.x;
+    const int h = n - 9;
+    double e[4] = {0.0, 0.0, 0.0, 0.0};
+    if (w >= 4 && w < h) {
+        for (int b = t; b < chunks; b += 64) {
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                e[c] += fixed_partials[((uint64_t)(w - 4) * chunks + b) * 4 + c];
+            }
+        }
+    } else {
+        const int col = w < 3        ? 4 * w
+                        : w == 3     ? 48
+                        : w < h + 5  ? 4 * (3 + w - h)
+                                     : 32 + 4 * (w - h - 5);
+        for (int b = t; b < blocks; b += 64) {
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                e[c] += blk_partials[(uint64_t)b * kTopL0Cols + col + c];
+            }
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        e[c] = wave_sum(e[c]);
+    }
+    if (t == 0) {
+        layer0_wire_result(lc0[w], e, w, grad, scale, energy_out);
+    }
+}
+
+void launch_bwd_top_l0(const double2 *lam_in, const WireCoef *layer_coef, const PassGeom &g,
+                       double *partials, int grid, hipStream_t s, const CMat2 *gen_tab,
+                       const CMat2 *gen_utab, const WireCoef *lc0, double2 *phi_c,
+                       double2 *tile_sums, double *l0_partials) {
+    if (g.a != 3 || g.h0 + 9 != g.n || g.h0 < 4 || g.rot_lo < 3 || g.lay_out.count != 0 ||
+        !lay_fits(g.lay_in, g.n, g.h0 + 5, 4)) {
+        throw std::invalid_argument("layer-0 contraction in a high pass: top group tiles only");
+    }
+    layer0_top_factor_kernel<<<(unsigned)((g.num_tiles + 255) / 256), 256, 0, s>>>(
+        lc0, g.h0, g.num_tiles, phi_c);
+    bwd_top_l0_kernel<<<grid, Tile<4>::THREADS, 0, s>>>(lam_in, layer_coef, g, partials, gen_tab,
+                                                        gen_utab, lc0, phi_c, tile_sums,
+                                                        l0_partials);
+}
+
+void launch_layer0_top_finish(const double2 *tile_sums, uint64_t num_tiles, const double *blk_partials,
+                              int blocks, const WireCoef *lc0, int n, double *fixed_partials,
+                              int chunks, double *grad, const double *scale, hipStream_t s,
+                              double *energy_out) {
+    const int h = n - 9;
+    if (h > 4) {
+        dim3 grid(chunks, h - 4);
+        layer0_fixed_env_kernel<<<grid, 256, 0, s>>>(tile_sums, Tile<4>::WAVES, num_tiles, lc0, h,
+                                                      fixed_partials);
+    }
+    layer0_finish_top_kernel<<<n, 64, 0, s>>>(blk_partials, blocks, fixed_partials, chunks, lc0, n,
+                                              grad, scale, energy_out);
 }
 
 // ---------------------------------------------------------------------------

@@ -330,6 +330,27 @@ void launch_layer0_high_env(const double2 *tile_sums, uint64_t num_tiles, const 
                             int n, double *partials, int chunks, hipStream_t s);
 // energy_out (optional): Re <chi|mu>, the energy when lambda = H psi was carried
 // back complete (fused Hamiltonian: the pair terms are all added by then).
+// Waves of a 256-thread workgroup (register shape 4): the fused top pass
+// writes one sum per wave and tile.
+constexpr uint64_t Tile4Waves = 4;
+// Layer-0 values per workgroup of the fused top pass (see launch_bwd_top_l0).
+constexpr int kTopL0Cols = 52;
+// The last high pass of a two-layer evaluation whose high passes run from the
+// lowest group up (the top group's tiles, h0 = n - 9; register shape 4): the
+// lambda-only sweep of its wires, then, instead of storing lambda, the layer-0
+// contraction of its tiles (per-block partials: 2T layer-1 gradients in
+// partials, kTopL0Cols = 52 layer-0 values in l0_partials; per tile 4 wave
+// sums in tile_sums; phi_c: one value per tile, written first).
+void launch_bwd_top_l0(const double2 *lam_in, const WireCoef *layer_coef, const PassGeom &g,
+                       double *partials, int grid, hipStream_t s, const CMat2 *gen_tab,
+                       const CMat2 *gen_utab, const WireCoef *lc0, double2 *phi_c,
+                       double2 *tile_sums, double *l0_partials);
+// The fixed wires' environments from tile_sums and the layer-0 gradients (and
+// energy) from the fused pass's partials.
+void launch_layer0_top_finish(const double2 *tile_sums, uint64_t num_tiles, const double *blk_partials,
+                              int blocks, const WireCoef *lc0, int n, double *fixed_partials,
+                              int chunks, double *grad, const double *scale, hipStream_t s,
+                              double *energy_out);
 void launch_layer0_finish(const double *low_partials, int low_blocks, const double *high_partials,
                           int high_chunks, const WireCoef *lc0, int n, double *grad,
                           const double *scale, hipStream_t s, double *energy_out = nullptr);

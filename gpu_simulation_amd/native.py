@@ -52,6 +52,8 @@ C_ABI_SYMBOLS = (
     "rie_engine_double_buffer",
     "rie_engine_set_lambda_layouts",
     "rie_engine_lambda_layouts",
+    "rie_engine_set_layer0_in_pass",
+    "rie_engine_layer0_in_pass",
     "rie_engine_set_kernel_timing",
     "rie_engine_reset_kernel_stats",
     "rie_engine_kernel_stat_count",

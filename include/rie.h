@@ -232,14 +232,29 @@ int rie_engine_generated_start(const rie_engine *engine);
  * whose passes keep every vector in index order).  Between the passes of a
  * generated-start evaluation lambda is stored in the order that makes the
  * next high pass's tiles contiguous 64 KiB blocks (index bits 0..2 and that
- * pass's nine high bits at the bottom), so every high pass reads whole
- * blocks and only its writes are 128-byte runs; the last high pass writes
- * the natural order.  Results are bitwise identical either way.  enabled: 1
+ * pass's nine high bits at the bottom, the writing pass's tile bits right
+ * above them), so every high pass reads whole blocks and writes 128-byte
+ * runs within 32 MB per tile; the last high pass writes the natural order (or
+ * nothing, see rie_engine_set_layer0_in_pass).  Results are bitwise identical
+ * either way.  enabled: 1
  * on (default), 0 off; rie_engine_lambda_layouts reports whether gradient
  * evaluations use them (the generated start in use and the pass geometry
  * suited to them). */
 rie_status rie_engine_set_lambda_layouts(rie_engine *engine, int enabled);
 int rie_engine_lambda_layouts(const rie_engine *engine);
+
+/* Layer 0 inside the last high pass of the generated start (no counterpart
+ * in the reference, whose layer-0 gradients come from the backward sweep of
+ * layer 0).  The generated start runs its high passes from the lowest group
+ * up, so the last one holds the top group's tiles, which under the inverse
+ * ring CNOT are product domains of the layer-0 coordinates (up to one parity
+ * bit): that pass contracts lambda with the layer-0 product state instead of
+ * storing it, and the separate 1 S layer-0 contraction pass is not run.
+ * enabled: 1 on (default), 0 off; rie_engine_layer0_in_pass reports whether
+ * gradient evaluations use it (the generated start in use, register shape 4
+ * for the high passes). */
+rie_status rie_engine_set_layer0_in_pass(rie_engine *engine, int enabled);
+int rie_engine_layer0_in_pass(const rie_engine *engine);
 
 /* Double-buffered adjoint high passes.  Replaces the double_buffer flag of
  * RingIsingCudaBackend (cpp/ising_cuda_backend.cu:374-419), which selects a

@@ -125,14 +125,17 @@ def _chain_y(sites, chunks, ua, ub, i, base):
     return y @ t1[(i >> f1) & ((1 << n1) - 1)]
 
 
-def _plan_cases(n):
+def _plan_cases(n, bottom_up=False):
     """(tile low bits, first high index bit, rotated wires) of every table set
     as the engine plans them: the forward's low tile (all wires rotated), then
     one adjoint high pass per high group.  The n - 12 high wires split into
     ceil((n - 12) / 9) groups of near-equal size; group j's tile takes 9 wires
-    from h0 = min(g0, n - 9), and its state still carries the layer-1
-    rotations of groups 0 .. j (wires [12, end of group j)), the low wires and
-    the groups above j being un-rotated already."""
+    from h0 = min(g0, n - 9).  With the high passes running from the top group
+    down (the stored schedule, the walk) group j's state still carries the
+    layer-1 rotations of groups 0 .. j (wires [12, end of group j)), the low
+    wires and the groups above j being un-rotated already; with the passes
+    running from the lowest group up (the generated start, bottom_up) it
+    carries groups j .. K-1 (wires [g0 of group j, n))."""
     full = (1 << n) - 1
     a = 3
     cases = [(T, T, full)]
@@ -142,7 +145,10 @@ def _plan_cases(n):
     for gi in range(groups):
         bg = rest // groups + (1 if gi < rest % groups else 0)
         h0 = min(g0, n - width)
-        cases.append((a, h0, full & ~((1 << T) - 1) & ((1 << (g0 + bg)) - 1)))
+        if bottom_up:
+            cases.append((a, h0, full & ~((1 << g0) - 1)))
+        else:
+            cases.append((a, h0, full & ~((1 << T) - 1) & ((1 << (g0 + bg)) - 1)))
         g0 += bg
     return cases
 
@@ -166,11 +172,18 @@ def test_plan_cases_follow_the_engine_groups():
     c = _plan_cases(31)
     assert [(ta, h0) for ta, h0, _ in c] == [(12, 12), (3, 12), (3, 19), (3, 22)]
     assert c[2][2] == ((1 << 25) - 1) & ~((1 << 12) - 1)
+    # the generated start's order: the top group last, with only its own wires
+    c = _plan_cases(30, bottom_up=True)
+    assert c[1][2] == ((1 << 30) - 1) & ~((1 << 12) - 1)
+    assert c[2][2] == ((1 << 30) - 1) & ~((1 << 21) - 1)
+    c = _plan_cases(31, bottom_up=True)
+    assert [(ta, h0) for ta, h0, _ in c] == [(12, 12), (3, 12), (3, 19), (3, 22)]
+    assert c[3][2] == ((1 << 31) - 1) & ~((1 << 25) - 1)
 
 
-def _check_sampled(rng, n, R, params, want_at, tiles=4, threads=24):
+def _check_sampled(rng, n, R, params, want_at, tiles=4, threads=24, bottom_up=False):
     l0, l1 = _coefs(params[0]), _coefs(params[1])
-    for ta, th0, rot in _plan_cases(n):
+    for ta, th0, rot in _plan_cases(n, bottom_up):
         want = want_at(rot)
         sites, r0, run, chunks, ua, ub = _set(n, R, ta, th0, rot, l0, l1)
         gap = th0 - ta
@@ -192,7 +205,8 @@ def _check_sampled(rng, n, R, params, want_at, tiles=4, threads=24):
 
 @pytest.mark.parametrize("n", [13, 16, 22])
 @pytest.mark.parametrize("R", [3, 4])
-def test_tile_tables_reproduce_states(oracle, n, R):
+@pytest.mark.parametrize("bottom_up", [False, True])
+def test_tile_tables_reproduce_states(oracle, n, R, bottom_up):
     # sampled tiles and threads of every table set against the oracle's states
     rng = np.random.default_rng(60 + n + R)
     params = rng.uniform(0.0, 2.0 * np.pi, size=(2, n, 2))
@@ -201,12 +215,13 @@ def test_tile_tables_reproduce_states(oracle, n, R):
         v = _rotated_reference(oracle, n, params, rot)
         return lambda i: v[i]
 
-    _check_sampled(rng, n, R, params, want_at)
+    _check_sampled(rng, n, R, params, want_at, bottom_up=bottom_up)
 
 
 @pytest.mark.parametrize("n", [30, 31])
 @pytest.mark.parametrize("R", [3, 4])
-def test_tile_tables_at_the_largest_geometries(n, R):
+@pytest.mark.parametrize("bottom_up", [False, True])
+def test_tile_tables_at_the_largest_geometries(n, R, bottom_up):
     # n = 30 (two full groups, h0 == g0) and n = 31 (three groups, overlapping
     # tiles): sampled tiles and threads against the direct per-wire trace
     rng = np.random.default_rng(90 + n + R)
@@ -217,7 +232,7 @@ def test_tile_tables_at_the_largest_geometries(n, R):
         sites = [[_site(w, q, l0, l1, (rot >> w) & 1) for q in (0, 1)] for w in range(n)]
         return lambda i: _direct_trace(sites, n, i)
 
-    _check_sampled(rng, n, R, params, want_at, tiles=3, threads=6)
+    _check_sampled(rng, n, R, params, want_at, tiles=3, threads=6, bottom_up=bottom_up)
 
 
 def _ring_fwd(i, n):

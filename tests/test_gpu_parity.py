@@ -456,8 +456,13 @@ def test_lambda_layouts(oracle, backend_module, n, r_low, r_high, r_l0):
     print(f"case lambda layouts n={n} r_low={r_low} r_high={r_high} r_l0={r_l0}", flush=True)
     assert eng.generated_start and eng.lambda_layouts
     out = eng.energy_and_grad(params)
+    # the same pass order without layouts (where the order from the lowest
+    # group up would lose the layouts, the engine runs from the top down; off,
+    # it would switch)
+    fused = eng.layer0_in_pass
     eng.set_lambda_layouts(0)
-    assert not eng.lambda_layouts
+    eng.set_layer0_in_pass(1 if fused else 0)
+    assert not eng.lambda_layouts and eng.layer0_in_pass == fused
     ref = eng.energy_and_grad(params)
     assert out["energy"] == ref["energy"]
     assert np.array_equal(np.asarray(out["gradient"]), np.asarray(ref["gradient"]))
@@ -483,6 +488,61 @@ def test_lambda_layouts_scope(backend_module):
     assert eng.lambda_layouts
     eng.set_generated_start(0)
     assert not eng.lambda_layouts
+
+
+@pytest.mark.parametrize("n,layouts", [(13, 1), (16, 1), (19, 1), (21, 0), (22, 1), (25, 1),
+                                       (26, 0), (30, 1)])
+def test_layer0_in_pass(oracle, backend_module, n, layouts):
+    """Two-layer evaluations whose last high pass (the top group, high passes
+    from the lowest group up) contracts layer 0 instead of storing lambda:
+    against the separate contraction (the high passes from the top group
+    down) at 1e-12, and the oracle (n <= 25), at two fields, one, two and
+    three high groups, with and without lambda layouts, special angles
+    included."""
+    for field, seed in [(1.0, 8400 + n), (0.61, 8500 + n)]:
+        params = _params(seed, n, 2)
+        if field != 1.0:
+            # zeros in the layer-0 factors of wires 3, n - 1 and a fixed wire
+            params[0, 3, 0] = np.pi
+            params[0, n - 1, 0] = 0.0
+            params[0, min(5, n - 10), 0] = np.pi
+        params = np.ascontiguousarray(params.reshape(-1))
+        eng = backend_module.RingIsingCudaBackend(n, 2, field, "structured_adjoint", 8, False)
+        eng.set_lambda_layouts(layouts)
+        print(f"case layer0 in pass n={n} layouts={layouts} h={field}", flush=True)
+        assert eng.generated_start and eng.layer0_in_pass
+        out = eng.energy_and_grad(params)
+        eng.set_layer0_in_pass(0)
+        assert not eng.layer0_in_pass
+        ref = eng.energy_and_grad(params)
+        assert abs(out["energy"] - ref["energy"]) <= 1e-12
+        assert np.max(np.abs(np.asarray(out["gradient"]) - np.asarray(ref["gradient"]))) <= 1e-12
+        if n <= 25:
+            e_ref, g_ref = oracle.energy_and_grad(n, 2, field, params, form="structured")
+            assert abs(out["energy"] - e_ref) <= TOL
+            assert np.max(np.abs(np.asarray(out["gradient"]) - g_ref)) <= TOL
+        del eng
+
+
+def test_layer0_in_pass_scope(backend_module):
+    # on by default where the top group's pass can take it; off with the
+    # 3-register shape for the high passes, and at 28 qubits, where the
+    # passes from the lowest group up would lose the lambda layouts
+    eng = backend_module.RingIsingCudaBackend(16, 2, 1.0, "structured_adjoint", 8, False)
+    assert eng.layer0_in_pass
+    shapes = list(eng.pass_shapes)
+    shapes[5] = 3
+    eng.set_pass_shapes(shapes)
+    assert not eng.layer0_in_pass
+    eng = backend_module.RingIsingCudaBackend(28, 2, 1.0, "structured_adjoint", 8, False)
+    assert eng.lambda_layouts and not eng.layer0_in_pass
+    eng.set_lambda_layouts(0)
+    assert eng.layer0_in_pass
+    del eng
+    eng = backend_module.RingIsingCudaBackend(16, 3, 1.0, "structured_adjoint", 8, False)
+    assert not eng.layer0_in_pass
+    with pytest.raises(ValueError):
+        eng.set_layer0_in_pass(2)
 
 
 def test_generated_start_scope(backend_module):
