@@ -235,8 +235,8 @@ Engine::Engine(size_t num_qubits, size_t num_layers, double field, const std::st
         const uint64_t tiles = passes[0].g.num_tiles;
         // per-tile sums: one per tile for the low-tile contraction; for the
         // contraction inside the top group's pass one per wave of a tile plus
-        // the tile's fixed-wire factor
-        tile_sum_count = L == 2 ? tiles * (Tile4Waves + 1) : tiles;
+        // the tile's fixed-wire factor and lambda address (16 B each)
+        tile_sum_count = L == 2 ? tiles * (Tile4Waves + 2) : tiles;
         d_tile_sums = (double2 *)device_alloc(sizeof(double2) * tile_sum_count, "alloc tile sums");
         d_chain = (CMat2 *)device_alloc(sizeof(CMat2) * kChainMaxSets * kChainTabEntries,
                                         "alloc chain tables");
@@ -860,16 +860,17 @@ void Engine::structured_eval_gen_start(hipStream_t s, const double *scale, doubl
                 const uint64_t tiles = pp.g.num_tiles;
                 require_partials((size_t)pp.grid * (2 * T + kTopL0Cols));
                 if ((size_t)high_env_chunks * 4 * (size_t)std::max(1, n - T) > partial2_count ||
-                    tile_sum_count < tiles * (Tile4Waves + 1)) {
+                    tile_sum_count < tiles * (Tile4Waves + 2)) {
                     fail(RIE_DEVICE_ERROR, "internal: layer-0 partial buffer too small");
                 }
                 double *l0_part = d_partials + (size_t)pp.grid * 2 * T;
                 double2 *phi_c = d_tile_sums + tiles * Tile4Waves;
+                uint64_t *lam_tile = reinterpret_cast<uint64_t *>(phi_c + tiles);
                 kt_begin(s);
                 launch_bwd_top_l0(lam, d_coef + n, gj, d_partials, pp.grid, s,
                                   d_chain + (size_t)(1 + j) * kChainTabEntries,
                                   d_chain_u + (size_t)(1 + j) * 2 * passes[0].g.num_tiles, d_coef,
-                                  phi_c, d_tile_sums, l0_part);
+                                  phi_c, lam_tile, d_tile_sums, l0_part);
                 check_launch("bwd_pass high gen layer0");
                 kt_end(s, "bwd_pass_high_gen_layer0", S);
                 kt_begin(s);

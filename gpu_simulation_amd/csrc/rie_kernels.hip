@@ -1323,6 +1323,7 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_la
     const TileAddr ta_in = lay_tile_addr(ta, kGather ? LamLayout{} : G.lay_in);
     const TileAddr ta_out = lay_tile_addr(ta, kGather ? LamLayout{} : G.lay_out);
     static_assert(!(kPrefetch && kGather && kGen), "the gather pass reads the state");
+
     double2 x_next[kPrefetch ? TL::REGS : 1];
     auto prefetch_tile = [&](uint64_t gn, const TileAddr &a, uint32_t tt) {
         if constexpr (kPrefetch) {
@@ -1719,6 +1720,7 @@ static void launch_bwd_pass_r(bool gather, bool write_cur, bool write_lam, const
             // the sweep takes tile bit 3 across lanes and no other bit of window 1
             throw std::invalid_argument("high pass geometry: tile bits 0..2 must ride along");
         }
+
 #define RIE_BWD_LAM(GA, HM, GN, PF)                                                          \
     if (gather == GA && ham == HM && gen == GN && prefetch == PF) {                          \
         bwd_lam_kernel<R, GA, HM, GN, PF><<<grid, TH, 0, s>>>(                               \
@@ -2598,19 +2600,23 @@ __device__ __forceinline__ int top_wire(int b, int h) {
     return b < 3 ? b : h + (b - 3);
 }
 
-// phi_c[g] = prod_{v = 4}^{h-1} u_v(j_v ^ j_{v-1}), j_{3+m} = bit m of g.
+// Per top tile g: phi_c[g] = prod_{v = 4}^{h-1} u_v(j_v ^ j_{v-1}) (j_{3+m} =
+// bit m of g) and lam_tile[g], the address of the tile's base index in the
+// layout the pass reads.
 __global__ __launch_bounds__(256) void layer0_top_factor_kernel(const WireCoef *__restrict__ lc0,
-                                                                int h, uint64_t num_tiles,
-                                                                double2 *__restrict__ phi_c) {
+                                                                PassGeom G,
+                                                                double2 *__restrict__ phi_c,
+                                                                uint64_t *__restrict__ lam_tile) {
     const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= num_tiles) {
+    if (g >= G.num_tiles) {
         return;
     }
     double2 f = make_double2(1.0, 0.0);
-    for (int v = 4; v < h; ++v) {
+    for (int v = 4; v < G.h0; ++v) {
         f = cmul(f, u_of(lc0[v], (int)(((g >> (v - 3)) ^ (g >> (v - 4))) & 1u)));
     }
     phi_c[g] = f;
+    lam_tile[g] = lay_addr(tile_base(g, G, Tile<4>::T), G.lay_in);
 }
 
 // Per-block partials of the fused pass: layer-1 gradients as bwd_lam_kernel
@@ -2623,8 +2629,8 @@ __global__ __launch_bounds__(256, 2) void bwd_top_l0_kernel(
     const double2 *lam_in, const WireCoef *__restrict__ lc, PassGeom G,
     double *__restrict__ partials, const CMat2 *__restrict__ gen_tab,
     const CMat2 *__restrict__ gen_utab, const WireCoef *__restrict__ lc0,
-    const double2 *__restrict__ phi_c, double2 *__restrict__ tile_sums,
-    double *__restrict__ l0_partials) {
+    const double2 *__restrict__ phi_c, const uint64_t *__restrict__ lam_tile,
+    double2 *__restrict__ tile_sums, double *__restrict__ l0_partials) {
     constexpr int R = 4;
     using TL = Tile<R>;
     constexpr int T = TL::T;
@@ -2687,9 +2693,11 @@ __global__ __launch_bounds__(256, 2) void bwd_top_l0_kernel(
         const uint64_t base = tile_base(g, G, T);
         double2 c[TL::REGS];
         double2 l[TL::REGS];
+        // lambda's address of the tile in its layout from the per-tile table
+        // (layer0_top_factor_kernel), so the layout is not evaluated here
         bwd_load_tile<R, false, false, true>(c, l, nullptr, lam_in, base, ta, G, lds, tl, 0.0,
-                                             gen_tab, gen_chunks, gen_utab + 2 * g,
-                                             lay_addr(base, G.lay_in), &ta_in);
+                                             gen_tab, gen_chunks, gen_utab + 2 * g, lam_tile[g],
+                                             &ta_in);
         // the sweep of the lambda-only high pass (bwd_lam_kernel, R = 4)
         grad_window<R, 0>(c, l, lc, G, row_acc, lane, rz_tab);
         exchange<R, 0, 2>(l, lds, tl);
@@ -2896,15 +2904,15 @@ __global__ __launch_bounds__(64) void layer0_finish_top_kernel(
 void launch_bwd_top_l0(const double2 *lam_in, const WireCoef *layer_coef, const PassGeom &g,
                        double *partials, int grid, hipStream_t s, const CMat2 *gen_tab,
                        const CMat2 *gen_utab, const WireCoef *lc0, double2 *phi_c,
-                       double2 *tile_sums, double *l0_partials) {
+                       uint64_t *lam_tile, double2 *tile_sums, double *l0_partials) {
     if (g.a != 3 || g.h0 + 9 != g.n || g.h0 < 4 || g.rot_lo < 3 || g.lay_out.count != 0 ||
         !lay_fits(g.lay_in, g.n, g.h0 + 5, 4)) {
         throw std::invalid_argument("layer-0 contraction in a high pass: top group tiles only");
     }
-    layer0_top_factor_kernel<<<(unsigned)((g.num_tiles + 255) / 256), 256, 0, s>>>(
-        lc0, g.h0, g.num_tiles, phi_c);
+    layer0_top_factor_kernel<<<(unsigned)((g.num_tiles + 255) / 256), 256, 0, s>>>(lc0, g, phi_c,
+                                                                                  lam_tile);
     bwd_top_l0_kernel<<<grid, Tile<4>::THREADS, 0, s>>>(lam_in, layer_coef, g, partials, gen_tab,
-                                                        gen_utab, lc0, phi_c, tile_sums,
+                                                        gen_utab, lc0, phi_c, lam_tile, tile_sums,
                                                         l0_partials);
 }
 

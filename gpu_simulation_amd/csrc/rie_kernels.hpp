@@ -52,13 +52,18 @@ struct PassGeom {
 
 // Address of index i in layout L (a bit permutation, so lay_addr(x | y) =
 // lay_addr(x) | lay_addr(y) for disjoint x, y, and strides map bit by bit).
+// (Unrolled over all eight segments with constant indices, so a kernel keeps
+// the layout in scalar registers instead of indexing a private copy.)
 __host__ __device__ inline uint64_t lay_addr(uint64_t i, const LamLayout &L) {
     if (L.count == 0) {
         return i;
     }
     uint64_t r = 0;
-    for (int s = 0; s < L.count; ++s) {
-        r |= ((i >> L.src[s]) & ((uint64_t{1} << L.len[s]) - 1)) << L.dst[s];
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+        if (s < L.count) {
+            r |= ((i >> L.src[s]) & ((uint64_t{1} << L.len[s]) - 1)) << L.dst[s];
+        }
     }
     return r;
 }
@@ -340,11 +345,11 @@ constexpr int kTopL0Cols = 52;
 // lambda-only sweep of its wires, then, instead of storing lambda, the layer-0
 // contraction of its tiles (per-block partials: 2T layer-1 gradients in
 // partials, kTopL0Cols = 52 layer-0 values in l0_partials; per tile 4 wave
-// sums in tile_sums; phi_c: one value per tile, written first).
+// sums in tile_sums; phi_c and lam_tile: one value per tile, written first).
 void launch_bwd_top_l0(const double2 *lam_in, const WireCoef *layer_coef, const PassGeom &g,
                        double *partials, int grid, hipStream_t s, const CMat2 *gen_tab,
                        const CMat2 *gen_utab, const WireCoef *lc0, double2 *phi_c,
-                       double2 *tile_sums, double *l0_partials);
+                       uint64_t *lam_tile, double2 *tile_sums, double *l0_partials);
 // The fixed wires' environments from tile_sums and the layer-0 gradients (and
 // energy) from the fused pass's partials.
 void launch_layer0_top_finish(const double2 *tile_sums, uint64_t num_tiles, const double *blk_partials,

@@ -71,8 +71,9 @@ def fused_environments(lam, u, n):
     thread bits 0..7, register bits 8..11), lambda permuted to q, the register
     product contraction pk and environments env, the thread factor a_t and
     parity par_t, the tile factor Phi_C(g) over the fixed wires 4..h-1, per
-    thread sums zc[x] (x = j_3 ^ par_t) and benv, per tile S(g); then the
-    fixed wires' environments over the tiles."""
+    thread sums zc[x] (x = j_3 ^ par_t), the register wires' environments with
+    the thread factor applied per tile, per tile S(g); then the fixed wires'
+    environments over the tiles."""
     h = n - 9
     wire = [0, 1, 2] + list(range(h, n))  # wire of q bit b
     o = np.arange(1 << T)
@@ -109,8 +110,8 @@ def fused_environments(lam, u, n):
             phi_c *= u[v, ((base >> v) ^ (base >> (v - 1))) & 1]
         x = j3 ^ par_t
         zc[np.arange(256), x] += phi_c * pk
-        bgw = phi_c * u[3, x]
-        benv += bgw[:, None, None] * env
+        # register wires: a_t u_3(x) phi_c env, summed over the threads per tile
+        benv += (a_t * u[3, x] * phi_c)[:, None, None] * env
         S[g] = np.sum(a_t * u[3, x] * pk)
     E = np.zeros((n, 2), dtype=complex)
     # thread-held wires (q bits 0..7)
@@ -129,7 +130,7 @@ def fused_environments(lam, u, n):
     # register wires
     for e in range(4):
         for b in range(2):
-            E[wire[8 + e], b] = np.sum(a_t * benv[:, e, b])
+            E[wire[8 + e], b] = np.sum(benv[:, e, b])
     # fixed wires 4..h-1 from the tile sums
     g = np.arange(tiles)
     base = g << 3
