@@ -39,6 +39,10 @@ if str(ROOT) not in sys.path:
 HBM_PEAK_GBS = 8000.0
 METRIC = "energy_and_grad evals/s, % HBM roofline; batched vs single-launch @|\u0394E|\u22641e-10"
 TRAFFIC_FILE = ROOT / "profiles" / "hbm_traffic_structured_30q2l.json"
+# executed FP64 operations per launch of each kernel (rocprofv3 counter
+# SQ_INSTS_VALU_FLOPS_FP64), for the compute side of the dominant kernel
+FLOPS_FILE = ROOT / "profiles" / "fp64_flops_structured_30q2l.json"
+FP64_PEAK_TFLOPS = 78.6
 PARITY_TOL = 1e-10
 
 
@@ -465,7 +469,27 @@ def main() -> int:
         "bytes_per_launch": dom_bytes,
         "eval_achieved": round(total_bytes / args.steps / (ms_per_step / 1e3) / 1e9, 1),
         "state_passes_per_eval": round(total_bytes / args.steps / (16.0 * 2 ** n), 2),
+        # the state passes of the reference's minimum schedule (one pass per
+        # wire group and layer, 6L + 1 = 13 S at two layers) over this
+        # evaluation's time: the HBM rate a streaming schedule would need
+        "min_schedule_passes": 6 * layers + 1,
+        "min_schedule_frac": round((6 * layers + 1) * 16.0 * 2 ** n
+                                   / (ms_per_step / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
     }
+    # the dominant kernel's compute side: executed FP64 operations per launch
+    # (profile) over its live average launch time, against the FP64 peak
+    flops = None
+    if FLOPS_FILE.exists():
+        try:
+            flops = json.loads(FLOPS_FILE.read_text()).get(dom["name"])
+        except (ValueError, OSError):
+            flops = None
+    if flops:
+        tf = flops / dom_avg_s / 1e12
+        roofline["fp64"] = {"achieved": round(tf, 2), "peak": FP64_PEAK_TFLOPS,
+                            "unit": "TFLOP/s", "frac": round(tf / FP64_PEAK_TFLOPS, 4),
+                            "flops_per_launch": flops,
+                            "source": str(FLOPS_FILE.relative_to(ROOT))}
     result = {
         "metric": METRIC,
         "value": round(1e3 / ms_per_step, 4),
