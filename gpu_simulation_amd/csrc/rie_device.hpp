@@ -195,15 +195,17 @@ __device__ __forceinline__ double2 cmat_trace_mul(const CMat2 &p, const CMat2 &q
 // w and w + 1, and z = y_{n-1} closing the ring:
 //   M_0[q](z, x_0) = U_0[q, x_0 ^ z] u_0(x_0),
 //   M_w[q](x, y)   = U_w[q, y] u_w(y ^ x)   (w >= 1).
-// k0: the wire's layer-0 coefficients, k1: layer 1's; rot: w in S.
+// k0: the wire's layer-0 coefficients, k1: layer 1's; rot: w in S; rz: with
+// the wire's RZ (false: U_w = RY(theta_w) alone, the state with that wire's
+// phase e^{-+ i phi/2} divided out, as an adjoint pass uses it).
 __device__ __forceinline__ CMat2 chain_site(int w, int q, const WireCoef &k0, const WireCoef &k1,
-                                            bool rot) {
+                                            bool rot, bool rz = true) {
     const double2 u[2] = {u0_of(k0), u1_of(k0)};
     double2 U[2];  // U_w[q, 0], U_w[q, 1]
     if (rot) {
         // RZ(phi) RY(theta) = [[c e^-, -s e^-], [s e^+, c e^+]], e^-+ = e^{-+ i phi/2}
-        const double2 em = make_double2(k1.ch, -k1.sh);
-        const double2 ep = make_double2(k1.ch, k1.sh);
+        const double2 em = rz ? make_double2(k1.ch, -k1.sh) : make_double2(1.0, 0.0);
+        const double2 ep = rz ? make_double2(k1.ch, k1.sh) : make_double2(1.0, 0.0);
         if (q == 0) {
             U[0] = make_double2(k1.c * em.x, k1.c * em.y);
             U[1] = make_double2(-k1.s * em.x, -k1.s * em.y);

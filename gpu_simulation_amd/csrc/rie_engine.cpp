@@ -613,7 +613,7 @@ void Engine::structured_hamiltonian(hipStream_t s, bool write_lam) {
 
 void Engine::launch_bwd(int l, bool gather, bool write_cur, const double2 *cur_in,
                         const double2 *lam_in, double2 *cur_out, double2 *lam_out, size_t p,
-                        hipStream_t s, int gen_set, const PassGeom *geom) {
+                        hipStream_t s, int gen_set, const PassGeom *geom, bool phase_free) {
     const WireCoef *lc = d_coef + (size_t)l * n;
     const PassPlan &pp = passes[p];
     const PassGeom &pg = geom != nullptr ? *geom : pp.g;
@@ -633,7 +633,7 @@ void Engine::launch_bwd(int l, bool gather, bool write_cur, const double2 *cur_i
     const bool prefetch = double_buffer && !write_cur;
     launch_bwd_pass(pass_r[gather ? RIE_PASS_BWD_LOW : RIE_PASS_BWD_HIGH], gather, write_cur, true,
                     cur_in, lam_in, cur_out, lam_out, lc, pg, d_partials, pp.grid, s, pairs,
-                    pairs ? h : 0.0, gen_tab, gen_utab, prefetch);
+                    pairs ? h : 0.0, gen_tab, gen_utab, prefetch, phase_free);
 }
 
 // The generated start's lambda layouts.  The high passes run either from the
@@ -749,6 +749,9 @@ void Engine::build_chain_tables(hipStream_t s, int r_set0, bool bottom_up) {
         st.rot = bottom_up ? all & ~((uint64_t{1} << passes[p].first_wire) - 1)
                            : all & ~((uint64_t{1} << T) - 1) &
                                  ((uint64_t{1} << passes[p].end_wire) - 1);
+        // from the lowest group up a pass takes its own wires' phases off
+        // lambda once per tile, so its state is generated without them
+        st.ry_only = bottom_up ? st.rot & ((uint64_t{1} << passes[p].end_wire) - 1) : 0;
         st.c0_first = 0;
         st.c0_len = g.a;
         st.c1_first = g.h0;
@@ -886,7 +889,7 @@ void Engine::structured_eval_gen_start(hipStream_t s, const double *scale, doubl
             require_partials((size_t)pp.grid * 2 * T);
             kt_begin(s);
             launch_bwd(1, false, false, nullptr, lam, nullptr, lam_other, (size_t)(1 + j), s, 1 + j,
-                       &gj);
+                       &gj, bottom_up);
             check_launch("bwd_pass");
             kt_end(s, "bwd_pass_high_gen", 2 * S);
             std::swap(lam, lam_other);

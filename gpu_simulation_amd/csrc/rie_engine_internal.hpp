@@ -206,7 +206,8 @@ struct Engine {
                                bool staged);
     void launch_bwd(int l, bool gather, bool write_cur, const double2 *cur_in,
                     const double2 *lam_in, double2 *cur_out, double2 *lam_out, size_t p,
-                    hipStream_t s, int gen_set = -1, const PassGeom *geom = nullptr);
+                    hipStream_t s, int gen_set = -1, const PassGeom *geom = nullptr,
+                    bool phase_free = false);
     bool gen_ready() const;
     bool gen_start_ready() const;
     void build_chain_tables(hipStream_t s, int r_set0 = -1, bool bottom_up = false);

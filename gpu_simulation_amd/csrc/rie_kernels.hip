@@ -580,7 +580,8 @@ __global__ __launch_bounds__(256) void chain_tables_kernel(const WireCoef *__res
     const int n = cs.n;
     for (int i = (int)threadIdx.x; i < 2 * n; i += (int)blockDim.x) {
         const int w = i >> 1;
-        site[i] = chain_site(w, i & 1, lc0[w], lc1[w], ((st.rot >> w) & 1u) != 0);
+        site[i] = chain_site(w, i & 1, lc0[w], lc1[w], ((st.rot >> w) & 1u) != 0,
+                             ((st.ry_only >> w) & 1u) == 0);
     }
     __syncthreads();
     CMat2 *tab = tabs + (size_t)blockIdx.x * kChainTabEntries;
@@ -637,7 +638,8 @@ __global__ __launch_bounds__(256) void chain_uniform_kernel(const WireCoef *__re
     const int n = cs.n;
     for (int i = (int)threadIdx.x; i < 2 * n; i += (int)blockDim.x) {
         const int w = i >> 1;
-        site[i] = chain_site(w, i & 1, lc0[w], lc1[w], ((st.rot >> w) & 1u) != 0);
+        site[i] = chain_site(w, i & 1, lc0[w], lc1[w], ((st.rot >> w) & 1u) != 0,
+                             ((st.ry_only >> w) & 1u) == 0);
     }
     __syncthreads();
     const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -673,7 +675,8 @@ __global__ __launch_bounds__(256) void chain_flip_kernel(const WireCoef *__restr
     const int n = cs.n;
     for (int i = (int)threadIdx.x; i < 2 * n; i += (int)blockDim.x) {
         const int w = i >> 1;
-        site[i] = chain_site(w, i & 1, lc0[w], lc1[w], ((st.rot >> w) & 1u) != 0);
+        site[i] = chain_site(w, i & 1, lc0[w], lc1[w], ((st.rot >> w) & 1u) != 0,
+                             ((st.ry_only >> w) & 1u) == 0);
     }
     __syncthreads();
     const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1092,7 +1095,10 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_pa
 // the gradients, and cur never gets RY^dagger; a second sweep applies
 // RY^dagger to lam alone.  Against the one-sweep pass: no RY on cur (8 of
 // 28 FP64 operations per amplitude pair and wire), two more exchanges of lam.
-template <int R, int M>
+// kPhased = false: the window's RZ phases are already divided out of both
+// arrays (a state generated without them and lambda multiplied by the tile's
+// combined phase, apply_tile_phase), so the window takes its gradients only.
+template <int R, int M, bool kPhased = true>
 __device__ __forceinline__ void grad_window(double2 (&c)[Tile<R>::REGS],
                                             double2 (&l)[Tile<R>::REGS],
                                             const WireCoef *__restrict__ lc, const PassGeom &G,
@@ -1109,10 +1115,12 @@ __device__ __forceinline__ void grad_window(double2 (&c)[Tile<R>::REGS],
     }
 #pragma unroll
     for (int q = 0; q < Tile<R>::REGS; ++q) {
-        const double2 z = rz_tab[M * Tile<R>::REGS + q];
-        const double2 zc = make_double2(z.x, -z.y);
-        c[q] = cmul(c[q], zc);
-        l[q] = cmul(l[q], zc);
+        if constexpr (kPhased) {
+            const double2 z = rz_tab[M * Tile<R>::REGS + q];
+            const double2 zc = make_double2(z.x, -z.y);
+            c[q] = cmul(c[q], zc);
+            l[q] = cmul(l[q], zc);
+        }
         const double im = l[q].x * c[q].y - l[q].y * c[q].x;
 #pragma unroll
         for (int rb = 0; rb < R; ++rb) {
@@ -1174,7 +1182,7 @@ __device__ __forceinline__ double2 lane_xor8(double2 v) {
 
 // RZ^dagger of tile bit 3 on both states (window 1's phase table at register
 // index 8 or 0), then its gradients as in grad_window.
-template <int R>
+template <int R, bool kPhased = true>
 __device__ __forceinline__ void grad_lane_bit3(double2 (&c)[Tile<R>::REGS],
                                                double2 (&l)[Tile<R>::REGS],
                                                double *__restrict__ row_acc, int lane,
@@ -1186,8 +1194,10 @@ __device__ __forceinline__ void grad_lane_bit3(double2 (&c)[Tile<R>::REGS],
     double dp = 0.0;
 #pragma unroll
     for (int q = 0; q < Tile<R>::REGS; ++q) {
-        c[q] = cmul(c[q], zc);
-        l[q] = cmul(l[q], zc);
+        if constexpr (kPhased) {
+            c[q] = cmul(c[q], zc);
+            l[q] = cmul(l[q], zc);
+        }
         const double im = l[q].x * c[q].y - l[q].y * c[q].x;
         dp += hi ? -im : im;
         // bit set: Re(l1* c0); clear: -Re(l0* c1)
@@ -1276,13 +1286,34 @@ __device__ __forceinline__ void load_lam_high(double2 (&l)[Tile<R>::REGS],
     }
 }
 
+// lambda of a high pass's tile (mapping 0, R = 4) times the conjugate of the
+// combined RZ phase of every rotated tile bit: window 0's (register bits
+// 8..11), window 1's (thread bits 0..3) and window 2's (thread bits 4..7)
+// factors from rz_tab.  With a state generated without those phases, the
+// windows then take their gradients with no phase work (kPhased = false).
+template <int R>
+__device__ __forceinline__ void apply_tile_phase(double2 (&l)[Tile<R>::REGS], const double2 *rz_tab,
+                                                 uint32_t t) {
+    static_assert(R == 4, "three register windows: bits 8..11, 0..3, 4..7");
+    const double2 pt = cmul(rz_tab[Tile<R>::REGS + (t & 15u)],
+                            rz_tab[2 * Tile<R>::REGS + ((t >> 4) & 15u)]);
+#pragma unroll
+    for (int k = 0; k < Tile<R>::REGS; ++k) {
+        const double2 z = cmul(rz_tab[k], pt);
+        l[k] = cmul(l[k], make_double2(z.x, -z.y));
+    }
+}
+
 // kPrefetch: once the tile's state is no longer needed (after its
 // gradients), the registers it held receive the workgroup's next tile --
 // lam for a high pass, the state for the low (gather) pass -- so that load is
 // in flight during the lam-only sweep, the store and the next tile's start
 // instead of stalling it (a register double buffer: the state and the
 // prefetched array share registers).
-template <int R, bool kGather, bool kHam, bool kGen, bool kPrefetch = false>
+// kPhaseFree (generated high passes, R = 4): the table set generates the
+// state without the RZ of the pass's own wires, so lambda takes the tile's
+// combined phase once (apply_tile_phase) and the windows none.
+template <int R, bool kGather, bool kHam, bool kGen, bool kPrefetch = false, bool kPhaseFree = false>
 __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_lam_kernel(
     const double2 *cur_in, const double2 *lam_in, double2 *lam_out,
     const WireCoef *__restrict__ lc, PassGeom G, double *__restrict__ partials, double field,
@@ -1323,6 +1354,9 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_la
     const TileAddr ta_in = lay_tile_addr(ta, kGather ? LamLayout{} : G.lay_in);
     const TileAddr ta_out = lay_tile_addr(ta, kGather ? LamLayout{} : G.lay_out);
     static_assert(!(kPrefetch && kGather && kGen), "the gather pass reads the state");
+    static_assert(!kPhaseFree || (kGen && !kGather && !kHam && TL::P == 3),
+                  "phase-free sweep: generated high passes of register shape 4 without pairs");
+    constexpr bool kPhased = !kPhaseFree;
 
     double2 x_next[kPrefetch ? TL::REGS : 1];
     auto prefetch_tile = [&](uint64_t gn, const TileAddr &a, uint32_t tt) {
@@ -1443,7 +1477,10 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_la
             }
         } else {
             constexpr uint32_t w0 = window_mask<R, 0>();
-            grad_window<R, 0>(c, l, lc, G, row_acc, lane, rz_tab);
+            if constexpr (kPhaseFree) {
+                apply_tile_phase<R>(l, rz_tab, tl);
+            }
+            grad_window<R, 0, kPhased>(c, l, lc, G, row_acc, lane, rz_tab);
             if constexpr (TL::P == 3) {
                 // windows 0 and 2, tile bit 3 across lanes (window 1 has no
                 // other rotated bit: its pairs with bit 4 were added at window
@@ -1454,10 +1491,10 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_la
                 if constexpr (kHam) {
                     add_pair_terms<R, 2>(c, l, lds, tl, G, w0, field);
                 }
-                grad_window<R, 2>(c, l, lc, G, row_acc, lane, rz_tab);
+                grad_window<R, 2, kPhased>(c, l, lc, G, row_acc, lane, rz_tab);
                 // unconditional: an unrotated bit 3 has phase 1 and the
                 // identity in tile_coef, and its gradient slot is not read
-                grad_lane_bit3<R>(c, l, row_acc, lane, rz_tab);
+                grad_lane_bit3<R, kPhased>(c, l, row_acc, lane, rz_tab);
                 prefetch_next();
                 ry_inv_window<R, 2>(l, tile_coef);
                 ry_inv_lane_bit3<R>(l, tile_coef[3], lane);
@@ -1698,7 +1735,7 @@ static void launch_bwd_pass_r(bool gather, bool write_cur, bool write_lam, const
                               const double2 *lam_in, double2 *cur_out, double2 *lam_out,
                               const WireCoef *lc, const PassGeom &g, double *partials, int grid,
                               hipStream_t s, bool ham, double field, const CMat2 *gen_tab,
-                              const CMat2 *gen_utab, bool prefetch) {
+                              const CMat2 *gen_utab, bool prefetch, bool phase_free) {
     constexpr int TH = Tile<R>::THREADS;
     const bool gen = gen_tab != nullptr;
     if (gen && (gen_utab == nullptr || gather || g.a > 4 || Tile<R>::T - R - g.a > 6 ||
@@ -1715,10 +1752,27 @@ static void launch_bwd_pass_r(bool gather, bool write_cur, bool write_lam, const
             throw std::invalid_argument("lambda layouts: pass or geometry outside their scope");
         }
     }
+    if (phase_free && (R != 4 || gather || !gen || ham || write_cur || !write_lam)) {
+        // the tables leave the pass's own phases out only for this sweep
+        throw std::invalid_argument(
+            "phase-free sweep: generated lambda-only high pass of shape 4 without pair terms");
+    }
     if (!write_cur && write_lam) {  // the state is not rewritten: lambda-only sweep
         if (!gather && R == 4 && (g.a != 3 || g.rot_lo < 3)) {
             // the sweep takes tile bit 3 across lanes and no other bit of window 1
             throw std::invalid_argument("high pass geometry: tile bits 0..2 must ride along");
+        }
+        if constexpr (R == 4) {
+            if (phase_free && prefetch) {
+                bwd_lam_kernel<R, false, false, true, true, true><<<grid, TH, 0, s>>>(
+                    cur_in, lam_in, lam_out, lc, g, partials, field, gen_tab, gen_utab);
+                return;
+            }
+            if (phase_free) {
+                bwd_lam_kernel<R, false, false, true, false, true><<<grid, TH, 0, s>>>(
+                    cur_in, lam_in, lam_out, lc, g, partials, field, gen_tab, gen_utab);
+                return;
+            }
         }
 
 #define RIE_BWD_LAM(GA, HM, GN, PF)                                                          \
@@ -1769,15 +1823,15 @@ void launch_bwd_pass(int R, bool gather, bool write_cur, bool write_lam, const d
                      const double2 *lam_in, double2 *cur_out, double2 *lam_out,
                      const WireCoef *layer_coef, const PassGeom &g, double *partials, int grid,
                      hipStream_t s, bool ham_pairs, double field, const CMat2 *gen_tab,
-                     const CMat2 *gen_utab, bool prefetch) {
+                     const CMat2 *gen_utab, bool prefetch, bool phase_free) {
     if (R == 4) {
         launch_bwd_pass_r<4>(gather, write_cur, write_lam, cur_in, lam_in, cur_out, lam_out,
                              layer_coef, g, partials, grid, s, ham_pairs, field, gen_tab,
-                             gen_utab, prefetch);
+                             gen_utab, prefetch, phase_free);
     } else {
         launch_bwd_pass_r<3>(gather, write_cur, write_lam, cur_in, lam_in, cur_out, lam_out,
                              layer_coef, g, partials, grid, s, ham_pairs, field, gen_tab,
-                             gen_utab, prefetch);
+                             gen_utab, prefetch, phase_free);
     }
 }
 
@@ -2698,12 +2752,14 @@ __global__ __launch_bounds__(256, 2) void bwd_top_l0_kernel(
         bwd_load_tile<R, false, false, true>(c, l, nullptr, lam_in, base, ta, G, lds, tl, 0.0,
                                              gen_tab, gen_chunks, gen_utab + 2 * g, lam_tile[g],
                                              &ta_in);
-        // the sweep of the lambda-only high pass (bwd_lam_kernel, R = 4)
-        grad_window<R, 0>(c, l, lc, G, row_acc, lane, rz_tab);
+        // the sweep of the lambda-only high pass (bwd_lam_kernel, R = 4), the
+        // state generated without the group's phases
+        apply_tile_phase<R>(l, rz_tab, tl);
+        grad_window<R, 0, false>(c, l, lc, G, row_acc, lane, rz_tab);
         exchange<R, 0, 2>(l, lds, tl);
         exchange<R, 0, 2>(c, lds, tl);
-        grad_window<R, 2>(c, l, lc, G, row_acc, lane, rz_tab);
-        grad_lane_bit3<R>(c, l, row_acc, lane, rz_tab);
+        grad_window<R, 2, false>(c, l, lc, G, row_acc, lane, rz_tab);
+        grad_lane_bit3<R, false>(c, l, row_acc, lane, rz_tab);
         ry_inv_window<R, 2>(l, tile_coef);
         ry_inv_lane_bit3<R>(l, tile_coef[3], lane);
         exchange<R, 2, 0>(l, lds, tl);

@@ -246,6 +246,7 @@ struct ChainSet {
     int r0;
     int R;
     uint64_t rot;
+    uint64_t ry_only;  // rotated wires generated without their RZ (chain_site)
     int c0_first;  // chunk C0: wires [c0_first, c0_first + c0_len)
     int c0_len;
     int c1_first;  // chunk C1
@@ -310,7 +311,7 @@ void launch_bwd_pass(int R, bool gather, bool write_cur, bool write_lam, const d
                      const WireCoef *layer_coef, const PassGeom &g, double *partials, int grid,
                      hipStream_t s, bool ham_pairs = false, double field = 0.0,
                      const CMat2 *gen_tab = nullptr, const CMat2 *gen_utab = nullptr,
-                     bool prefetch = false);
+                     bool prefetch = false, bool phase_free = false);
 
 // Hamiltonian tile pass: adds the X terms of the pass's wires (and the ZZ
 // diagonal if diag) of H psi, optionally into lam, and writes per-workgroup

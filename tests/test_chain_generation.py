@@ -27,12 +27,13 @@ def _coefs(layer):
     return np.cos(th / 2), np.sin(th / 2), np.cos(ph / 2), np.sin(ph / 2)
 
 
-def _site(w, q, l0, l1, rot):
+def _site(w, q, l0, l1, rot, rz=True):
+    """rz = False: the wire's RY alone (the RZ phase left out of the state)."""
     c0, s0, ch0, sh0 = (x[w] for x in l0)
     u = (c0 * (ch0 - 1j * sh0), s0 * (ch0 + 1j * sh0))
     if rot:
         c, s, ch, sh = (x[w] for x in l1)
-        em, ep = ch - 1j * sh, ch + 1j * sh
+        em, ep = (ch - 1j * sh, ch + 1j * sh) if rz else (1.0, 1.0)
         U = np.array([[c * em, -s * em], [s * ep, c * ep]])
     else:
         U = np.eye(2)
@@ -79,6 +80,80 @@ def test_chain_trace_equals_layer1_states(oracle, n):
         if rot == full:
             # every wire of layer 1 rotated: P^-1 of the two-layer state
             assert np.max(np.abs(got - phi)) <= 1e-14
+
+
+@pytest.mark.parametrize("n", [4, 5, 7])
+def test_chain_trace_without_the_rz_of_a_wire_subset(oracle, n):
+    # a high pass's table set leaves the RZ of the pass's own wires out: the
+    # trace is then the layer-1 state with those wires' phases divided out
+    rng = np.random.default_rng(60 + n)
+    params = rng.uniform(0.0, 2.0 * np.pi, size=(2, n, 2))
+    l0, l1 = _coefs(params[0]), _coefs(params[1])
+    full = (1 << n) - 1
+    idx = np.arange(1 << n)
+    for rot, ry in ((full, 0b110), (full & ~1, full & ~1), (full & ~3, 0b1100), (full, full)):
+        ph = params[1, :, 1]
+        div = np.ones(1 << n, complex)
+        for w in range(n):
+            if (ry >> w) & 1:
+                div *= np.where((idx >> w) & 1, np.exp(0.5j * ph[w]), np.exp(-0.5j * ph[w]))
+        want = _rotated_reference(oracle, n, params, rot) / div
+        got = np.empty(1 << n, complex)
+        for i in range(1 << n):
+            m = np.eye(2)
+            for w in range(n):
+                m = m @ _site(w, (i >> w) & 1, l0, l1, (rot >> w) & 1, not (ry >> w) & 1)
+            got[i] = np.trace(m)
+        assert np.max(np.abs(got - want)) <= 1e-14, (rot, ry)
+
+
+def _sweep_grads(c, l, phases, order):
+    """The gradient inputs a lambda-only sweep takes per rotated tile bit b:
+    sum_x (-1)^{x_b} Im(conj(l[x]) c[x]) and sum_{x_b = 0} Re(conj(l[x | b])
+    c[x]) - Re(conj(l[x]) c[x | b]), after the RZ^dagger of the bit groups in
+    ``order`` (applied to both arrays group by group; ``phases[b]`` the bit's
+    e^{-i phi/2}, e^{+i phi/2}).  Returns the per-bit values and the final l."""
+    c, l = c.copy(), l.copy()
+    x = np.arange(c.size)
+    out = {}
+    for group in order:
+        for b in group:
+            z = np.where((x >> b) & 1, phases[b][1], phases[b][0])
+            c *= np.conj(z)
+            l *= np.conj(z)
+        for b in group:
+            sgn = np.where((x >> b) & 1, -1.0, 1.0)
+            im = np.sum(sgn * (np.conj(l) * c).imag)
+            lo = x[((x >> b) & 1) == 0]
+            hi = lo | (1 << b)
+            re = np.sum((np.conj(l[hi]) * c[lo]).real - (np.conj(l[lo]) * c[hi]).real)
+            out[b] = (im, re)
+    return out, l
+
+
+def test_phase_free_sweep_equals_the_phased_one():
+    # the state generated without the pass's phases and lambda times the
+    # conjugate of the tile's combined phase (once) give the gradients and the
+    # lambda of the window-by-window RZ^dagger (window 0: tile bits 8..11,
+    # window 2: bits 4..7, then bit 3; bits 0..2 ride along unrotated)
+    rng = np.random.default_rng(91)
+    x = np.arange(1 << T)
+    c_true = rng.normal(size=1 << T) + 1j * rng.normal(size=1 << T)
+    l = rng.normal(size=1 << T) + 1j * rng.normal(size=1 << T)
+    phi = rng.uniform(0.0, 2.0 * np.pi, T)
+    phases = {b: (np.exp(-0.5j * phi[b]), np.exp(0.5j * phi[b])) for b in range(3, T)}
+    order = [range(8, 12), range(4, 8), [3]]
+    want, l_want = _sweep_grads(c_true, l, phases, order)
+    tile_phase = np.ones(1 << T, complex)
+    for b in range(3, T):
+        tile_phase *= np.where((x >> b) & 1, phases[b][1], phases[b][0])
+    c_gen = c_true / tile_phase
+    # the phase-free windows take the same gradient formulas with no phase work
+    got, l_got = _sweep_grads(c_gen, l * np.conj(tile_phase),
+                              {b: (1.0, 1.0) for b in range(3, T)}, order)
+    for b in range(3, T):
+        assert np.allclose(got[b], want[b], rtol=0, atol=1e-10), b
+    assert np.max(np.abs(l_got - l_want)) <= 1e-13
 
 
 def _omap(R, M, t, k):
