@@ -258,6 +258,46 @@ __device__ __forceinline__ double row_sum(double v) {
     return v;
 }
 
+// Row sums of eight values at once (transpose and add): a lane holds u[m],
+// the pair sum over lanes {s, s ^ 1} of v[2m + (s & 1)] -- the first step,
+// row_sums8_first, run by the caller as each pair of values is ready -- and
+// row_sums8_rest halves the values twice more (partner lane ^ 2 by
+// quad_perm [2,3,0,1], then the lane 4 along by row_ror:4, which keeps lane
+// bits 0 and 1) and adds the lane 8 along (row_ror:8).  Lane s of a row
+// returns the sum over the row's 16 lanes of v[s & 7]: 52 VALU instructions
+// against 96 for eight row_sum calls.
+__device__ __forceinline__ double row_sums8_first(double v_even, double v_odd, bool b0) {
+    const double keep = b0 ? v_odd : v_even;
+    const double send = b0 ? v_even : v_odd;
+    return keep + dpp_move<0xB1>(send);
+}
+
+__device__ __forceinline__ double row_sums8_rest(const double (&u)[4], int lane) {
+    const bool b1 = (lane & 2) != 0;
+    const bool b2 = (lane & 4) != 0;
+    double w[2];
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+        const double keep = b1 ? u[2 * p + 1] : u[2 * p];
+        const double send = b1 ? u[2 * p] : u[2 * p + 1];
+        w[p] = keep + dpp_move<0x4E>(send);
+    }
+    const double keep = b2 ? w[1] : w[0];
+    const double send = b2 ? w[0] : w[1];
+    const double x = keep + dpp_move<0x124>(send);
+    return x + dpp_move<0x128>(x);
+}
+
+// Row sums of two values: lane s of a row returns the sum over the row's 16
+// lanes of v0 (s even) or v1 (s odd).
+__device__ __forceinline__ double row_sums2(double v0, double v1, int lane) {
+    double x = row_sums8_first(v0, v1, (lane & 1) != 0);
+    x += dpp_move<0x4E>(x);
+    x += dpp_move<0x124>(x);
+    x += dpp_move<0x128>(x);
+    return x;
+}
+
 // Sum over the 64 lanes of a (fully active) wave, returned to every lane in a
 // fixed order: the four row sums read from lanes 0, 16, 32, 48.
 __device__ __forceinline__ double wave_sum(double v) {

@@ -1108,6 +1108,73 @@ __device__ __forceinline__ void grad_window(double2 (&c)[Tile<R>::REGS],
     // no early exit for a window without rotated bits: its phases are 1 and
     // its gradients are not read, and branch-free code keeps the register
     // arrays free of copies at control-flow joins
+    if constexpr (R == 4) {
+        // dps[b] = sum_q (-1)^{q_b} im_q from the partial sums of each quad
+        // of registers (q_3 q_2 fixed): 40 adds against 64
+        double dps[4];
+#pragma unroll
+        for (int x = 0; x < 4; ++x) {
+            double im[4];
+#pragma unroll
+            for (int y = 0; y < 4; ++y) {
+                const int q = 4 * x + y;
+                if constexpr (kPhased) {
+                    const double2 z = rz_tab[M * Tile<R>::REGS + q];
+                    const double2 zc = make_double2(z.x, -z.y);
+                    c[q] = cmul(c[q], zc);
+                    l[q] = cmul(l[q], zc);
+                }
+                im[y] = l[q].x * c[q].y - l[q].y * c[q].x;
+            }
+            const double a0 = im[0] + im[1];
+            const double a1 = im[2] + im[3];
+            const double d = (im[0] - im[1]) + (im[2] - im[3]);
+            const double s = a0 + a1;
+            const double e = a0 - a1;
+            if (x == 0) {
+                dps[0] = d;
+                dps[1] = e;
+                dps[2] = s;
+                dps[3] = s;
+            } else {
+                dps[0] += d;
+                dps[1] += e;
+                dps[2] += (x & 1) ? -s : s;
+                dps[3] += (x & 2) ? -s : s;
+            }
+        }
+        // dt per bit, each (dt, dp) pair folded into the first step of the
+        // transposed row sums at once
+        const bool b0 = (lane & 1) != 0;
+        double u[4];
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb) {
+            const int j = P0 + rb;
+            double dt = 0.0;
+            if (j >= G.rot_lo && j < G.rot_hi) {
+#pragma unroll
+                for (int q = 0; q < Tile<R>::REGS; ++q) {
+                    if (!(q & (1 << rb))) {
+                        const double2 c0 = c[q];
+                        const double2 c1 = c[q | (1 << rb)];
+                        const double2 l0 = l[q];
+                        const double2 l1 = l[q | (1 << rb)];
+                        dt += (l1.x * c0.x + l1.y * c0.y) - (l0.x * c1.x + l0.y * c1.y);
+                    }
+                }
+            }
+            u[rb] = row_sums8_first(dt, dps[rb], b0);
+        }
+        // lane s (s < 8) of each row holds slot 2 * P0 + s: bit P0 + s / 2,
+        // dt for s even, dp for s odd.  The slots of unrotated bits are
+        // accumulated too and never read (reduce_grad takes the columns of
+        // [rot_lo, rot_hi) only).
+        const double r = row_sums8_rest(u, lane);
+        if ((lane & 15) < 8) {
+            row_acc[2 * P0 + (lane & 7)] += r;
+        }
+        return;
+    }
     double dps[R];
 #pragma unroll
     for (int rb = 0; rb < R; ++rb) {
@@ -1205,11 +1272,9 @@ __device__ __forceinline__ void grad_lane_bit3(double2 (&c)[Tile<R>::REGS],
         const double re = l[q].x * cp.x + l[q].y * cp.y;
         dt += hi ? re : -re;
     }
-    dt = row_sum(dt);
-    dp = row_sum(dp);
-    if ((lane & 15) == 0) {
-        row_acc[2 * 3] += dt;
-        row_acc[2 * 3 + 1] += dp;
+    const double r = row_sums2(dt, dp, lane);
+    if ((lane & 15) < 2) {
+        row_acc[2 * 3 + (lane & 1)] += r;
     }
 }
 
