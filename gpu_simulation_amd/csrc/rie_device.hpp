@@ -138,6 +138,34 @@ __device__ __forceinline__ void ry_inv_pair(double2 &a0, double2 &a1, const Wire
     a0.y = b0y;
 }
 
+// RY^dagger in tangent form: [[c, s], [-s, c]] = c [[1, t], [-t, 1]] with
+// t = s / c (form A, |c| >= |s|) or s [[t, 1], [-1, t]] with t = c / s
+// (form B): one FMA per output component instead of a product and an FMA,
+// the factor c or s taken out (a pass collects the factors of its wires into
+// one scale and applies it once).  |t| <= 1 in both forms.
+__device__ __forceinline__ void ry_inv_pair_tan_a(double2 &a0, double2 &a1, double t) {
+    const double x0 = fma(t, a1.x, a0.x);
+    const double y0 = fma(t, a1.y, a0.y);
+    a1.x = fma(-t, a0.x, a1.x);
+    a1.y = fma(-t, a0.y, a1.y);
+    a0.x = x0;
+    a0.y = y0;
+}
+
+__device__ __forceinline__ void ry_inv_pair_tan_b(double2 &a0, double2 &a1, double t) {
+    const double x0 = fma(t, a0.x, a1.x);
+    const double y0 = fma(t, a0.y, a1.y);
+    a1.x = fma(t, a1.x, -a0.x);
+    a1.y = fma(t, a1.y, -a0.y);
+    a0.x = x0;
+    a0.y = y0;
+}
+
+// The tangent form's choice for one wire: form B where |s| > |c|.
+__device__ __forceinline__ bool ry_tan_form_b(const WireCoef &k) {
+    return fabs(k.s) > fabs(k.c);
+}
+
 // Components of U|0> for one wire: u0 = c e^{-i phi/2}, u1 = s e^{+i phi/2}
 // (reference: cpp/ising_cuda_rotation_kernels.cu:686-703).
 __device__ __forceinline__ double2 u0_of(const WireCoef &k) {

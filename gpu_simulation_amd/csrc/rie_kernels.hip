@@ -1238,6 +1238,73 @@ __device__ __forceinline__ void ry_inv_window(double2 (&l)[Tile<R>::REGS],
     }
 }
 
+// RY^dagger of a window in tangent form (ry_inv_pair_tan_a / _b): tan_t[j]
+// the tangent of tile bit j (0 for an unrotated bit: the identity, factor 1),
+// bit j of form_b set where that wire takes form B.  form_b is uniform, so
+// each wire's choice is a scalar branch.
+template <int R, int M>
+__device__ __forceinline__ void ry_inv_window_tan(double2 (&l)[Tile<R>::REGS],
+                                                  const double *tan_t, uint32_t form_b) {
+    constexpr int P0 = phase_first_bit<R, M>();
+#pragma unroll
+    for (int rb = 0; rb < R; ++rb) {
+        const double tt = tan_t[P0 + rb];
+        if ((form_b >> (P0 + rb)) & 1u) {
+#pragma unroll
+            for (int q = 0; q < Tile<R>::REGS; ++q) {
+                if (!(q & (1 << rb))) {
+                    ry_inv_pair_tan_b(l[q], l[q | (1 << rb)], tt);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < Tile<R>::REGS; ++q) {
+                if (!(q & (1 << rb))) {
+                    ry_inv_pair_tan_a(l[q], l[q | (1 << rb)], tt);
+                }
+            }
+        }
+    }
+}
+
+// The tangent form's tables for the tile bits [lo, T) of a pass (tile_coef
+// by tile bit, the identity outside the rotated bits): thread j < T writes
+// tan_t[j] for bits in [lo, T) (bits below lo keep the full form); every
+// thread returns the uniform form mask, thread 0 writes the scale sigma (the
+// product of the factors taken out, |sigma| >= 2^(-(T - lo) / 2)) to *sigma.
+// tile_coef is read once the caller's barrier has published it; the caller
+// places a barrier after this call as well.
+template <int T>
+__device__ __forceinline__ uint32_t ry_tan_tables(const WireCoef *tile_coef, int lo,
+                                                  double *tan_t, double *sigma, uint32_t t) {
+    if (t < (uint32_t)T && (int)t >= lo) {
+        const WireCoef k = tile_coef[t];
+        tan_t[t] = ry_tan_form_b(k) ? k.c / k.s : k.s / k.c;
+    }
+    uint32_t form_b = 0;
+    double sg = 1.0;
+    for (int j = lo; j < T; ++j) {
+        const WireCoef k = tile_coef[j];
+        const bool fb = ry_tan_form_b(k);
+        form_b |= (fb ? 1u : 0u) << j;
+        sg *= fb ? k.s : k.c;
+    }
+    if (t == 0) {
+        *sigma = sg;
+    }
+    return __builtin_amdgcn_readfirstlane(form_b);
+}
+
+// lambda times the tangent form's scale
+template <int R>
+__device__ __forceinline__ void scale_regs(double2 (&l)[Tile<R>::REGS], double sg) {
+#pragma unroll
+    for (int k = 0; k < Tile<R>::REGS; ++k) {
+        l[k].x *= sg;
+        l[k].y *= sg;
+    }
+}
+
 // High passes: window 1 of the 256-thread shape holds tile bits 0..3, of which
 // only bit 3 can be a rotated wire (bits 0..2 are the contiguous low index
 // bits that ride along).  In mappings 0 and 2 tile bit 3 is lane bit 3, so
@@ -1392,6 +1459,10 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_la
     __shared__ double acc[ROWS][2 * T];
     __shared__ double2 rz_tab[TL::P * TL::REGS];
     __shared__ WireCoef tile_coef[T];
+    // phase-free sweep: RY^dagger of lambda's window bits (4..11) in tangent
+    // form, its scale applied once before the store
+    __shared__ double tan_t[kPhaseFree ? T : 1];
+    __shared__ double ry_sigma;
     // chunk products of the generating table set: read at every tile start,
     // where LDS latency is short next to L2 behind the streaming traffic
     __shared__ CMat2 gen_chunks[kGen ? kChainTabEntries - kChainC0 : 1];
@@ -1412,6 +1483,11 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_la
         }
     }
     __syncthreads();
+    uint32_t form_b = 0;
+    if constexpr (kPhaseFree) {
+        form_b = ry_tan_tables<T>(tile_coef, 4, tan_t, &ry_sigma, t);
+        __syncthreads();
+    }
 
     const TileAddr ta = tile_addr<R>(t, G);
     // lambda's layouts (high passes; the gather pass reads and writes the
@@ -1561,10 +1637,19 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_la
                 // identity in tile_coef, and its gradient slot is not read
                 grad_lane_bit3<R, kPhased>(c, l, row_acc, lane, rz_tab);
                 prefetch_next();
-                ry_inv_window<R, 2>(l, tile_coef);
+                if constexpr (kPhaseFree) {
+                    ry_inv_window_tan<R, 2>(l, tan_t, form_b);
+                } else {
+                    ry_inv_window<R, 2>(l, tile_coef);
+                }
                 ry_inv_lane_bit3<R>(l, tile_coef[3], lane);
                 exchange<R, 2, 0>(l, lds, tl);
-                ry_inv_window<R, 0>(l, tile_coef);
+                if constexpr (kPhaseFree) {
+                    ry_inv_window_tan<R, 0>(l, tan_t, form_b);
+                    scale_regs<R>(l, ry_sigma);
+                } else {
+                    ry_inv_window<R, 0>(l, tile_coef);
+                }
                 store_lam<R, 0, false>(l, lam_out, lay_addr(base, G.lay_out), ta_out, tl);
             } else {
                 uint32_t done = w0;
@@ -1657,6 +1742,10 @@ __global__ __launch_bounds__(Tile<R>::THREADS, low_gen_min_waves<R>()) void bwd_
     __shared__ double acc[ROWS][2 * T];
     __shared__ double2 rz_tab[TL::P * TL::REGS];
     __shared__ WireCoef tile_coef[T];
+    // RY^dagger of lambda in tangent form for the 4-register shape
+    constexpr bool kTan = R == 4;
+    __shared__ double tan_t[kTan ? T : 1];
+    __shared__ double ry_sigma;
     // chunk products C0 (index bits 0..3) and C1 (bits 4 .. T - R - 1): in LDS
     // for R = 4, read from the table in global memory (L2) for R = 3
     constexpr bool kChunksLds = R == 4;
@@ -1678,6 +1767,11 @@ __global__ __launch_bounds__(Tile<R>::THREADS, low_gen_min_waves<R>()) void bwd_
         }
     }
     __syncthreads();
+    uint32_t form_b = 0;
+    if constexpr (kTan) {
+        form_b = ry_tan_tables<T>(tile_coef, 0, tan_t, &ry_sigma, t);
+        __syncthreads();
+    }
     for (uint64_t g = blockIdx.x; g < G.num_tiles; g += gridDim.x) {
         uint32_t tl = t;
         asm volatile("" : "+v"(tl));
@@ -1741,12 +1835,14 @@ __global__ __launch_bounds__(Tile<R>::THREADS, low_gen_min_waves<R>()) void bwd_
         add_pair_terms<R, 2>(c, l, lds, tl, G, w0 | w1, field);
         grad_window<R, 2>(c, l, lc, G, row_acc, lane, rz_tab);
         if constexpr (TL::P == 3) {
-            // sweep 2: lambda back to mapping 0 under RY^dagger
-            ry_inv_window<R, 2>(l, tile_coef);
+            // sweep 2: lambda back to mapping 0 under RY^dagger (tangent
+            // form, its scale applied once before the store)
+            ry_inv_window_tan<R, 2>(l, tan_t, form_b);
             exchange<R, 2, 1>(l, lds, tl);
-            ry_inv_window<R, 1>(l, tile_coef);
+            ry_inv_window_tan<R, 1>(l, tan_t, form_b);
             exchange<R, 1, 0>(l, lds, tl);
-            ry_inv_window<R, 0>(l, tile_coef);
+            ry_inv_window_tan<R, 0>(l, tan_t, form_b);
+            scale_regs<R>(l, ry_sigma);
         } else {
             constexpr uint32_t w2 = window_mask<R, 2>();
             exchange<R, 2, 3>(l, lds, tl);
@@ -2758,6 +2854,10 @@ __global__ __launch_bounds__(256, 2) void bwd_top_l0_kernel(
     __shared__ double acc[ROWS][2 * T];
     __shared__ double2 rz_tab[TL::P * TL::REGS];
     __shared__ WireCoef tile_coef[T];
+    // RY^dagger of lambda's window bits (4..11) in tangent form, its scale
+    // applied once before the contraction
+    __shared__ double tan_t[T];
+    __shared__ double ry_sigma;
     __shared__ CMat2 gen_chunks[kChainTabEntries - kChainC0];
     // u_{n-4+j}(b) at 2j + b (the register wires), u_3(b) at 8 + b
     __shared__ double2 l0_u[10];
@@ -2782,6 +2882,8 @@ __global__ __launch_bounds__(256, 2) void bwd_top_l0_kernel(
         const int w = t < 8 ? n - 4 + (int)(t >> 1) : 3;
         l0_u[t] = u_of(lc0[w], (int)(t & 1u));
     }
+    __syncthreads();
+    const uint32_t form_b = ry_tan_tables<T>(tile_coef, 4, tan_t, &ry_sigma, t);
     __syncthreads();
 
     const TileAddr ta = tile_addr<R>(t, G);
@@ -2825,10 +2927,11 @@ __global__ __launch_bounds__(256, 2) void bwd_top_l0_kernel(
         exchange<R, 0, 2>(c, lds, tl);
         grad_window<R, 2, false>(c, l, lc, G, row_acc, lane, rz_tab);
         grad_lane_bit3<R, false>(c, l, row_acc, lane, rz_tab);
-        ry_inv_window<R, 2>(l, tile_coef);
+        ry_inv_window_tan<R, 2>(l, tan_t, form_b);
         ry_inv_lane_bit3<R>(l, tile_coef[3], lane);
         exchange<R, 2, 0>(l, lds, tl);
-        ry_inv_window<R, 0>(l, tile_coef);
+        ry_inv_window_tan<R, 0>(l, tan_t, form_b);
+        scale_regs<R>(l, ry_sigma);
         // lambda to the layer-0 coordinates q (mapping 0 of q: thread bits 0..7)
         const uint32_t cflip = (uint32_t)((base >> (h - 1)) & 1u) << 3;
         __syncthreads();
