@@ -1314,16 +1314,22 @@ __device__ __forceinline__ double2 lane_xor8(double2 v) {
     return make_double2(dpp_move<0x128>(v.x), dpp_move<0x128>(v.y));
 }
 
-// RZ^dagger of tile bit 3 on both states (window 1's phase table at register
-// index 8 or 0), then its gradients as in grad_window.
+// Tile bit 3: RZ^dagger on both states (window 1's phase table at register
+// index 8 or 0; none with kPhased = false), its gradients as in grad_window,
+// then RY^dagger of bit 3 on lam (before window 2's: they commute).  One lane
+// move of lam serves both: a pair's lanes hold (c0, l0) and (c1, l1), and
+// Re(l1* c0) - Re(l0* c1) is summed as Re(p* c) on the lane with bit 3 clear
+// and -Re(p* c) on the other, p the partner's lam, which is also the partner
+// value RY^dagger takes.
 template <int R, bool kPhased = true>
-__device__ __forceinline__ void grad_lane_bit3(double2 (&c)[Tile<R>::REGS],
-                                               double2 (&l)[Tile<R>::REGS],
-                                               double *__restrict__ row_acc, int lane,
-                                               const double2 *rz_tab) {
+__device__ __forceinline__ void grad_ry_lane_bit3(double2 (&c)[Tile<R>::REGS],
+                                                  double2 (&l)[Tile<R>::REGS],
+                                                  double *__restrict__ row_acc, int lane,
+                                                  const double2 *rz_tab, const WireCoef &k) {
     const bool hi = ((lane >> 3) & 1) != 0;
     const double2 z = rz_tab[Tile<R>::REGS + (hi ? 8 : 0)];
     const double2 zc = make_double2(z.x, -z.y);
+    const double s = hi ? -k.s : k.s;
     double dt = 0.0;
     double dp = 0.0;
 #pragma unroll
@@ -1334,27 +1340,15 @@ __device__ __forceinline__ void grad_lane_bit3(double2 (&c)[Tile<R>::REGS],
         }
         const double im = l[q].x * c[q].y - l[q].y * c[q].x;
         dp += hi ? -im : im;
-        // bit set: Re(l1* c0); clear: -Re(l0* c1)
-        const double2 cp = lane_xor8(c[q]);
-        const double re = l[q].x * cp.x + l[q].y * cp.y;
-        dt += hi ? re : -re;
+        const double2 p = lane_xor8(l[q]);
+        const double re = p.x * c[q].x + p.y * c[q].y;
+        dt += hi ? -re : re;
+        l[q].x = k.c * l[q].x + s * p.x;
+        l[q].y = k.c * l[q].y + s * p.y;
     }
     const double r = row_sums2(dt, dp, lane);
     if ((lane & 15) < 2) {
         row_acc[2 * 3 + (lane & 1)] += r;
-    }
-}
-
-// RY^dagger of tile bit 3 on lam (ry_inv_pair across the two lanes).
-template <int R>
-__device__ __forceinline__ void ry_inv_lane_bit3(double2 (&l)[Tile<R>::REGS],
-                                                 const WireCoef &k, int lane) {
-    const double s = ((lane >> 3) & 1) != 0 ? -k.s : k.s;
-#pragma unroll
-    for (int q = 0; q < Tile<R>::REGS; ++q) {
-        const double2 p = lane_xor8(l[q]);
-        l[q].x = k.c * l[q].x + s * p.x;
-        l[q].y = k.c * l[q].y + s * p.y;
     }
 }
 
@@ -1635,14 +1629,13 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_la
                 grad_window<R, 2, kPhased>(c, l, lc, G, row_acc, lane, rz_tab);
                 // unconditional: an unrotated bit 3 has phase 1 and the
                 // identity in tile_coef, and its gradient slot is not read
-                grad_lane_bit3<R, kPhased>(c, l, row_acc, lane, rz_tab);
+                grad_ry_lane_bit3<R, kPhased>(c, l, row_acc, lane, rz_tab, tile_coef[3]);
                 prefetch_next();
                 if constexpr (kPhaseFree) {
                     ry_inv_window_tan<R, 2>(l, tan_t, form_b);
                 } else {
                     ry_inv_window<R, 2>(l, tile_coef);
                 }
-                ry_inv_lane_bit3<R>(l, tile_coef[3], lane);
                 exchange<R, 2, 0>(l, lds, tl);
                 if constexpr (kPhaseFree) {
                     ry_inv_window_tan<R, 0>(l, tan_t, form_b);
@@ -1800,11 +1793,22 @@ __global__ __launch_bounds__(Tile<R>::THREADS, low_gen_min_waves<R>()) void bwd_
             for (int k = 0; k < TL::REGS; ++k) {
                 l[k] = make_double2(0.0, 0.0);
             }
+            // H's diagonal (ham_diag_pre_ring) of register k: the register
+            // bits T - R .. T - 1 (all >= 2) add pk = popcount(k) to
+            // popcount(i >> 2) and flip the ring term's parity pk times, so
+            // d = dpar[pk & 1] + 2 pk with two values per thread
+            const uint64_t i0 = base | row;
+            const int hi0 = (int)__popcll(i0 >> 2);
+            const int b0 = (int)(i0 & 1u);
+            const int e0 = b0 ^ (hi0 & 1);
+            const double dpar0 = (double)(2 * (hi0 + b0 + e0) - G.n);
+            const double dpar1 = (double)(2 * (hi0 + b0 + (e0 ^ 1)) - G.n);
 #pragma unroll
             for (int k = 0; k < TL::REGS; ++k) {
                 const CMat2 rk = tab[k];
                 c[k] = cmat_trace_mul(rk, y);
-                const double d = ham_diag_pre_ring(base | omap<R, 0>(tl, k), G.n);
+                const int pk = __builtin_popcount((unsigned)k);
+                const double d = ((pk & 1) ? dpar1 : dpar0) + (double)(2 * pk);
                 const double2 v1 = cmat_trace_mul(rk, ysum);
                 const double2 v0 = cmat_trace_mul(rk, y0);
                 l[k].x += d * c[k].x - field * v1.x;
@@ -2926,9 +2930,8 @@ __global__ __launch_bounds__(256, 2) void bwd_top_l0_kernel(
         exchange<R, 0, 2>(l, lds, tl);
         exchange<R, 0, 2>(c, lds, tl);
         grad_window<R, 2, false>(c, l, lc, G, row_acc, lane, rz_tab);
-        grad_lane_bit3<R, false>(c, l, row_acc, lane, rz_tab);
+        grad_ry_lane_bit3<R, false>(c, l, row_acc, lane, rz_tab, tile_coef[3]);
         ry_inv_window_tan<R, 2>(l, tan_t, form_b);
-        ry_inv_lane_bit3<R>(l, tile_coef[3], lane);
         exchange<R, 2, 0>(l, lds, tl);
         ry_inv_window_tan<R, 0>(l, tan_t, form_b);
         scale_regs<R>(l, ry_sigma);

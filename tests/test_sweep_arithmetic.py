@@ -10,6 +10,9 @@ against the direct forms it replaces.
   lanes; lane s of each row must end with the row sum of value s & 7.
 * The signed phase sums of a 4-register window from quad partials
   (``grad_window``).
+* The generating low pass's H diagonal from two values per thread.
+* Tile bit 3's gradient from the partner lane's lambda, the same lane move
+  that its RY^dagger takes (``grad_ry_lane_bit3``).
 
 The GPU kernels themselves are checked against the oracle in
 ``test_gpu_parity.py``; these tests pin the algebra on the CPU, and a planted
@@ -149,3 +152,52 @@ def test_phase_sums_from_quad_partials():
     k = np.arange(16)
     want = [np.sum(np.where((k >> b) & 1, -im, im)) for b in range(4)]
     assert np.allclose(dps, want, rtol=0, atol=1e-12)
+
+
+def _ham_diag_pre_ring(i, n):
+    """H's diagonal at P(i) from the pre-CNOT index i (ham_diag_pre_ring)."""
+    hi = bin(i >> 2).count("1")
+    b0 = i & 1
+    return 2 * (hi + (b0 ^ (hi & 1)) + b0) - n
+
+
+def test_diagonal_from_two_values_per_thread():
+    # the generating low pass forms H's diagonal of register k (index bits
+    # 8..11) from two per-thread values: d = dpar[popcount(k) & 1] + 2 popcount(k)
+    rng = np.random.default_rng(13)
+    for _ in range(2000):
+        n = int(rng.integers(13, 35))
+        i0 = int(rng.integers(0, 1 << n)) & ~(0xF << 8)
+        hi0 = bin(i0 >> 2).count("1")
+        b0 = i0 & 1
+        e0 = b0 ^ (hi0 & 1)
+        dpar = (2 * (hi0 + b0 + e0) - n, 2 * (hi0 + b0 + (e0 ^ 1)) - n)
+        for k in range(16):
+            pk = bin(k).count("1")
+            assert dpar[pk & 1] + 2 * pk == _ham_diag_pre_ring(i0 | (k << 8), n)
+
+
+def test_bit3_gradient_from_the_partner_lambda():
+    # tile bit 3 is lane bit 3: a pair's lanes hold (c0, l0) and (c1, l1).
+    # The sweep moves lam only: Re(p* c) on the lane with bit 3 clear and
+    # -Re(p* c) on the other (p the partner's lam) sum to the pair's
+    # Re(l1* c0) - Re(l0* c1), and p is what RY^dagger of bit 3 takes.
+    rng = np.random.default_rng(14)
+    c = rng.normal(size=(64, 16)) + 1j * rng.normal(size=(64, 16))
+    l = rng.normal(size=(64, 16)) + 1j * rng.normal(size=(64, 16))
+    lane = np.arange(64)
+    partner = (lane & ~15) | ((lane + 8) & 15)
+    hi = ((lane >> 3) & 1) != 0
+    p = l[partner]
+    re = (np.conj(p) * c).real
+    got = np.sum(np.where(hi[:, None], -re, re))
+    lo = lane[~hi]
+    want = np.sum((np.conj(l[lo | 8]) * c[lo]).real - (np.conj(l[lo]) * c[lo | 8]).real)
+    assert abs(got - want) <= 1e-11 * max(1.0, abs(want))
+    # RY^dagger across the lanes equals the pair rotation
+    th = 1.234
+    cc, ss = np.cos(th / 2), np.sin(th / 2)
+    s_lane = np.where(hi, -ss, ss)[:, None]
+    out = cc * l + s_lane * p
+    a, b = l[lo], l[lo | 8]
+    assert np.allclose(out[lo], cc * a + ss * b) and np.allclose(out[lo | 8], -ss * a + cc * b)
