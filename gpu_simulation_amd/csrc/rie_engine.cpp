@@ -238,6 +238,10 @@ Engine::Engine(size_t num_qubits, size_t num_layers, double field, const std::st
         // the tile's fixed-wire factor and lambda address (16 B each)
         tile_sum_count = L == 2 ? tiles * (Tile4Waves + 2) : tiles;
         d_tile_sums = (double2 *)device_alloc(sizeof(double2) * tile_sum_count, "alloc tile sums");
+        if (L == 2) {
+            // the generated start's per-tile lambda addresses (read and write)
+            d_lam_tab = (uint64_t *)device_alloc(sizeof(uint64_t) * 2 * tiles, "alloc tile table");
+        }
         d_chain = (CMat2 *)device_alloc(sizeof(CMat2) * kChainMaxSets * kChainTabEntries,
                                         "alloc chain tables");
         if (L >= 2 && n <= kChainMaxQubits && passes.size() <= (size_t)kChainMaxSets) {
@@ -284,6 +288,7 @@ Engine::~Engine() {
     device_free(d_epartials);
     device_free(d_drift);
     device_free(d_tile_sums);
+    device_free(d_lam_tab);
     device_free(d_chain);
     device_free(d_chain_u);
     device_free(d_chain_x);
@@ -842,7 +847,12 @@ void Engine::structured_eval_gen_start(hipStream_t s, const double *scale, doubl
     l0_energy_out = energy_out;
     try {
         require_partials((size_t)passes[0].grid * 2 * T);
+        if (d_lam_tab == nullptr) {
+            fail(RIE_DEVICE_ERROR, "internal: no lambda tile table");
+        }
         kt_begin(s);
+        launch_lam_tile_table(g0, g0.lay_out, d_lam_tab, s);
+        g0.tile_out = d_lam_tab;
         launch_bwd_low_gen(r_low, lam, d_coef + n, g0, d_partials, passes[0].grid, s, h, d_chain,
                            d_chain_u, d_chain_x);
         check_launch("bwd_pass low gen");
@@ -888,6 +898,13 @@ void Engine::structured_eval_gen_start(hipStream_t s, const double *scale, doubl
             }
             require_partials((size_t)pp.grid * 2 * T);
             kt_begin(s);
+            if (bottom_up) {
+                // the phase-free sweep reads its tiles' lambda addresses from tables
+                launch_lam_tile_table(gj, gj.lay_in, d_lam_tab, s);
+                launch_lam_tile_table(gj, gj.lay_out, d_lam_tab + pp.g.num_tiles, s);
+                gj.tile_in = d_lam_tab;
+                gj.tile_out = d_lam_tab + pp.g.num_tiles;
+            }
             launch_bwd(1, false, false, nullptr, lam, nullptr, lam_other, (size_t)(1 + j), s, 1 + j,
                        &gj, bottom_up);
             check_launch("bwd_pass");

@@ -137,6 +137,7 @@ struct Engine {
     double *d_drift = nullptr;
     double2 *d_tile_sums = nullptr;  // layer-0 contraction: per-tile sums (tile_sum_count)
     uint64_t tile_sum_count = 0;
+    uint64_t *d_lam_tab = nullptr;  // generated start: per-tile lambda addresses (2 x tiles)
     double *d_partials2 = nullptr;   // layer-0 contraction: high-wire partials
     int high_env_chunks = 1;
     size_t partial_count = 0;    // capacity of d_partials (doubles)

@@ -1499,7 +1499,9 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_la
             if constexpr (kGather) {
                 load_cur_gather<R>(x_next, cur_in, tile_base(gn, G, T), G, tt);
             } else {
-                load_lam_high<R>(x_next, lam_in, lay_addr(tile_base(gn, G, T), G.lay_in), a);
+                load_lam_high<R>(x_next, lam_in,
+                                 kPhaseFree ? G.tile_in[gn] : lay_addr(tile_base(gn, G, T), G.lay_in),
+                                 a);
             }
         }
     };
@@ -1539,7 +1541,9 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_la
         bwd_load_tile<R, kGather, kHam, kGen, kPrefetch>(c, l, cur_in, lam_in, base, tla, G, lds,
                                                          tl, field, gen_tab, gen_chunks,
                                                          gen_utab + 2 * g,
-                                                         lay_addr(base, G.lay_in), &tla_in);
+                                                         kPhaseFree ? G.tile_in[g]
+                                                                    : lay_addr(base, G.lay_in),
+                                                         &tla_in);
         // the next tile, into the registers the state frees (called right
         // after the state's last use; scheduling barriers keep the loads from
         // being hoisted above it, where both would need registers)
@@ -1643,7 +1647,9 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_la
                 } else {
                     ry_inv_window<R, 0>(l, tile_coef);
                 }
-                store_lam<R, 0, false>(l, lam_out, lay_addr(base, G.lay_out), ta_out, tl);
+                store_lam<R, 0, false>(l, lam_out,
+                                      kPhaseFree ? G.tile_out[g] : lay_addr(base, G.lay_out),
+                                      ta_out, tl);
             } else {
                 uint32_t done = w0;
                 const bool w1_rot = G.rot_lo < R;
@@ -1684,7 +1690,9 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_la
                     exchange<R, 2, 0>(l, lds, tl);
                 }
                 ry_inv_window<R, 0>(l, tile_coef);
-                store_lam<R, 0, false>(l, lam_out, lay_addr(base, G.lay_out), ta_out, tl);
+                store_lam<R, 0, false>(l, lam_out,
+                                      kPhaseFree ? G.tile_out[g] : lay_addr(base, G.lay_out),
+                                      ta_out, tl);
             }
         }
     }
@@ -1866,7 +1874,7 @@ __global__ __launch_bounds__(Tile<R>::THREADS, low_gen_min_waves<R>()) void bwd_
         TileAddr ta{};
         ta.offA = lay_addr(omap<R, 0>(tl, 0), G.lay_out);
         ta.strideA = lay_addr(uint64_t{1} << (T - R), G.lay_out);
-        store_lam<R, 0, false>(l, lam_out, lay_addr(base, G.lay_out), ta, tl);
+        store_lam<R, 0, false>(l, lam_out, G.tile_out[g], ta, tl);
     }
     __syncthreads();
     if (t < 2 * T) {
@@ -1885,6 +1893,9 @@ void launch_bwd_low_gen(int R, double2 *lam_out, const WireCoef *layer_coef, con
     if (g.a != Tile<4>::T || g.h0 != Tile<4>::T || g.rot_lo != 0 || g.rot_hi != Tile<4>::T ||
         g.n <= Tile<4>::T || g.lay_in.count != 0 || !lay_fits(g.lay_out, g.n, Tile<4>::T - R, R)) {
         throw std::invalid_argument("generating low pass: geometry outside the low tile");
+    }
+    if (g.tile_out == nullptr) {
+        throw std::invalid_argument("generating low pass: no per-tile lambda address table");
     }
     if (R == 4) {
         bwd_low_gen_kernel<4><<<grid, Tile<4>::THREADS, 0, s>>>(lam_out, layer_coef, g, partials,
@@ -1921,6 +1932,9 @@ static void launch_bwd_pass_r(bool gather, bool write_cur, bool write_lam, const
         // the tables leave the pass's own phases out only for this sweep
         throw std::invalid_argument(
             "phase-free sweep: generated lambda-only high pass of shape 4 without pair terms");
+    }
+    if (phase_free && (g.tile_in == nullptr || g.tile_out == nullptr)) {
+        throw std::invalid_argument("phase-free sweep: no per-tile lambda address tables");
     }
     if (!write_cur && write_lam) {  // the state is not rewritten: lambda-only sweep
         if (!gather && R == 4 && (g.a != 3 || g.rot_lo < 3)) {
@@ -2822,6 +2836,18 @@ __device__ __forceinline__ int top_wire(int b, int h) {
 // Per top tile g: phi_c[g] = prod_{v = 4}^{h-1} u_v(j_v ^ j_{v-1}) (j_{3+m} =
 // bit m of g) and lam_tile[g], the address of the tile's base index in the
 // layout the pass reads.
+__global__ __launch_bounds__(256) void lam_tile_table_kernel(PassGeom G, LamLayout L,
+                                                             uint64_t *__restrict__ out) {
+    const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < G.num_tiles) {
+        out[g] = lay_addr(tile_base(g, G, Tile<4>::T), L);
+    }
+}
+
+void launch_lam_tile_table(const PassGeom &g, const LamLayout &L, uint64_t *out, hipStream_t s) {
+    lam_tile_table_kernel<<<(unsigned)((g.num_tiles + 255) / 256), 256, 0, s>>>(g, L, out);
+}
+
 __global__ __launch_bounds__(256) void layer0_top_factor_kernel(const WireCoef *__restrict__ lc0,
                                                                 PassGeom G,
                                                                 double2 *__restrict__ phi_c,

@@ -48,6 +48,12 @@ struct PassGeom {
     // read, lay_out the layout written; see make_lam_layout.
     LamLayout lay_in;
     LamLayout lay_out;
+    // Per-tile lambda addresses, tile_in[g] = lay_addr(tile_base(g), lay_in)
+    // and tile_out[g] likewise (launch_lam_tile_table): the generated start's
+    // phase-free high passes read them (tile_in, tile_out) and its low pass
+    // stores through tile_out, so their tile loops do not evaluate a layout.
+    const uint64_t *tile_in = nullptr;
+    const uint64_t *tile_out = nullptr;
 };
 
 // Address of index i in layout L (a bit permutation, so lay_addr(x | y) =
@@ -347,6 +353,9 @@ constexpr int kTopL0Cols = 52;
 // contraction of its tiles (per-block partials: 2T layer-1 gradients in
 // partials, kTopL0Cols = 52 layer-0 values in l0_partials; per tile 4 wave
 // sums in tile_sums; phi_c and lam_tile: one value per tile, written first).
+// out[g] = lay_addr(tile_base(g, g_, T), L) for every tile of pass geometry g_.
+void launch_lam_tile_table(const PassGeom &g, const LamLayout &L, uint64_t *out, hipStream_t s);
+
 void launch_bwd_top_l0(const double2 *lam_in, const WireCoef *layer_coef, const PassGeom &g,
                        double *partials, int grid, hipStream_t s, const CMat2 *gen_tab,
                        const CMat2 *gen_utab, const WireCoef *lc0, double2 *phi_c,
