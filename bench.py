@@ -99,6 +99,8 @@ def parse_args() -> argparse.Namespace:
     p.add_argument("--batch-count", type=int, default=2048)
     p.add_argument("--no-warm-start", action="store_true",
                    help="skip the learned warm-start measurement")
+    p.add_argument("--no-reference-cases", action="store_true",
+                   help="skip timing the reference's published cases")
     return p.parse_args()
 
 
@@ -223,6 +225,83 @@ def plain_small(torch, steps: int) -> dict:
         "speedup_device_vs_cpu": round(cpu_s / dev_s, 1),
         "max_abs_dE": abs(float(out["energy"]) - e_ref),
         "max_abs_dgrad": float(np.max(np.abs(np.asarray(out["gradient"]) - g_ref))),
+    }
+
+
+# The reference's published single-evaluation timings on its own cases (NVIDIA
+# RTX 6000 Ada, CUDA 13.2; context, not targets): the final structured_adjoint
+# stage totals (docs/adjoint_diff_strategy_summary.md:195-201) and the per-step
+# training times of structured_adjoint and dense_scan
+# (docs/adjoint_diff_strategy_summary.md:46-58).
+REFERENCE_CASES = [
+    # (n, L, {reference strategy: ms})
+    (8, 8, {"structured_adjoint": 0.413}),
+    (12, 8, {"structured_adjoint": 0.600}),
+    (16, 4, {"structured_adjoint": 0.598}),
+    (20, 2, {"structured_adjoint": 3.699}),
+    (24, 1, {"structured_adjoint": 33.711}),
+    (4, 32, {"structured_adjoint": 0.467, "dense_scan": 0.181}),
+    (4, 512, {"structured_adjoint": 7.146, "dense_scan": 0.590}),
+    (5, 512, {"structured_adjoint": 9.710, "dense_scan": 0.977}),
+    (6, 512, {"structured_adjoint": 12.840, "dense_scan": 2.998}),
+    (7, 128, {"structured_adjoint": 4.369, "dense_scan": 4.213}),
+    (7, 2048, {"structured_adjoint": 81.393, "dense_scan": 83.105}),
+    (8, 128, {"structured_adjoint": 5.589, "dense_scan": 28.819}),
+    (8, 512, {"structured_adjoint": 22.638, "dense_scan": 114.143}),
+]
+
+
+def reference_cases(torch) -> dict:
+    """The reference's published cases on this engine: one energy_and_grad per
+    call through the host-array entry point (parameters from the host, energy
+    and gradient back to it, as the reference's bindings take them), median
+    over repeated calls, for structured_adjoint and, at n <= 8, dense_scan;
+    beside the reference's figures.  Energy and gradient are checked against
+    the oracle where it is quick (n <= 16)."""
+    from gpu_simulation_amd import native
+    from oracle import oracle as cpu_oracle
+
+    ext = native.load_extension()
+    cpu_oracle.build()
+    out = []
+    for n, layers, ref in REFERENCE_CASES:
+        params = np.random.default_rng(1000 + 37 * n + layers).uniform(
+            0.0, 2.0 * math.pi, size=(layers, n, 2))
+        flat = np.ascontiguousarray(params.reshape(-1))
+        ours = {}
+        err = None
+        for strategy in (["structured_adjoint", "dense_scan"] if n <= 8
+                         else ["structured_adjoint"]):
+            eng = ext.RingIsingCudaBackend(n, layers, 1.0, strategy, 8, False)
+            res = eng.energy_and_grad(flat)
+            for _ in range(3):
+                eng.energy_and_grad(flat)
+            t0 = time.perf_counter()
+            eng.energy_and_grad(flat)
+            one = time.perf_counter() - t0
+            reps = int(min(200, max(5, 0.3 / max(one, 1e-6))))
+            times = []
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                eng.energy_and_grad(flat)
+                times.append(time.perf_counter() - t0)
+            ours[strategy] = round(1e3 * statistics.median(times), 4)
+            if n <= 16:
+                e_ref, g_ref = cpu_oracle.energy_and_grad(n, layers, 1.0, params)
+                d = max(abs(float(res["energy"]) - e_ref),
+                        float(np.max(np.abs(np.asarray(res["gradient"]) - g_ref))))
+                err = d if err is None else max(err, d)
+            del eng
+        best_ours = min(ours.values())
+        best_ref = min(ref.values())
+        out.append({"case": f"{n}x{layers}", "ours_ms": ours, "reference_ms": ref,
+                    "speedup_vs_reference_best": round(best_ref / best_ours, 2),
+                    "max_abs_err_vs_oracle": err})
+    return {
+        "timing": "median ms per energy_and_grad through the host-array call (MI355X); "
+                  "reference figures: RTX 6000 Ada, final structured_adjoint stage totals and "
+                  "per-step training times of structured_adjoint and dense_scan",
+        "cases": out,
     }
 
 
@@ -528,6 +607,8 @@ def main() -> int:
     if not args.no_extras:
         result["plain_12q3l"] = plain_small(torch, args.steps)
         result["drift"] = drift_report(torch)
+        if not args.no_reference_cases:
+            result["reference_cases"] = reference_cases(torch)
     if not args.no_batched and not args.no_extras:
         result["batched"] = batched_mix(args.batch_count, args.steps, torch)
     if not args.no_warm_start and not args.no_extras:
