@@ -14,7 +14,11 @@ The printed JSON line carries
     launch stream during the timed region, against the 8 TB/s HBM3E peak;
   * cpu_baseline = the FP64 CPU oracle (gate-level adjoint, PennyLane gate
     order, OpenMP) timed on this host's cores on a bounded smaller-n sample
-    and scaled to the 30-qubit workload by its 2^n * n cost.
+    and scaled to the 30-qubit workload by its 2^n * n cost;
+  * reference_cases / gd20_cases = the reference's own published cases and
+    its 20-step gradient-descent scaling run (n = 8 .. 24, 2 layers) timed on
+    this engine beside the reference's figures, with the error against the
+    oracle or the reference's stored final energies.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -303,6 +307,41 @@ def reference_cases(torch) -> dict:
                   "per-step training times of structured_adjoint and dense_scan",
         "cases": out,
     }
+
+
+def gd20_cases() -> dict:
+    """The reference's 20-step gradient-descent runs at 2 layers, n = 8 .. 24
+    (RunConfig defaults: h = 1, seed 7, parameters 0.3 N(0, 1), step size 0.08),
+    through this package's run(RunConfig(backend="standalone")): the average
+    step time (loop wall time over the steps, the better of two runs) beside
+    the reference's structured_adjoint figure on an RTX 6000 Ada, and the final
+    energy against the reference's stored one (the fixture in tests/golden)."""
+    from gpu_simulation_amd.ring_ising import RunConfig, run
+
+    golden = json.loads((ROOT / "tests" / "golden" / "gd20_final_energies.json").read_text())
+    cfg = golden["run_config"]
+    rows = []
+    for row in golden["rows"]:
+        if row["method"] != "structured_adjoint":
+            continue
+        n = row["num_qubits"]
+        best = None
+        energy = None
+        for _ in range(2):
+            res = run(RunConfig(backend="standalone", num_qubits=n, layers=cfg["layers"],
+                                field=cfg["field"], steps=cfg["steps"], stepsize=cfg["stepsize"],
+                                seed=cfg["seed"], init_scale=cfg["init_scale"],
+                                gradient_strategy="structured_adjoint", verbose=False,
+                                show_progress=False))
+            ms = 1e3 * res.wall_s / cfg["steps"]
+            best = ms if best is None else min(best, ms)
+            energy = res.final_energy
+        rows.append({"n": n, "avg_step_ms": round(best, 4),
+                     "reference_avg_step_ms": round(row["avg_step_ms_rtx6000ada"], 4),
+                     "speedup": round(row["avg_step_ms_rtx6000ada"] / best, 2),
+                     "abs_dE_vs_reference": abs(energy - row["final_energy"])})
+    return {"workload": "20 gradient-descent steps, 2 layers, RunConfig defaults",
+            "reference": golden["source"], "rows": rows}
 
 
 def drift_report(torch) -> dict:
@@ -609,6 +648,7 @@ def main() -> int:
         result["drift"] = drift_report(torch)
         if not args.no_reference_cases:
             result["reference_cases"] = reference_cases(torch)
+            result["gd20_cases"] = gd20_cases()
     if not args.no_batched and not args.no_extras:
         result["batched"] = batched_mix(args.batch_count, args.steps, torch)
     if not args.no_warm_start and not args.no_extras:
