@@ -1707,16 +1707,6 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_la
     }
 }
 
-// H's diagonal -sum_w Z_w Z_{w+1} at P(i), from i (the pre-CNOT index):
-// bits k, k + 1 of P(i) differ where i_{k+1} = 1 (1 <= k <= n - 2), bits n - 1
-// and 0 where i_0 = 1, bits 0 and 1 where i_0 xor parity(i >> 2) = 1.
-__device__ __forceinline__ double ham_diag_pre_ring(uint64_t i, int n) {
-    const int hi = (int)__popcll(i >> 2);
-    const int b0 = (int)(i & 1u);
-    const int walls = hi + (b0 ^ (hi & 1)) + b0;
-    return (double)(2 * walls - n);
-}
-
 // Generating low pass of a two-layer circuit (see launch_bwd_low_gen).  The
 // tile is generated in mapping 0 and swept 0, 1, .., P - 1 (phases, pair
 // terms and gradients), then lambda alone goes back to mapping 0 under RY^dagger
@@ -1801,10 +1791,14 @@ __global__ __launch_bounds__(Tile<R>::THREADS, low_gen_min_waves<R>()) void bwd_
             for (int k = 0; k < TL::REGS; ++k) {
                 l[k] = make_double2(0.0, 0.0);
             }
-            // H's diagonal (ham_diag_pre_ring) of register k: the register
-            // bits T - R .. T - 1 (all >= 2) add pk = popcount(k) to
-            // popcount(i >> 2) and flip the ring term's parity pk times, so
-            // d = dpar[pk & 1] + 2 pk with two values per thread
+            // H's diagonal -sum_w Z_w Z_{w+1} at P(i) from the pre-CNOT index
+            // i: bits k, k + 1 of P(i) differ where i_{k+1} = 1
+            // (1 <= k <= n - 2), bits n - 1 and 0 where i_0 = 1, bits 0 and 1
+            // where i_0 xor parity(i >> 2) = 1, so d = 2 walls - n with
+            // walls = popcount(i >> 2) + (i_0 ^ parity(i >> 2)) + i_0.  The
+            // register bits T - R .. T - 1 (all >= 2) of register k add
+            // pk = popcount(k) to popcount(i >> 2) and flip that parity pk
+            // times: d = dpar[pk & 1] + 2 pk with two values per thread
             const uint64_t i0 = base | row;
             const int hi0 = (int)__popcll(i0 >> 2);
             const int b0 = (int)(i0 & 1u);

@@ -155,10 +155,30 @@ def test_phase_sums_from_quad_partials():
 
 
 def _ham_diag_pre_ring(i, n):
-    """H's diagonal at P(i) from the pre-CNOT index i (ham_diag_pre_ring)."""
+    """H's diagonal at P(i) from the pre-CNOT index i, per index."""
     hi = bin(i >> 2).count("1")
     b0 = i & 1
     return 2 * (hi + (b0 ^ (hi & 1)) + b0) - n
+
+
+def _ring_fwd(i, n):
+    """P(i): bit k >= 1 the parity of bits 0..k, bit 0 the parity of bits 1..n-1."""
+    p = i
+    for sh in (1, 2, 4, 8, 16, 32):
+        p ^= p << sh
+    p &= (1 << n) - 1
+    return (p & ~1) | (bin(i >> 1).count("1") & 1)
+
+
+def test_pre_ring_diagonal_equals_the_walls_of_the_image():
+    # -sum_w Z_w Z_{w+1 mod n} at j = P(i) is 2 (domain walls of j) - n
+    rng = np.random.default_rng(15)
+    for _ in range(3000):
+        n = int(rng.integers(3, 35))
+        i = int(rng.integers(0, 1 << n))
+        j = _ring_fwd(i, n)
+        rot = ((j << 1) | (j >> (n - 1))) & ((1 << n) - 1)
+        assert _ham_diag_pre_ring(i, n) == 2 * bin(j ^ rot).count("1") - n
 
 
 def test_diagonal_from_two_values_per_thread():
