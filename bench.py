@@ -365,10 +365,12 @@ def drift_report(torch) -> dict:
 
 def batched_mix(count: int, steps: int, torch) -> dict:
     """Uneven mix of `count` circuits (n in 4..14, L in {8, 32, 128}, h in
-    U[0.2, 2]; n <= 8 on dense_scan): one persistent launch replayed from a
-    HIP graph vs one launch per circuit (same kernels) vs one engine call per
-    circuit (the reference's calling pattern), each an energy_and_grad of every
-    circuit."""
+    U[0.2, 2]; circuits with n <= 8 may take dense_scan, the autotuner choosing
+    which): one persistent launch replayed from a HIP graph vs one launch per
+    circuit (same kernels and split) vs one engine call per circuit (the
+    reference's calling pattern, n <= 8 on dense_scan), each an energy_and_grad
+    of every circuit; beside it the same mix with the split fixed at n <= 8 and
+    with every circuit on the whole-circuit kernel."""
     from gpu_simulation_amd import native
     from gpu_simulation_amd.batch import RingIsingBatch, uneven_mix
 
@@ -424,11 +426,21 @@ def batched_mix(count: int, steps: int, torch) -> dict:
     de_sv = float(np.max(np.abs(np.asarray(e_sv) - np.asarray(e_b))))
     dg_sv = max(float(np.max(np.abs(a - b))) for a, b in zip(g_sv, g_b))
     del sv
-    n_dense = sum(1 for s_ in specs if s_.num_qubits <= 8)
+    # the split fixed at n <= 8 on dense_scan (shapes still tuned)
+    fx = RingIsingBatch(specs, dense_max_qubits=8)
+    fx.set_params(params)
+    fx.autotune(repeats=2, choose_split=False)
+    fx.native.set_use_graph(True)
+    fx_s = timed(lambda: fx.native.run_device(True, stream), max(3, steps))
+    e_fx, _ = fx.results()
+    de_fx = float(np.max(np.abs(np.asarray(e_fx) - np.asarray(e_b))))
+    del fx
+    n_dense = sum(1 for st in batch.strategies if st == "dense_scan")
     return {
         "workload": (f"{count} circuits, n ~ U{{4..14}}, L ~ U{{8,32,128}}, h ~ U[0.2,2.0], "
-                     f"params U[0,2pi), rng default_rng([3, i]); {n_dense} on dense_scan (n <= 8), "
-                     f"{count - n_dense} on structured_adjoint"),
+                     f"params U[0,2pi), rng default_rng([3, i]); autotuned split: {n_dense} on "
+                     f"dense_scan (n <= {cfg['dense_max_qubits']}), {count - n_dense} on "
+                     "structured_adjoint"),
         "batched_evals_per_s": round(count / batched_s, 1),
         "batched_ms": round(batched_s * 1e3, 3),
         "per_circuit_launch_evals_per_s": round(count / per_circuit_s, 1),
@@ -444,8 +456,13 @@ def batched_mix(count: int, steps: int, torch) -> dict:
         "all_statevector_evals_per_s": round(count / sv_s, 1),
         "all_statevector_max_abs_dE_vs_batched": de_sv,
         "all_statevector_max_abs_dgrad_vs_batched": dg_sv,
+        "fixed_split_ms": round(fx_s * 1e3, 3),
+        "fixed_split_evals_per_s": round(count / fx_s, 1),
+        "fixed_split_max_abs_dE_vs_batched": de_fx,
         "autotune_s": round(tune_s, 2),
-        "config": {k: cfg[k] for k in ("threads", "grid", "dense_tile", "chunk_by_n")},
+        "config": {k: cfg[k] for k in ("threads", "grid", "dense_tile", "dense_max_qubits",
+                                        "chunk_by_n")},
+        "split_ms": cfg["autotune"].get("split_ms"),
     }
 
 

@@ -57,7 +57,11 @@ class RingIsingBatch:
                                      [s.layers for s in self.specs],
                                      [float(s.field) for s in self.specs], int(dense_max_qubits))
         self.offsets = np.asarray(self._b.param_offsets(), dtype=np.int64)
-        self.strategies = list(self._b.strategies())
+
+    @property
+    def strategies(self) -> list[str]:
+        """Each circuit's strategy; autotune() may move circuits between them."""
+        return list(self._b.strategies())
 
     @property
     def native(self):
@@ -108,7 +112,10 @@ class RingIsingBatch:
         return np.asarray(self._b.optimize(optimizer, float(lr), int(steps), bool(use_graph),
                                            int(stream)))
 
-    def autotune(self, repeats: int = 3) -> dict:
+    def autotune(self, repeats: int = 3, choose_split: bool = True) -> dict:
+        """Tune kernel shapes, and with choose_split which circuits with
+        num_qubits <= dense_max_qubits take dense_scan, on this batch."""
+        self._b.set_tune_split(int(bool(choose_split)))
         return self._b.autotune(int(repeats))
 
     def config(self) -> dict:

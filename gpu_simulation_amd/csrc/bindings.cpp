@@ -448,6 +448,8 @@ class RingIsingBatch {
         return config();
     }
 
+    void set_tune_split(int enabled) { check(rie_batch_set_tune_split(batch_, enabled)); }
+
     py::object config() const {
         size_t need = 0;
         check(rie_batch_config(batch_, nullptr, 0, &need));
@@ -654,6 +656,7 @@ PYBIND11_MODULE(_cuda_backend, m) {
         .def("optimize", &RingIsingBatch::optimize, py::arg("optimizer"), py::arg("lr"),
              py::arg("steps"), py::arg("use_graph") = true, py::arg("stream") = 0)
         .def("autotune", &RingIsingBatch::autotune, py::arg("repeats") = 3)
+        .def("set_tune_split", &RingIsingBatch::set_tune_split, py::arg("enabled"))
         .def("config", &RingIsingBatch::config)
         .def("set_config", &RingIsingBatch::set_config, py::arg("threads"),
              py::arg("chunk_by_n") = py::none(), py::arg("dense_tile") = 64,

@@ -59,7 +59,9 @@ struct Batch {
     std::vector<char> dense_flag;
     std::vector<size_t> poff;  // count + 1 parameter offsets
     size_t total = 0;
-    int dense_max = 8;
+    int dense_max = 8;    // the caller's bound: n <= dense_max may take dense_scan
+    int dense_split = 8;  // the split in force: n <= dense_split takes dense_scan
+    bool tune_split = true;  // the autotuner chooses dense_split in [0, dense_max]
 
     // device
     int device = 0;
@@ -116,6 +118,7 @@ struct Batch {
 
     void build_structured();
     void build_dense();
+    void repartition(int split);
     void release_structured();
     void invalidate_graph();
     void issue_eval(hipStream_t s, bool want_grad);
@@ -133,7 +136,7 @@ struct Batch {
 };
 
 Batch::Batch(size_t count_, const int *nq, const int *nl, const double *fields, int dense_max_qubits)
-    : count(count_), dense_max(dense_max_qubits) {
+    : count(count_), dense_max(dense_max_qubits), dense_split(dense_max_qubits) {
     if (count == 0) {
         fail(RIE_INVALID_ARGUMENT, "a batch needs at least one circuit.");
     }
@@ -193,6 +196,21 @@ Batch::Batch(size_t count_, const int *nq, const int *nl, const double *fields, 
     for (size_t i = 0; i < count; ++i) {
         (dense_flag[i] ? dids : sids).push_back((int)i);
     }
+    build_structured();
+    build_dense();
+}
+
+// Circuits with n <= split on dense_scan, the others on the whole-circuit
+// kernel; both parts rebuilt (the per-circuit comparison plans as well).
+void Batch::repartition(int split) {
+    dense_split = split;
+    dids.clear();
+    sids.clear();
+    for (size_t i = 0; i < count; ++i) {
+        dense_flag[i] = n[i] <= split ? 1 : 0;
+        (dense_flag[i] ? dids : sids).push_back((int)i);
+    }
+    dense_single.clear();
     build_structured();
     build_dense();
 }
@@ -531,6 +549,23 @@ double Batch::time_ms(size_t repeats, const std::function<void(hipStream_t)> &fn
 void Batch::autotune(size_t repeats) {
     repeats = std::max<size_t>(1, std::min<size_t>(repeats, 50));
     invalidate_graph();
+    // the split candidates: every qubit count <= dense_max present, and none
+    std::vector<int> splits;
+    if (tune_split) {
+        for (int q = dense_max; q >= 1; --q) {
+            if (std::find(n.begin(), n.end(), q) != n.end()) {
+                splits.push_back(q);
+            }
+        }
+        if (!splits.empty()) {
+            splits.push_back(0);
+            // every circuit on the whole-circuit kernel while its costs are probed
+            repartition(0);
+        }
+    }
+    if (splits.empty()) {
+        splits.push_back(dense_split);
+    }
     std::ostringstream rep;
     rep.precision(6);
     rep << "{\"per_layer_us\": {";
@@ -633,63 +668,79 @@ void Batch::autotune(size_t repeats) {
         }
         measured = true;
     }
-    // dense tile for D >= 64
-    bool big_dense = false;
-    for (int i : dids) {
-        big_dense = big_dense || n[(size_t)i] >= 6;
-    }
-    double dense_ms[2] = {0.0, 0.0};
-    if (big_dense) {
-        const int tiles[2] = {32, 64};
-        for (int k = 0; k < 2; ++k) {
-            dense_tile = tiles[k];
-            build_dense();
-            dense_ms[k] = time_ms(repeats, [&](hipStream_t s) {
-                launch_coef(d_params, d_coef, (int)(total / 2), s);
-                dense.run(s, true);
-            });
-        }
-        dense_tile = dense_ms[0] < dense_ms[1] ? 32 : 64;
-        build_dense();
-    }
-    // structured grid: with a dense part running beside it, leave it CUs
-    std::vector<int> grid_opts;
-    if (!sids.empty()) {
-        if (!dids.empty()) {
-            for (int f : {1, 2, 3, 4}) {
-                grid_opts.push_back(num_cus * f / 4);
-            }
-        } else {
-            grid_opts.push_back(num_cus);
-            if (max_lds_n <= 11) {
-                grid_opts.push_back(2 * num_cus);  // two workgroups per CU fit in LDS
-            }
-        }
-    }
+    // per split: the dense tile for D >= 64 and the structured launch's grid
+    // (with a dense part running beside it, leave it CUs); the fastest whole
+    // evaluation wins
     double best_ms = -1.0;
     int best_grid = 0;
-    rep << ", \"grid_ms\": {";
-    for (size_t k = 0; k < grid_opts.size(); ++k) {
-        grid_limit = grid_opts[k];
-        build_structured();
-        const double ms = time_ms(repeats, [&](hipStream_t s) { issue_eval(s, true); });
-        rep << (k ? ", " : "") << "\"" << grid_limit << "\": " << ms;
-        if (best_ms < 0.0 || ms < best_ms) {
-            best_ms = ms;
-            best_grid = grid_limit;
+    int best_split = splits.front();
+    int best_tile = dense_tile;
+    rep << ", \"split_ms\": {";
+    for (size_t si = 0; si < splits.size(); ++si) {
+        const int split = splits[si];
+        if (split != dense_split || si == 0) {
+            repartition(split);
         }
+        bool big_dense = false;
+        for (int i : dids) {
+            big_dense = big_dense || n[(size_t)i] >= 6;
+        }
+        if (big_dense) {
+            const int tiles[2] = {32, 64};
+            double dense_ms[2] = {0.0, 0.0};
+            for (int k = 0; k < 2; ++k) {
+                dense_tile = tiles[k];
+                build_dense();
+                dense_ms[k] = time_ms(repeats, [&](hipStream_t s) {
+                    launch_coef(d_params, d_coef, (int)(total / 2), s);
+                    dense.run(s, true);
+                });
+            }
+            dense_tile = dense_ms[0] < dense_ms[1] ? 32 : 64;
+            build_dense();
+        }
+        std::vector<int> grid_opts;
+        if (!sids.empty()) {
+            if (!dids.empty()) {
+                for (int f : {1, 2, 3, 4}) {
+                    grid_opts.push_back(num_cus * f / 4);
+                }
+            } else {
+                grid_opts.push_back(num_cus);
+                if (max_lds_n <= 11) {
+                    grid_opts.push_back(2 * num_cus);  // two workgroups per CU fit in LDS
+                }
+            }
+        } else {
+            grid_opts.push_back(0);
+        }
+        rep << (si ? ", " : "") << "\"" << split << "\": {";
+        for (size_t k = 0; k < grid_opts.size(); ++k) {
+            grid_limit = grid_opts[k];
+            build_structured();
+            const double ms = time_ms(repeats, [&](hipStream_t s) { issue_eval(s, true); });
+            rep << (k ? ", " : "") << "\"" << grid_limit << "\": " << ms;
+            if (best_ms < 0.0 || ms < best_ms) {
+                best_ms = ms;
+                best_grid = grid_limit;
+                best_split = split;
+                best_tile = dense_tile;
+            }
+        }
+        rep << "}";
     }
     rep << "}";
     grid_limit = best_grid;
-    build_structured();
-    rep << ", \"dense_tile_ms\": [" << dense_ms[0] << ", " << dense_ms[1] << "]";
-    rep << ", \"batch_ms\": " << best_ms << "}";
+    dense_tile = best_tile;
+    repartition(best_split);
+    rep << ", \"dense_max_qubits\": " << best_split << ", \"batch_ms\": " << best_ms << "}";
     report = rep.str();
 }
 
 std::string Batch::config_json() const {
     std::ostringstream o;
     o << "{\"threads\": " << threads << ", \"grid\": " << grid << ", \"grid_limit\": " << grid_limit
+      << ", \"dense_max_qubits\": " << dense_split << ", \"tune_split\": " << (tune_split ? "true" : "false")
       << ", \"dense_tile\": " << dense_tile << ", \"measured\": " << (measured ? "true" : "false")
       << ", \"structured_circuits\": " << sids.size() << ", \"dense_circuits\": " << dids.size()
       << ", \"chunk_by_n\": {";
@@ -900,6 +951,10 @@ rie_status rie_batch_autotune(rie_batch *batch, size_t repeats) {
         rie::check(hipDeviceSynchronize(), "sync");
         b->autotune(repeats);
     });
+}
+
+rie_status rie_batch_set_tune_split(rie_batch *batch, int enabled) {
+    return guard([&]() { as_batch(batch)->tune_split = enabled != 0; });
 }
 
 rie_status rie_batch_set_config(rie_batch *batch, int threads, const int *chunk_by_n,

@@ -77,6 +77,7 @@ C_ABI_SYMBOLS = (
     "rie_batch_run_per_circuit",
     "rie_batch_optimize",
     "rie_batch_autotune",
+    "rie_batch_set_tune_split",
     "rie_batch_set_config",
     "rie_batch_config",
     "rie_mlp_create",

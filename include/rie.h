@@ -394,10 +394,19 @@ rie_status rie_batch_optimize(rie_batch *batch, int optimizer, double learning_r
                               int use_graph, double *energy_history_out, void *stream);
 
 /* Autotuner: times the whole-circuit kernel per qubit count for every
- * workgroup size (256/512/1024) and rotation-chunk width (1..4), the dense
- * GEMM tile (32/64) and the share of CUs given to the structured launch,
- * keeps the fastest, and rebuilds the schedule with the measured costs. */
+ * workgroup size (256/512/1024) and rotation-chunk width (1..4), then, for
+ * each split of the circuits between dense_scan (num_qubits <= s) and the
+ * whole-circuit kernel, s from the dense_max_qubits given at creation down to
+ * 0 (see rie_batch_set_tune_split), the dense GEMM tile (32/64) and the share
+ * of CUs given to the structured launch; keeps the fastest whole evaluation
+ * and rebuilds the schedule with the measured costs.  The reference fixes the
+ * strategy per call and its kernel shapes by qubit-count thresholds
+ * (cpp/ising_cuda_rotation_kernels.cu:562-575). */
 rie_status rie_batch_autotune(rie_batch *batch, size_t repeats);
+/* Whether rie_batch_autotune may move circuits between dense_scan and the
+ * whole-circuit kernel (default 1; 0 keeps num_qubits <= dense_max_qubits on
+ * dense_scan).  No counterpart in the reference, which has no batch. */
+rie_status rie_batch_set_tune_split(rie_batch *batch, int enabled);
 /* Manual configuration: threads 256/512/1024; chunk_by_n (15 values,
  * indexed by num_qubits, each 1..4) or NULL to keep; dense_tile 32/64;
  * structured_grid = workgroups of the persistent launch (0: one per CU). */

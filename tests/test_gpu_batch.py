@@ -140,9 +140,37 @@ def test_autotune_keeps_results(oracle):
           flush=True)
     assert cfg["measured"] is True
     assert cfg["threads"] in (256, 512, 1024)
+    # the split the autotuner kept: a timed candidate, and every circuit on the
+    # strategy it names
+    split = cfg["dense_max_qubits"]
+    assert 0 <= split <= 8 and str(split) in cfg["autotune"]["split_ms"]
+    assert all((st == "dense_scan") == (s.num_qubits <= split)
+               for st, s in zip(batch.strategies, specs))
     energies, grads = batch.energy_and_grad(params)
     assert np.max(np.abs(energies - before_e)) <= 1e-12
     _check_against_oracle(oracle, batch, params, energies, grads)
+
+
+def test_autotune_split_choices_keep_results(oracle):
+    # the split fixed by the caller, and each split the autotuner may pick
+    # (forced through dense_max_qubits), give the oracle's results
+    specs, params = uneven_mix(40, seed=16, qubits=(3, 10), layer_choices=(1, 3, 8))
+    fixed = RingIsingBatch(specs, dense_max_qubits=8)
+    cfg = fixed.autotune(repeats=1, choose_split=False)
+    print("fixed split", cfg["dense_max_qubits"], flush=True)
+    assert cfg["dense_max_qubits"] == 8 and cfg["tune_split"] is False
+    assert all((st == "dense_scan") == (s.num_qubits <= 8)
+               for st, s in zip(fixed.strategies, specs))
+    energies, grads = fixed.energy_and_grad(params)
+    _check_against_oracle(oracle, fixed, params, energies, grads)
+    for bound in (8, 5, 0):
+        batch = RingIsingBatch(specs, dense_max_qubits=bound)
+        cfg = batch.autotune(repeats=1)
+        print("tuned split", bound, cfg["dense_max_qubits"], flush=True)
+        assert cfg["dense_max_qubits"] <= bound
+        e, g = batch.energy_and_grad(params)
+        _check_against_oracle(oracle, batch, params, e, g)
+        assert np.max(np.abs(e - energies)) <= 1e-10
 
 
 def _oracle_gd(oracle, spec, p0, lr, steps):
