@@ -802,7 +802,9 @@ constexpr bool gather_pairs_reach() {
     return phase_first_bit<R, M>() + R - 1 >= Tile<R>::T - 4;
 }
 
-template <int R, int M>
+// kFullTile: pair_lo = 0 and pair_hi = T (the generating low pass), so the
+// range tests are resolved at compile time instead of held as uniform masks.
+template <int R, int M, bool kFullTile = false>
 __device__ __forceinline__ void add_pair_terms(const double2 (&c)[Tile<R>::REGS],
                                                double2 (&l)[Tile<R>::REGS], const double2 *lds,
                                                uint32_t t, const PassGeom &G, uint32_t done,
@@ -818,7 +820,8 @@ __device__ __forceinline__ void add_pair_terms(const double2 (&c)[Tile<R>::REGS]
             continue;
         }
         const uint32_t m = 3u << j;
-        if (j >= G.pair_lo && j + 1 < G.pair_hi && (m & done) == 0) {
+        const bool in_range = kFullTile || (j >= G.pair_lo && j + 1 < G.pair_hi);
+        if (in_range && (m & done) == 0) {
 #pragma unroll
             for (int k = 0; k < Tile<R>::REGS; ++k) {
                 const double2 p = lds[lds_slot<R>(omap<R, M>(t, k) ^ m)];
@@ -831,7 +834,8 @@ __device__ __forceinline__ void add_pair_terms(const double2 (&c)[Tile<R>::REGS]
 #pragma unroll
     for (int rb = 0; rb + 1 < R; ++rb) {
         const int j = lo + rb;
-        if (j >= G.pair_lo && j + 1 < G.pair_hi && ((3u << j) & done) == 0) {
+        const bool in_range = kFullTile || (j >= G.pair_lo && j + 1 < G.pair_hi);
+        if (in_range && ((3u << j) & done) == 0) {
 #pragma unroll
             for (int k = 0; k < Tile<R>::REGS; ++k) {
                 const double2 p = c[k ^ (3 << rb)];
@@ -1098,7 +1102,9 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_pa
 // kPhased = false: the window's RZ phases are already divided out of both
 // arrays (a state generated without them and lambda multiplied by the tile's
 // combined phase, apply_tile_phase), so the window takes its gradients only.
-template <int R, int M, bool kPhased = true>
+// kFullTile: every tile bit is rotated (rot_lo = 0, rot_hi = T: the
+// generating low pass), resolved at compile time.
+template <int R, int M, bool kPhased = true, bool kFullTile = false>
 __device__ __forceinline__ void grad_window(double2 (&c)[Tile<R>::REGS],
                                             double2 (&l)[Tile<R>::REGS],
                                             const WireCoef *__restrict__ lc, const PassGeom &G,
@@ -1151,7 +1157,7 @@ __device__ __forceinline__ void grad_window(double2 (&c)[Tile<R>::REGS],
         for (int rb = 0; rb < 4; ++rb) {
             const int j = P0 + rb;
             double dt = 0.0;
-            if (j >= G.rot_lo && j < G.rot_hi) {
+            if (kFullTile || (j >= G.rot_lo && j < G.rot_hi)) {
 #pragma unroll
                 for (int q = 0; q < Tile<R>::REGS; ++q) {
                     if (!(q & (1 << rb))) {
@@ -1197,7 +1203,7 @@ __device__ __forceinline__ void grad_window(double2 (&c)[Tile<R>::REGS],
 #pragma unroll
     for (int rb = R - 1; rb >= 0; --rb) {
         const int j = P0 + rb;
-        if (j >= G.rot_lo && j < G.rot_hi) {
+        if (kFullTile || (j >= G.rot_lo && j < G.rot_hi)) {
             double dt = 0.0;
 #pragma unroll
             for (int q = 0; q < Tile<R>::REGS; ++q) {
@@ -1820,7 +1826,7 @@ __global__ __launch_bounds__(Tile<R>::THREADS, low_gen_min_waves<R>()) void bwd_
             }
         }
         // pair terms of window 0 (its lower edge from an LDS copy), sweep 1
-        if (G.pair_lo <= T - R - 1 && G.pair_hi >= T - R + 1) {
+        if constexpr (T - R - 1 >= 0) {  // every pair in the tile (pair_lo = 0, pair_hi = T)
             __syncthreads();
 #pragma unroll
             for (int k = 0; k < TL::REGS; ++k) {
@@ -1828,18 +1834,18 @@ __global__ __launch_bounds__(Tile<R>::THREADS, low_gen_min_waves<R>()) void bwd_
             }
             __syncthreads();
         }
-        add_pair_terms<R, 0>(c, l, lds, tl, G, 0u, field);
-        grad_window<R, 0>(c, l, lc, G, row_acc, lane, rz_tab);
+        add_pair_terms<R, 0, true>(c, l, lds, tl, G, 0u, field);
+        grad_window<R, 0, true, true>(c, l, lc, G, row_acc, lane, rz_tab);
         constexpr uint32_t w0 = window_mask<R, 0>();
         constexpr uint32_t w1 = window_mask<R, 1>();
         exchange<R, 0, 1>(l, lds, tl);
         exchange<R, 0, 1>(c, lds, tl);
-        add_pair_terms<R, 1>(c, l, lds, tl, G, w0, field);
-        grad_window<R, 1>(c, l, lc, G, row_acc, lane, rz_tab);
+        add_pair_terms<R, 1, true>(c, l, lds, tl, G, w0, field);
+        grad_window<R, 1, true, true>(c, l, lc, G, row_acc, lane, rz_tab);
         exchange<R, 1, 2>(l, lds, tl);
         exchange<R, 1, 2>(c, lds, tl);
-        add_pair_terms<R, 2>(c, l, lds, tl, G, w0 | w1, field);
-        grad_window<R, 2>(c, l, lc, G, row_acc, lane, rz_tab);
+        add_pair_terms<R, 2, true>(c, l, lds, tl, G, w0 | w1, field);
+        grad_window<R, 2, true, true>(c, l, lc, G, row_acc, lane, rz_tab);
         if constexpr (TL::P == 3) {
             // sweep 2: lambda back to mapping 0 under RY^dagger (tangent
             // form, its scale applied once before the store)
@@ -1853,8 +1859,8 @@ __global__ __launch_bounds__(Tile<R>::THREADS, low_gen_min_waves<R>()) void bwd_
             constexpr uint32_t w2 = window_mask<R, 2>();
             exchange<R, 2, 3>(l, lds, tl);
             exchange<R, 2, 3>(c, lds, tl);
-            add_pair_terms<R, 3>(c, l, lds, tl, G, w0 | w1 | w2, field);
-            grad_window<R, 3>(c, l, lc, G, row_acc, lane, rz_tab);
+            add_pair_terms<R, 3, true>(c, l, lds, tl, G, w0 | w1 | w2, field);
+            grad_window<R, 3, true, true>(c, l, lc, G, row_acc, lane, rz_tab);
             ry_inv_window<R, 3>(l, tile_coef);
             exchange<R, 3, 2>(l, lds, tl);
             ry_inv_window<R, 2>(l, tile_coef);
@@ -1890,6 +1896,10 @@ void launch_bwd_low_gen(int R, double2 *lam_out, const WireCoef *layer_coef, con
     }
     if (g.tile_out == nullptr) {
         throw std::invalid_argument("generating low pass: no per-tile lambda address table");
+    }
+    if (g.pair_lo != 0 || g.pair_hi != Tile<4>::T) {
+        // the kernel adds every in-tile pair (resolved at compile time)
+        throw std::invalid_argument("generating low pass: the pair range must be the whole tile");
     }
     if (R == 4) {
         bwd_low_gen_kernel<4><<<grid, Tile<4>::THREADS, 0, s>>>(lam_out, layer_coef, g, partials,
