@@ -1102,9 +1102,11 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_pa
 // kPhased = false: the window's RZ phases are already divided out of both
 // arrays (a state generated without them and lambda multiplied by the tile's
 // combined phase, apply_tile_phase), so the window takes its gradients only.
-// kFullTile: every tile bit is rotated (rot_lo = 0, rot_hi = T: the
-// generating low pass), resolved at compile time.
-template <int R, int M, bool kPhased = true, bool kFullTile = false>
+// kAllBits: the window's gradients of every bit, without the uniform
+// rot_lo / rot_hi tests: the generating low pass rotates every tile bit, and
+// the generated start's high passes leave only gradient slots that
+// reduce_grad never reads (it takes the columns of [rot_lo, rot_hi)).
+template <int R, int M, bool kPhased = true, bool kAllBits = false>
 __device__ __forceinline__ void grad_window(double2 (&c)[Tile<R>::REGS],
                                             double2 (&l)[Tile<R>::REGS],
                                             const WireCoef *__restrict__ lc, const PassGeom &G,
@@ -1157,7 +1159,7 @@ __device__ __forceinline__ void grad_window(double2 (&c)[Tile<R>::REGS],
         for (int rb = 0; rb < 4; ++rb) {
             const int j = P0 + rb;
             double dt = 0.0;
-            if (kFullTile || (j >= G.rot_lo && j < G.rot_hi)) {
+            if (kAllBits || (j >= G.rot_lo && j < G.rot_hi)) {
 #pragma unroll
                 for (int q = 0; q < Tile<R>::REGS; ++q) {
                     if (!(q & (1 << rb))) {
@@ -1203,7 +1205,7 @@ __device__ __forceinline__ void grad_window(double2 (&c)[Tile<R>::REGS],
 #pragma unroll
     for (int rb = R - 1; rb >= 0; --rb) {
         const int j = P0 + rb;
-        if (kFullTile || (j >= G.rot_lo && j < G.rot_hi)) {
+        if (kAllBits || (j >= G.rot_lo && j < G.rot_hi)) {
             double dt = 0.0;
 #pragma unroll
             for (int q = 0; q < Tile<R>::REGS; ++q) {
@@ -1625,7 +1627,7 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_la
             if constexpr (kPhaseFree) {
                 apply_tile_phase<R>(l, rz_tab, tl);
             }
-            grad_window<R, 0, kPhased>(c, l, lc, G, row_acc, lane, rz_tab);
+            grad_window<R, 0, kPhased, kPhaseFree>(c, l, lc, G, row_acc, lane, rz_tab);
             if constexpr (TL::P == 3) {
                 // windows 0 and 2, tile bit 3 across lanes (window 1 has no
                 // other rotated bit: its pairs with bit 4 were added at window
@@ -1636,7 +1638,7 @@ __global__ __launch_bounds__(Tile<R>::THREADS, tile_min_blocks<R>()) void bwd_la
                 if constexpr (kHam) {
                     add_pair_terms<R, 2>(c, l, lds, tl, G, w0, field);
                 }
-                grad_window<R, 2, kPhased>(c, l, lc, G, row_acc, lane, rz_tab);
+                grad_window<R, 2, kPhased, kPhaseFree>(c, l, lc, G, row_acc, lane, rz_tab);
                 // unconditional: an unrotated bit 3 has phase 1 and the
                 // identity in tile_coef, and its gradient slot is not read
                 grad_ry_lane_bit3<R, kPhased>(c, l, row_acc, lane, rz_tab, tile_coef[3]);
@@ -2956,10 +2958,10 @@ __global__ __launch_bounds__(256, 2) void bwd_top_l0_kernel(
         // the sweep of the lambda-only high pass (bwd_lam_kernel, R = 4), the
         // state generated without the group's phases
         apply_tile_phase<R>(l, rz_tab, tl);
-        grad_window<R, 0, false>(c, l, lc, G, row_acc, lane, rz_tab);
+        grad_window<R, 0, false, true>(c, l, lc, G, row_acc, lane, rz_tab);
         exchange<R, 0, 2>(l, lds, tl);
         exchange<R, 0, 2>(c, lds, tl);
-        grad_window<R, 2, false>(c, l, lc, G, row_acc, lane, rz_tab);
+        grad_window<R, 2, false, true>(c, l, lc, G, row_acc, lane, rz_tab);
         grad_ry_lane_bit3<R, false>(c, l, row_acc, lane, rz_tab, tile_coef[3]);
         ry_inv_window_tan<R, 2>(l, tan_t, form_b);
         exchange<R, 2, 0>(l, lds, tl);
