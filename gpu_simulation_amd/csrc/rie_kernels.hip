@@ -1745,6 +1745,13 @@ __global__ __launch_bounds__(Tile<R>::THREADS, low_gen_min_waves<R>()) void bwd_
     constexpr bool kTan = R == 4;
     __shared__ double tan_t[kTan ? T : 1];
     __shared__ double ry_sigma;
+    // lambda's store addresses in layout G.lay_out for the 4-register shape:
+    // each thread's row offset and the register step (tile independent: read
+    // per tile instead of keeping the layout live across the loop; the
+    // 512-thread shape keeps its LDS within two workgroups per CU instead)
+    constexpr bool kLayRows = R == 4;
+    __shared__ uint32_t lay_row[kLayRows ? TL::THREADS : 1];
+    __shared__ uint64_t lay_step;
     // chunk products C0 (index bits 0..3) and C1 (bits 4 .. T - R - 1): in LDS
     // for R = 4, read from the table in global memory (L2) for R = 3
     constexpr bool kChunksLds = R == 4;
@@ -1759,6 +1766,13 @@ __global__ __launch_bounds__(Tile<R>::THREADS, low_gen_min_waves<R>()) void bwd_
     build_rz_tables<R>(rz_tab, lc, G, t);
     if (t < (uint32_t)T) {
         tile_coef[t] = lc[t];  // every bit of the low tile is a rotated wire
+    }
+    if constexpr (kLayRows) {
+        // (offsets within the low tile map to address bits below n <= 32)
+        lay_row[t] = (uint32_t)lay_addr(omap<R, 0>(t, 0), G.lay_out);
+        if (t == 0) {
+            lay_step = lay_addr(uint64_t{1} << (T - R), G.lay_out);
+        }
     }
     if constexpr (kChunksLds) {
         for (int i = (int)t; i < kChainTabEntries - kChainC0; i += TL::THREADS) {
@@ -1874,8 +1888,13 @@ __global__ __launch_bounds__(Tile<R>::THREADS, low_gen_min_waves<R>()) void bwd_
         // lambda in layout G.lay_out (a bit permutation: mapping 0's offset and
         // its register step, tile bit T - R, map bit by bit)
         TileAddr ta{};
-        ta.offA = lay_addr(omap<R, 0>(tl, 0), G.lay_out);
-        ta.strideA = lay_addr(uint64_t{1} << (T - R), G.lay_out);
+        if constexpr (kLayRows) {
+            ta.offA = lay_row[tl];
+            ta.strideA = lay_step;
+        } else {
+            ta.offA = lay_addr(omap<R, 0>(tl, 0), G.lay_out);
+            ta.strideA = lay_addr(uint64_t{1} << (T - R), G.lay_out);
+        }
         store_lam<R, 0, false>(l, lam_out, G.tile_out[g], ta, tl);
     }
     __syncthreads();
@@ -1902,6 +1921,10 @@ void launch_bwd_low_gen(int R, double2 *lam_out, const WireCoef *layer_coef, con
     if (g.pair_lo != 0 || g.pair_hi != Tile<4>::T) {
         // the kernel adds every in-tile pair (resolved at compile time)
         throw std::invalid_argument("generating low pass: the pair range must be the whole tile");
+    }
+    if (g.n > 32) {
+        // the threads' store offsets in the layout are held in 32 bits
+        throw std::invalid_argument("generating low pass: at most 32 qubits");
     }
     if (R == 4) {
         bwd_low_gen_kernel<4><<<grid, Tile<4>::THREADS, 0, s>>>(lam_out, layer_coef, g, partials,
