@@ -3099,16 +3099,26 @@ __global__ __launch_bounds__(256, 2) void bwd_top_l0_kernel(
 // Environments of the fixed wires v in [4, h) of the top-group tiles from the
 // per-tile sums (waves partials per tile):
 // E_v[b] = sum_{g : i_v(g) = b} S(g) prod_{u in [4, h), u != v} u_u(i_u(g)),
-// i_u(g) = j_u ^ j_{u-1}, j_{3+m} = bit m of g.  Block (v - 4, chunk) writes
-// 4 doubles.
+// i_u(g) = j_u ^ j_{u-1}, j_{3+m} = bit m of g.  One thread per tile forms all
+// of them from prefix and suffix products of the tile's factors (one read of
+// the tile sums, O(h) products); block `chunk` writes 4 doubles per wire at
+// ((v - 4) * chunks + chunk) * 4.
+constexpr int kMaxFixedWires = 18;  // h - 4 = n - 13 for n <= 31
+
 __global__ __launch_bounds__(256) void layer0_fixed_env_kernel(
     const double2 *__restrict__ tile_sums, int waves, uint64_t num_tiles,
     const WireCoef *__restrict__ lc0, int h, double *__restrict__ partials) {
-    __shared__ double red[4][4];
-    const int v = 4 + (int)blockIdx.y;
+    constexpr int M = kMaxFixedWires;
+    __shared__ double red[4][4 * M];
+    const int m = h - 4;
     const int t = threadIdx.x;
-    double2 e0 = make_double2(0.0, 0.0);
-    double2 e1 = make_double2(0.0, 0.0);
+    double2 e0[M];
+    double2 e1[M];
+#pragma unroll
+    for (int i = 0; i < M; ++i) {
+        e0[i] = make_double2(0.0, 0.0);
+        e1[i] = make_double2(0.0, 0.0);
+    }
     const uint64_t stride = (uint64_t)gridDim.x * 256;
     for (uint64_t g = (uint64_t)blockIdx.x * 256 + t; g < num_tiles; g += stride) {
         double2 f = tile_sums[g * waves];
@@ -3117,31 +3127,49 @@ __global__ __launch_bounds__(256) void layer0_fixed_env_kernel(
             f.x += p.x;
             f.y += p.y;
         }
-        for (int u = 4; u < h; ++u) {
-            if (u != v) {
-                f = cmul(f, u_of(lc0[u], (int)(((g >> (u - 3)) ^ (g >> (u - 4))) & 1u)));
+        // prefix[i] = f * prod_{k < i} u_{4+k}, then the suffix products
+        // from the top: E_{4+i} takes prefix[i] * prod_{k > i} u_{4+k}
+        double2 uu[M];
+        double2 pre[M];
+#pragma unroll
+        for (int i = 0; i < M; ++i) {
+            if (i < m) {
+                uu[i] = u_of(lc0[4 + i], (int)(((g >> (i + 1)) ^ (g >> i)) & 1u));
+                pre[i] = f;
+                f = cmul(f, uu[i]);
             }
         }
-        if (((g >> (v - 3)) ^ (g >> (v - 4))) & 1u) {
-            e1.x += f.x;
-            e1.y += f.y;
-        } else {
-            e0.x += f.x;
-            e0.y += f.y;
+        double2 suf = make_double2(1.0, 0.0);
+#pragma unroll
+        for (int i = M - 1; i >= 0; --i) {
+            if (i < m) {
+                const double2 val = cmul(pre[i], suf);
+                const bool bit = (((g >> (i + 1)) ^ (g >> i)) & 1u) != 0;
+                e0[i].x += bit ? 0.0 : val.x;
+                e0[i].y += bit ? 0.0 : val.y;
+                e1[i].x += bit ? val.x : 0.0;
+                e1[i].y += bit ? val.y : 0.0;
+                suf = cmul(suf, uu[i]);
+            }
         }
     }
-    const double vals[4] = {e0.x, e0.y, e1.x, e1.y};
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
-        const double r = wave_sum(vals[c]);
-        if ((t & 63) == 0) {
-            red[t >> 6][c] = r;
+    for (int i = 0; i < M; ++i) {
+        if (i < m) {
+            const double vals[4] = {e0[i].x, e0[i].y, e1[i].x, e1[i].y};
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const double r = wave_sum(vals[c]);
+                if ((t & 63) == 0) {
+                    red[t >> 6][4 * i + c] = r;
+                }
+            }
         }
     }
     __syncthreads();
-    if (t < 4) {
-        partials[((uint64_t)blockIdx.y * gridDim.x + blockIdx.x) * 4 + t] =
-            red[0][t] + red[1][t] + red[2][t] + red[3][t];
+    for (int k = t; k < 4 * m; k += 256) {
+        partials[((uint64_t)(k >> 2) * gridDim.x + blockIdx.x) * 4 + (k & 3)] =
+            red[0][k] + red[1][k] + red[2][k] + red[3][k];
     }
 }
 
@@ -3203,10 +3231,12 @@ void launch_layer0_top_finish(const double2 *tile_sums, uint64_t num_tiles, cons
                               int chunks, double *grad, const double *scale, hipStream_t s,
                               double *energy_out) {
     const int h = n - 9;
+    if (h - 4 > kMaxFixedWires) {
+        throw std::invalid_argument("layer-0 contraction in the top pass: too many fixed wires");
+    }
     if (h > 4) {
-        dim3 grid(chunks, h - 4);
-        layer0_fixed_env_kernel<<<grid, 256, 0, s>>>(tile_sums, Tile<4>::WAVES, num_tiles, lc0, h,
-                                                      fixed_partials);
+        layer0_fixed_env_kernel<<<chunks, 256, 0, s>>>(tile_sums, Tile<4>::WAVES, num_tiles, lc0, h,
+                                                       fixed_partials);
     }
     layer0_finish_top_kernel<<<n, 64, 0, s>>>(blk_partials, blocks, fixed_partials, chunks, lc0, n,
                                               grad, scale, energy_out);
