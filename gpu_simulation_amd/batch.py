@@ -125,7 +125,7 @@ class RingIsingBatch:
 def main(argv=None) -> None:
     """Evaluate an uneven mix a few times (a fixed workload for profilers):
 
-        python -m gpu_simulation_amd.batch --count 2048 --evals 3 [--dense-only]
+        python -m gpu_simulation_amd.batch --count 2048 --evals 3 [--dense-only] [--dense-max N]
     """
     import argparse
     import time
@@ -135,6 +135,9 @@ def main(argv=None) -> None:
     p.add_argument("--seed", type=int, default=3)
     p.add_argument("--evals", type=int, default=3)
     p.add_argument("--dense-only", action="store_true", help="keep only the n <= 8 circuits")
+    p.add_argument("--dense-max", type=int, default=8,
+                   help="circuits with n <= this take dense_scan (the autotuner's choice on the "
+                        "2048-circuit mix is 4)")
     p.add_argument("--threads", type=int, default=256)
     p.add_argument("--chunks", default="3,3,3,3,3,3,3,3,3,2,2,3,4,4,4",
                    help="rotation chunk width per qubit count 0..14")
@@ -146,7 +149,7 @@ def main(argv=None) -> None:
     if args.dense_only:
         keep = [i for i, s in enumerate(specs) if s.num_qubits <= 8]
         specs, params = [specs[i] for i in keep], [params[i] for i in keep]
-    batch = RingIsingBatch(specs)
+    batch = RingIsingBatch(specs, dense_max_qubits=args.dense_max)
     batch.set_params(params)
     if args.autotune:
         batch.autotune(repeats=2)
